@@ -1,0 +1,191 @@
+"""ctypes front-end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE, NOT PRODUCT CODE: imported only by tests/,
+__graft_entry__.smoke() and the cpu_baseline leg of bench.py, always as the
+checker.  The oracle is a clean-room restatement of the reference em2d
+timestep (openmp/em2d); see zpic_oracle.hpp for the per-function citations.
+
+Also holds the reader for the tagged-record golden files written by
+oracle/ref_driver.cpp (u32 name length, name, dtype char, u64 count, payload).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+
+UDIST = {"none": 0, "cold": 1, "thermal": 2, "thermal_corr": 3}
+FILTER = {"none": 0, "binomial": 1, "compensated": 2}
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "oracle"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp, i, f, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [i, i, i, i, i, f, f, d]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_filter.argtypes = [vp, i, i, i]
+        L.orc_add_species.argtypes = [vp, f, i, i, i, vp, vp, i, f]
+        L.orc_advance.argtypes = [vp, i]
+        L.orc_stage.argtypes = [vp, i, i]
+        L.orc_geom.argtypes = [vp, vp]
+        L.orc_np_total.restype = ctypes.c_longlong
+        L.orc_np_total.argtypes = [vp, i]
+        L.orc_nmove.restype = ctypes.c_ulonglong
+        L.orc_nmove.argtypes = [vp, i]
+        L.orc_get_particles.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        L.orc_set_particles.argtypes = [vp, i, vp, vp, vp, vp]
+        L.orc_get_field.argtypes = [vp, i, vp]
+        L.orc_set_field.argtypes = [vp, i, vp]
+        L.orc_energies.argtypes = [vp, vp]
+        L.orc_raw_energy.restype = d
+        L.orc_raw_energy.argtypes = [vp, i]
+        L.orc_charge.restype = ctypes.c_longlong
+        L.orc_charge.argtypes = [vp, i, vp]
+        L.orc_iter.argtypes = [vp]
+        L.orc_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class OracleSim:
+    """em2d simulation on the CPU oracle (prec 4 = FP32 reference arithmetic, 8 = FP64)."""
+
+    STAGES = {"push": 0, "move": 1, "sort": 2, "species": 3, "zero_j": 4,
+              "current": 5, "yee": 6, "add_gc": 7, "copy_gc": 8, "filter": 9}
+
+    def __init__(self, ntiles, nx, box, dt, prec=4, filter=None):
+        L = lib()
+        self.prec = prec
+        self.real = np.float32 if prec == 4 else np.float64
+        h = L.orc_create(prec, ntiles[0], ntiles[1], nx[0], nx[1], box[0], box[1], dt)
+        if not h:
+            raise RuntimeError(L.orc_last_error().decode())
+        self.h = h
+        g = np.zeros(8, np.int32)
+        L.orc_geom(h, _p(g))
+        self.ntx, self.nty, self.nx, self.ny, self.ext_x, self.ext_y, self.vol, self.off = map(int, g)
+        self.ntiles = self.ntx * self.nty
+        self.nspecies = 0
+        if filter:
+            self.set_filter(*filter)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def set_filter(self, kind, direction=0, order=1):
+        self._chk(lib().orc_set_filter(self.h, FILTER[kind] if isinstance(kind, str) else kind,
+                                       direction, order))
+
+    def add_species(self, m_q, ppc, udist="thermal_corr", uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2, n0=1.0):
+        uth = np.asarray(uth, np.float32)
+        ufl = np.asarray(ufl, np.float32)
+        self._chk(lib().orc_add_species(self.h, m_q, ppc[0], ppc[1], UDIST[udist], _p(uth), _p(ufl), npmin, n0))
+        self.nspecies += 1
+
+    def advance(self, n=1):
+        self._chk(lib().orc_advance(self.h, n))
+
+    def stage(self, what, k=0):
+        self._chk(lib().orc_stage(self.h, self.STAGES[what], k))
+
+    def np_total(self, k):
+        return int(lib().orc_np_total(self.h, k))
+
+    def nmove(self, k):
+        return int(lib().orc_nmove(self.h, k))
+
+    def particles(self, k):
+        n = self.np_total(k)
+        np_ = np.zeros(self.ntiles, np.int32)
+        off = np.zeros(self.ntiles, np.int32)
+        ix = np.zeros((n, 2), np.int32)
+        x = np.zeros((n, 2), self.real)
+        u = np.zeros((n, 3), self.real)
+        self._chk(lib().orc_get_particles(self.h, k, _p(np_), _p(off), _p(ix), _p(x), _p(u)))
+        return {"np": np_, "offset": off, "ix": ix, "x": x, "u": u}
+
+    def set_particles(self, k, np_, ix, x, u):
+        np_ = np.ascontiguousarray(np_, np.int32)
+        ix = np.ascontiguousarray(ix, np.int32)
+        x = np.ascontiguousarray(x, self.real)
+        u = np.ascontiguousarray(u, self.real)
+        self._chk(lib().orc_set_particles(self.h, k, _p(np_), _p(ix), _p(x), _p(u)))
+
+    def field(self, which):
+        out = np.zeros((self.ntiles * self.vol, 3), self.real)
+        self._chk(lib().orc_get_field(self.h, "EBJ".index(which), _p(out)))
+        return out
+
+    def set_field(self, which, data):
+        data = np.ascontiguousarray(data, self.real)
+        assert data.size == self.ntiles * self.vol * 3
+        self._chk(lib().orc_set_field(self.h, "EBJ".index(which), _p(data)))
+
+    def energies(self):
+        out = np.zeros(self.nspecies + 6, np.float64)
+        self._chk(lib().orc_energies(self.h, _p(out)))
+        return out
+
+    def raw_energy(self, k):
+        return float(lib().orc_raw_energy(self.h, k))
+
+    def charge(self, k):
+        ext_x, ext_y = self.nx + 1, self.ny + 1
+        vol = (ext_x * ext_y + 3) & ~3
+        out = np.zeros(self.ntiles * vol, self.real)
+        lib().orc_charge(self.h, k, _p(out))
+        return out, (ext_x, ext_y, vol)
+
+
+def weibel(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), box=None, dt=0.07, uth=(0.1, 0.1, 0.1),
+           ufl=(0.0, 0.0, 0.6), filter=None, prec=4):
+    """The reference Weibel setup (openmp/em2d/main.cpp:179-259) on the oracle."""
+    if box is None:
+        box = (np.float32(ntiles[0] * nx[0] / 10.0), np.float32(ntiles[1] * nx[1] / 10.0))
+    sim = OracleSim(ntiles, nx, box, float(np.float32(dt)), prec=prec, filter=filter)
+    sim.add_species(-1.0, ppc, "thermal_corr", uth, ufl)
+    sim.add_species(+1.0, ppc, "thermal_corr", uth, tuple(-v for v in ufl))
+    return sim
+
+
+def read_records(path):
+    """Reads a tagged-record file into {name: ndarray}."""
+    out = {}
+    dt = {"i": np.int32, "f": np.float32, "d": np.float64, "q": np.uint64}
+    with open(path, "rb") as fp:
+        buf = fp.read()
+    pos = 0
+    while pos < len(buf):
+        n = int(np.frombuffer(buf, np.uint32, 1, pos)[0]); pos += 4
+        name = buf[pos:pos + n].decode(); pos += n
+        code = chr(buf[pos]); pos += 1
+        cnt = int(np.frombuffer(buf, np.uint64, 1, pos)[0]); pos += 8
+        t = np.dtype(dt[code])
+        out[name] = np.frombuffer(buf, t, cnt, pos).copy()
+        pos += cnt * t.itemsize
+    return out

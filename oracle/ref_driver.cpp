@@ -1,0 +1,213 @@
+// Test-infrastructure driver for the REFERENCE em2d OpenMP code.
+//
+// This file is compiled against the unmodified reference sources
+// (openmp/em2d/*.cpp, read in place from the reference checkout by
+// oracle/Makefile) into oracle/_ref/ref_driver.  It is a checker only: it
+// produces golden vectors (dump mode) that pin the oracle restatement, and it
+// times the reference CPU path (bench mode) as the CPU baseline.  Nothing in
+// the product links or runs it.
+//
+// It reproduces the call sequence of the reference Weibel driver
+// (openmp/em2d/main.cpp:179-259): Simulation(ntiles, nx, box, dt) with
+// `float dt` widened to double, two ThermalCorr species added in the order
+// electrons (id 1) then positrons (id 2), then Simulation::advance() in a loop.
+//
+// Usage:
+//   ref_driver dump  <out.bin> key=value ...
+//   ref_driver bench key=value ...
+// keys: ntiles=X,Y nx=X,Y ppc=X,Y box=X,Y dt=F uth=X,Y,Z ufl=X,Y,Z
+//       filter=none|bx<N>|by<N>|cx<N>|cy<N>  snap=s0,s1,...  steps=N warmup=N
+//       energies=1 (record energies after every step up to the last snapshot)
+
+#include "simulation.h"
+#include "species.h"
+#include "udist.h"
+#include "filter.h"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <map>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+// Exposes the protected particle buffer of the reference Species class.
+class ProbeSpecies : public Species {
+  public:
+    using Species::Species;
+    Particles * parts() { return particles; }
+};
+
+struct Args {
+    uint2 ntiles = make_uint2( 8, 8 );
+    uint2 nx = make_uint2( 16, 16 );
+    uint2 ppc = make_uint2( 8, 8 );
+    float2 box = make_float2( 12.8f, 12.8f );
+    float dt = 0.07f;
+    float3 uth = make_float3( 0.1f, 0.1f, 0.1f );
+    float3 ufl = make_float3( 0.0f, 0.0f, 0.6f );
+    std::string filter = "none";
+    std::vector<int> snap;
+    int steps = 10;
+    int warmup = 1;
+    int energies = 0;
+};
+
+static std::vector<std::string> split( const std::string & s, char c ) {
+    std::vector<std::string> out; std::string cur;
+    for ( char ch : s ) { if ( ch == c ) { out.push_back( cur ); cur.clear(); } else cur += ch; }
+    out.push_back( cur );
+    return out;
+}
+
+static Args parse( int argc, char ** argv, int first ) {
+    Args a;
+    for ( int i = first; i < argc; i++ ) {
+        std::string kv = argv[i];
+        auto p = kv.find( '=' );
+        if ( p == std::string::npos ) { fprintf( stderr, "bad arg %s\n", argv[i] ); exit( 2 ); }
+        std::string k = kv.substr( 0, p ), v = kv.substr( p + 1 );
+        auto f = split( v, ',' );
+        if ( k == "ntiles" ) a.ntiles = make_uint2( atoi( f[0].c_str() ), atoi( f[1].c_str() ) );
+        else if ( k == "nx" ) a.nx = make_uint2( atoi( f[0].c_str() ), atoi( f[1].c_str() ) );
+        else if ( k == "ppc" ) a.ppc = make_uint2( atoi( f[0].c_str() ), atoi( f[1].c_str() ) );
+        else if ( k == "box" ) a.box = make_float2( strtof( f[0].c_str(), 0 ), strtof( f[1].c_str(), 0 ) );
+        else if ( k == "dt" ) a.dt = strtof( v.c_str(), 0 );
+        else if ( k == "uth" ) a.uth = make_float3( strtof( f[0].c_str(), 0 ), strtof( f[1].c_str(), 0 ), strtof( f[2].c_str(), 0 ) );
+        else if ( k == "ufl" ) a.ufl = make_float3( strtof( f[0].c_str(), 0 ), strtof( f[1].c_str(), 0 ), strtof( f[2].c_str(), 0 ) );
+        else if ( k == "filter" ) a.filter = v;
+        else if ( k == "snap" ) { for ( auto & s : f ) a.snap.push_back( atoi( s.c_str() ) ); }
+        else if ( k == "steps" ) a.steps = atoi( v.c_str() );
+        else if ( k == "warmup" ) a.warmup = atoi( v.c_str() );
+        else if ( k == "energies" ) a.energies = atoi( v.c_str() );
+        else { fprintf( stderr, "unknown key %s\n", k.c_str() ); exit( 2 ); }
+    }
+    return a;
+}
+
+// Tagged record: u32 name length, name bytes, 1 dtype char, u64 count, payload.
+static void put( FILE * fp, const std::string & name, char dtype, uint64_t count, const void * data ) {
+    uint32_t n = name.size();
+    size_t esize = ( dtype == 'd' || dtype == 'q' ) ? 8 : 4;
+    fwrite( &n, 4, 1, fp ); fwrite( name.data(), 1, n, fp );
+    fwrite( &dtype, 1, 1, fp ); fwrite( &count, 8, 1, fp );
+    if ( count ) fwrite( data, esize, count, fp );
+}
+
+static void set_filter( Simulation & sim, const std::string & f ) {
+    if ( f == "none" ) return;
+    unsigned order = atoi( f.c_str() + 2 );
+    coord::cart dir = ( f[1] == 'x' ) ? coord::x : coord::y;
+    if ( f[0] == 'b' ) sim.current.set_filter( Filter::Binomial( dir, order ) );
+    else if ( f[0] == 'c' ) sim.current.set_filter( Filter::Compensated( dir, order ) );
+    else { fprintf( stderr, "bad filter %s\n", f.c_str() ); exit( 2 ); }
+}
+
+static void snapshot( FILE * fp, int step, Simulation & sim, std::vector<ProbeSpecies*> & sp ) {
+    std::string pre = "s" + std::to_string( step ) + "/";
+    for ( size_t k = 0; k < sp.size(); k++ ) {
+        Particles * p = sp[k]->parts();
+        unsigned nt = p->ntiles.x * p->ntiles.y;
+        std::string sn = pre + "sp" + std::to_string( k ) + "/";
+        put( fp, sn + "np", 'i', nt, p->np );
+        put( fp, sn + "offset", 'i', nt, p->offset );
+        std::vector<int> ix; std::vector<float> x, u;
+        for ( unsigned t = 0; t < nt; t++ ) {
+            for ( int i = p->offset[t]; i < p->offset[t] + p->np[t]; i++ ) {
+                ix.push_back( p->ix[i].x ); ix.push_back( p->ix[i].y );
+                x.push_back( p->x[i].x ); x.push_back( p->x[i].y );
+                u.push_back( p->u[i].x ); u.push_back( p->u[i].y ); u.push_back( p->u[i].z );
+            }
+        }
+        put( fp, sn + "ix", 'i', ix.size(), ix.data() );
+        put( fp, sn + "x", 'f', x.size(), x.data() );
+        put( fp, sn + "u", 'f', u.size(), u.data() );
+        double e = sp[k]->get_energy();
+        put( fp, sn + "energy", 'd', 1, &e );
+        uint64_t nm = sp[k]->get_nmove();
+        put( fp, sn + "nmove", 'q', 1, &nm );
+        // charge density as Species::save_charge builds it (species.cpp:2508-2522)
+        bnd<unsigned int> cgc;
+        cgc.x = { 0, 1 };
+        cgc.y = { 0, 1 };
+        grid<float> charge( p->ntiles, p->nx, cgc );
+        charge.zero();
+        sp[k]->deposit_charge( charge );
+        charge.add_from_gc();
+        put( fp, sn + "charge", 'f', charge.buffer_size(), charge.d_buffer );
+    }
+    size_t fsz = sim.emf.E->buffer_size() * 3;
+    put( fp, pre + "E", 'f', fsz, sim.emf.E->d_buffer );
+    put( fp, pre + "B", 'f', fsz, sim.emf.B->d_buffer );
+    put( fp, pre + "J", 'f', fsz, sim.current.J->d_buffer );
+    double3 eE, eB;
+    sim.emf.get_energy( eE, eB );
+    double fe[6] = { eE.x, eE.y, eE.z, eB.x, eB.y, eB.z };
+    put( fp, pre + "field_energy", 'd', 6, fe );
+}
+
+int main( int argc, char ** argv ) {
+    if ( argc < 2 ) { fprintf( stderr, "usage: ref_driver dump|bench ...\n" ); return 2; }
+    std::string mode = argv[1];
+    bool dump = ( mode == "dump" );
+    Args a = parse( argc, argv, dump ? 3 : 2 );
+
+    Simulation sim( a.ntiles, a.nx, a.box, (double) a.dt );
+    set_filter( sim, a.filter );
+
+    ProbeSpecies electrons( "electrons", -1.0f, a.ppc );
+    electrons.set_udist( UDistribution::ThermalCorr( a.uth, a.ufl ) );
+    sim.add_species( electrons );
+
+    ProbeSpecies positrons( "positrons", +1.0f, a.ppc );
+    positrons.set_udist( UDistribution::ThermalCorr( a.uth, make_float3( -a.ufl.x, -a.ufl.y, -a.ufl.z ) ) );
+    sim.add_species( positrons );
+
+    std::vector<ProbeSpecies*> sp = { &electrons, &positrons };
+
+    if ( dump ) {
+        FILE * fp = fopen( argv[2], "wb" );
+        if ( !fp ) { perror( "fopen" ); return 1; }
+        int last = 0;
+        for ( int s : a.snap ) last = std::max( last, s );
+        std::vector<double> ene;
+        for ( int step = 0; step <= last; step++ ) {
+            if ( step > 0 ) sim.advance();
+            for ( int s : a.snap ) if ( s == step ) snapshot( fp, step, sim, sp );
+            if ( a.energies && step > 0 ) {
+                double3 eE, eB;
+                sim.emf.get_energy( eE, eB );
+                ene.push_back( electrons.get_energy() );
+                ene.push_back( positrons.get_energy() );
+                ene.push_back( eE.x ); ene.push_back( eE.y ); ene.push_back( eE.z );
+                ene.push_back( eB.x ); ene.push_back( eB.y ); ene.push_back( eB.z );
+            }
+        }
+        if ( a.energies ) put( fp, "energies", 'd', ene.size(), ene.data() );
+        fclose( fp );
+        return 0;
+    }
+
+    // bench: warmup steps, then `steps` timed steps of Simulation::advance()
+    for ( int i = 0; i < a.warmup; i++ ) sim.advance();
+    uint64_t n0 = sim.get_nmove();
+    auto t0 = std::chrono::steady_clock::now();
+    for ( int i = 0; i < a.steps; i++ ) sim.advance();
+    auto t1 = std::chrono::steady_clock::now();
+    uint64_t moved = sim.get_nmove() - n0;
+    double sec = std::chrono::duration<double>( t1 - t0 ).count();
+    int threads = 1;
+#ifdef _OPENMP
+    threads = omp_get_max_threads();
+#endif
+    printf( "{\"particles_per_ns\": %.6g, \"ms_per_step\": %.6g, \"steps\": %d, \"threads\": %d, "
+            "\"particles\": %llu}\n",
+            moved / sec / 1e9, 1e3 * sec / a.steps, a.steps, threads,
+            (unsigned long long) ( electrons.np_total() + positrons.np_total() ) );
+    return 0;
+}

@@ -1,0 +1,92 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/ref_driver (the unmodified reference openmp/em2d sources,
+built by `make -C oracle ref`) single-threaded, so that the within-tile
+particle order and every floating-point summation order are the reference's
+deterministic serial ones.  Outputs:
+
+  weibel_t2x2_n16_ppc2.npz  full states (particles per tile, E, B, J,
+                             energies) at steps 0, 1 and 5 of a 32x32 Weibel
+                             run (2x2 tiles of 16x16, ppc 2x2).
+  digests.json              SHA-256 of every state array (raw little-endian
+                             bytes in tile order) and exact energies for the
+                             larger cases listed in CASES.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import pyoracle  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+FULL = dict(name="weibel_t2x2_n16_ppc2",
+            args=["ntiles=2,2", "nx=16,16", "ppc=2,2", "box=3.2,3.2", "dt=0.07",
+                  "uth=0.1,0.1,0.1", "ufl=0,0,0.6", "filter=none"],
+            snap=[0, 1, 5])
+
+# name -> (driver args, snapshot steps)
+CASES = {
+    # the 128x128 Weibel problem with 8x8 ppc (2.1 M particles)
+    "weibel_c1": (["ntiles=8,8", "nx=16,16", "ppc=8,8", "box=12.8,12.8", "dt=0.07",
+                   "uth=0.1,0.1,0.1", "ufl=0,0,0.6", "filter=none"], [0, 1, 10]),
+    # non-square tiles and ppc, 3x2 tiles, x binomial filter
+    "odd_tiles_bx1": (["ntiles=3,2", "nx=8,12", "ppc=3,2", "box=2.4,2.4", "dt=0.07",
+                       "uth=0.1,0.1,0.1", "ufl=0,0,0.6", "filter=bx1"], [0, 1, 20]),
+    # one tile along x: every x neighbour is the tile itself; y filter order 2
+    "wrap_1x3_by2": (["ntiles=1,3", "nx=8,8", "ppc=2,2", "box=0.8,2.4", "dt=0.05",
+                      "uth=0.1,0.1,0.1", "ufl=0,0,0.6", "filter=by2"], [0, 1, 20]),
+    # compensated binomial filter along x
+    "comp_cx1": (["ntiles=2,2", "nx=16,16", "ppc=2,2", "box=3.2,3.2", "dt=0.07",
+                  "uth=0.1,0.1,0.1", "ufl=0,0,0.6", "filter=cx1"], [0, 1, 10]),
+    # hot, fast plasma on small tiles: many tile and double cell crossings
+    "hot_small_tiles": (["ntiles=4,4", "nx=4,4", "ppc=4,4", "box=1.6,1.6", "dt=0.07",
+                         "uth=0.3,0.3,0.3", "ufl=0.5,0.2,0.9", "filter=none"], [1, 10, 30]),
+}
+
+
+def run_driver(args, snap, path):
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    cmd = [pyoracle.REF_DRIVER, "dump", path] + args + ["snap=" + ",".join(map(str, snap)), "energies=1"]
+    subprocess.check_call(cmd, env=env, stdout=subprocess.DEVNULL)
+    return pyoracle.read_records(path)
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    if not os.path.exists(pyoracle.REF_DRIVER):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
+    with tempfile.TemporaryDirectory() as td:
+        rec = run_driver(FULL["args"], FULL["snap"], os.path.join(td, "full.bin"))
+        arrays = {k.replace("/", "__"): v for k, v in rec.items()}
+        arrays["args"] = np.array(FULL["args"])
+        np.savez_compressed(os.path.join(OUT, FULL["name"] + ".npz"), **arrays)
+
+        dig = {}
+        for name, (args, snap) in CASES.items():
+            rec = run_driver(args, snap, os.path.join(td, name + ".bin"))
+            entry = {"args": args, "snap": snap, "sha256": {}, "values": {}}
+            for k, v in rec.items():
+                if k.endswith("energy") or k.endswith("nmove") or k == "energies" or k.endswith("field_energy"):
+                    entry["values"][k] = [float(x) if v.dtype.kind == "f" else int(x) for x in v]
+                if k != "energies":
+                    entry["sha256"][k] = digest(v)
+            dig[name] = entry
+        with open(os.path.join(OUT, "digests.json"), "w") as fp:
+            json.dump(dig, fp, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
