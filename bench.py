@@ -1,0 +1,179 @@
+"""Benchmark of the MI355X em2d PIC timestep (push + deposit + sort + FDTD).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload weibel2048|weibel128]
+
+Workload (default): the 2D Weibel instability at single-GPU production size,
+2048x2048 cells (128x128 tiles of 16x16, dx = dy = 0.1, dt = 0.07, periodic),
+two species (electrons / positrons) with ppc 8x8 each (536,870,912
+particles, FP32 SoA), ThermalCorr momenta uth = 0.1, drift uz = +-0.6, and
+the binomial current filter along x (order 1).  Synthetic data: the initial
+state is the reference's deterministic lattice + seeded thermal load, built
+on the host and uploaded before the timed region.
+
+One JSON line is printed.  `value` = particles moved per ns over the whole
+timed steps (all stages, inputs resident in HBM).  The per-stage split comes
+from HIP events recorded on the library's stream inside the timed region;
+the roofline entry refers to the fused push + move + deposit kernel.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "zpic-parallel_amd"))
+
+METRIC = "particles/ns (push+deposit+sort) 2D Weibel; wall-time at ≤5% growth-rate err"
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    "weibel2048": dict(ntiles=(128, 128), nx=(16, 16), ppc=(8, 8), filter=("binomial", 0, 1)),
+    "weibel128": dict(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), filter=None),
+}
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_push_move_deposit_weibel2048.json")
+
+
+def cpu_baseline():
+    """Reference openmp/em2d (compiled from its sources into oracle/_ref) on
+    the host cores: the same Weibel physics on a bounded 512x512 sample."""
+    threads = max(1, min(16, os.cpu_count() or 1))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="close")
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    port = os.path.join(ROOT, "oracle", "oracle_bench")
+    sample = "Weibel 512x512 (32x32 tiles of 16x16), ppc 8x8, 2 species (33.5M particles), binomial x filter, 1 warm-up + 10 timed steps"
+    try:
+        if os.path.exists(ref):
+            cmd = [ref, "bench", "ntiles=32,32", "nx=16,16", "ppc=8,8", "box=51.2,51.2", "dt=0.07",
+                   "filter=bx1", "steps=10", "warmup=1"]
+            kind = "reference"
+        elif os.path.exists(port):
+            cmd = [port, "32", "32", "16", "16", "8", "8", "10", "1"]
+            kind = "port"
+            sample = sample.replace(", binomial x filter", "")
+        else:
+            return None
+        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=300)
+        line = [l for l in out.stdout.splitlines() if l.startswith("{")][-1]
+        r = json.loads(line)
+        model = ""
+        try:
+            for l in open("/proc/cpuinfo"):
+                if l.startswith("model name"):
+                    model = l.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        return {"value": r["particles_per_ns"], "unit": "particles/ns", "cores": r["threads"], "kind": kind,
+                "sample": sample + f"; {r['ms_per_step']:.1f} ms/step on {r['threads']} threads ({model})"}
+    except Exception as e:  # the baseline is reported, never fatal
+        return {"value": None, "unit": "particles/ns", "cores": threads, "kind": "reference",
+                "sample": f"failed: {e}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="weibel2048", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the step would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import pyzpic
+
+    wl = WORKLOADS[args.workload]
+    print(f"[bench] building {args.workload}: ntiles {wl['ntiles']} tile {wl['nx']} ppc {wl['ppc']}",
+          file=sys.stderr, flush=True)
+    t_init = time.perf_counter()
+    sim = pyzpic.weibel(ntiles=wl["ntiles"], nx=wl["nx"], ppc=wl["ppc"], filter=wl["filter"])
+    sim.sync()
+    t_init = time.perf_counter() - t_init
+    npart = sum(s.np_total() for s in sim.species)
+    ncells = wl["ntiles"][0] * wl["nx"][0] * wl["ntiles"][1] * wl["nx"][1]
+    print(f"[bench] {npart} particles, {ncells} cells, init {t_init:.1f} s; warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+
+    sim.advance(args.warmup)
+    sim.sync()
+    sim.set_timing(True)
+    n0 = sim.get_nmove()
+    t0 = time.perf_counter()
+    sim.advance(args.steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    moved = sim.get_nmove() - n0
+    tm = sim.get_timing()
+    sec = t1 - t0
+    ms_step = 1e3 * sec / args.steps
+    print(f"[bench] {args.steps} steps, {ms_step:.3f} ms/step, split {tm}", file=sys.stderr, flush=True)
+
+    nsp = len(sim.species)
+    pds_ms = tm["push_move"] + tm["sort"]
+    # dominant kernel: fused push + move + deposit, one launch per species
+    launch_ms = tm["push_move"] / nsp
+    bytes_per_launch = 56.0 * (npart / nsp) + 36.0 * ncells
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            traffic = json.load(open(TRAFFIC_FILE)).get(args.workload)
+        except Exception:
+            traffic = None
+    # whole-step algorithmic bytes: 56 B per particle + 56 B per tile-crossing
+    # particle (not counted here: conservative) + (96 + 24 n_filter) B per cell
+    nfilt = 1 if wl["filter"] else 0
+    step_bytes = 56.0 * npart + (96.0 + 24.0 * nfilt) * ncells
+
+    result = {
+        "metric": METRIC,
+        "value": moved / (sec * 1e9),
+        "unit": "particles/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload, "grid": [wl["ntiles"][0] * wl["nx"][0], wl["ntiles"][1] * wl["nx"][1]],
+            "tiles": list(wl["ntiles"]), "tile": list(wl["nx"]), "ppc": list(wl["ppc"]), "species": nsp,
+            "particles": npart, "dt": 0.07, "dx": 0.1, "filter": "binomial x order 1" if wl["filter"] else "none",
+            "parallelism": "single GPU",
+        },
+        "push_deposit_sort_particles_per_ns": moved / args.steps / (pds_ms * 1e6) if pds_ms > 0 else None,
+        "stage_ms": {k: round(v, 4) for k, v in tm.items()},
+        "step_hbm_fraction": step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS,
+        "roofline": {
+            "kernel": "k_particles<push,move> (fused push + move + deposit)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": traffic,
+            "launch_ms": launch_ms,
+            "bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
+        "host": platform.node(),
+    }
+    sim.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

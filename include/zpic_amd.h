@@ -1,0 +1,243 @@
+/*
+ * zpic_amd.h -- C ABI of the MI355X em2d particle-in-cell timestep.
+ *
+ * Two layers, both plain C (pointers, sizes, PODs; no HIP or torch types):
+ *
+ *  1. Stage entry points on device buffers.  Each one is one stage of the
+ *     reference's Simulation::advance() and takes an opaque HIP stream
+ *     (`void *stream`, NULL = the library's default stream).  They return 0 on
+ *     success and a negative ZPIC_E* code otherwise; zpic_last_error() gives
+ *     the message.  Work is enqueued only; nothing synchronises the host.
+ *
+ *  2. Object entry points mirroring the reference C++ classes
+ *     (Simulation / Species / EMF / Current), i.e. exactly what the reference's
+ *     own FFI (the Cython declarations openmp/python/em2d/simulation.pxd,
+ *     species.pxd, emf.pxd, current.pxd) binds.  They own device memory and one
+ *     HIP stream per simulation.
+ *
+ * Layouts (the reference's, openmp/em2d/particles.h:82-105, grid.h:77-92):
+ *   particles  tile-major SoA: ix int2[max_part] (cell inside the tile),
+ *              x float2[max_part] (offset in [-0.5,0.5) from the cell centre),
+ *              u float3[max_part] (proper velocity u = gamma v / c);
+ *              tile t owns [offset[t], offset[t] + np[t]).
+ *   fields     tile-major float3 with guard cells: tile t occupies elements
+ *              [t*tile_vol, (t+1)*tile_vol); cell (i,j) of tile t is element
+ *              t*tile_vol + offset + j*ext_x + i, i in [-gc_lower, nx+gc_upper).
+ */
+#ifndef ZPIC_AMD_H_
+#define ZPIC_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ codes */
+#define ZPIC_OK 0
+#define ZPIC_EINVAL (-1)   /* invalid argument / configuration             */
+#define ZPIC_EHIP (-2)     /* HIP runtime error                             */
+#define ZPIC_ERANGE (-3)   /* size beyond a buffer or a stated maximum      */
+#define ZPIC_ESTATE (-4)   /* call not valid in the object's current state  */
+
+/* Last error message of the calling thread ("" if none). */
+const char *zpic_last_error(void);
+
+/* ------------------------------------------------------------- geometry */
+/* Tiled grid geometry (grid<T> fields, grid.h:47-92). */
+typedef struct {
+    int ntiles_x, ntiles_y; /* number of tiles                               */
+    int nx, ny;             /* cells per tile                                */
+    int gc_lower, gc_upper; /* guard cells below / above (same in x and y)   */
+    int ext_x, ext_y;       /* tile size including guard cells               */
+    int tile_vol;           /* elements per tile: ext_x*ext_y rounded to 4   */
+    int offset;             /* element of cell (0,0): gc_lower*ext_x+gc_lower */
+} zpic_geom;
+
+/* Fills a geometry from tile count, tile size and guard cells. */
+int zpic_geom_init(zpic_geom *g, int ntiles_x, int ntiles_y, int nx, int ny, int gc_lower, int gc_upper);
+
+/* Device particle buffer (Particles / ParticleData, particles.h:82-206). */
+typedef struct {
+    int *np;        /* [ntiles] particles per tile                */
+    int *offset;    /* [ntiles] first particle of each tile       */
+    int *ix;        /* int2[max_part]                             */
+    float *x;       /* float2[max_part]                           */
+    float *u;       /* float3[max_part]                           */
+    uint32_t max_part;
+} zpic_particles;
+
+/* Device scratch of the tile sort (ParticleSort, particles.h:20-65): leaver
+ * lists, per-tile direction counts, new counts, per-wave segment counts and
+ * the staged-particle counts, with the sizes given per member. */
+typedef struct {
+    int *idx;       /* [max_part]   leaver list (tile-local index << 4 | direction) */
+    int *npt;       /* [9*ntiles]   particles per direction (4 = staying)           */
+    int *new_np;    /* [ntiles]     particles per tile after the sort               */
+    int *seg;       /* [16*ntiles]  leavers found by each wave segment              */
+    int *tmp_np;    /* [ntiles]     staged particles per tile                       */
+    int *total;     /* [1]          particles after the sort                        */
+} zpic_sort;
+
+/* ----------------------------------------------- stage entry points (1) */
+
+/* Species::push (species.cpp:2302-2335, push_kernel :1626-1682): linear E/B
+ * interpolation + relativistic Boris push of every particle; writes each
+ * tile's sum of u^2/(gamma+1) (double) into tile_energy[ntiles]. */
+int zpic_push(const zpic_particles *p, const zpic_geom *fld, const float *E, const float *B,
+              float alpha, double *tile_energy, int deterministic, void *stream);
+
+/* Species::move(J) (species.cpp:2138-2164, move_deposit_kernel :1400-1493):
+ * position advance, split-segment charge-conserving current deposit into the
+ * tile's J region (accumulated on top of J), cell update, and classification
+ * of tile leavers into `s` for the following zpic_tile_sort. */
+int zpic_move_deposit(const zpic_particles *p, const zpic_geom *fld, float *J,
+                      float dt_dx, float dt_dy, float q, float qnx, float qny,
+                      zpic_sort *s, int deterministic, void *stream);
+
+/* push + move of Species::advance(emf, current) (species.cpp:2051-2068) fused
+ * into one pass over the particles (same results as zpic_push then
+ * zpic_move_deposit). */
+int zpic_push_move_deposit(const zpic_particles *p, const zpic_geom *fld, const float *E,
+                           const float *B, float *J, float alpha, float dt_dx, float dt_dy,
+                           float q, float qnx, float qny, double *tile_energy, zpic_sort *s,
+                           int deterministic, void *stream);
+
+/* Particles::tile_sort (particles.cpp:855-904: bnd_check, update_tile_info,
+ * copy_out, copy_in) as per-tile counting + prefix scan.  Uses the
+ * classification left in `s` by zpic_move_deposit / zpic_push_move_deposit.
+ * `tmp` supplies the staging buffers and receives the new offsets.  The
+ * resulting within-tile order is the reference's single-thread order. */
+int zpic_tile_sort(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, int ntiles_x, int ntiles_y,
+                   int nx, int ny, int periodic_x, int periodic_y, void *stream);
+
+/* grid<T>::add_from_gc / copy_to_gc (grid.h:253-445) on a tile-major grid
+ * of `ncomp` floats per element (3 for vec3grid<float3>, 1 for grid<float>). */
+int zpic_add_from_gc(float *buf, const zpic_geom *g, int ncomp, int periodic_x, int periodic_y, void *stream);
+int zpic_copy_to_gc(float *buf, const zpic_geom *g, int ncomp, int periodic_x, int periodic_y, void *stream);
+
+/* grid<T>::kernel3_x / kernel3_y (grid.h:495-593) on vec3 grids: one pass of
+ * the 3-point convolution [a,b,c] along dir (0 = x, 1 = y), then the guard
+ * copy along that direction. */
+int zpic_kernel3(float *buf, const zpic_geom *g, int dir, float a, float b, float c,
+                 int periodic_x, int periodic_y, void *stream);
+
+/* Current::advance (current.cpp:227-246) for periodic boundaries: guard add,
+ * guard copy and the digital filter (filter.h:9-78).  filter_kind 0 none,
+ * 1 binomial, 2 compensated; filter_dir 0 x / 1 y. */
+int zpic_current_advance(float *J, const zpic_geom *g, int filter_kind, int filter_dir,
+                         int filter_order, int periodic_x, int periodic_y, void *stream);
+
+/* EMF::advance(Current&) (emf.cpp:513-580): B half step, E step with J,
+ * B half step per tile, then guard copy of E and B. */
+int zpic_yee(float *E, float *B, const float *J, const zpic_geom *g, float dt_dx, float dt_dy,
+             float dt, int periodic_x, int periodic_y, void *stream);
+
+/* EMF::get_energy (emf.cpp:662-713): sums of E_c^2, B_c^2 over interior
+ * cells in the reference's order.  out6 is a device pointer to 6 doubles;
+ * tile_scratch holds 6*ntiles doubles. */
+int zpic_field_energy(const float *E, const float *B, const zpic_geom *g, double *tile_scratch,
+                      double *out6, void *stream);
+
+/* Sum of per-tile doubles (the OpenMP reduction of Species::push,
+ * species.cpp:2311): in tile order when deterministic != 0 (the reference's
+ * single-thread result), as a parallel tree otherwise.  out is a device
+ * pointer. */
+int zpic_sum_tiles(const double *tile_values, int ntiles, double *out, int deterministic, void *stream);
+
+/* Species::deposit_charge (species.cpp:2348-2410) into a scalar grid with
+ * guard cells (accumulates), e.g. the gc {0,1} grid of save_charge. */
+int zpic_deposit_charge(const zpic_particles *p, const zpic_geom *g, float *rho, float q,
+                        int deterministic, void *stream);
+
+/* vec3grid::gather (vec3grid.h:76-158): component fc (0,1,2) of the interior
+ * cells into a contiguous [ny*ntiles_y][nx*ntiles_x] device array. */
+int zpic_gather_component(const float *buf, const zpic_geom *g, int ncomp, int fc, float *out, void *stream);
+
+/* Fills n floats with a value (grid::zero / set). */
+int zpic_fill(float *buf, size_t n, float value, void *stream);
+
+/* --------------------------------------------- object entry points (2) */
+typedef struct zpic_simulation zpic_simulation;
+typedef struct zpic_species zpic_species;
+
+/* Simulation(ntiles, nx, box, dt) (simulation.h:39-43).  NULL on error
+ * (e.g. the Courant condition of EMF::EMF, emf.cpp:22-28). */
+zpic_simulation *zpic_simulation_new(unsigned ntiles_x, unsigned ntiles_y, unsigned nx, unsigned ny,
+                                     float box_x, float box_y, double dt);
+void zpic_simulation_free(zpic_simulation *sim);
+
+/* Species(name, m_q, ppc) (species.h:147) with its density / udist setters
+ * (species.h:172-194, udist.h:19-58, density.h:32-60).
+ * udist kind: 0 None, 1 Cold(ufl), 2 Thermal(uth, ufl), 3 ThermalCorr(uth, ufl, npmin). */
+zpic_species *zpic_species_new(const char *name, float m_q, unsigned ppc_x, unsigned ppc_y);
+void zpic_species_free(zpic_species *sp);
+int zpic_species_set_udist(zpic_species *sp, int kind, const float uth[3], const float ufl[3], int npmin);
+int zpic_species_set_density_uniform(zpic_species *sp, float n0);
+
+/* Simulation::add_species (simulation.h:68-71): the simulation keeps a
+ * non-owning reference; the species must outlive it. */
+int zpic_simulation_add_species(zpic_simulation *sim, zpic_species *sp);
+
+/* Simulation::advance (simulation.h:90-107), n times. */
+int zpic_simulation_advance(zpic_simulation *sim, int n);
+
+/* Stages of one Simulation::advance, for stage-level parity checks:
+ * 0 push(k), 1 move(k), 2 tile_sort(k), 3 Species::advance(k),
+ * 4 Current::zero, 5 Current::advance, 6 EMF::advance(current),
+ * 7 J add_from_gc, 8 J copy_to_gc, 9 J filter. */
+int zpic_simulation_stage(zpic_simulation *sim, int stage, int species);
+
+/* Current::set_filter (current.h:137) with Filter::None/Binomial/Compensated. */
+int zpic_current_set_filter(zpic_simulation *sim, int kind, int dir, int order);
+
+/* 1 = deterministic deposit and energy order (bit-for-bit the reference's
+ * single-thread results), 0 = fast atomic order (default). */
+int zpic_simulation_set_deterministic(zpic_simulation *sim, int on);
+
+unsigned zpic_simulation_get_iter(const zpic_simulation *sim);
+double zpic_simulation_get_t(const zpic_simulation *sim);
+uint64_t zpic_simulation_get_nmove(zpic_simulation *sim);
+
+/* Species::get_energy (species.h:365-368) and EMF::get_energy (emf.cpp:662). */
+double zpic_species_get_energy(zpic_species *sp);
+int zpic_emf_get_energy(zpic_simulation *sim, double ene_E[3], double ene_B[3]);
+
+uint64_t zpic_species_get_nmove(const zpic_species *sp);
+uint64_t zpic_species_np_total(zpic_species *sp);
+
+/* Host copies of a species' particles in tile order (compact): np/offset
+ * [ntiles], ix [2n], x [2n], u [3n]; any pointer may be NULL. */
+int zpic_species_get_particles(zpic_species *sp, int *np, int *offset, int *ix, float *x, float *u);
+/* Replaces a species' particles (same compact layout). */
+int zpic_species_set_particles(zpic_species *sp, const int *np, const int *ix, const float *x, const float *u);
+
+/* Whole tile-major field buffers (guards included): which 0 = E, 1 = B, 2 = J. */
+int zpic_field_get(zpic_simulation *sim, int which, float *host_out);
+int zpic_field_set(zpic_simulation *sim, int which, const float *host_in);
+int zpic_field_geom(const zpic_simulation *sim, zpic_geom *g);
+
+/* Species::save_charge's density (species.cpp:2508-2522): tile-major
+ * gc {0,1} scalar grid after the guard add, host copy; returns the element
+ * count or a negative code. */
+long long zpic_species_charge(zpic_species *sp, float *host_out);
+
+/* Species::gather(part::quant, float*) (species.h:515): x, y (global
+ * position in simulation units), ux, uy, uz for quant 0..4, host copy. */
+int zpic_species_gather(zpic_species *sp, int quant, float *host_out);
+
+/* Average kernel time (ms) of the last step's stages measured with HIP
+ * events when timing is enabled: out[0] push+move+deposit, out[1] sort,
+ * out[2] current, out[3] yee, out[4] whole step. */
+int zpic_simulation_set_timing(zpic_simulation *sim, int on);
+int zpic_simulation_get_timing(zpic_simulation *sim, double out[5]);
+
+/* Device synchronisation of the simulation's stream. */
+int zpic_simulation_sync(zpic_simulation *sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZPIC_AMD_H_ */

@@ -1,0 +1,325 @@
+"""Parity of the HIP path (libzpic_amd.so, called through its C ABI) with
+the CPU oracle and with the golden vectors of the compiled reference.
+
+Bars (from the north star):
+* integer / index work and per-particle arithmetic: bit-exact;
+* deterministic mode (lane-serialised deposit): every array bit-exact with
+  the reference's single-thread run, for many steps;
+* fast mode (LDS atomics): particles bit-exact after one step, E, B, J and
+  energies within relative 1e-5 after one step and 1e-3 over 500 Weibel
+  steps (norm-wise errors ||d||_2/||ref||_2 for the grids).
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import driver_args_to_config
+from gridutil import gather, global_positions
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def zp():
+    import pyzpic
+    pyzpic.lib()
+    return pyzpic
+
+
+def gpu_weibel(zp, cfg, det=False):
+    return zp.weibel(ntiles=cfg["ntiles"], nx=cfg["nx"], ppc=cfg["ppc"],
+                     box=tuple(np.float32(b) for b in cfg["box"]), dt=cfg["dt"], uth=cfg["uth"],
+                     ufl=cfg["ufl"], filter=cfg["filter"], deterministic=det)
+
+
+def cpu_weibel(cfg, prec=4):
+    return po.weibel(ntiles=cfg["ntiles"], nx=cfg["nx"], ppc=cfg["ppc"],
+                     box=tuple(np.float32(b) for b in cfg["box"]), dt=cfg["dt"], uth=cfg["uth"],
+                     ufl=cfg["ufl"], filter=cfg["filter"], prec=prec)
+
+
+def cfg_of(ntiles, nx, ppc, uth=(0.1, 0.1, 0.1), ufl=(0, 0, 0.6), dt=0.07, filter=None):
+    box = (ntiles[0] * nx[0] / 10.0, ntiles[1] * nx[1] / 10.0)
+    return {"ntiles": ntiles, "nx": nx, "ppc": ppc, "box": box, "dt": dt, "uth": uth, "ufl": ufl, "filter": filter}
+
+
+def gpu_state(sim, step):
+    out = {}
+    e = sim.energies()
+    for k, s in enumerate(sim.species):
+        p = s.particles()
+        pre = f"s{step}/sp{k}/"
+        for name in ("np", "offset", "ix", "x", "u"):
+            out[pre + name] = p[name].ravel()
+        out[pre + "energy"] = np.array([e[k]])
+        out[pre + "nmove"] = np.array([s.get_nmove()], np.uint64)
+        out[pre + "charge"] = s.charge()
+    for f in "EBJ":
+        out[f"s{step}/{f}"] = sim.field(f).ravel()
+    out[f"s{step}/field_energy"] = e[2:]
+    return out
+
+
+def relnorm(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def same_bits(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and a.dtype == b.dtype and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def copy_state_to_oracle(sim, osim):
+    for k, s in enumerate(sim.species):
+        p = s.particles()
+        osim.set_particles(k, p["np"], p["ix"], p["x"], p["u"])
+    for f in "EB":
+        osim.set_field(f, sim.field(f))
+
+
+# --------------------------------------------------------------------------
+def test_deterministic_full_state_matches_reference(zp):
+    gold = np.load(os.path.join(GOLD, "weibel_t2x2_n16_ppc2.npz"))
+    cfg = driver_args_to_config([str(a) for a in gold["args"]])
+    say("deterministic 32x32 ppc 2x2 vs golden, steps 0 1 5")
+    sim = gpu_weibel(zp, cfg, det=True)
+    done = 0
+    for step in (0, 1, 5):
+        sim.advance(step - done)
+        done = step
+        for key, val in gpu_state(sim, step).items():
+            ref = gold[key.replace("/", "__")]
+            assert same_bits(ref, val), f"{key} differs (max abs {np.abs(ref.astype(np.float64) - val).max()})"
+
+
+with open(os.path.join(GOLD, "digests.json")) as fp:
+    DIGESTS = json.load(fp)
+
+
+@pytest.mark.parametrize("case", sorted(DIGESTS))
+def test_deterministic_digests_match_reference(zp, case):
+    entry = DIGESTS[case]
+    cfg = driver_args_to_config(entry["args"])
+    say(f"deterministic digest case {case} {entry['args']}")
+    sim = gpu_weibel(zp, cfg, det=True)
+    energies = []
+    for step in range(0, max(entry["snap"]) + 1):
+        if step > 0:
+            sim.advance(1)
+            energies.extend(sim.energies().tolist())
+        if step in entry["snap"]:
+            for key, val in gpu_state(sim, step).items():
+                h = hashlib.sha256(np.ascontiguousarray(val).tobytes()).hexdigest()
+                assert h == entry["sha256"][key], f"{case}: {key} differs"
+    assert energies == entry["values"]["energies"]
+
+
+def test_fast_first_step(zp):
+    cfg = cfg_of((4, 4), (16, 16), (4, 4))
+    say("fast 64x64 ppc 4x4 one step vs oracle")
+    sim = gpu_weibel(zp, cfg)
+    osim = cpu_weibel(cfg)
+    sim.advance(1)
+    osim.advance(1)
+    for k in range(2):
+        p, q = sim.species[k].particles(), osim.particles(k)
+        for name in ("np", "offset", "ix", "x", "u"):
+            assert same_bits(p[name], q[name]), f"species {k} {name}"
+    for f in "JEB":
+        err = relnorm(sim.field(f), osim.field(f))
+        assert err <= 1e-6, f"{f}: {err}"
+    eg, eo = sim.energies(), osim.energies()
+    np.testing.assert_allclose(eg, eo, rtol=1e-5, atol=1e-12)
+    assert sim.get_nmove() == osim.nmove(0) + osim.nmove(1)
+
+
+def test_fast_step_from_evolved_state(zp):
+    """One step from the GPU's own state after 30 steps: the oracle gets the
+    same particles and fields, so cells, positions and momenta must agree bit
+    for bit and J, E, B to summation-order rounding."""
+    cfg = cfg_of((4, 4), (16, 16), (4, 4), uth=(0.2, 0.2, 0.2), ufl=(0.1, 0.0, 0.6))
+    say("fast 64x64 ppc 4x4: 30 steps, then one step GPU vs oracle from the same state")
+    sim = gpu_weibel(zp, cfg)
+    sim.advance(30)
+    osim = cpu_weibel(cfg)
+    copy_state_to_oracle(sim, osim)
+    sim.advance(1)
+    osim.advance(1)
+    for k in range(2):
+        p, q = sim.species[k].particles(), osim.particles(k)
+        for name in ("np", "offset", "ix", "x", "u"):
+            assert same_bits(p[name], q[name]), f"species {k} {name}"
+    for f in "JEB":
+        err = relnorm(sim.field(f), osim.field(f))
+        assert err <= 1e-5, f"{f}: {err}"
+    np.testing.assert_allclose(sim.energies()[:2], osim.energies()[:2], rtol=1e-9)
+
+
+@pytest.mark.parametrize("case", ["hot_small_tiles", "wrap_1x3_by2", "odd_tiles_bx1", "comp_cx1"])
+def test_fast_sort_edge_cases(zp, case):
+    cfg = driver_args_to_config(DIGESTS[case]["args"])
+    say(f"fast mode edge case {case}: 3 steps, particles vs oracle")
+    sim = gpu_weibel(zp, cfg)
+    osim = cpu_weibel(cfg)
+    sim.advance(1)
+    osim.advance(1)
+    for k in range(2):
+        p, q = sim.species[k].particles(), osim.particles(k)
+        for name in ("np", "offset", "ix", "x", "u"):
+            assert same_bits(p[name], q[name]), f"{case} species {k} {name}"
+    for _ in range(2):
+        copy_state_to_oracle(sim, osim)
+        sim.advance(1)
+        osim.advance(1)
+        for k in range(2):
+            p, q = sim.species[k].particles(), osim.particles(k)
+            for name in ("np", "offset", "ix", "x", "u"):
+                assert same_bits(p[name], q[name]), f"{case} species {k} {name}"
+        for f in "JEB":
+            assert relnorm(sim.field(f), osim.field(f)) <= 1e-5, f
+
+
+def test_fast_weibel_500_steps(zp):
+    """128x128 Weibel, ppc 8x8, 500 steps: energies within 1e-3 of the oracle."""
+    cfg = cfg_of((8, 8), (16, 16), (8, 8))
+    say("fast 128x128 ppc 8x8 Weibel, 500 steps vs oracle (energies every 50 steps)")
+    sim = gpu_weibel(zp, cfg)
+    osim = cpu_weibel(cfg)
+    worst = 0.0
+    for blk in range(10):
+        sim.advance(50)
+        osim.advance(50)
+        eg, eo = sim.energies(), osim.energies()
+        kin_err = np.abs(eg[:2] - eo[:2]) / np.abs(eo[:2])
+        fld_err = abs(eg[2:].sum() - eo[2:].sum()) / eo[2:].sum()
+        bz_err = abs(eg[5:].sum() - eo[5:].sum()) / eo[5:].sum()
+        worst = max(worst, kin_err.max(), fld_err, bz_err)
+        say(f"  step {50 * (blk + 1)}: kinetic rel err {kin_err.max():.2e}, field {fld_err:.2e}, B {bz_err:.2e}")
+    assert worst <= 1e-3, worst
+
+
+def test_push_stage_bit_exact(zp):
+    cfg = cfg_of((3, 2), (16, 16), (3, 3), uth=(0.3, 0.3, 0.3), ufl=(0.2, -0.1, 0.5))
+    say("push stage with random E, B")
+    sim = gpu_weibel(zp, cfg)
+    osim = cpu_weibel(cfg)
+    rng = np.random.default_rng(7)
+    n = sim.ntiles * sim.geom.tile_vol
+    for f in "EB":
+        v = rng.uniform(-0.3, 0.3, (n, 3)).astype(np.float32)
+        sim.set_field(f, v)
+        osim.set_field(f, v)
+    for k in range(2):
+        sim.stage("push", k)
+        osim.stage("push", k)
+        p, q = sim.species[k].particles(), osim.particles(k)
+        assert same_bits(p["u"], q["u"])
+        np.testing.assert_allclose(sim.species[k].get_energy(), osim.energies()[k], rtol=1e-12)
+
+
+@pytest.mark.parametrize("det", [False, True])
+def test_move_deposit_stage(zp, det):
+    cfg = cfg_of((2, 3), (8, 16), (2, 3), uth=(0.4, 0.4, 0.4), ufl=(0.5, -0.4, 0.3))
+    say(f"move+deposit stage, deterministic={det}")
+    sim = gpu_weibel(zp, cfg, det=det)
+    osim = cpu_weibel(cfg)
+    sim.stage("zero_j")
+    osim.stage("zero_j")
+    for k in range(2):
+        sim.stage("move", k)
+        osim.stage("move", k)
+        p, q = sim.species[k].particles(), osim.particles(k)
+        assert same_bits(p["ix"], q["ix"]) and same_bits(p["x"], q["x"])
+    if det:
+        assert same_bits(sim.field("J"), osim.field("J"))
+    else:
+        assert relnorm(sim.field("J"), osim.field("J")) <= 1e-6
+
+
+def test_charge_continuity_gpu(zp):
+    """Discrete continuity of the GPU deposit: FP32 rounding level, against the
+    FP64 oracle's residual on the same state."""
+    cfg = cfg_of((2, 2), (16, 16), (3, 2), uth=(0.4, 0.4, 0.4), ufl=(0.5, -0.3, 0.6))
+    say("charge continuity on the GPU fast path")
+    sim = gpu_weibel(zp, cfg)
+    sim.advance(3)
+    geo = ((2, 2), (16, 16))
+
+    def rho(s):
+        tot = 0
+        for sp in s.species:
+            tot = tot + gather(sp.charge(), geo[0], geo[1], 17, (17 * 17 + 3) & ~3, 0, ncomp=1)
+        return tot
+
+    r0 = rho(sim)
+    sim.stage("zero_j")
+    for k in range(2):
+        sim.stage("species", k)
+    sim.stage("add_gc")
+    r1 = rho(sim)
+    J = gather(sim.field("J"), geo[0], geo[1], 19, 364, 20)
+    dtdx = np.float32(0.07) / (np.float32(3.2) / np.float32(32))
+    res = (r1 - r0) + dtdx * (J[..., 0] - np.roll(J[..., 0], 1, 1)) + dtdx * (J[..., 1] - np.roll(J[..., 1], 1, 0))
+    scale = np.abs(r1).max()
+    say(f"  max residual / max rho = {np.abs(res).max() / scale:.2e}")
+    assert np.abs(res).max() <= 3e-6 * scale
+
+
+@pytest.mark.parametrize("filt", [("binomial", 0, 1), ("binomial", 1, 2), ("compensated", 0, 1),
+                                  ("compensated", 1, 2)])
+def test_current_filter_bit_exact(zp, filt):
+    cfg = cfg_of((3, 2), (8, 12), (1, 1), filter=filt)
+    say(f"current advance with filter {filt}")
+    sim = gpu_weibel(zp, cfg)
+    osim = cpu_weibel(cfg)
+    rng = np.random.default_rng(3)
+    J = rng.standard_normal((sim.ntiles * sim.geom.tile_vol, 3)).astype(np.float32)
+    sim.set_field("J", J)
+    osim.set_field("J", J)
+    sim.stage("current")
+    osim.stage("current")
+    assert same_bits(sim.field("J"), osim.field("J"))
+
+
+def test_yee_and_field_energy_bit_exact(zp):
+    cfg = cfg_of((3, 2), (16, 8), (1, 1))
+    say("Yee update and field energy with random E, B, J")
+    sim = gpu_weibel(zp, cfg)
+    osim = cpu_weibel(cfg)
+    rng = np.random.default_rng(5)
+    n = sim.ntiles * sim.geom.tile_vol
+    for f in "EBJ":
+        v = rng.standard_normal((n, 3)).astype(np.float32)
+        sim.set_field(f, v)
+        osim.set_field(f, v)
+    sim.stage("yee")
+    osim.stage("yee")
+    for f in "EB":
+        assert same_bits(sim.field(f), osim.field(f)), f
+    assert np.array_equal(sim.energies()[2:], osim.energies()[2:])
+
+
+def test_error_paths(zp):
+    say("error paths")
+    with pytest.raises(zp.ZpicError, match="courant"):
+        zp.Simulation((2, 2), (16, 16), (3.2, 3.2), 0.08)
+    with pytest.raises(zp.ZpicError, match="m_q"):
+        zp.Species("bad", 0.0, (2, 2))
+    with pytest.raises(zp.ZpicError, match="ppc"):
+        zp.Species("bad", 1.0, (0, 2))

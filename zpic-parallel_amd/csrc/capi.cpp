@@ -1,0 +1,285 @@
+// Object entry points of zpic_amd.h: C handles over the C++ classes of
+// zpic.h, the surface the reference's Cython layer binds
+// (openmp/python/em2d/simulation.pxd, species.pxd, emf.pxd, current.pxd).
+// Exceptions never cross the boundary: they become ZPIC_E* codes with the
+// message kept for zpic_last_error().
+#include "common.h"
+#include "zpic.h"
+
+#include <cstring>
+#include <memory>
+
+struct zpic_simulation {
+    std::unique_ptr<Simulation> sim;
+    bool deterministic = false;
+};
+
+struct zpic_species {
+    std::unique_ptr<Species> sp;
+    std::unique_ptr<UDistribution::Type> udist;
+    bool added = false;
+};
+
+namespace {
+
+template <class F>
+int guarded( F && fn ) {
+    try {
+        return fn();
+    } catch ( std::exception & e ) {
+        return zpic::set_error( ZPIC_EINVAL, e.what() );
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+zpic_simulation * zpic_simulation_new( unsigned ntx, unsigned nty, unsigned nx, unsigned ny, float box_x, float box_y,
+                                       double dt ) {
+    try {
+        auto * h = new zpic_simulation();
+        h->sim.reset( new Simulation( make_uint2( ntx, nty ), make_uint2( nx, ny ), make_float2( box_x, box_y ), dt ) );
+        return h;
+    } catch ( std::exception & e ) {
+        zpic::set_error( ZPIC_EINVAL, e.what() );
+        return nullptr;
+    }
+}
+
+void zpic_simulation_free( zpic_simulation * h ) { delete h; }
+
+zpic_species * zpic_species_new( const char * name, float m_q, unsigned ppc_x, unsigned ppc_y ) {
+    try {
+        auto * h = new zpic_species();
+        h->sp.reset( new Species( name ? name : "species", m_q, make_uint2( ppc_x, ppc_y ) ) );
+        return h;
+    } catch ( std::exception & e ) {
+        zpic::set_error( ZPIC_EINVAL, e.what() );
+        return nullptr;
+    }
+}
+
+void zpic_species_free( zpic_species * h ) { delete h; }
+
+int zpic_species_set_udist( zpic_species * h, int kind, const float uth[3], const float ufl[3], int npmin ) {
+    return guarded( [&] {
+        if ( !h ) return zpic::set_error( ZPIC_EINVAL, "null species" );
+        if ( h->added ) return zpic::set_error( ZPIC_ESTATE, "udist must be set before add_species" );
+        const float3 th = uth ? make_float3( uth[0], uth[1], uth[2] ) : make_float3( 0, 0, 0 );
+        const float3 fl = ufl ? make_float3( ufl[0], ufl[1], ufl[2] ) : make_float3( 0, 0, 0 );
+        switch ( kind ) {
+        case 0: h->sp->set_udist( UDistribution::None() ); break;
+        case 1: h->sp->set_udist( UDistribution::Cold( fl ) ); break;
+        case 2: h->sp->set_udist( UDistribution::Thermal( th, fl ) ); break;
+        case 3: h->sp->set_udist( UDistribution::ThermalCorr( th, fl, npmin ) ); break;
+        default: return zpic::set_error( ZPIC_EINVAL, "unknown udist kind" );
+        }
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_species_set_density_uniform( zpic_species * h, float n0 ) {
+    return guarded( [&] {
+        if ( !h ) return zpic::set_error( ZPIC_EINVAL, "null species" );
+        if ( h->added ) return zpic::set_error( ZPIC_ESTATE, "density must be set before add_species" );
+        h->sp->set_density( Density::Uniform( n0 ) );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_add_species( zpic_simulation * s, zpic_species * h ) {
+    return guarded( [&] {
+        if ( !s || !h ) return zpic::set_error( ZPIC_EINVAL, "null handle" );
+        if ( h->added ) return zpic::set_error( ZPIC_ESTATE, "species already added to a simulation" );
+        s->sim->add_species( *h->sp );
+        h->sp->set_deterministic( s->deterministic );
+        h->added = true;
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_advance( zpic_simulation * s, int n ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        for ( int i = 0; i < n; i++ ) s->sim->advance();
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_stage( zpic_simulation * s, int stage, int k ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        Simulation & sim = *s->sim;
+        if ( stage <= 3 && ( k < 0 || k >= (int) sim.species.size() ) )
+            return zpic::set_error( ZPIC_EINVAL, "bad species index" );
+        switch ( stage ) {
+        case 0: sim.species[k]->push( sim.emf.E, sim.emf.B ); break;
+        case 1: sim.species[k]->move( sim.current.J ); break;
+        case 2: sim.species[k]->tile_sort(); break;
+        case 3: sim.species[k]->advance( sim.emf, sim.current ); break;
+        case 4: sim.current.zero(); break;
+        case 5: sim.current.advance(); break;
+        case 6: sim.emf.advance( sim.current ); break;
+        case 7: sim.current.J->add_from_gc(); break;
+        case 8: sim.current.J->copy_to_gc(); break;
+        case 9: sim.current.filter->apply( *sim.current.J ); break;
+        default: return zpic::set_error( ZPIC_EINVAL, "bad stage" );
+        }
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_current_set_filter( zpic_simulation * s, int kind, int dir, int order ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        const coord::cart d = dir ? coord::y : coord::x;
+        switch ( kind ) {
+        case 0: s->sim->current.set_filter( Filter::None() ); break;
+        case 1: s->sim->current.set_filter( Filter::Binomial( d, order ) ); break;
+        case 2: s->sim->current.set_filter( Filter::Compensated( d, order ) ); break;
+        default: return zpic::set_error( ZPIC_EINVAL, "unknown filter kind" );
+        }
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_set_deterministic( zpic_simulation * s, int on ) {
+    if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+    s->deterministic = on != 0;
+    s->sim->set_deterministic( on != 0 );
+    return ZPIC_OK;
+}
+
+unsigned zpic_simulation_get_iter( const zpic_simulation * s ) { return s ? s->sim->get_iter() : 0; }
+double zpic_simulation_get_t( const zpic_simulation * s ) { return s ? s->sim->get_t() : 0.0; }
+uint64_t zpic_simulation_get_nmove( zpic_simulation * s ) { return s ? s->sim->get_nmove() : 0; }
+
+double zpic_species_get_energy( zpic_species * h ) {
+    try {
+        return h ? h->sp->get_energy() : 0.0;
+    } catch ( std::exception & e ) {
+        zpic::set_error( ZPIC_EHIP, e.what() );
+        return 0.0;
+    }
+}
+
+int zpic_emf_get_energy( zpic_simulation * s, double ene_E[3], double ene_B[3] ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        double3 e, b;
+        s->sim->emf.get_energy( e, b );
+        ene_E[0] = e.x; ene_E[1] = e.y; ene_E[2] = e.z;
+        ene_B[0] = b.x; ene_B[1] = b.y; ene_B[2] = b.z;
+        return ZPIC_OK;
+    } );
+}
+
+uint64_t zpic_species_get_nmove( const zpic_species * h ) { return h ? h->sp->get_nmove() : 0; }
+
+uint64_t zpic_species_np_total( zpic_species * h ) {
+    try {
+        return h && h->added ? h->sp->np_total() : 0;
+    } catch ( std::exception & e ) {
+        zpic::set_error( ZPIC_EHIP, e.what() );
+        return 0;
+    }
+}
+
+int zpic_species_get_particles( zpic_species * h, int * np, int * offset, int * ix, float * x, float * u ) {
+    return guarded( [&] {
+        if ( !h || !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        h->sp->get_particles( np, offset, ix, x, u );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_species_set_particles( zpic_species * h, const int * np, const int * ix, const float * x, const float * u ) {
+    return guarded( [&] {
+        if ( !h || !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        if ( !np || !ix || !x || !u ) return zpic::set_error( ZPIC_EINVAL, "null buffer" );
+        h->sp->set_particles( np, ix, x, u );
+        return ZPIC_OK;
+    } );
+}
+
+static vec3grid<float3> * field_of( zpic_simulation * s, int which ) {
+    switch ( which ) {
+    case 0: return s->sim->emf.E;
+    case 1: return s->sim->emf.B;
+    case 2: return s->sim->current.J;
+    }
+    return nullptr;
+}
+
+int zpic_field_get( zpic_simulation * s, int which, float * out ) {
+    return guarded( [&] {
+        vec3grid<float3> * f = s ? field_of( s, which ) : nullptr;
+        if ( !f || !out ) return zpic::set_error( ZPIC_EINVAL, "bad field request" );
+        f->download( reinterpret_cast<float3 *>( out ) );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_field_set( zpic_simulation * s, int which, const float * in ) {
+    return guarded( [&] {
+        vec3grid<float3> * f = s ? field_of( s, which ) : nullptr;
+        if ( !f || !in ) return zpic::set_error( ZPIC_EINVAL, "bad field request" );
+        f->upload( reinterpret_cast<const float3 *>( in ) );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_field_geom( const zpic_simulation * s, zpic_geom * g ) {
+    if ( !s || !g ) return zpic::set_error( ZPIC_EINVAL, "null handle" );
+    *g = s->sim->emf.E->geom();
+    return ZPIC_OK;
+}
+
+long long zpic_species_charge( zpic_species * h, float * out ) {
+    try {
+        if ( !h || !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        bnd<unsigned int> gc;
+        gc.x = { 0, 1 };
+        gc.y = { 0, 1 };
+        grid<float> rho( h->sp->get_ntiles(), h->sp->get_nx(), gc );
+        rho.zero();
+        h->sp->deposit_charge( rho );
+        rho.add_from_gc();
+        if ( out ) rho.download( out );
+        return (long long) rho.buffer_size();
+    } catch ( std::exception & e ) {
+        return zpic::set_error( ZPIC_EINVAL, e.what() );
+    }
+}
+
+int zpic_species_gather( zpic_species * h, int quant, float * out ) {
+    return guarded( [&] {
+        if ( !h || !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        if ( quant < 0 || quant > 4 || !out ) return zpic::set_error( ZPIC_EINVAL, "bad gather request" );
+        h->sp->gather( static_cast<part::quant>( quant ), out );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_set_timing( zpic_simulation * s, int on ) {
+    if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+    s->sim->set_timing( on != 0 );
+    return ZPIC_OK;
+}
+
+int zpic_simulation_get_timing( zpic_simulation * s, double out[5] ) {
+    if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+    s->sim->get_timing( out );
+    return ZPIC_OK;
+}
+
+int zpic_simulation_sync( zpic_simulation * s ) {
+    (void) s;
+    return guarded( [&] {
+        zpic::hip_check( hipStreamSynchronize( zpic::stream() ), "hipStreamSynchronize" );
+        return ZPIC_OK;
+    } );
+}
+
+}  // extern "C"

@@ -1,0 +1,441 @@
+// Grid stages of the em2d step on gfx950: guard-cell exchange, digital
+// filter, Yee FDTD update, field energy, gather.  Tile-major float3 grids
+// with guard cells, one workgroup per tile; stencils read the tile (guards
+// included) from LDS.  All arithmetic keeps the reference's operation order
+// (built with -ffp-contract=off), so every result is bit-identical to the
+// reference's.
+#include "common.h"
+
+#include <mutex>
+
+namespace zpic {
+
+namespace {
+thread_local std::string g_error;
+}
+
+int set_error( int code, const std::string & msg ) {
+    g_error = msg;
+    return code;
+}
+
+namespace {
+
+struct G {
+    int ntx, nty, nx, ny, gl, gu, ext_x, ext_y, vol, off;
+};
+
+G make( const zpic_geom * g ) {
+    return G{ g->ntiles_x, g->ntiles_y, g->nx, g->ny, g->gc_lower, g->gc_upper,
+              g->ext_x, g->ext_y, g->tile_vol, g->offset };
+}
+
+__device__ inline int nbr( int tx, int ty, int dx, int dy, const G & g, int px, int py ) {
+    int x = tx + dx, y = ty + dy;
+    if ( px ) { if ( x < 0 ) x += g.ntx; if ( x >= g.ntx ) x -= g.ntx; }
+    if ( py ) { if ( y < 0 ) y += g.nty; if ( y >= g.nty ) y -= g.nty; }
+    if ( x < 0 || x >= g.ntx || y < 0 || y >= g.nty ) return -1;
+    return y * g.ntx + x;
+}
+
+constexpr int GC_BLOCK = 128;
+
+// add_from_gc, x pass (grid.h:367-401): lower neighbour's upper guards into
+// the first interior columns, then upper neighbour's lower guards into the
+// last interior columns.
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( float * __restrict__ buf, G g, int px, int py ) {
+    const int t = blockIdx.x, tx = t % g.ntx, ty = t / g.ntx;
+    float * loc = buf + (size_t) t * g.vol * NC;
+    const int lo = nbr( tx, ty, -1, 0, g, px, py ), hi = nbr( tx, ty, +1, 0, g, px, py );
+    const int E = g.ext_x;
+    if ( lo >= 0 ) {
+        const float * src = buf + (size_t) lo * g.vol * NC;
+        for ( int k = threadIdx.x; k < g.ext_y * g.gu * NC; k += blockDim.x ) {
+            const int c = k % NC, e = k / NC, i = e % g.gu, j = e / g.gu;
+            loc[( g.gl + i + j * E ) * NC + c] += src[( g.gl + g.nx + i + j * E ) * NC + c];
+        }
+    }
+    __syncthreads();
+    if ( hi >= 0 ) {
+        const float * src = buf + (size_t) hi * g.vol * NC;
+        for ( int k = threadIdx.x; k < g.ext_y * g.gl * NC; k += blockDim.x ) {
+            const int c = k % NC, e = k / NC, i = e % g.gl, j = e / g.gl;
+            loc[( g.nx + i + j * E ) * NC + c] += src[( i + j * E ) * NC + c];
+        }
+    }
+}
+
+// add_from_gc, y pass (grid.h:406-444): full rows of the y neighbours' guards.
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( float * __restrict__ buf, G g, int px, int py ) {
+    const int t = blockIdx.x, tx = t % g.ntx, ty = t / g.ntx;
+    float * loc = buf + (size_t) t * g.vol * NC;
+    const int lo = nbr( tx, ty, 0, -1, g, px, py ), hi = nbr( tx, ty, 0, +1, g, px, py );
+    const int E = g.ext_x;
+    if ( lo >= 0 ) {
+        const float * src = buf + (size_t) lo * g.vol * NC;
+        for ( int k = threadIdx.x; k < g.gu * E * NC; k += blockDim.x ) {
+            const int c = k % NC, e = k / NC, i = e % E, j = e / E;
+            loc[( i + ( g.gl + j ) * E ) * NC + c] += src[( i + ( g.gl + g.ny + j ) * E ) * NC + c];
+        }
+    }
+    __syncthreads();
+    if ( hi >= 0 ) {
+        const float * src = buf + (size_t) hi * g.vol * NC;
+        for ( int k = threadIdx.x; k < g.gl * E * NC; k += blockDim.x ) {
+            const int c = k % NC, e = k / NC, i = e % E, j = e / E;
+            loc[( i + ( g.ny + j ) * E ) * NC + c] += src[( i + j * E ) * NC + c];
+        }
+    }
+}
+
+// copy_to_gc_x (grid.h:253-297): x guard columns from the neighbours' edges.
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( float * __restrict__ buf, G g, int px, int py ) {
+    const int t = blockIdx.x, tx = t % g.ntx, ty = t / g.ntx;
+    float * loc = buf + (size_t) t * g.vol * NC;
+    const int lo = nbr( tx, ty, -1, 0, g, px, py ), hi = nbr( tx, ty, +1, 0, g, px, py );
+    const int E = g.ext_x;
+    if ( lo >= 0 ) {
+        const float * src = buf + (size_t) lo * g.vol * NC;
+        for ( int k = threadIdx.x; k < g.ext_y * g.gl * NC; k += blockDim.x ) {
+            const int c = k % NC, e = k / NC, i = e % g.gl, j = e / g.gl;
+            loc[( i + j * E ) * NC + c] = src[( g.nx + i + j * E ) * NC + c];
+        }
+    }
+    if ( hi >= 0 ) {
+        const float * src = buf + (size_t) hi * g.vol * NC;
+        for ( int k = threadIdx.x; k < g.ext_y * g.gu * NC; k += blockDim.x ) {
+            const int c = k % NC, e = k / NC, i = e % g.gu, j = e / g.gu;
+            loc[( g.gl + g.nx + i + j * E ) * NC + c] = src[( g.gl + i + j * E ) * NC + c];
+        }
+    }
+}
+
+// copy_to_gc_y (grid.h:303-340): y guard rows (full width) from the neighbours.
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_y( float * __restrict__ buf, G g, int px, int py ) {
+    const int t = blockIdx.x, tx = t % g.ntx, ty = t / g.ntx;
+    float * loc = buf + (size_t) t * g.vol * NC;
+    const int lo = nbr( tx, ty, 0, -1, g, px, py ), hi = nbr( tx, ty, 0, +1, g, px, py );
+    const int E = g.ext_x;
+    if ( lo >= 0 ) {
+        const float * src = buf + (size_t) lo * g.vol * NC;
+        for ( int k = threadIdx.x; k < g.gl * E * NC; k += blockDim.x ) {
+            const int c = k % NC, e = k / NC, i = e % E, j = e / E;
+            loc[( i + j * E ) * NC + c] = src[( i + ( g.ny + j ) * E ) * NC + c];
+        }
+    }
+    if ( hi >= 0 ) {
+        const float * src = buf + (size_t) hi * g.vol * NC;
+        for ( int k = threadIdx.x; k < g.gu * E * NC; k += blockDim.x ) {
+            const int c = k % NC, e = k / NC, i = e % E, j = e / E;
+            loc[( i + ( g.gl + g.ny + j ) * E ) * NC + c] = src[( i + ( g.gl + j ) * E ) * NC + c];
+        }
+    }
+}
+
+constexpr int TILE_BLOCK = 256;
+
+// kernel3_x / kernel3_y (grid.h:495-593): B = A[-1]*a + A[0]*b + A[+1]*c per
+// component over all rows (x) or all columns (y) of the tile, A in LDS.
+__global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( float * __restrict__ buf, G g, int dir, float a, float b,
+                                                         float c ) {
+    extern __shared__ float A[];
+    const int t = blockIdx.x;
+    float * tb = buf + (size_t) t * g.vol * 3;
+    for ( int k = threadIdx.x; k < 3 * g.vol; k += blockDim.x ) A[k] = tb[k];
+    __syncthreads();
+    const int E = g.ext_x;
+    const int step = ( dir == 0 ) ? 3 : 3 * E;
+    const int x0 = ( dir == 0 ) ? g.gl : 0, nxr = ( dir == 0 ) ? g.nx : E;
+    const int y0 = ( dir == 0 ) ? 0 : g.gl, nyr = ( dir == 0 ) ? g.ext_y : g.ny;
+    for ( int k = threadIdx.x; k < nxr * nyr * 3; k += blockDim.x ) {
+        const int comp = k % 3, e = k / 3;
+        const int ix = x0 + e % nxr, iy = y0 + e / nxr;
+        const int q = ( iy * E + ix ) * 3 + comp;
+        tb[q] = A[q - step] * a + A[q] * b + A[q + step] * c;
+    }
+}
+
+// EMF::advance(Current&) per tile (emf.cpp:513-566, yee_b :67-81,
+// yeeJ_e :486-506): E and B of the tile in LDS planes, J from global.
+__global__ void __launch_bounds__( TILE_BLOCK ) k_yee( float * __restrict__ Eg, float * __restrict__ Bg,
+                                                     const float * __restrict__ Jg, G g, float dtdx, float dtdy,
+                                                     float dt ) {
+    extern __shared__ float sm[];
+    const int t = blockIdx.x;
+    const int vol = g.vol, ys = g.ext_x;
+    float * Ex = sm; float * Ey = Ex + vol; float * Ez = Ey + vol;
+    float * Bx = Ez + vol; float * By = Bx + vol; float * Bz = By + vol;
+    float * Et = Eg + (size_t) t * vol * 3;
+    float * Bt = Bg + (size_t) t * vol * 3;
+    const float * Jt = Jg ? Jg + (size_t) t * vol * 3 : nullptr;
+    for ( int k = threadIdx.x; k < 3 * vol; k += blockDim.x ) {
+        const int el = k / 3, c = k - 3 * el;
+        Ex[c * vol + el] = Et[k];
+        Bx[c * vol + el] = Bt[k];
+    }
+    __syncthreads();
+    const float hx = dtdx / 2, hy = dtdy / 2;
+    const int nb = ( g.nx + 2 ) * ( g.ny + 2 );
+    auto bstep = [&]() {
+        for ( int e = threadIdx.x; e < nb; e += blockDim.x ) {
+            const int ix = e % ( g.nx + 2 ) - 1, iy = e / ( g.nx + 2 ) - 1;
+            const int k = g.off + ix + iy * ys;
+            Bx[k] += ( -hy * ( Ez[k + ys] - Ez[k] ) );
+            By[k] += ( hx * ( Ez[k + 1] - Ez[k] ) );
+            Bz[k] += ( -hx * ( Ey[k + 1] - Ey[k] ) + hy * ( Ex[k + ys] - Ex[k] ) );
+        }
+    };
+    bstep();
+    __syncthreads();
+    for ( int e = threadIdx.x; e < nb; e += blockDim.x ) {
+        const int ix = e % ( g.nx + 2 ), iy = e / ( g.nx + 2 );
+        const int k = g.off + ix + iy * ys;
+        const float jx = Jt ? Jt[3 * k] : 0.0f, jy = Jt ? Jt[3 * k + 1] : 0.0f, jz = Jt ? Jt[3 * k + 2] : 0.0f;
+        Ex[k] += ( dtdy * ( Bz[k] - Bz[k - ys] ) ) - dt * jx;
+        Ey[k] += ( -dtdx * ( Bz[k] - Bz[k - 1] ) ) - dt * jy;
+        Ez[k] += ( dtdx * ( By[k] - By[k - 1] ) - dtdy * ( Bx[k] - Bx[k - ys] ) ) - dt * jz;
+    }
+    __syncthreads();
+    bstep();
+    __syncthreads();
+    for ( int k = threadIdx.x; k < 3 * vol; k += blockDim.x ) {
+        const int el = k / 3, c = k - 3 * el;
+        Et[k] = Ex[c * vol + el];
+        Bt[k] = Bx[c * vol + el];
+    }
+}
+
+// EMF::get_energy per tile (emf.cpp:680-705): one thread per (tile,
+// component), cells in the reference's order; the inner bound is nx.y as in
+// the reference loop.
+__global__ void k_energy_tiles( const float * __restrict__ E, const float * __restrict__ B, G g,
+                                double * __restrict__ out ) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = g.ntx * g.nty;
+    if ( gid >= 6 * nt ) return;
+    const int t = gid / 6, comp = gid % 6;
+    const float * F = ( comp < 3 ) ? E : B;
+    const int c = comp % 3;
+    const float * base = F + ( (size_t) t * g.vol + g.off ) * 3;
+    double s = 0;
+    for ( int iy = 0; iy < g.ny; iy++ )
+        for ( int ix = 0; ix < g.ny; ix++ ) {
+            const float v = base[( iy * g.ext_x + ix ) * 3 + c];
+            s += v * v;
+        }
+    out[6 * t + comp] = s;
+}
+
+// Sums tile values in tile order, out[c] = 0 + ((v0 + v1) + ...): the block
+// stages 1024 values at a time in LDS and thread c adds them in order.
+constexpr int SUM_BLOCK = 1024;
+__global__ void __launch_bounds__( SUM_BLOCK ) k_sum_tiles( const double * __restrict__ v, int n, int stride,
+                                                            double * __restrict__ out ) {
+    __shared__ double buf[SUM_BLOCK];
+    for ( int c = 0; c < stride; c++ ) {
+        double acc = 0;
+        for ( int base = 0; base < n; base += SUM_BLOCK ) {
+            const int t = base + threadIdx.x;
+            buf[threadIdx.x] = t < n ? v[(size_t) t * stride + c] : 0.0;
+            __syncthreads();
+            if ( threadIdx.x == 0 ) {
+                const int m = min( SUM_BLOCK, n - base );
+                for ( int k = 0; k < m; k++ ) acc += buf[k];
+            }
+            __syncthreads();
+        }
+        if ( threadIdx.x == 0 ) out[c] = 0.0 + acc;
+    }
+}
+
+// Unordered parallel sum (per-thread runs, then a tree): for the fast mode.
+__global__ void __launch_bounds__( SUM_BLOCK ) k_sum_tiles_fast( const double * __restrict__ v, int n,
+                                                                 double * __restrict__ out ) {
+    __shared__ double buf[SUM_BLOCK];
+    double acc = 0;
+    for ( int t = threadIdx.x; t < n; t += SUM_BLOCK ) acc += v[t];
+    buf[threadIdx.x] = acc;
+    __syncthreads();
+    for ( int o = SUM_BLOCK / 2; o > 0; o >>= 1 ) {
+        if ( (int) threadIdx.x < o ) buf[threadIdx.x] += buf[threadIdx.x + o];
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 ) out[0] = buf[0];
+}
+
+__global__ void k_gather( const float * __restrict__ buf, G g, int ncomp, int fc, float * __restrict__ out ) {
+    const int dimx = g.nx * g.ntx;
+    const size_t n = (size_t) dimx * g.ny * g.nty;
+    for ( size_t k = blockIdx.x * (size_t) blockDim.x + threadIdx.x; k < n; k += (size_t) gridDim.x * blockDim.x ) {
+        const int gx = k % dimx, gy = k / dimx;
+        const int tx = gx / g.nx, ty = gy / g.ny, ix = gx % g.nx, iy = gy % g.ny;
+        const size_t t = (size_t) ty * g.ntx + tx;
+        out[k] = buf[( t * g.vol + g.off + iy * g.ext_x + ix ) * ncomp + fc];
+    }
+}
+
+__global__ void k_fill( float * __restrict__ buf, size_t n, float v ) {
+    for ( size_t k = blockIdx.x * (size_t) blockDim.x + threadIdx.x; k < n; k += (size_t) gridDim.x * blockDim.x )
+        buf[k] = v;
+}
+
+int check( const zpic_geom * g ) {
+    if ( !g || g->ntiles_x < 1 || g->ntiles_y < 1 || g->nx < 1 || g->ny < 1 || g->gc_lower < 0 || g->gc_upper < 0 )
+        return set_error( ZPIC_EINVAL, "invalid grid geometry" );
+    if ( g->gc_upper > g->nx || g->gc_lower > g->nx || g->gc_upper > g->ny || g->gc_lower > g->ny )
+        return set_error( ZPIC_EINVAL, "guard cells wider than the tile" );
+    return ZPIC_OK;
+}
+
+template <int NC>
+int gc_pass( int which, float * buf, const zpic_geom * gg, int px, int py, hipStream_t st ) {
+    const G g = make( gg );
+    const int nt = g.ntx * g.nty;
+    switch ( which ) {
+    case 0: hipLaunchKernelGGL( k_add_gc_x<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, buf, g, px, py ); break;
+    case 1: hipLaunchKernelGGL( k_add_gc_y<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, buf, g, px, py ); break;
+    case 2: hipLaunchKernelGGL( k_copy_gc_x<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, buf, g, px, py ); break;
+    case 3: hipLaunchKernelGGL( k_copy_gc_y<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, buf, g, px, py ); break;
+    }
+    ZPIC_LAUNCH_CHECK( "guard cell kernel" );
+    return ZPIC_OK;
+}
+
+int gc( int which, float * buf, const zpic_geom * g, int ncomp, int px, int py, void * stream ) {
+    if ( int rc = check( g ) ) return rc;
+    if ( !buf ) return set_error( ZPIC_EINVAL, "null buffer" );
+    hipStream_t st = (hipStream_t) stream;
+    if ( ncomp == 3 ) return gc_pass<3>( which, buf, g, px ? 1 : 0, py ? 1 : 0, st );
+    if ( ncomp == 1 ) return gc_pass<1>( which, buf, g, px ? 1 : 0, py ? 1 : 0, st );
+    return set_error( ZPIC_EINVAL, "ncomp must be 1 or 3" );
+}
+
+}  // namespace
+}  // namespace zpic
+
+using namespace zpic;
+
+extern "C" {
+
+const char * zpic_last_error( void ) { return g_error.c_str(); }
+
+int zpic_geom_init( zpic_geom * g, int ntx, int nty, int nx, int ny, int gl, int gu ) {
+    if ( !g || ntx < 1 || nty < 1 || nx < 1 || ny < 1 || gl < 0 || gu < 0 )
+        return set_error( ZPIC_EINVAL, "invalid grid geometry" );
+    g->ntiles_x = ntx; g->ntiles_y = nty; g->nx = nx; g->ny = ny;
+    g->gc_lower = gl; g->gc_upper = gu;
+    g->ext_x = gl + nx + gu;
+    g->ext_y = gl + ny + gu;
+    g->tile_vol = ( g->ext_x * g->ext_y + 3 ) & ~3;
+    g->offset = gl * g->ext_x + gl;
+    return ZPIC_OK;
+}
+
+int zpic_add_from_gc( float * buf, const zpic_geom * g, int ncomp, int px, int py, void * stream ) {
+    if ( int rc = gc( 0, buf, g, ncomp, px, py, stream ) ) return rc;
+    return gc( 1, buf, g, ncomp, px, py, stream );
+}
+
+int zpic_copy_to_gc( float * buf, const zpic_geom * g, int ncomp, int px, int py, void * stream ) {
+    if ( int rc = gc( 2, buf, g, ncomp, px, py, stream ) ) return rc;
+    return gc( 3, buf, g, ncomp, px, py, stream );
+}
+
+int zpic_kernel3( float * buf, const zpic_geom * gg, int dir, float a, float b, float c, int px, int py,
+                  void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( gg->gc_lower < 1 || gg->gc_upper < 1 )
+        return set_error( ZPIC_EINVAL, "kernel3 requires at least 1 guard cell at both boundaries" );
+    const G g = make( gg );
+    const size_t lds = sizeof( float ) * 3 * g.vol;
+    if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    hipStream_t st = (hipStream_t) stream;
+    hipLaunchKernelGGL( k_kernel3, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, g, dir, a, b, c );
+    ZPIC_LAUNCH_CHECK( "k_kernel3" );
+    return gc( dir == 0 ? 2 : 3, buf, gg, 3, px, py, stream );
+}
+
+int zpic_current_advance( float * J, const zpic_geom * g, int kind, int dir, int order, int px, int py,
+                          void * stream ) {
+    if ( int rc = zpic_add_from_gc( J, g, 3, px, py, stream ) ) return rc;
+    if ( int rc = zpic_copy_to_gc( J, g, 3, px, py, stream ) ) return rc;
+    if ( kind == 0 ) return ZPIC_OK;
+    const unsigned n = order > 0 ? order : 1;
+    for ( unsigned i = 0; i < n; i++ )
+        if ( int rc = zpic_kernel3( J, g, dir, 0.25f, 0.5f, 0.25f, px, py, stream ) ) return rc;
+    if ( kind == 2 ) {
+        // Filter::Compensated (filter.h:58-76)
+        const float a = -1.0f;
+        const float b = ( 4.0 + 2.0 * n ) / n;
+        const float norm = 2 * a + b;
+        if ( int rc = zpic_kernel3( J, g, dir, a / norm, b / norm, a / norm, px, py, stream ) ) return rc;
+    }
+    return ZPIC_OK;
+}
+
+int zpic_yee( float * E, float * B, const float * J, const zpic_geom * gg, float dtdx, float dtdy, float dt, int px,
+              int py, void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( gg->gc_lower < 1 || gg->gc_upper < 2 ) return set_error( ZPIC_EINVAL, "Yee update needs guard cells {1,2}" );
+    if ( !E || !B ) return set_error( ZPIC_EINVAL, "null buffer" );
+    const G g = make( gg );
+    const size_t lds = sizeof( float ) * 6 * g.vol;
+    if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    hipStream_t st = (hipStream_t) stream;
+    hipLaunchKernelGGL( k_yee, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, E, B, J, g, dtdx, dtdy, dt );
+    ZPIC_LAUNCH_CHECK( "k_yee" );
+    if ( int rc = zpic_copy_to_gc( E, gg, 3, px, py, stream ) ) return rc;
+    return zpic_copy_to_gc( B, gg, 3, px, py, stream );
+}
+
+int zpic_field_energy( const float * E, const float * B, const zpic_geom * gg, double * scratch, double * out6,
+                       void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !E || !B || !scratch || !out6 ) return set_error( ZPIC_EINVAL, "null buffer" );
+    // the reference's inner bound nx.y must stay inside the tile
+    if ( gg->offset + ( gg->ny - 1 ) * gg->ext_x + gg->ny - 1 >= gg->tile_vol )
+        return set_error( ZPIC_ERANGE, "field energy loop would leave the tile (ny too large for nx)" );
+    const G g = make( gg );
+    const int nt = g.ntx * g.nty;
+    hipStream_t st = (hipStream_t) stream;
+    hipLaunchKernelGGL( k_energy_tiles, dim3( ( 6 * nt + 255 ) / 256 ), dim3( 256 ), 0, st, E, B, g, scratch );
+    ZPIC_LAUNCH_CHECK( "k_energy_tiles" );
+    hipLaunchKernelGGL( k_sum_tiles, dim3( 1 ), dim3( SUM_BLOCK ), 0, st, scratch, nt, 6, out6 );
+    ZPIC_LAUNCH_CHECK( "k_sum_tiles" );
+    return ZPIC_OK;
+}
+
+int zpic_sum_tiles( const double * v, int n, double * out, int deterministic, void * stream ) {
+    if ( !v || !out || n < 0 ) return set_error( ZPIC_EINVAL, "invalid arguments" );
+    if ( deterministic )
+        hipLaunchKernelGGL( k_sum_tiles, dim3( 1 ), dim3( SUM_BLOCK ), 0, (hipStream_t) stream, v, n, 1, out );
+    else
+        hipLaunchKernelGGL( k_sum_tiles_fast, dim3( 1 ), dim3( SUM_BLOCK ), 0, (hipStream_t) stream, v, n, out );
+    ZPIC_LAUNCH_CHECK( "k_sum_tiles" );
+    return ZPIC_OK;
+}
+
+int zpic_gather_component( const float * buf, const zpic_geom * gg, int ncomp, int fc, float * out, void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !buf || !out || fc < 0 || fc >= ncomp ) return set_error( ZPIC_EINVAL, "invalid gather arguments" );
+    const G g = make( gg );
+    hipLaunchKernelGGL( k_gather, dim3( 1024 ), dim3( 256 ), 0, (hipStream_t) stream, buf, g, ncomp, fc, out );
+    ZPIC_LAUNCH_CHECK( "k_gather" );
+    return ZPIC_OK;
+}
+
+int zpic_fill( float * buf, size_t n, float v, void * stream ) {
+    if ( !buf && n ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( n == 0 ) return ZPIC_OK;
+    size_t blocks = ( n + 255 ) / 256;
+    if ( blocks > 8192 ) blocks = 8192;
+    hipLaunchKernelGGL( k_fill, dim3( blocks ), dim3( 256 ), 0, (hipStream_t) stream, buf, n, v );
+    ZPIC_LAUNCH_CHECK( "k_fill" );
+    return ZPIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,617 @@
+// Host side of the zpic C++ API (zpic.h): object lifetimes, device buffers,
+// initial particle load, and the call sequence of the reference timestep,
+// each stage dispatched to a HIP kernel through the stage entry points of
+// zpic_amd.h.  The initial load (density injection and momentum
+// distribution) runs on the host exactly as the reference does, one worker
+// per tile range, and is uploaded once; every per-step stage runs on the GPU.
+#include "zpic.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <limits>
+#include <mutex>
+#include <thread>
+#include <cstdlib>
+
+namespace zpic {
+
+hipStream_t stream() {
+    static hipStream_t s = nullptr;
+    static std::once_flag once;
+    std::call_once( once, [] {
+        if ( hipStreamCreateWithFlags( &s, hipStreamNonBlocking ) != hipSuccess ) s = nullptr;
+    } );
+    if ( !s ) throw std::runtime_error( "(*error*) no HIP device available: the zpic library needs an MI355X GPU" );
+    return s;
+}
+
+void check( int rc ) {
+    if ( rc != ZPIC_OK ) throw std::runtime_error( zpic_last_error() );
+}
+
+void hip_check( hipError_t e, const char * what ) {
+    if ( e != hipSuccess ) throw std::runtime_error( std::string( what ) + ": " + hipGetErrorString( e ) );
+}
+
+namespace {
+
+template <class T>
+void to_device( T * dst, const T * src, size_t n ) {
+    if ( n ) hip_check( hipMemcpyAsync( dst, src, n * sizeof( T ), hipMemcpyHostToDevice, stream() ), "upload" );
+}
+template <class T>
+void to_host( T * dst, const T * src, size_t n ) {
+    if ( n ) hip_check( hipMemcpyAsync( dst, src, n * sizeof( T ), hipMemcpyDeviceToHost, stream() ), "download" );
+}
+void sync() { hip_check( hipStreamSynchronize( stream() ), "hipStreamSynchronize" ); }
+
+// Runs fn(t) for t in [0, n) on the host worker threads.
+// Worker count: ZPIC_HOST_THREADS if set, else the hardware threads capped at 16.
+template <class F>
+void parallel_tiles( int n, F && fn ) {
+    int nw = (int) std::thread::hardware_concurrency();
+    if ( nw > 16 ) nw = 16;
+    if ( const char * env = std::getenv( "ZPIC_HOST_THREADS" ) ) nw = std::atoi( env );
+    if ( nw < 1 ) nw = 1;
+    if ( nw > n ) nw = n;
+    if ( nw <= 1 ) { for ( int t = 0; t < n; t++ ) fn( t ); return; }
+    std::vector<std::thread> th;
+    for ( int w = 0; w < nw; w++ )
+        th.emplace_back( [&, w] { for ( int t = w; t < n; t += nw ) fn( t ); } );
+    for ( auto & x : th ) x.join();
+}
+
+// Marsaglia multiply-with-carry generator and polar Box-Muller normal
+// deviates (zrandom, random.h:23-155); one independent stream per tile.
+class TileRng {
+    uint32_t a, b;
+    double spare;
+    static constexpr double none = std::numeric_limits<double>::max();
+
+  public:
+    TileRng( unsigned tile, uint32_t sa, uint32_t sb )
+        : a( sa + tile ), b( sb + ( tile ^ 0xFFFFFFFFu ) ), spare( none ) {}
+    uint32_t bits() {
+        a = 36969u * ( a & 0xFFFFu ) + ( a >> 16 );
+        b = 18000u * ( b & 0xFFFFu ) + ( b >> 16 );
+        return ( a << 16 ) + b;
+    }
+    double normal() {
+        if ( spare != none ) {
+            const double r = spare;
+            spare = none;
+            return r;
+        }
+        double p, q, s;
+        do {
+            p = ( bits() + 1.0 ) * 0x1.fffffffep-33 * 2.0 - 1.0;
+            q = ( bits() + 1.0 ) * 0x1.fffffffep-33 * 2.0 - 1.0;
+            s = p * p + q * q;
+        } while ( s == 0.0 || s >= 1.0 );
+        const double f = std::sqrt( -2.0 * std::log( s ) / s );
+        spare = p * f;
+        return q * f;
+    }
+};
+
+}  // namespace
+}  // namespace zpic
+
+using namespace zpic;
+
+// ------------------------------------------------------------------ Particles
+Particles::Particles( uint2 const ntiles_, uint2 const nx_, uint32_t const max_part_ )
+    : ntiles( ntiles_ ), nx( nx_ ), periodic( make_int2( 1, 1 ) ),
+      dims( make_uint2( ntiles_.x * nx_.x, ntiles_.y * nx_.y ) ), max_part( max_part_ ) {
+    const size_t nt = (size_t) ntiles.x * ntiles.y;
+    np = device_alloc<int>( nt );
+    offset = device_alloc<int>( nt );
+    ix = device_alloc<int2>( max_part );
+    x = device_alloc<float2>( max_part );
+    u = device_alloc<float3>( max_part );
+    hip_check( hipMemsetAsync( np, 0, nt * sizeof( int ), stream() ), "memset" );
+    hip_check( hipMemsetAsync( offset, 0, nt * sizeof( int ), stream() ), "memset" );
+}
+
+Particles::~Particles() {
+    device_free( u ); device_free( x ); device_free( ix );
+    device_free( offset ); device_free( np );
+}
+
+zpic_particles Particles::view() const {
+    zpic_particles v;
+    v.np = np; v.offset = offset;
+    v.ix = reinterpret_cast<int *>( ix );
+    v.x = reinterpret_cast<float *>( x );
+    v.u = reinterpret_cast<float *>( u );
+    v.max_part = max_part;
+    return v;
+}
+
+std::vector<int> Particles::get_np() const {
+    std::vector<int> h( ntiles.x * ntiles.y );
+    to_host( h.data(), np, h.size() );
+    sync();
+    return h;
+}
+std::vector<int> Particles::get_offset() const {
+    std::vector<int> h( ntiles.x * ntiles.y );
+    to_host( h.data(), offset, h.size() );
+    sync();
+    return h;
+}
+uint64_t Particles::np_total() const {
+    uint64_t s = 0;
+    for ( int v : get_np() ) s += v;
+    return s;
+}
+
+ParticleSort::ParticleSort( uint2 const nt2, uint32_t const max_part ) : ntiles( nt2.x * nt2.y ) {
+    buf.idx = device_alloc<int>( max_part );
+    buf.npt = device_alloc<int>( 9 * (size_t) ntiles );
+    buf.new_np = device_alloc<int>( ntiles );
+    buf.seg = device_alloc<int>( 16 * (size_t) ntiles );
+    buf.tmp_np = device_alloc<int>( ntiles );
+    buf.total = device_alloc<int>( 1 );
+}
+ParticleSort::~ParticleSort() {
+    device_free( buf.total ); device_free( buf.tmp_np ); device_free( buf.seg );
+    device_free( buf.new_np ); device_free( buf.npt ); device_free( buf.idx );
+}
+
+// ------------------------------------------------------------ Density::Uniform
+// Whole-box injection: every cell of every tile gets ppc.x*ppc.y particles on
+// a regular lattice, in the "mk2" order of density.cpp:91-186 (all cells for
+// lattice point 0, then all cells for lattice point 1, ...).
+void Density::Uniform::np_inject( uint2 ntiles, uint2 nx, uint2 ppc, int * np ) const {
+    for ( unsigned t = 0; t < ntiles.x * ntiles.y; t++ ) np[t] = nx.x * nx.y * ppc.x * ppc.y;
+}
+
+void Density::Uniform::inject( uint2 ntiles, uint2 nx, uint2 ppc, const int * offset, int2 * ix, float2 * x ) const {
+    const int cells = nx.x * nx.y;
+    const double sx = 1.0 / ppc.x, sy = 1.0 / ppc.y;
+    parallel_tiles( ntiles.x * ntiles.y, [&]( int t ) {
+        int n = offset[t];
+        for ( unsigned ly = 0; ly < ppc.y; ly++ )
+            for ( unsigned lx = 0; lx < ppc.x; lx++ ) {
+                const float2 pos = make_float2( (float) ( sx * ( lx + 0.5 ) - 0.5 ), (float) ( sy * ( ly + 0.5 ) - 0.5 ) );
+                for ( int c = 0; c < cells; c++, n++ ) {
+                    ix[n] = make_int2( c % (int) nx.x, c / (int) nx.x );
+                    x[n] = pos;
+                }
+            }
+    } );
+}
+
+// --------------------------------------------------------- UDistribution
+// (udist.cpp:17-136).  Normal deviates of one particle are drawn in the order
+// z, y, x, the evaluation order of the reference build's make_float3 call.
+void UDistribution::None::set( int nt, const int * np, const int * off, const int2 *, uint2, float3 * u,
+                               unsigned ) const {
+    for ( int t = 0; t < nt; t++ )
+        for ( int i = 0; i < np[t]; i++ ) u[off[t] + i] = make_float3( 0, 0, 0 );
+}
+
+void UDistribution::Cold::set( int nt, const int * np, const int * off, const int2 *, uint2, float3 * u,
+                               unsigned ) const {
+    for ( int t = 0; t < nt; t++ )
+        for ( int i = 0; i < np[t]; i++ ) u[off[t] + i] = ufl;
+}
+
+void UDistribution::Thermal::set( int nt, const int * np, const int * off, const int2 *, uint2, float3 * u,
+                                  unsigned seed ) const {
+    parallel_tiles( nt, [&]( int t ) {
+        TileRng rng( t, 12345u + seed, 67890u );
+        for ( int i = 0; i < np[t]; i++ ) {
+            const double nz = rng.normal(), ny = rng.normal(), nxv = rng.normal();
+            u[off[t] + i] = make_float3( (float) ( ufl.x + uth.x * nxv ), (float) ( ufl.y + uth.y * ny ),
+                                         (float) ( ufl.z + uth.z * nz ) );
+        }
+    } );
+}
+
+void UDistribution::ThermalCorr::set( int nt, const int * np, const int * off, const int2 * ix, uint2 nx, float3 * u,
+                                      unsigned seed ) const {
+    const int cells = nx.x * nx.y;
+    parallel_tiles( nt, [&]( int t ) {
+        TileRng rng( t, 12345u + seed, 67890u );
+        std::vector<float> sx( cells, 0.f ), sy( cells, 0.f ), sz( cells, 0.f );
+        std::vector<int> cnt( cells, 0 );
+        float3 * ut = u + off[t];
+        const int2 * it = ix + off[t];
+        for ( int i = 0; i < np[t]; i++ ) {
+            const double nz = rng.normal(), ny = rng.normal(), nxv = rng.normal();
+            const float3 v = make_float3( (float) ( uth.x * nxv ), (float) ( uth.y * ny ), (float) ( uth.z * nz ) );
+            ut[i] = v;
+            const int c = it[i].x + (int) nx.x * it[i].y;
+            cnt[c] += 1;
+            sx[c] += v.x; sy[c] += v.y; sz[c] += v.z;
+        }
+        for ( int c = 0; c < cells; c++ ) {
+            if ( cnt[c] > npmin ) {
+                sx[c] /= cnt[c]; sy[c] /= cnt[c]; sz[c] /= cnt[c];
+            } else {
+                sx[c] = sy[c] = sz[c] = 0.f;
+            }
+        }
+        for ( int i = 0; i < np[t]; i++ ) {
+            const int c = it[i].x + (int) nx.x * it[i].y;
+            ut[i].x += ufl.x - sx[c];
+            ut[i].y += ufl.y - sy[c];
+            ut[i].z += ufl.z - sz[c];
+        }
+    } );
+}
+
+// ------------------------------------------------------------------ Current
+Current::Current( uint2 const ntiles, uint2 const nx, float2 const box_, double const dt_ )
+    : dx( make_float2( box_.x / ( nx.x * ntiles.x ), box_.y / ( nx.y * ntiles.y ) ) ), dt( dt_ ), iter( 0 ),
+      box( box_ ) {
+    bnd<unsigned int> gc;
+    gc.x = { 1, 2 };
+    gc.y = { 1, 2 };
+    J = new vec3grid<float3>( ntiles, nx, gc );
+    J->name = "Current density";
+    J->zero();
+    bc = current::bc_type( current::bc::periodic );
+    filter = new Filter::None();
+}
+
+Current::~Current() {
+    delete J;
+    delete filter;
+}
+
+void Current::set_bc( current::bc_type new_bc ) {
+    if ( new_bc.x.lower != current::bc::periodic || new_bc.x.upper != current::bc::periodic ||
+         new_bc.y.lower != current::bc::periodic || new_bc.y.upper != current::bc::periodic )
+        throw std::runtime_error( "(*error*) only periodic current boundaries are available in this build" );
+    bc = new_bc;
+    J->set_periodic( make_int2( 1, 1 ) );
+}
+
+// Current::advance (current.cpp:227-246)
+void Current::advance() {
+    J->add_from_gc();
+    J->copy_to_gc();
+    filter->apply( *J );
+    iter++;
+}
+
+void Current::zero() { J->zero(); }
+
+// ---------------------------------------------------------------------- EMF
+EMF::EMF( uint2 const ntiles, uint2 const nx, float2 const box_, double const dt_ )
+    : dx( make_float2( box_.x / ( nx.x * ntiles.x ), box_.y / ( nx.y * ntiles.y ) ) ), dt( dt_ ), iter( 0 ),
+      box( box_ ) {
+    const float cour = std::sqrt( 1.0f / ( 1.0f / ( dx.x * dx.x ) + 1.0f / ( dx.y * dx.y ) ) );
+    if ( dt >= cour )
+        throw std::runtime_error( "(*error*) Invalid timestep, courant condition violation. For the current "
+                                  "resolution the maximum timestep is dt = " + std::to_string( cour ) );
+    bnd<unsigned int> gc;
+    gc.x = { 1, 2 };
+    gc.y = { 1, 2 };
+    E = new vec3grid<float3>( ntiles, nx, gc );
+    E->name = "Electric field";
+    B = new vec3grid<float3>( ntiles, nx, gc );
+    B->name = "Magnetic field";
+    E->zero();
+    B->zero();
+    bc = emf::bc_type( emf::bc::periodic );
+    d_energy = device_alloc<double>( 6 * (size_t) ntiles.x * ntiles.y + 6 );
+}
+
+EMF::~EMF() {
+    device_free( d_energy );
+    delete E;
+    delete B;
+}
+
+void EMF::set_bc( emf::bc_type new_bc ) {
+    if ( new_bc.x.lower != emf::bc::periodic || new_bc.x.upper != emf::bc::periodic ||
+         new_bc.y.lower != emf::bc::periodic || new_bc.y.upper != emf::bc::periodic )
+        throw std::runtime_error( "(*error*) only periodic EMF boundaries are available in this build" );
+    bc = new_bc;
+}
+
+// EMF::advance(Current&) (emf.cpp:513-580): dt/dx is evaluated in double and
+// narrowed, dt is narrowed to float, as in the reference.
+void EMF::advance( Current & current ) {
+    const float dtdx = dt / dx.x, dtdy = dt / dx.y;
+    zpic_geom g = E->geom();
+    const int2 p = E->get_periodic();
+    check( zpic_yee( E->fbuf(), B->fbuf(), current.J->fbuf(), &g, dtdx, dtdy, (float) dt, p.x, p.y, stream() ) );
+    iter += 1;
+}
+
+void EMF::advance() {
+    const float dtdx = dt / dx.x, dtdy = dt / dx.y;
+    zpic_geom g = E->geom();
+    const int2 p = E->get_periodic();
+    check( zpic_yee( E->fbuf(), B->fbuf(), nullptr, &g, dtdx, dtdy, (float) dt, p.x, p.y, stream() ) );
+    iter += 1;
+}
+
+void EMF::get_energy( double3 & ene_E, double3 & ene_B ) {
+    zpic_geom g = E->geom();
+    const size_t nt = (size_t) g.ntiles_x * g.ntiles_y;
+    check( zpic_field_energy( E->fbuf(), B->fbuf(), &g, d_energy, d_energy + 6 * nt, stream() ) );
+    double h[6];
+    to_host( h, d_energy + 6 * nt, 6 );
+    sync();
+    ene_E = make_double3( h[0], h[1], h[2] );
+    ene_B = make_double3( h[3], h[4], h[5] );
+}
+
+// ------------------------------------------------------------------ Species
+Species::Species( std::string const name_, float const m_q_, uint2 const ppc_ )
+    : ppc( ppc_ ), particles( nullptr ), tmp( nullptr ), sort( nullptr ), d_tile_energy( nullptr ),
+      d_energy_dev( nullptr ), d_nmove( 0 ), np_count( 0 ), deterministic( false ), name( name_ ), m_q( m_q_ ),
+      push_type( species::boris ) {
+    if ( m_q == 0 ) throw std::runtime_error( "(*error*) Invalid m_q value, must be not 0, aborting..." );
+    if ( ppc.x < 1 || ppc.y < 1 )
+        throw std::runtime_error( "(*error*) Invalid ppc value, must be >= 1 in all directions" );
+    density = new Density::Uniform( 1.0 );
+    udist = new UDistribution::None();
+    bc = species::bc_type( species::bc::periodic );
+    id = 0; q = 0; dt = 0; iter = 0;
+    dx = make_float2( 0, 0 );
+    box = make_float2( 0, 0 );
+}
+
+Species::~Species() {
+    device_free( d_energy_dev );
+    device_free( d_tile_energy );
+    delete sort;
+    delete tmp;
+    delete particles;
+    delete density;
+    delete udist;
+}
+
+void Species::set_bc( species::bc_type new_bc ) {
+    if ( new_bc.x.lower != species::bc::periodic || new_bc.x.upper != species::bc::periodic ||
+         new_bc.y.lower != species::bc::periodic || new_bc.y.upper != species::bc::periodic )
+        throw std::runtime_error( "(*error*) only periodic species boundaries are available in this build" );
+    bc = new_bc;
+}
+
+// Species::initialize (species.cpp:1730-1788)
+void Species::initialize( float2 const box_, uint2 const ntiles, uint2 const nx, double const dt_, int const id_ ) {
+    if ( push_type != species::boris )
+        throw std::runtime_error( "(*error*) only the Boris pusher is available in this build" );
+    box = box_;
+    dt = dt_;
+    id = id_;
+    q = std::copysign( density->n0, m_q ) / ( ppc.x * ppc.y );
+    const float2 dims = make_float2( nx.x * ntiles.x, nx.y * ntiles.y );
+    dx.x = box.x / dims.x;
+    dx.y = box.y / dims.y;
+    const uint32_t max_part = 1.2 * dims.x * dims.y * ppc.x * ppc.y;
+
+    particles = new Particles( ntiles, nx, max_part );
+    particles->periodic = make_int2( bc.x.lower == species::bc::periodic, bc.y.lower == species::bc::periodic );
+    tmp = new Particles( ntiles, nx, max_part );
+    sort = new ParticleSort( ntiles, max_part );
+    const int nt = ntiles.x * ntiles.y;
+    d_tile_energy = device_alloc<double>( nt );
+    d_energy_dev = device_alloc<double>( 1 );
+    hip_check( hipMemsetAsync( d_tile_energy, 0, nt * sizeof( double ), stream() ), "memset" );
+    hip_check( hipMemsetAsync( d_energy_dev, 0, sizeof( double ), stream() ), "memset" );
+    d_nmove = 0;
+    iter = 0;
+
+    std::vector<int> np( nt ), off( nt );
+    density->np_inject( ntiles, nx, ppc, np.data() );
+    size_t total = 0;
+    for ( int t = 0; t < nt; t++ ) { off[t] = (int) total; total += np[t]; }
+    if ( total > max_part ) throw std::runtime_error( "(*error*) particle buffer too small for the injection" );
+    std::vector<int2> hix( total );
+    std::vector<float2> hx( total );
+    std::vector<float3> hu( total );
+    density->inject( ntiles, nx, ppc, off.data(), hix.data(), hx.data() );
+    udist->set( nt, np.data(), off.data(), hix.data(), nx, hu.data(), id );
+
+    to_device( particles->np, np.data(), nt );
+    to_device( particles->offset, off.data(), nt );
+    to_device( particles->ix, hix.data(), total );
+    to_device( particles->x, hx.data(), total );
+    to_device( particles->u, hu.data(), total );
+    sync();
+    np_count = total;
+}
+
+// Species::push (species.cpp:2302-2335)
+void Species::push( vec3grid<float3> * const E, vec3grid<float3> * const B ) {
+    const float alpha = 0.5 * dt / m_q;
+    zpic_geom g = E->geom();
+    zpic_particles p = particles->view();
+    check( zpic_push( &p, &g, E->fbuf(), B->fbuf(), alpha, d_tile_energy, deterministic, stream() ) );
+    check( zpic_sum_tiles( d_tile_energy, g.ntiles_x * g.ntiles_y, d_energy_dev, deterministic, stream() ) );
+}
+
+// Species::move(J) (species.cpp:2138-2164)
+void Species::move( vec3grid<float3> * const J ) {
+    const float2 dt_dx = make_float2( dt / dx.x, dt / dx.y );
+    const float2 qnx = make_float2( q * dx.x / dt, q * dx.y / dt );
+    zpic_geom g = J->geom();
+    zpic_particles p = particles->view();
+    check( zpic_move_deposit( &p, &g, J->fbuf(), dt_dx.x, dt_dx.y, q, qnx.x, qnx.y, &sort->buf, deterministic,
+                              stream() ) );
+    d_nmove += np_count;
+}
+
+void Species::tile_sort() {
+    zpic_particles p = particles->view(), t = tmp->view();
+    check( zpic_tile_sort( &p, &t, &sort->buf, particles->ntiles.x, particles->ntiles.y, particles->nx.x,
+                           particles->nx.y, particles->periodic.x, particles->periodic.y, stream() ) );
+}
+
+// Species::advance(emf, current) (species.cpp:2051-2068): the push and the
+// move run as one pass over the particles.
+void Species::advance( EMF const & emf, Current & current ) { advance_timed( emf, current, nullptr ); }
+
+void Species::advance_timed( EMF const & emf, Current & current, hipEvent_t after_push ) {
+    const float alpha = 0.5 * dt / m_q;
+    const float2 dt_dx = make_float2( dt / dx.x, dt / dx.y );
+    const float2 qnx = make_float2( q * dx.x / dt, q * dx.y / dt );
+    zpic_geom g = emf.E->geom();
+    zpic_particles p = particles->view();
+    check( zpic_push_move_deposit( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha, dt_dx.x, dt_dx.y,
+                                   q, qnx.x, qnx.y, d_tile_energy, &sort->buf, deterministic, stream() ) );
+    if ( after_push ) hip_check( hipEventRecord( after_push, stream() ), "hipEventRecord" );
+    d_nmove += np_count;
+    iter++;
+    tile_sort();
+    check( zpic_sum_tiles( d_tile_energy, g.ntiles_x * g.ntiles_y, d_energy_dev, deterministic, stream() ) );
+}
+
+double Species::get_energy() const {
+    double e = 0;
+    to_host( &e, d_energy_dev, 1 );
+    sync();
+    return e * q * m_q * dx.x * dx.y;
+}
+
+void Species::deposit_charge( grid<float> & charge ) const {
+    zpic_geom g = charge.geom();
+    zpic_particles p = particles->view();
+    check( zpic_deposit_charge( &p, &g, charge.fbuf(), q, deterministic, stream() ) );
+}
+
+void Species::get_particles( int * np, int * offset, int * ix, float * x, float * u ) const {
+    const int nt = particles->ntiles.x * particles->ntiles.y;
+    std::vector<int> hnp = particles->get_np(), hoff = particles->get_offset();
+    size_t n = 0;
+    for ( int t = 0; t < nt; t++ ) {
+        if ( np ) np[t] = hnp[t];
+        if ( offset ) offset[t] = (int) n;
+        if ( ix ) to_host( reinterpret_cast<int2 *>( ix ) + n, particles->ix + hoff[t], hnp[t] );
+        if ( x ) to_host( reinterpret_cast<float2 *>( x ) + n, particles->x + hoff[t], hnp[t] );
+        if ( u ) to_host( reinterpret_cast<float3 *>( u ) + n, particles->u + hoff[t], hnp[t] );
+        n += hnp[t];
+    }
+    sync();
+}
+
+void Species::set_particles( const int * np, const int * ix, const float * x, const float * u ) {
+    const int nt = particles->ntiles.x * particles->ntiles.y;
+    std::vector<int> off( nt );
+    size_t n = 0;
+    for ( int t = 0; t < nt; t++ ) { off[t] = (int) n; n += np[t]; }
+    if ( n > particles->max_part ) throw std::runtime_error( "(*error*) too many particles for the buffer" );
+    to_device( particles->np, np, nt );
+    to_device( particles->offset, off.data(), nt );
+    to_device( particles->ix, reinterpret_cast<const int2 *>( ix ), n );
+    to_device( particles->x, reinterpret_cast<const float2 *>( x ), n );
+    to_device( particles->u, reinterpret_cast<const float3 *>( u ), n );
+    sync();
+    np_count = n;
+}
+
+// Particles::gather (particles.cpp:88-150) with Species::gather's scales
+void Species::gather( part::quant quant, float * out ) const {
+    const int nt = particles->ntiles.x * particles->ntiles.y;
+    const uint64_t n = particles->np_total();
+    std::vector<int> np( nt ), off( nt ), ix( 2 * n );
+    std::vector<float> x( 2 * n ), u( 3 * n );
+    get_particles( np.data(), off.data(), ix.data(), x.data(), u.data() );
+    const uint2 tnx = particles->nx;
+    for ( int t = 0; t < nt; t++ ) {
+        const unsigned tx = t % particles->ntiles.x, ty = t / particles->ntiles.x;
+        for ( int i = off[t]; i < off[t] + np[t]; i++ ) {
+            float v = 0;
+            switch ( quant ) {
+            case part::x: v = std::fma( dx.x, (float) ( tx * tnx.x + ix[2 * i] ) + ( 0.5f + x[2 * i] ), 0.0f ); break;
+            case part::y: v = std::fma( dx.y, (float) ( ty * tnx.y + ix[2 * i + 1] ) + ( 0.5f + x[2 * i + 1] ), 0.0f ); break;
+            case part::ux: v = u[3 * i]; break;
+            case part::uy: v = u[3 * i + 1]; break;
+            case part::uz: v = u[3 * i + 2]; break;
+            }
+            out[i] = v;
+        }
+    }
+}
+
+// --------------------------------------------------------------- Simulation
+Simulation::Simulation( uint2 const ntiles_, uint2 const nx_, float2 const box_, double dt_ )
+    : iter( 0 ), timing( false ), t_steps( 0 ), ntiles( ntiles_ ), nx( nx_ ), box( box_ ), dt( dt_ ),
+      emf( ntiles_, nx_, box_, dt_ ), current( ntiles_, nx_, box_, dt_ ) {
+    for ( double & v : t_acc ) v = 0;
+}
+
+Simulation::~Simulation() {
+    for ( auto e : ev ) (void) hipEventDestroy( e );
+}
+
+void Simulation::set_timing( bool on ) {
+    timing = on;
+    for ( double & v : t_acc ) v = 0;
+    t_steps = 0;
+}
+
+// Simulation::advance (simulation.h:90-107)
+void Simulation::advance() {
+    const size_t nsp = species.size();
+    const size_t nev = 2 * nsp + 3;
+    if ( timing && ev.size() < nev ) {
+        for ( size_t i = ev.size(); i < nev; i++ ) {
+            hipEvent_t e;
+            hip_check( hipEventCreate( &e ), "hipEventCreate" );
+            ev.push_back( e );
+        }
+    }
+    current.zero();
+    if ( timing ) hip_check( hipEventRecord( ev[0], stream() ), "hipEventRecord" );
+    for ( size_t k = 0; k < nsp; k++ ) {
+        species[k]->advance_timed( emf, current, timing ? ev[1 + 2 * k] : nullptr );
+        if ( timing ) hip_check( hipEventRecord( ev[2 + 2 * k], stream() ), "hipEventRecord" );
+    }
+    current.advance();
+    if ( timing ) hip_check( hipEventRecord( ev[2 * nsp + 1], stream() ), "hipEventRecord" );
+    emf.advance( current );
+    if ( timing ) {
+        hip_check( hipEventRecord( ev[2 * nsp + 2], stream() ), "hipEventRecord" );
+        hip_check( hipEventSynchronize( ev[2 * nsp + 2] ), "hipEventSynchronize" );
+        float ms;
+        double pm = 0, so = 0;
+        for ( size_t k = 0; k < nsp; k++ ) {
+            hip_check( hipEventElapsedTime( &ms, k ? ev[2 * k] : ev[0], ev[1 + 2 * k] ), "elapsed" );
+            pm += ms;
+            hip_check( hipEventElapsedTime( &ms, ev[1 + 2 * k], ev[2 + 2 * k] ), "elapsed" );
+            so += ms;
+        }
+        t_acc[0] += pm;
+        t_acc[1] += so;
+        hip_check( hipEventElapsedTime( &ms, ev[2 * nsp], ev[2 * nsp + 1] ), "elapsed" );
+        t_acc[2] += ms;
+        hip_check( hipEventElapsedTime( &ms, ev[2 * nsp + 1], ev[2 * nsp + 2] ), "elapsed" );
+        t_acc[3] += ms;
+        hip_check( hipEventElapsedTime( &ms, ev[0], ev[2 * nsp + 2] ), "elapsed" );
+        t_acc[4] += ms;
+        t_steps++;
+    }
+    iter++;
+}
+
+void Simulation::get_timing( double out[5] ) {
+    for ( int i = 0; i < 5; i++ ) out[i] = t_steps ? t_acc[i] / t_steps : 0.0;
+}
+
+void Simulation::energy_info() {
+    std::cout << "(*info*) Energy at n = " << iter << ", t = " << iter * double( dt ) << '\n';
+    double part_ene = 0;
+    for ( auto * s : species ) {
+        const double kin = s->get_energy();
+        std::cout << "(*info*) " << s->name << " = " << kin << '\n';
+        part_ene += kin;
+    }
+    if ( species.size() > 1 ) std::cout << "(*info*) Total particle energy = " << part_ene << '\n';
+    double3 eE, eB;
+    emf.get_energy( eE, eB );
+    std::cout << "(*info*) Electric field = " << eE.x + eE.y + eE.z << '\n';
+    std::cout << "(*info*) Magnetic field = " << eB.x + eB.y + eB.z << '\n';
+    std::cout << "(*info*) total = " << part_ene + eE.x + eE.y + eE.z + eB.x + eB.y + eB.z << '\n';
+}

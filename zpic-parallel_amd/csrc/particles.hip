@@ -1,0 +1,762 @@
+// Particle stages of the em2d step on gfx950: field interpolation + Boris
+// push, split-segment charge-conserving current deposit, and the tile sort.
+//
+// One workgroup per tile.  The tile's E and B (with guard cells) are staged in
+// LDS as six float planes; the tile's current J is accumulated in LDS with
+// ds_add_f32 and flushed once, with plain adds, into the tile's own region of
+// the global J grid (each tile owns its guard cells, so the flush needs no
+// global atomics).
+//
+// Arithmetic is the reference's, operation for operation: no contraction
+// (built with -ffp-contract=off), fmaf exactly where the reference calls
+// ops::fma (species.cpp:22-26, 117-178), correctly rounded division and
+// square root.  Per-particle results are therefore bit-identical to the
+// reference's; only the order of the J additions (and of the energy sum)
+// differs in the fast mode.  The deterministic mode runs one wavefront per
+// tile and serialises the deposit and the energy sum over lanes in particle
+// order, which reproduces the reference's single-thread results exactly.
+#include "common.h"
+
+namespace zpic {
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int FAST_BLOCK = 256;  // 4 wavefronts per tile in the fast mode
+constexpr int MAX_SEG = 15;      // wave segments recorded per tile (slot 15 = chunk size)
+
+struct PartView {
+    int * np;
+    int * off;
+    int2 * ix;
+    float2 * x;
+    float3 * u;
+};
+
+struct SortView {
+    int * idx;
+    int * npt;
+    int * new_np;
+    int * seg;
+    int * tmp_np;
+    int * total;
+};
+
+struct Geo {
+    int nx, ny, ext_x, ext_y, vol, off;
+};
+
+__device__ inline unsigned long long lanes_below() {
+    const unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : ( ~0ull >> ( 64 - lane ) );
+}
+
+// ---- interpolate_fld (species.cpp:43-92) on LDS float planes
+__device__ inline void interpolate( const float * __restrict__ Ex, const float * __restrict__ Ey,
+                                    const float * __restrict__ Ez, const float * __restrict__ Bx,
+                                    const float * __restrict__ By, const float * __restrict__ Bz, int ys,
+                                    int2 c, float2 x, float3 & e, float3 & b ) {
+    const int i = c.x, j = c.y;
+    const float s0x = 0.5f - x.x, s1x = 0.5f + x.x;
+    const float s0y = 0.5f - x.y, s1y = 0.5f + x.y;
+    const int hx = x.x < 0, hy = x.y < 0;
+    const int ih = i - hx, jh = j - hy;
+    const float s0xh = ( 1 - hx ) - x.x, s1xh = hx + x.x;
+    const float s0yh = ( 1 - hy ) - x.y, s1yh = hy + x.y;
+
+    const int k00 = ih + j * ys;  // (ih, j)
+    const int kjh = i + jh * ys;  // (i, jh)
+    const int kij = i + j * ys;   // (i, j)
+    const int khh = ih + jh * ys; // (ih, jh)
+
+    e.x = ( Ex[k00] * s0xh + Ex[k00 + 1] * s1xh ) * s0y + ( Ex[k00 + ys] * s0xh + Ex[k00 + ys + 1] * s1xh ) * s1y;
+    e.y = ( Ey[kjh] * s0x + Ey[kjh + 1] * s1x ) * s0yh + ( Ey[kjh + ys] * s0x + Ey[kjh + ys + 1] * s1x ) * s1yh;
+    e.z = ( Ez[kij] * s0x + Ez[kij + 1] * s1x ) * s0y + ( Ez[kij + ys] * s0x + Ez[kij + ys + 1] * s1x ) * s1y;
+
+    b.x = ( Bx[kjh] * s0x + Bx[kjh + 1] * s1x ) * s0yh + ( Bx[kjh + ys] * s0x + Bx[kjh + ys + 1] * s1x ) * s1yh;
+    b.y = ( By[k00] * s0xh + By[k00 + 1] * s1xh ) * s0y + ( By[k00 + ys] * s0xh + By[k00 + ys + 1] * s1xh ) * s1y;
+    b.z = ( Bz[khh] * s0xh + Bz[khh + 1] * s1xh ) * s0yh + ( Bz[khh + ys] * s0xh + Bz[khh + ys + 1] * s1xh ) * s1yh;
+}
+
+// ---- dudt_boris (species.cpp:117-178); returns u and the energy term
+__device__ inline float3 boris( float alpha, float3 e, float3 b, float3 u, float & ene ) {
+    e.x *= alpha; e.y *= alpha; e.z *= alpha;
+    float3 ut = make_float3( u.x + e.x, u.y + e.y, u.z + e.z );
+    {
+        const float usq = fmaf( ut.z, ut.z, fmaf( ut.y, ut.y, ut.x * ut.x ) );
+        const float g = sqrtf( 1.0f + usq );
+        ene = usq / ( g + 1.0f );
+        const float ag = alpha / g;
+        b.x *= ag; b.y *= ag; b.z *= ag;
+    }
+    float3 v;
+    v.x = fmaf( b.z, ut.y, ut.x );
+    v.y = fmaf( b.x, ut.z, ut.y );
+    v.z = fmaf( b.y, ut.x, ut.z );
+    v.x = fmaf( -b.y, ut.z, v.x );
+    v.y = fmaf( -b.z, ut.x, v.y );
+    v.z = fmaf( -b.x, ut.y, v.z );
+    {
+        const float s = 2.0f / fmaf( b.z, b.z, fmaf( b.y, b.y, fmaf( b.x, b.x, 1.0f ) ) );
+        b.x *= s; b.y *= s; b.z *= s;
+    }
+    ut.x = fmaf( b.z, v.y, ut.x );
+    ut.y = fmaf( b.x, v.z, ut.y );
+    ut.z = fmaf( b.y, v.x, ut.z );
+    ut.x = fmaf( -b.y, v.z, ut.x );
+    ut.y = fmaf( -b.z, v.x, ut.y );
+    ut.z = fmaf( -b.x, v.y, ut.z );
+    ut.x += e.x; ut.y += e.y; ut.z += e.z;
+    return ut;
+}
+
+// ---- rgamma (species.cpp:22-26): true division by a rounded sqrt, not rsqrt
+__device__ inline float inv_gamma( float3 u ) {
+    return 1.0f / sqrtf( fmaf( u.z, u.z, fmaf( u.y, u.y, fmaf( u.x, u.x, 1.0f ) ) ) );
+}
+
+struct Seg {
+    int2 c;
+    float2 a, b;
+    float qvz;
+};
+
+// ---- split2d (species.cpp:336-430): up to three straight segments, one per cell
+__device__ inline int split_path( int2 c0, float2 x0, float2 x1, float2 d, float qvz, int2 & di, Seg s[3] ) {
+    di.x = ( x1.x >= 0.5f ) - ( x1.x < -0.5f );
+    di.y = ( x1.y >= 0.5f ) - ( x1.y < -0.5f );
+    const bool cx = di.x != 0, cy = di.y != 0;
+    s[0].c = c0; s[0].a = x0; s[0].b = x1; s[0].qvz = qvz;
+    if ( !cx && !cy ) return 1;
+    if ( !cx ) {
+        const float yb = 0.5f * di.y;
+        const float f = ( yb - x0.y ) / d.y;
+        const float xb = x0.x + d.x * f;
+        s[0].b = make_float2( xb, yb );
+        s[0].qvz = qvz * f;
+        s[1].c = make_int2( c0.x, c0.y + di.y );
+        s[1].a = make_float2( xb, -yb );
+        s[1].b = make_float2( x1.x, x1.y - di.y );
+        s[1].qvz = qvz * ( 1 - f );
+        return 2;
+    }
+    const float xb = 0.5f * di.x;
+    float f = ( xb - x0.x ) / d.x;
+    const float yb = x0.y + d.y * f;
+    s[0].b = make_float2( xb, yb );
+    s[0].qvz = qvz * f;
+    s[1].c = make_int2( c0.x + di.x, c0.y );
+    s[1].a = make_float2( -xb, yb );
+    s[1].b = make_float2( x1.x - di.x, x1.y );
+    s[1].qvz = qvz * ( 1 - f );
+    if ( !cy ) return 2;
+    const float yb2 = 0.5f * di.y;
+    if ( yb >= -0.5f && yb < 0.5f ) {
+        // y crossing on the second segment
+        f = ( yb2 - yb ) / ( x1.y - yb );
+        const float xb2 = -xb + ( x1.x - xb ) * f;
+        s[2].c = make_int2( s[1].c.x, s[1].c.y + di.y );
+        s[2].a = make_float2( xb2, -yb2 );
+        s[2].b = make_float2( s[1].b.x, s[1].b.y - di.y );
+        s[2].qvz = s[1].qvz * ( 1 - f );
+        s[1].b = make_float2( xb2, yb2 );
+        s[1].qvz *= f;
+    } else {
+        // y crossing on the first segment
+        f = ( yb2 - x0.y ) / ( yb - x0.y );
+        const float xb2 = x0.x + ( xb - x0.x ) * f;
+        s[2].c = make_int2( s[0].c.x, s[0].c.y + di.y );
+        s[2].a = make_float2( xb2, -yb2 );
+        s[2].b = make_float2( s[0].b.x, s[0].b.y - di.y );
+        s[2].qvz = s[0].qvz * ( 1 - f );
+        s[0].b = make_float2( xb2, yb2 );
+        s[0].qvz *= f;
+        s[1].c.y += di.y;
+        s[1].a.y -= di.y;
+        s[1].b.y -= di.y;
+    }
+    return 3;
+}
+
+template <bool ATOMIC>
+__device__ inline void lds_add( float * p, float v ) {
+    if constexpr ( ATOMIC ) {
+        __hip_atomic_fetch_add( p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+    } else {
+        *p += v;
+    }
+}
+
+// ---- dep_current_seg (species.cpp:267-310) into LDS planes (J at cell (0,0))
+template <bool ATOMIC>
+__device__ inline void deposit( const Seg & s, float qnx, float qny, float * Jx, float * Jy, float * Jz, int ys ) {
+    const float A0x = 0.5f - s.a.x, A1x = 0.5f + s.a.x, B0x = 0.5f - s.b.x, B1x = 0.5f + s.b.x;
+    const float A0y = 0.5f - s.a.y, A1y = 0.5f + s.a.y, B0y = 0.5f - s.b.y, B1y = 0.5f + s.b.y;
+    const float jx = qnx * ( s.b.x - s.a.x );
+    const float jy = qny * ( s.b.y - s.a.y );
+    const float wy0 = 0.5f * ( A0y + B0y ), wy1 = 0.5f * ( A1y + B1y );
+    const float wx0 = 0.5f * ( A0x + B0x ), wx1 = 0.5f * ( A1x + B1x );
+    const int k = s.c.x + ys * s.c.y;
+    const float q = s.qvz;
+    lds_add<ATOMIC>( &Jx[k], jx * wy0 );
+    lds_add<ATOMIC>( &Jy[k], jy * wx0 );
+    lds_add<ATOMIC>( &Jz[k], q * ( A0x * A0y + B0x * B0y + ( A0x * B0y - B0x * A0y ) / 2.0f ) );
+    lds_add<ATOMIC>( &Jy[k + 1], jy * wx1 );
+    lds_add<ATOMIC>( &Jz[k + 1], q * ( A1x * A0y + B1x * B0y + ( A1x * B0y - B1x * A0y ) / 2.0f ) );
+    lds_add<ATOMIC>( &Jx[k + ys], jx * wy1 );
+    lds_add<ATOMIC>( &Jz[k + ys], q * ( A0x * A1y + B0x * B1y + ( A0x * B1y - B0x * A1y ) / 2.0f ) );
+    lds_add<ATOMIC>( &Jz[k + ys + 1], q * ( A1x * A1y + B1x * B1y + ( A1x * B1y - B1x * A1y ) / 2.0f ) );
+}
+
+struct MoveArgs {
+    float dtdx, dtdy, q, qnx, qny;
+};
+
+// ---------------------------------------------------------------------------
+// Particle pass.  PUSH: interpolation + Boris.  MOVE: position advance,
+// deposit, cell update and leaver classification.  DET: one wavefront, lane
+// serialised deposit and energy sum (reference order).
+//
+// Wave w of W handles the contiguous chunk [w*chunk, (w+1)*chunk) of the
+// tile's particles, so that the leaver indices it appends to the tile's list
+// (idx[offset + w*chunk ...]) are in increasing order; seg[16 t + w] keeps the
+// count of each wave's segment, seg[16 t + 15] the chunk.
+template <bool PUSH, bool MOVE, bool DET>
+__global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, const float3 * __restrict__ E,
+                                                          const float3 * __restrict__ B, float3 * __restrict__ J,
+                                                          float alpha, MoveArgs m, double * __restrict__ tile_energy,
+                                                          SortView S ) {
+    extern __shared__ float smem[];
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int vol = g.vol;
+    const int ys = g.ext_x;
+
+    float * Ex = smem;
+    float * Ey = Ex + vol;
+    float * Ez = Ey + vol;
+    float * Bx = Ez + vol;
+    float * By = Bx + vol;
+    float * Bz = By + vol;
+    float * Jx = PUSH ? Bz + vol : smem;
+    float * Jy = Jx + vol;
+    float * Jz = Jy + vol;
+    __shared__ int s_cnt[9];
+    __shared__ double s_en[FAST_BLOCK / WAVE];
+    __shared__ int s_nl[FAST_BLOCK / WAVE];
+
+    if constexpr ( PUSH ) {
+        const float * Et = reinterpret_cast<const float *>( E + (size_t) t * vol );
+        const float * Bt = reinterpret_cast<const float *>( B + (size_t) t * vol );
+        for ( int k = tid; k < 3 * vol; k += nthr ) {
+            const int el = k / 3, c = k - 3 * el;
+            Ex[c * vol + el] = Et[k];
+            Bx[c * vol + el] = Bt[k];
+        }
+    }
+    if constexpr ( MOVE ) {
+        for ( int k = tid; k < 3 * vol; k += nthr ) Jx[k] = 0.0f;
+        if ( tid < 9 ) s_cnt[tid] = 0;
+    }
+    __syncthreads();
+
+    const int off = P.off[t];
+    const int np = P.np[t];
+    const int W = nthr / WAVE;
+    const int w = tid / WAVE;
+    const int lane = tid % WAVE;
+    const int chunk = ( ( np + W - 1 ) / W + WAVE - 1 ) & ~( WAVE - 1 );
+    const int beg = w * chunk;
+    const int end = min( np, beg + chunk );
+
+    const float * e0x = Ex + g.off; const float * e0y = Ey + g.off; const float * e0z = Ez + g.off;
+    const float * b0x = Bx + g.off; const float * b0y = By + g.off; const float * b0z = Bz + g.off;
+    float * j0x = Jx + g.off; float * j0y = Jy + g.off; float * j0z = Jz + g.off;
+
+    double en = 0.0;
+    int nl = 0;
+
+    for ( int i0 = beg; i0 < end; i0 += WAVE ) {
+        const int i = i0 + lane;
+        const bool act = i < end;
+        int2 c = make_int2( 0, 0 );
+        float2 x = make_float2( 0.f, 0.f );
+        float3 u = make_float3( 0.f, 0.f, 0.f );
+        if ( act ) {
+            c = P.ix[off + i];
+            x = P.x[off + i];
+            u = P.u[off + i];
+        }
+        if constexpr ( PUSH ) {
+            float ene = 0.0f;
+            if ( act ) {
+                float3 e, b;
+                interpolate( e0x, e0y, e0z, b0x, b0y, b0z, ys, c, x, e, b );
+                u = boris( alpha, e, b, u, ene );
+                P.u[off + i] = u;
+            }
+            if constexpr ( DET ) {
+                // energy += term, in particle order
+                for ( int l = 0; l < WAVE; l++ ) {
+                    const float v = __shfl( ene, l );
+                    if ( i0 + l < end ) en += v;
+                }
+            } else {
+                en += ene;
+            }
+        }
+        if constexpr ( MOVE ) {
+            int dir = 4;
+            int ns = 0;
+            Seg sg[3];
+            if ( act ) {
+                const float rg = inv_gamma( u );
+                const float2 d = make_float2( m.dtdx * rg * u.x, m.dtdy * rg * u.y );
+                const float qvz = m.q * u.z * rg * 0.5f;
+                float2 x1 = make_float2( x.x + d.x, x.y + d.y );
+                int2 di;
+                ns = split_path( c, x, x1, d, qvz, di, sg );
+                if constexpr ( !DET ) {
+                    deposit<true>( sg[0], m.qnx, m.qny, j0x, j0y, j0z, ys );
+                    if ( ns > 1 ) deposit<true>( sg[1], m.qnx, m.qny, j0x, j0y, j0z, ys );
+                    if ( ns > 2 ) deposit<true>( sg[2], m.qnx, m.qny, j0x, j0y, j0z, ys );
+                }
+                x1.x -= di.x;
+                x1.y -= di.y;
+                const int2 c1 = make_int2( c.x + di.x, c.y + di.y );
+                P.x[off + i] = x1;
+                P.ix[off + i] = c1;
+                const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
+                const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
+                dir = ( yc + 1 ) * 3 + ( xc + 1 );
+            }
+            if constexpr ( DET ) {
+                for ( int l = 0; l < WAVE; l++ ) {
+                    if ( lane == l && ns > 0 ) {
+                        deposit<false>( sg[0], m.qnx, m.qny, j0x, j0y, j0z, ys );
+                        if ( ns > 1 ) deposit<false>( sg[1], m.qnx, m.qny, j0x, j0y, j0z, ys );
+                        if ( ns > 2 ) deposit<false>( sg[2], m.qnx, m.qny, j0x, j0y, j0z, ys );
+                    }
+                }
+            }
+            const bool leave = act && dir != 4;
+            const unsigned long long mask = __ballot( leave );
+            if ( leave ) {
+                const int pos = nl + __popcll( mask & lanes_below() );
+                S.idx[off + beg + pos] = ( i << 4 ) | dir;
+                atomicAdd( &s_cnt[dir], 1 );
+            }
+            nl += __popcll( mask );
+        }
+    }
+
+    if constexpr ( PUSH ) {
+        if constexpr ( !DET ) {
+            for ( int o = WAVE / 2; o > 0; o >>= 1 ) en += __shfl_xor( en, o );
+        }
+        if ( lane == 0 ) s_en[w] = en;
+    }
+    if constexpr ( MOVE ) {
+        if ( lane == 0 ) s_nl[w] = nl;
+    }
+    __syncthreads();
+
+    if constexpr ( MOVE ) {
+        // flush: J(tile) += J(LDS), guard cells included (species.cpp:1488-1492)
+        float * Jt = reinterpret_cast<float *>( J + (size_t) t * vol );
+        for ( int k = tid; k < 3 * g.ext_x * g.ext_y; k += nthr ) {
+            const int el = k / 3, c = k - 3 * el;
+            Jt[k] += Jx[c * vol + el];
+        }
+        if ( tid == 0 ) {
+            int tot = 0;
+            for ( int v = 0; v < W; v++ ) {
+                S.seg[16 * t + v] = s_nl[v];
+                tot += s_nl[v];
+            }
+            for ( int v = W; v < MAX_SEG; v++ ) S.seg[16 * t + v] = 0;
+            S.seg[16 * t + MAX_SEG] = chunk;
+            for ( int d = 0; d < 9; d++ ) S.npt[9 * t + d] = s_cnt[d];
+            S.npt[9 * t + 4] = np - tot;
+        }
+    }
+    if constexpr ( PUSH ) {
+        if ( tid == 0 ) {
+            double s = 0.0;
+            for ( int v = 0; v < W; v++ ) s += s_en[v];
+            tile_energy[t] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile sort.  Directions d = 0..8 encode the motion (d%3-1, d/3-1) of a
+// particle from its source tile to its target tile; d = 4 is "stays".
+
+__device__ inline int wrap_tile( int tx, int ty, int ntx, int nty, int px, int py ) {
+    if ( px ) { if ( tx < 0 ) tx += ntx; if ( tx >= ntx ) tx -= ntx; }
+    if ( py ) { if ( ty < 0 ) ty += nty; if ( ty >= nty ) ty -= nty; }
+    if ( tx < 0 || tx >= ntx || ty < 0 || ty >= nty ) return -1;
+    return ty * ntx + tx;
+}
+
+struct SortGeo {
+    int ntx, nty, nx, ny, px, py;
+};
+
+// Tile receiving the particles that tile t sends in direction d (-1 if none).
+__device__ inline int target_of( int t, int d, const SortGeo & q ) {
+    return wrap_tile( t % q.ntx + ( d % 3 - 1 ), t / q.ntx + ( d / 3 - 1 ), q.ntx, q.nty, q.px, q.py );
+}
+// Tile that sends particles in direction d into tile t (-1 if none).
+__device__ inline int source_of( int t, int d, const SortGeo & q ) {
+    return wrap_tile( t % q.ntx - ( d % 3 - 1 ), t / q.ntx - ( d / 3 - 1 ), q.ntx, q.nty, q.px, q.py );
+}
+
+// New particle count per tile and its exclusive scan (bnd_check's new_np and
+// update_tile_info, particles.cpp:212-357).  One workgroup; each thread owns a
+// contiguous run of tiles.
+constexpr int SCAN_BLOCK = 1024;
+__global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortView S, int * __restrict__ new_off, SortGeo q,
+                                                              int ntiles, unsigned max_part ) {
+    __shared__ int s_sum[SCAN_BLOCK];
+    const int tid = threadIdx.x;
+    const int per = ( ntiles + SCAN_BLOCK - 1 ) / SCAN_BLOCK;
+    const int t0 = min( ntiles, tid * per ), t1 = min( ntiles, t0 + per );
+    int local = 0;
+    for ( int t = t0; t < t1; t++ ) {
+        int n = 0;
+        for ( int d = 0; d < 9; d++ ) {
+            const int s = source_of( t, d, q );
+            if ( s >= 0 ) n += S.npt[9 * s + d];
+        }
+        S.new_np[t] = n;
+        local += n;
+    }
+    s_sum[tid] = local;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the per-thread sums
+    for ( int o = 1; o < SCAN_BLOCK; o <<= 1 ) {
+        const int v = tid >= o ? s_sum[tid - o] : 0;
+        __syncthreads();
+        s_sum[tid] += v;
+        __syncthreads();
+    }
+    int base = s_sum[tid] - local;
+    for ( int t = t0; t < t1; t++ ) {
+        new_off[t] = base;
+        base += S.new_np[t];
+    }
+    if ( tid == SCAN_BLOCK - 1 ) S.total[0] = s_sum[tid];
+    (void) max_part;
+}
+
+// Stayers that move to the staging buffer because the tile's range moved
+// (copy_out, particles.cpp:409-423).
+__device__ inline int shifted_stayers( int t, const int * old_off, const int * new_off, const SortView & S ) {
+    const int o = old_off[t], no = new_off[t], n0 = S.npt[9 * t + 4], nn = S.new_np[t];
+    int ns;
+    if ( no >= o ) ns = no - o;
+    else { ns = ( o + n0 ) - ( no + nn ); if ( ns < 0 ) ns = 0; }
+    return ns > n0 ? n0 : ns;
+}
+
+// Position j of the tile's ordered leaver list.
+__device__ inline int leaver_at( int j, const int * seg_pre, int chunk, const int * idx, int off ) {
+    int w = 0;
+    while ( w + 1 < MAX_SEG && seg_pre[w + 1] <= j ) w++;
+    return idx[off + w * chunk + ( j - seg_pre[w] )];
+}
+
+constexpr int SORT_BLOCK = 256;
+
+// copy_out (particles.cpp:379-550): leavers to their staging slots, holes
+// below n0 refilled from the stayers above n0, shifted stayers staged.  The
+// staging slots of each target tile are filled in the reference's
+// single-thread reservation order: by source tile, the source's own shifted
+// stayers first, then directions 0..8.
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( PartView P, PartView T, SortView S, SortGeo q ) {
+    __shared__ int s_start[9];
+    __shared__ int s_pre[MAX_SEG + 1];
+    __shared__ int s_hdr[4];
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+
+    if ( tid < 9 ) {
+        const int d = tid;
+        const int tg = ( d == 4 ) ? s : target_of( s, d, q );
+        int start = -1;
+        if ( tg >= 0 ) {
+            const int mine = ( d == 4 ) ? -1 : d;
+            int before = 0, total = 0;
+            for ( int e = 0; e < 9; e++ ) {
+                const int se = ( e == 4 ) ? tg : source_of( tg, e, q );
+                if ( se < 0 ) continue;
+                const int cnt = ( e == 4 ) ? shifted_stayers( tg, P.off, T.off, S ) : S.npt[9 * se + e];
+                const int rank = ( e == 4 ) ? -1 : e;
+                if ( se < s || ( se == s && rank < mine ) ) before += cnt;
+                total += cnt;
+            }
+            start = T.off[tg] + before;
+            if ( d == 4 ) S.tmp_np[s] = total;
+        }
+        s_start[d] = start;
+    }
+    if ( tid == 0 ) {
+        int acc = 0;
+        for ( int w = 0; w < MAX_SEG; w++ ) { s_pre[w] = acc; acc += S.seg[16 * s + w]; }
+        s_pre[MAX_SEG] = acc;
+        s_hdr[0] = P.off[s];
+        s_hdr[1] = P.np[s];
+        s_hdr[2] = S.npt[9 * s + 4];
+        s_hdr[3] = S.seg[16 * s + MAX_SEG];
+    }
+    __syncthreads();
+
+    const int o = s_hdr[0], np = s_hdr[1], n0 = s_hdr[2], chunk = s_hdr[3];
+    const int nl = np - n0;
+
+    if ( tid < WAVE ) {
+        const int lane = tid;
+        // leavers to staging, rank within each direction = order in the list
+        int run[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+        for ( int jb = 0; jb < nl; jb += WAVE ) {
+            const int j = jb + lane;
+            const bool valid = j < nl;
+            const int ent = valid ? leaver_at( j, s_pre, chunk, S.idx, o ) : 0;
+            const int d = ent & 15;
+            int rank = 0;
+            for ( int dd = 0; dd < 9; dd++ ) {
+                if ( dd == 4 ) continue;
+                const unsigned long long mk = __ballot( valid && d == dd );
+                if ( d == dd ) rank = run[dd] + __popcll( mk & lanes_below() );
+                run[dd] += __popcll( mk );
+            }
+            if ( valid && s_start[d] >= 0 ) {
+                const int k = o + ( ent >> 4 );
+                const int2 c = P.ix[k];
+                const int xc = ( c.x >= q.nx ) - ( c.x < 0 );
+                const int yc = ( c.y >= q.ny ) - ( c.y < 0 );
+                const int dst = s_start[d] + rank;
+                T.ix[dst] = make_int2( c.x - xc * q.nx, c.y - yc * q.ny );
+                T.x[dst] = P.x[k];
+                T.u[dst] = P.u[k];
+            }
+        }
+        // holes below n0 <- stayers above n0, both in increasing order
+        int r0 = 0;
+        for ( int pb = n0; pb < np; pb += WAVE ) {
+            const int p = pb + lane;
+            bool stay = false;
+            if ( p < np ) {
+                const int2 c = P.ix[o + p];
+                stay = c.x >= 0 && c.x < q.nx && c.y >= 0 && c.y < q.ny;
+            }
+            const unsigned long long mk = __ballot( stay );
+            if ( stay ) {
+                const int r = r0 + __popcll( mk & lanes_below() );
+                const int hole = o + ( leaver_at( r, s_pre, chunk, S.idx, o ) >> 4 );
+                P.ix[hole] = P.ix[o + p];
+                P.x[hole] = P.x[o + p];
+                P.u[hole] = P.u[o + p];
+            }
+            r0 += __popcll( mk );
+        }
+    }
+    __syncthreads();
+
+    // shifted stayers (after the hole fill)
+    const int ns = shifted_stayers( s, P.off, T.off, S );
+    const int first = ( T.off[s] >= o ) ? 0 : n0 - ns;
+    const int dst0 = s_start[4];
+    for ( int i = tid; i < ns; i += blockDim.x ) {
+        T.ix[dst0 + i] = P.ix[o + first + i];
+        T.x[dst0 + i] = P.x[o + first + i];
+        T.u[dst0 + i] = P.u[o + first + i];
+    }
+    __syncthreads();
+    if ( tid == 0 ) P.np[s] = n0 - ns;
+}
+
+// copy_in (particles.cpp:559-614): staged particles appended after the
+// in-place stayers, or first into the gap when the range moved down.
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( PartView P, PartView T, SortView S ) {
+    __shared__ int s_hdr[4];
+    const int t = blockIdx.x;
+    if ( threadIdx.x == 0 ) {
+        s_hdr[0] = P.off[t];
+        s_hdr[1] = P.np[t];
+        s_hdr[2] = T.off[t];
+        s_hdr[3] = S.tmp_np[t];
+    }
+    __syncthreads();
+    const int o = s_hdr[0], n = s_hdr[1], no = s_hdr[2], nin = s_hdr[3];
+    int front = 0;
+    if ( no < o ) { front = o - no; if ( front > nin ) front = nin; }
+    for ( int i = threadIdx.x; i < nin; i += blockDim.x ) {
+        const int dst = ( i < front ) ? no + i : no + n + i;
+        P.ix[dst] = T.ix[no + i];
+        P.x[dst] = T.x[no + i];
+        P.u[dst] = T.u[no + i];
+    }
+    __syncthreads();
+    if ( threadIdx.x == 0 ) {
+        P.np[t] = n + nin;
+        P.off[t] = no;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dep_charge_kernel (species.cpp:2348-2395) into a scalar grid with guards.
+template <bool DET>
+__global__ void __launch_bounds__( FAST_BLOCK ) k_charge( PartView P, Geo g, float * __restrict__ rho, float q ) {
+    extern __shared__ float smem[];
+    const int t = blockIdx.x;
+    const int vol = g.vol, ys = g.ext_x;
+    for ( int k = threadIdx.x; k < vol; k += blockDim.x ) smem[k] = 0.0f;
+    __syncthreads();
+    float * c0 = smem + g.off;
+    const int off = P.off[t], np = P.np[t];
+    const int lane = threadIdx.x % WAVE;
+    const int nthr = blockDim.x;
+    for ( int i0 = ( threadIdx.x / WAVE ) * WAVE; i0 < np; i0 += nthr ) {
+        const int i = i0 + lane;
+        const bool act = i < np;
+        int k = 0;
+        float w00 = 0, w10 = 0, w01 = 0, w11 = 0;
+        if ( act ) {
+            const int2 c = P.ix[off + i];
+            const float2 x = P.x[off + i];
+            k = c.y * ys + c.x;
+            const float s0x = 0.5f - x.x, s1x = 0.5f + x.x, s0y = 0.5f - x.y, s1y = 0.5f + x.y;
+            w00 = s0y * s0x * q; w10 = s0y * s1x * q; w01 = s1y * s0x * q; w11 = s1y * s1x * q;
+        }
+        if constexpr ( DET ) {
+            for ( int l = 0; l < WAVE; l++ ) {
+                if ( lane == l && act ) {
+                    c0[k] += w00; c0[k + 1] += w10; c0[k + ys] += w01; c0[k + ys + 1] += w11;
+                }
+            }
+        } else if ( act ) {
+            lds_add<true>( &c0[k], w00 );
+            lds_add<true>( &c0[k + 1], w10 );
+            lds_add<true>( &c0[k + ys], w01 );
+            lds_add<true>( &c0[k + ys + 1], w11 );
+        }
+    }
+    __syncthreads();
+    float * dst = rho + (size_t) t * vol;
+    for ( int k = threadIdx.x; k < g.ext_x * g.ext_y; k += blockDim.x ) dst[k] += smem[k];
+}
+
+Geo make_geo( const zpic_geom * f ) {
+    Geo g;
+    g.nx = f->nx; g.ny = f->ny; g.ext_x = f->ext_x; g.ext_y = f->ext_y; g.vol = f->tile_vol; g.off = f->offset;
+    return g;
+}
+
+PartView view( const zpic_particles * p ) {
+    PartView v;
+    v.np = p->np; v.off = p->offset;
+    v.ix = reinterpret_cast<int2 *>( p->ix );
+    v.x = reinterpret_cast<float2 *>( p->x );
+    v.u = reinterpret_cast<float3 *>( p->u );
+    return v;
+}
+
+SortView view( const zpic_sort * s ) {
+    SortView v;
+    if ( s ) { v.idx = s->idx; v.npt = s->npt; v.new_np = s->new_np; v.seg = s->seg; v.tmp_np = s->tmp_np; v.total = s->total; }
+    else { v.idx = v.npt = v.new_np = v.seg = v.tmp_np = v.total = nullptr; }
+    return v;
+}
+
+int check_geom( const zpic_geom * g ) {
+    if ( !g || g->ntiles_x < 1 || g->ntiles_y < 1 || g->nx < 1 || g->ny < 1 )
+        return set_error( ZPIC_EINVAL, "invalid grid geometry" );
+    if ( g->gc_lower < 1 || g->gc_upper < 2 )
+        return set_error( ZPIC_EINVAL, "particle stages need guard cells {1,2}" );
+    return ZPIC_OK;
+}
+
+template <bool PUSH, bool MOVE>
+int launch_particles( const zpic_particles * p, const zpic_geom * fld, const float * E, const float * B, float * J,
+                      float alpha, MoveArgs m, double * tile_energy, zpic_sort * s, int det, void * stream ) {
+    if ( int rc = check_geom( fld ) ) return rc;
+    if ( !p || ( PUSH && ( !E || !B || !tile_energy ) ) || ( MOVE && ( !J || !s ) ) )
+        return set_error( ZPIC_EINVAL, "null buffer" );
+    const int nt = fld->ntiles_x * fld->ntiles_y;
+    const size_t lds = sizeof( float ) * fld->tile_vol * ( ( PUSH ? 6 : 0 ) + ( MOVE ? 3 : 0 ) );
+    if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    const Geo g = make_geo( fld );
+    hipStream_t st = (hipStream_t) stream;
+    if ( det )
+        hipLaunchKernelGGL( ( k_particles<PUSH, MOVE, true> ), dim3( nt ), dim3( WAVE ), lds, st, view( p ), g,
+                            (const float3 *) E, (const float3 *) B, (float3 *) J, alpha, m, tile_energy, view( s ) );
+    else
+        hipLaunchKernelGGL( ( k_particles<PUSH, MOVE, false> ), dim3( nt ), dim3( FAST_BLOCK ), lds, st, view( p ), g,
+                            (const float3 *) E, (const float3 *) B, (float3 *) J, alpha, m, tile_energy, view( s ) );
+    ZPIC_LAUNCH_CHECK( "k_particles" );
+    return ZPIC_OK;
+}
+
+}  // namespace
+}  // namespace zpic
+
+using namespace zpic;
+
+extern "C" {
+
+int zpic_push( const zpic_particles * p, const zpic_geom * fld, const float * E, const float * B, float alpha,
+               double * tile_energy, int deterministic, void * stream ) {
+    MoveArgs m{};
+    return launch_particles<true, false>( p, fld, E, B, nullptr, alpha, m, tile_energy, nullptr, deterministic, stream );
+}
+
+int zpic_move_deposit( const zpic_particles * p, const zpic_geom * fld, float * J, float dt_dx, float dt_dy, float q,
+                       float qnx, float qny, zpic_sort * s, int deterministic, void * stream ) {
+    MoveArgs m{ dt_dx, dt_dy, q, qnx, qny };
+    return launch_particles<false, true>( p, fld, nullptr, nullptr, J, 0.f, m, nullptr, s, deterministic, stream );
+}
+
+int zpic_push_move_deposit( const zpic_particles * p, const zpic_geom * fld, const float * E, const float * B,
+                            float * J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
+                            double * tile_energy, zpic_sort * s, int deterministic, void * stream ) {
+    MoveArgs m{ dt_dx, dt_dy, q, qnx, qny };
+    return launch_particles<true, true>( p, fld, E, B, J, alpha, m, tile_energy, s, deterministic, stream );
+}
+
+int zpic_tile_sort( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int ntiles_x, int ntiles_y, int nx,
+                    int ny, int periodic_x, int periodic_y, void * stream ) {
+    if ( !p || !tmp || !s || ntiles_x < 1 || ntiles_y < 1 || nx < 1 || ny < 1 )
+        return set_error( ZPIC_EINVAL, "invalid tile_sort arguments" );
+    const int nt = ntiles_x * ntiles_y;
+    SortGeo q{ ntiles_x, ntiles_y, nx, ny, periodic_x ? 1 : 0, periodic_y ? 1 : 0 };
+    hipStream_t st = (hipStream_t) stream;
+    const PartView P = view( p ), T = view( tmp );
+    const SortView S = view( s );
+    hipLaunchKernelGGL( k_sort_count, dim3( 1 ), dim3( SCAN_BLOCK ), 0, st, S, T.off, q, nt, p->max_part );
+    ZPIC_LAUNCH_CHECK( "k_sort_count" );
+    hipLaunchKernelGGL( k_sort_out, dim3( nt ), dim3( SORT_BLOCK ), 0, st, P, T, S, q );
+    ZPIC_LAUNCH_CHECK( "k_sort_out" );
+    hipLaunchKernelGGL( k_sort_in, dim3( nt ), dim3( SORT_BLOCK ), 0, st, P, T, S );
+    ZPIC_LAUNCH_CHECK( "k_sort_in" );
+    return ZPIC_OK;
+}
+
+int zpic_deposit_charge( const zpic_particles * p, const zpic_geom * g, float * rho, float q, int deterministic,
+                         void * stream ) {
+    if ( !p || !g || !rho ) return set_error( ZPIC_EINVAL, "null buffer" );
+    const int nt = g->ntiles_x * g->ntiles_y;
+    const size_t lds = sizeof( float ) * g->tile_vol;
+    if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    hipStream_t st = (hipStream_t) stream;
+    if ( deterministic )
+        hipLaunchKernelGGL( k_charge<true>, dim3( nt ), dim3( WAVE ), lds, st, view( p ), make_geo( g ), rho, q );
+    else
+        hipLaunchKernelGGL( k_charge<false>, dim3( nt ), dim3( FAST_BLOCK ), lds, st, view( p ), make_geo( g ), rho, q );
+    ZPIC_LAUNCH_CHECK( "k_charge" );
+    return ZPIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,505 @@
+// zpic C++ API for the MI355X em2d step: the reference's Simulation /
+// Species / EMF / Current / vec3grid / Filter / UDistribution / Density
+// classes (openmp/em2d) with the same names, signatures and argument
+// meanings, backed by device-resident buffers and the stage entry points of
+// zpic_amd.h.  Compile user code that includes this header with hipcc
+// (it uses HIP's int2/uint2/float2/float3/double3 vector types).
+//
+// Error behaviour: where the reference prints "(*error*) ..." and exits,
+// these classes throw std::runtime_error carrying the same message, so that
+// a library never terminates its host process.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "zpic_amd.h"
+
+namespace coord {
+enum cart { x = 0, y };
+}
+
+namespace fcomp {
+enum cart { x = 0, y, z };
+}
+
+namespace part {
+enum quant { x = 0, y, ux, uy, uz };
+}
+
+// bnd<T> (bnd.h:21-78): per-axis lower / upper values
+template <typename T>
+class bnd {
+  public:
+    struct { T lower; T upper; } x;
+    struct { T lower; T upper; } y;
+    bnd() : x{ T( 0 ), T( 0 ) }, y{ T( 0 ), T( 0 ) } {}
+    bnd( T v ) : x{ v, v }, y{ v, v } {}
+};
+
+namespace zpic {
+
+// Library stream shared by all objects (default priority).
+hipStream_t stream();
+// Throws std::runtime_error for a non-zero zpic_* return code.
+void check( int rc );
+void hip_check( hipError_t e, const char * what );
+
+template <class T>
+T * device_alloc( size_t n ) {
+    void * p = nullptr;
+    if ( n ) hip_check( hipMalloc( &p, n * sizeof( T ) ), "hipMalloc" );
+    return static_cast<T *>( p );
+}
+inline void device_free( void * p ) { if ( p ) (void) hipFree( p ); }
+
+}  // namespace zpic
+
+// ---------------------------------------------------------------- grid<T>
+// Tile-major grid with guard cells (grid.h:14-638); T is float or float3.
+template <class T>
+class grid {
+  protected:
+    uint2 ntiles;
+    int2 periodic;
+    const uint2 dims;
+
+  public:
+    T * d_buffer;
+    const uint2 nx;
+    const bnd<unsigned int> gc;
+    const uint2 ext_nx;
+    const unsigned int offset;
+    const unsigned int tile_vol;
+    std::string name;
+
+    grid( uint2 const ntiles_, uint2 const nx_, bnd<unsigned int> const gc_ )
+        : ntiles( ntiles_ ), periodic( make_int2( 1, 1 ) ), dims( make_uint2( ntiles_.x * nx_.x, ntiles_.y * nx_.y ) ),
+          d_buffer( nullptr ), nx( nx_ ), gc( gc_ ),
+          ext_nx( make_uint2( gc_.x.lower + nx_.x + gc_.x.upper, gc_.y.lower + nx_.y + gc_.y.upper ) ),
+          offset( gc_.y.lower * ext_nx.x + gc_.x.lower ), tile_vol( ( ext_nx.x * ext_nx.y + 3 ) & ~3u ), name( "grid" ) {
+        if ( ntiles.x == 0 || ntiles.y == 0 ) throw std::runtime_error( "Invalid number of tiles" );
+        if ( nx.x == 0 || nx.y == 0 ) throw std::runtime_error( "Invalid tile size" );
+        if ( gc.x.lower != gc.y.lower || gc.x.upper != gc.y.upper )
+            throw std::runtime_error( "guard cells must be the same along x and y" );
+        d_buffer = zpic::device_alloc<T>( buffer_size() );
+    }
+    grid( uint2 const ntiles_, uint2 const nx_ ) : grid( ntiles_, nx_, bnd<unsigned int>( 0 ) ) {}
+    ~grid() { zpic::device_free( d_buffer ); }
+    grid( const grid & ) = delete;
+    grid & operator=( const grid & ) = delete;
+
+    static constexpr int ncomp = sizeof( T ) / sizeof( float );
+
+    uint2 get_ntiles() const { return ntiles; }
+    uint2 get_dims() const { return dims; }
+    int2 get_periodic() const { return periodic; }
+    void set_periodic( int2 p ) { periodic = p; }
+    std::size_t buffer_size() const { return static_cast<std::size_t>( tile_vol ) * ( ntiles.x * ntiles.y ); }
+
+    zpic_geom geom() const {
+        zpic_geom g;
+        zpic_geom_init( &g, ntiles.x, ntiles.y, nx.x, nx.y, gc.x.lower, gc.x.upper );
+        return g;
+    }
+    float * fbuf() { return reinterpret_cast<float *>( d_buffer ); }
+    const float * fbuf() const { return reinterpret_cast<const float *>( d_buffer ); }
+
+    int zero() {
+        zpic::check( zpic_fill( fbuf(), buffer_size() * ncomp, 0.0f, zpic::stream() ) );
+        return 0;
+    }
+    void copy_to_gc() {
+        zpic_geom g = geom();
+        zpic::check( zpic_copy_to_gc( fbuf(), &g, ncomp, periodic.x, periodic.y, zpic::stream() ) );
+    }
+    void add_from_gc() {
+        zpic_geom g = geom();
+        zpic::check( zpic_add_from_gc( fbuf(), &g, ncomp, periodic.x, periodic.y, zpic::stream() ) );
+    }
+    void kernel3_x( float a, float b, float c ) {
+        zpic_geom g = geom();
+        zpic::check( zpic_kernel3( fbuf(), &g, 0, a, b, c, periodic.x, periodic.y, zpic::stream() ) );
+    }
+    void kernel3_y( float a, float b, float c ) {
+        zpic_geom g = geom();
+        zpic::check( zpic_kernel3( fbuf(), &g, 1, a, b, c, periodic.x, periodic.y, zpic::stream() ) );
+    }
+    // Whole tile-major buffer to / from the host (synchronous).
+    void download( T * host ) const {
+        zpic::hip_check( hipMemcpyAsync( host, d_buffer, buffer_size() * sizeof( T ), hipMemcpyDeviceToHost, zpic::stream() ), "download" );
+        zpic::hip_check( hipStreamSynchronize( zpic::stream() ), "sync" );
+    }
+    void upload( const T * host ) {
+        zpic::hip_check( hipMemcpyAsync( d_buffer, host, buffer_size() * sizeof( T ), hipMemcpyHostToDevice, zpic::stream() ), "upload" );
+        zpic::hip_check( hipStreamSynchronize( zpic::stream() ), "sync" );
+    }
+    // grid::gather (grid.h:222-247): interior cells, contiguous [dims.y][dims.x], host output
+    unsigned int gather( float * host_out, int fc = 0 ) const {
+        zpic_geom g = geom();
+        float * d = zpic::device_alloc<float>( (size_t) dims.x * dims.y );
+        zpic::check( zpic_gather_component( fbuf(), &g, ncomp, fc, d, zpic::stream() ) );
+        zpic::hip_check( hipMemcpyAsync( host_out, d, sizeof( float ) * dims.x * dims.y, hipMemcpyDeviceToHost, zpic::stream() ), "gather" );
+        zpic::hip_check( hipStreamSynchronize( zpic::stream() ), "sync" );
+        zpic::device_free( d );
+        return dims.x * dims.y;
+    }
+};
+
+// vec3grid<float3> (vec3grid.h:38-442)
+template <class V>
+class vec3grid : public grid<V> {
+  public:
+    using grid<V>::grid;
+    // component fc of the interior cells, contiguous (vec3grid.h:76-158)
+    unsigned int gather( const fcomp::cart fc, float * host_out ) const { return grid<V>::gather( host_out, fc ); }
+};
+
+// ----------------------------------------------------------- Filter:: (filter.h)
+namespace Filter {
+class Digital {
+  public:
+    virtual Digital * clone() const = 0;
+    virtual void apply( vec3grid<float3> & fld ) = 0;
+    virtual int kind() const = 0;
+    virtual ~Digital() = default;
+};
+class None : public Digital {
+  public:
+    None * clone() const override { return new None(); }
+    void apply( vec3grid<float3> & ) override {}
+    int kind() const override { return 0; }
+};
+class Binomial : public Digital {
+  protected:
+    unsigned int order;
+    coord::cart dir;
+
+  public:
+    Binomial( coord::cart dir_, unsigned int order_ = 0 ) : order( order_ > 0 ? order_ : 1 ), dir( dir_ ) {}
+    Binomial * clone() const override { return new Binomial( dir, order ); }
+    void apply( vec3grid<float3> & fld ) override {
+        for ( unsigned i = 0; i < order; i++ ) {
+            if ( dir == coord::x ) fld.kernel3_x( 0.25f, 0.5f, 0.25f );
+            else fld.kernel3_y( 0.25f, 0.5f, 0.25f );
+        }
+    }
+    int kind() const override { return 1; }
+    unsigned get_order() const { return order; }
+    coord::cart get_dir() const { return dir; }
+};
+class Compensated : public Binomial {
+  public:
+    Compensated( coord::cart dir_, unsigned int order_ = 0 ) : Binomial( dir_, order_ ) {}
+    Compensated * clone() const override { return new Compensated( dir, order ); }
+    void apply( vec3grid<float3> & fld ) override {
+        Binomial::apply( fld );
+        const float a = -1.0f;
+        const float b = ( 4.0 + 2.0 * order ) / order;
+        const float norm = 2 * a + b;
+        if ( dir == coord::x ) fld.kernel3_x( a / norm, b / norm, a / norm );
+        else fld.kernel3_y( a / norm, b / norm, a / norm );
+    }
+    int kind() const override { return 2; }
+};
+}  // namespace Filter
+
+// -------------------------------------------------------------- Particles
+class Particles {
+  public:
+    const uint2 ntiles;
+    const uint2 nx;
+    int2 periodic;
+    const uint2 dims;
+    int * np;
+    int * offset;
+    int2 * ix;
+    float2 * x;
+    float3 * u;
+    uint32_t max_part;
+
+    Particles( uint2 const ntiles, uint2 const nx, uint32_t const max_part );
+    ~Particles();
+    Particles( const Particles & ) = delete;
+    Particles & operator=( const Particles & ) = delete;
+
+    zpic_particles view() const;
+    uint64_t np_total() const;
+    std::vector<int> get_np() const;
+    std::vector<int> get_offset() const;
+};
+
+class ParticleSort {
+  public:
+    zpic_sort buf;
+    const int ntiles;
+    ParticleSort( uint2 const ntiles, uint32_t const max_part );
+    ~ParticleSort();
+    ParticleSort( const ParticleSort & ) = delete;
+    ParticleSort & operator=( const ParticleSort & ) = delete;
+};
+
+// --------------------------------------------------------- UDistribution
+namespace UDistribution {
+class Type {
+  public:
+    virtual Type * clone() const = 0;
+    // Fills the momenta of particles sorted by tile (host arrays), seed = species id.
+    virtual void set( int ntiles, const int * np, const int * offset, const int2 * ix, uint2 nx, float3 * u,
+                      unsigned int seed ) const = 0;
+    virtual ~Type() = default;
+};
+class None : public Type {
+  public:
+    None * clone() const override { return new None(); }
+    void set( int, const int *, const int *, const int2 *, uint2, float3 *, unsigned int ) const override;
+};
+class Cold : public Type {
+  public:
+    const float3 ufl;
+    Cold( float3 const ufl_ ) : ufl( ufl_ ) {}
+    Cold * clone() const override { return new Cold( ufl ); }
+    void set( int, const int *, const int *, const int2 *, uint2, float3 *, unsigned int ) const override;
+};
+class Thermal : public Type {
+  public:
+    const float3 uth, ufl;
+    Thermal( float3 const uth_, float3 const ufl_ ) : uth( uth_ ), ufl( ufl_ ) {}
+    Thermal * clone() const override { return new Thermal( uth, ufl ); }
+    void set( int, const int *, const int *, const int2 *, uint2, float3 *, unsigned int ) const override;
+};
+class ThermalCorr : public Type {
+  public:
+    const float3 uth, ufl;
+    const int npmin;
+    ThermalCorr( float3 const uth_, float3 const ufl_, int const npmin_ = 2 ) : uth( uth_ ), ufl( ufl_ ), npmin( npmin_ ) {
+        if ( npmin <= 1 ) throw std::runtime_error( "(*error*) invalid npmin parameter, must be > 1" );
+    }
+    ThermalCorr * clone() const override { return new ThermalCorr( uth, ufl, npmin ); }
+    void set( int, const int *, const int *, const int2 *, uint2, float3 *, unsigned int ) const override;
+};
+}  // namespace UDistribution
+
+// --------------------------------------------------------------- Density
+namespace Density {
+class Profile {
+  public:
+    const float n0;
+    Profile( float const n0_ ) : n0( n0_ < 0 ? -n0_ : n0_ ) {}
+    virtual Profile * clone() const = 0;
+    // particles injected per tile for the whole box
+    virtual void np_inject( uint2 ntiles, uint2 nx, uint2 ppc, int * np ) const = 0;
+    // positions in mk2 order (density.cpp:91-186) at [offset[t], offset[t]+np[t])
+    virtual void inject( uint2 ntiles, uint2 nx, uint2 ppc, const int * offset, int2 * ix, float2 * x ) const = 0;
+    virtual ~Profile() = default;
+};
+class None : public Profile {
+  public:
+    None() : Profile( 0.f ) {}
+    None * clone() const override { return new None(); }
+    void np_inject( uint2 ntiles, uint2, uint2, int * np ) const override {
+        for ( unsigned t = 0; t < ntiles.x * ntiles.y; t++ ) np[t] = 0;
+    }
+    void inject( uint2, uint2, uint2, const int *, int2 *, float2 * ) const override {}
+};
+class Uniform : public Profile {
+  public:
+    Uniform( float const n0_ ) : Profile( n0_ ) {}
+    Uniform * clone() const override { return new Uniform( n0 ); }
+    void np_inject( uint2 ntiles, uint2 nx, uint2 ppc, int * np ) const override;
+    void inject( uint2 ntiles, uint2 nx, uint2 ppc, const int * offset, int2 * ix, float2 * x ) const override;
+};
+}  // namespace Density
+
+namespace species {
+enum pusher { boris, euler };
+namespace bc {
+enum type { open = 0, periodic, reflecting };
+}
+typedef bnd<bc::type> bc_type;
+}  // namespace species
+
+namespace emf {
+enum field { e = 0, b };
+namespace bc {
+enum type { none = 0, periodic, pec, pmc };
+}
+typedef bnd<bc::type> bc_type;
+}  // namespace emf
+
+namespace current {
+namespace bc {
+enum type { none = 0, periodic, reflecting };
+}
+typedef bnd<bc::type> bc_type;
+}  // namespace current
+
+// ------------------------------------------------------------------ Current
+class Current {
+    float2 dx;
+    double dt;
+    current::bc_type bc;
+    int iter;
+
+  public:
+    vec3grid<float3> * J;
+    const float2 box;
+    Filter::Digital * filter;
+
+    Current( uint2 const ntiles, uint2 const nx, float2 const box, double const dt );
+    ~Current();
+    int get_iter() const { return iter; }
+    double get_dt() const { return dt; }
+    current::bc_type get_bc() const { return bc; }
+    void set_bc( current::bc_type new_bc );
+    void set_filter( Filter::Digital const & new_filter ) {
+        delete filter;
+        filter = new_filter.clone();
+    }
+    void advance();
+    void zero();
+};
+
+// ---------------------------------------------------------------------- EMF
+class EMF {
+    emf::bc_type bc;
+    const float2 dx;
+    const double dt;
+    int iter;
+    double * d_energy;   // [6*ntiles + 6] device scratch
+
+  public:
+    vec3grid<float3> * E;
+    vec3grid<float3> * B;
+    const float2 box;
+
+    EMF( uint2 const ntiles, uint2 const nx, float2 const box, double const dt );
+    ~EMF();
+    int get_iter() const { return iter; }
+    double get_dt() const { return dt; }
+    emf::bc_type get_bc() const { return bc; }
+    void set_bc( emf::bc_type new_bc );
+    void advance();
+    void advance( Current & current );
+    void get_energy( double3 & ene_E, double3 & ene_B );
+};
+
+// ------------------------------------------------------------------ Species
+class Species {
+  protected:
+    int id;
+    uint2 ppc;
+    float q;
+    float2 dx;
+    float2 box;
+    float dt;
+    int iter;
+    Particles * particles;
+    Particles * tmp;
+    ParticleSort * sort;
+    Density::Profile * density;
+
+  private:
+    species::bc_type bc;
+    UDistribution::Type * udist;
+    double * d_tile_energy;  // [ntiles] per-tile push energy sums
+    double * d_energy_dev;   // [1]
+    uint64_t d_nmove;
+    uint64_t np_count;       // particles held (constant for periodic boundaries)
+    bool deterministic;
+
+  public:
+    const std::string name;
+    const float m_q;
+    species::pusher push_type;
+
+    Species( std::string const name, float const m_q, uint2 const ppc );
+    virtual ~Species();
+    virtual void initialize( float2 const box, uint2 const ntiles, uint2 const nx, double const dt, int const id );
+
+    virtual void set_density( Density::Profile const & d ) { delete density; density = d.clone(); }
+    Density::Profile & get_density() { return *density; }
+    virtual void set_udist( UDistribution::Type const & u ) { delete udist; udist = u.clone(); }
+    UDistribution::Type & get_udist() { return *udist; }
+    void set_bc( species::bc_type new_bc );
+    species::bc_type get_bc() const { return bc; }
+
+    void push( vec3grid<float3> * const E, vec3grid<float3> * const B );
+    void move( vec3grid<float3> * const J );
+    void tile_sort();
+    virtual void advance( EMF const & emf, Current & current );
+    // advance() recording `after_push` (may be null) between the particle pass and the sort
+    void advance_timed( EMF const & emf, Current & current, hipEvent_t after_push );
+
+    // Deterministic (reference single-thread) deposit / energy order
+    void set_deterministic( bool on ) { deterministic = on; }
+
+    void deposit_charge( grid<float> & charge ) const;
+    double get_energy() const;
+    uint64_t get_nmove() const { return d_nmove; }
+    int get_iter() const { return iter; }
+    float get_dt() const { return dt; }
+    uint2 get_ntiles() const { return particles->ntiles; }
+    uint2 get_nx() const { return particles->nx; }
+    float2 get_box() const { return box; }
+    uint64_t np_total() const { return particles->np_total(); }
+    float get_q() const { return q; }
+    float2 get_dx() const { return dx; }
+    uint2 get_ppc() const { return ppc; }
+
+    // Host copies in tile order (compact)
+    void get_particles( int * np, int * offset, int * ix, float * x, float * u ) const;
+    void set_particles( const int * np, const int * ix, const float * x, const float * u );
+    // Species::gather (species.h:515): x, y in simulation units, or ux, uy, uz
+    void gather( part::quant quant, float * host_out ) const;
+
+    // Device access for callers that drive the stages themselves
+    Particles * get_particles_buffer() { return particles; }
+};
+
+// --------------------------------------------------------------- Simulation
+class Simulation {
+    unsigned int iter;
+    bool timing;
+    std::vector<hipEvent_t> ev;
+    double t_acc[5];
+    long long t_steps;
+
+  public:
+    const uint2 ntiles;
+    const uint2 nx;
+    const float2 box;
+    const double dt;
+    EMF emf;
+    Current current;
+    std::vector<Species *> species;
+
+    Simulation( uint2 const ntiles, uint2 const nx, float2 const box, double dt );
+    ~Simulation();
+
+    void add_species( Species & s ) {
+        species.push_back( &s );
+        s.initialize( box, ntiles, nx, dt, species.size() );
+    }
+    Species * get_species( std::string name ) {
+        for ( auto * s : species ) if ( s->name == name ) return s;
+        return nullptr;
+    }
+    void advance();
+    unsigned int get_iter() const { return iter; }
+    double get_t() const { return iter * double( dt ); }
+    void energy_info();
+    uint64_t get_nmove() const {
+        uint64_t n = 0;
+        for ( auto * s : species ) n += s->get_nmove();
+        return n;
+    }
+    void set_deterministic( bool on ) { for ( auto * s : species ) s->set_deterministic( on ); }
+    // HIP-event timing of the stages (push+move, sort, current, yee, step)
+    void set_timing( bool on );
+    void get_timing( double out[5] );
+};

@@ -1,0 +1,259 @@
+"""Python front-end of the MI355X em2d library (libzpic_amd.so) over ctypes.
+
+Mirrors the reference's Python layer (openmp/python/em2d/em2d.pyx:
+Simulation, Species, EMF, Current) on top of the object entry points of
+include/zpic_amd.h.  There is no CPU fallback: loading fails loudly when the
+library is missing, and creating a Simulation fails when no GPU is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libzpic_amd.so")
+
+UDIST = {"none": 0, "cold": 1, "thermal": 2, "thermal_corr": 3}
+FILTER = {"none": 0, "binomial": 1, "compensated": 2}
+STAGES = {"push": 0, "move": 1, "sort": 2, "species": 3, "zero_j": 4, "current": 5, "yee": 6,
+          "add_gc": 7, "copy_gc": 8, "filter": 9}
+
+
+class ZpicError(RuntimeError):
+    pass
+
+
+class Geom(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("ntiles_x", "ntiles_y", "nx", "ny", "gc_lower", "gc_upper", "ext_x", "ext_y", "tile_vol", "offset")]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libzpic_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ZpicError("libzpic_amd.so not built: run `python zpic-parallel_amd/build.py` "
+                        "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, u, f, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_float, ctypes.c_double
+    u64 = ctypes.c_uint64
+    sig = {
+        "zpic_last_error": (ctypes.c_char_p, []),
+        "zpic_geom_init": (i, [vp, i, i, i, i, i, i]),
+        "zpic_simulation_new": (vp, [u, u, u, u, f, f, d]),
+        "zpic_simulation_free": (None, [vp]),
+        "zpic_species_new": (vp, [ctypes.c_char_p, f, u, u]),
+        "zpic_species_free": (None, [vp]),
+        "zpic_species_set_udist": (i, [vp, i, vp, vp, i]),
+        "zpic_species_set_density_uniform": (i, [vp, f]),
+        "zpic_simulation_add_species": (i, [vp, vp]),
+        "zpic_simulation_advance": (i, [vp, i]),
+        "zpic_simulation_stage": (i, [vp, i, i]),
+        "zpic_current_set_filter": (i, [vp, i, i, i]),
+        "zpic_simulation_set_deterministic": (i, [vp, i]),
+        "zpic_simulation_get_iter": (u, [vp]),
+        "zpic_simulation_get_t": (d, [vp]),
+        "zpic_simulation_get_nmove": (u64, [vp]),
+        "zpic_species_get_energy": (d, [vp]),
+        "zpic_emf_get_energy": (i, [vp, vp, vp]),
+        "zpic_species_get_nmove": (u64, [vp]),
+        "zpic_species_np_total": (u64, [vp]),
+        "zpic_species_get_particles": (i, [vp, vp, vp, vp, vp, vp]),
+        "zpic_species_set_particles": (i, [vp, vp, vp, vp, vp]),
+        "zpic_field_get": (i, [vp, i, vp]),
+        "zpic_field_set": (i, [vp, i, vp]),
+        "zpic_field_geom": (i, [vp, vp]),
+        "zpic_species_charge": (ctypes.c_longlong, [vp, vp]),
+        "zpic_species_gather": (i, [vp, i, vp]),
+        "zpic_simulation_set_timing": (i, [vp, i]),
+        "zpic_simulation_get_timing": (i, [vp, vp]),
+        "zpic_simulation_sync": (i, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _chk(rc):
+    if rc != 0:
+        raise ZpicError(lib().zpic_last_error().decode())
+    return rc
+
+
+class Species:
+    """Species(name, m_q, ppc) with the reference's udist / density setters."""
+
+    def __init__(self, name, m_q, ppc, udist="none", uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2, n0=None):
+        L = lib()
+        h = L.zpic_species_new(name.encode(), m_q, ppc[0], ppc[1])
+        if not h:
+            raise ZpicError(L.zpic_last_error().decode())
+        self.h = h
+        self.name = name
+        self.sim = None
+        if n0 is not None:
+            _chk(L.zpic_species_set_density_uniform(h, n0))
+        self.set_udist(udist, uth, ufl, npmin)
+
+    def set_udist(self, kind, uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2):
+        uth = np.asarray(uth, np.float32)
+        ufl = np.asarray(ufl, np.float32)
+        _chk(lib().zpic_species_set_udist(self.h, UDIST[kind], _p(uth), _p(ufl), npmin))
+
+    def __del__(self):
+        if getattr(self, "h", None) and getattr(self, "sim", None) is None:
+            lib().zpic_species_free(self.h)
+            self.h = None
+
+    def get_energy(self):
+        return float(lib().zpic_species_get_energy(self.h))
+
+    def get_nmove(self):
+        return int(lib().zpic_species_get_nmove(self.h))
+
+    def np_total(self):
+        return int(lib().zpic_species_np_total(self.h))
+
+    def particles(self):
+        """Host copy in tile order: np, offset per tile, ix [n,2], x [n,2], u [n,3]."""
+        n = self.np_total()
+        nt = self.sim.ntiles
+        out = {"np": np.zeros(nt, np.int32), "offset": np.zeros(nt, np.int32),
+               "ix": np.zeros((n, 2), np.int32), "x": np.zeros((n, 2), np.float32),
+               "u": np.zeros((n, 3), np.float32)}
+        _chk(lib().zpic_species_get_particles(self.h, _p(out["np"]), _p(out["offset"]), _p(out["ix"]),
+                                              _p(out["x"]), _p(out["u"])))
+        return out
+
+    def set_particles(self, np_, ix, x, u):
+        np_ = np.ascontiguousarray(np_, np.int32)
+        ix = np.ascontiguousarray(ix, np.int32)
+        x = np.ascontiguousarray(x, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        _chk(lib().zpic_species_set_particles(self.h, _p(np_), _p(ix), _p(x), _p(u)))
+
+    def charge(self):
+        """Charge density on the (gc 0,1) tile-major grid after the guard add."""
+        n = lib().zpic_species_charge(self.h, None)
+        if n < 0:
+            raise ZpicError(lib().zpic_last_error().decode())
+        out = np.zeros(n, np.float32)
+        lib().zpic_species_charge(self.h, _p(out))
+        return out
+
+    def gather(self, quant):
+        q = {"x": 0, "y": 1, "ux": 2, "uy": 3, "uz": 4}[quant]
+        out = np.zeros(self.np_total(), np.float32)
+        _chk(lib().zpic_species_gather(self.h, q, _p(out)))
+        return out
+
+
+class Simulation:
+    """Simulation(ntiles, nx, box, dt) (simulation.h:39-43) on the GPU."""
+
+    def __init__(self, ntiles, nx, box, dt, filter=None, deterministic=False):
+        L = lib()
+        h = L.zpic_simulation_new(ntiles[0], ntiles[1], nx[0], nx[1], float(box[0]), float(box[1]), float(dt))
+        if not h:
+            raise ZpicError(L.zpic_last_error().decode())
+        self.h = h
+        self.species = []
+        g = Geom()
+        _chk(L.zpic_field_geom(h, ctypes.byref(g)))
+        self.geom = g
+        self.ntiles = g.ntiles_x * g.ntiles_y
+        if filter:
+            self.set_filter(*filter)
+        if deterministic:
+            self.set_deterministic(True)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zpic_simulation_free(self.h)
+            self.h = None
+            for s in self.species:
+                lib().zpic_species_free(s.h)
+                s.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_species(self, sp):
+        _chk(lib().zpic_simulation_add_species(self.h, sp.h))
+        sp.sim = self
+        self.species.append(sp)
+
+    def set_filter(self, kind, direction=0, order=1):
+        _chk(lib().zpic_current_set_filter(self.h, FILTER[kind] if isinstance(kind, str) else kind,
+                                           direction, order))
+
+    def set_deterministic(self, on=True):
+        _chk(lib().zpic_simulation_set_deterministic(self.h, 1 if on else 0))
+
+    def advance(self, n=1):
+        _chk(lib().zpic_simulation_advance(self.h, n))
+
+    def stage(self, what, k=0):
+        _chk(lib().zpic_simulation_stage(self.h, STAGES[what], k))
+
+    def sync(self):
+        _chk(lib().zpic_simulation_sync(self.h))
+
+    def get_iter(self):
+        return int(lib().zpic_simulation_get_iter(self.h))
+
+    def get_nmove(self):
+        return int(lib().zpic_simulation_get_nmove(self.h))
+
+    def field(self, which):
+        g = self.geom
+        out = np.zeros((self.ntiles * g.tile_vol, 3), np.float32)
+        _chk(lib().zpic_field_get(self.h, "EBJ".index(which), _p(out)))
+        return out
+
+    def set_field(self, which, data):
+        data = np.ascontiguousarray(data, np.float32)
+        assert data.size == self.ntiles * self.geom.tile_vol * 3
+        _chk(lib().zpic_field_set(self.h, "EBJ".index(which), _p(data)))
+
+    def emf_energy(self):
+        e = np.zeros(3)
+        b = np.zeros(3)
+        _chk(lib().zpic_emf_get_energy(self.h, _p(e), _p(b)))
+        return e, b
+
+    def energies(self):
+        """Kinetic energy per species, then E.x E.y E.z B.x B.y B.z."""
+        e, b = self.emf_energy()
+        return np.array([s.get_energy() for s in self.species] + list(e) + list(b))
+
+    def set_timing(self, on=True):
+        _chk(lib().zpic_simulation_set_timing(self.h, 1 if on else 0))
+
+    def get_timing(self):
+        out = np.zeros(5)
+        _chk(lib().zpic_simulation_get_timing(self.h, _p(out)))
+        return dict(zip(("push_move", "sort", "current", "yee", "step"), out.tolist()))
+
+
+def weibel(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), box=None, dt=0.07, uth=(0.1, 0.1, 0.1),
+           ufl=(0.0, 0.0, 0.6), filter=None, deterministic=False):
+    """The reference Weibel setup (openmp/em2d/main.cpp:179-259) on the GPU."""
+    if box is None:
+        box = (np.float32(ntiles[0] * nx[0] / 10.0), np.float32(ntiles[1] * nx[1] / 10.0))
+    sim = Simulation(ntiles, nx, box, float(np.float32(dt)), filter=filter, deterministic=deterministic)
+    sim.add_species(Species("electrons", -1.0, ppc, "thermal_corr", uth, ufl))
+    sim.add_species(Species("positrons", +1.0, ppc, "thermal_corr", uth, tuple(-v for v in ufl)))
+    return sim
