@@ -143,7 +143,9 @@ def test_fast_first_step(zp):
             assert same_bits(p[name], q[name]), f"species {k} {name}"
     for f in "JEB":
         err = relnorm(sim.field(f), osim.field(f))
-        assert err <= 1e-6, f"{f}: {err}"
+        say(f"  {f}: relative norm error {err:.2e}")
+        # the bar after one step: relative 1e-5 (J from atomics differs in summation order)
+        assert err <= 1e-5, f"{f}: {err}"
     eg, eo = sim.energies(), osim.energies()
     np.testing.assert_allclose(eg, eo, rtol=1e-5, atol=1e-12)
     assert sim.get_nmove() == osim.nmove(0) + osim.nmove(1)

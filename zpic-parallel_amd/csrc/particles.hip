@@ -177,34 +177,71 @@ __device__ inline int split_path( int2 c0, float2 x0, float2 x1, float2 d, float
     return 3;
 }
 
-template <bool ATOMIC>
-__device__ inline void lds_add( float * p, float v ) {
-    if constexpr ( ATOMIC ) {
-        __hip_atomic_fetch_add( p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
-    } else {
-        *p += v;
-    }
-}
-
-// ---- dep_current_seg (species.cpp:267-310) into LDS planes (J at cell (0,0))
-template <bool ATOMIC>
-__device__ inline void deposit( const Seg & s, float qnx, float qny, float * Jx, float * Jy, float * Jz, int ys ) {
+// ---- dep_current_seg (species.cpp:267-310): the 8 current contributions of
+// one segment, in the reference's order: Jx(k), Jy(k), Jz(k), Jy(k+1),
+// Jz(k+1), Jx(k+ys), Jz(k+ys), Jz(k+ys+1) with k the segment's cell.
+__device__ inline void segment_current( const Seg & s, float qnx, float qny, float v[8] ) {
     const float A0x = 0.5f - s.a.x, A1x = 0.5f + s.a.x, B0x = 0.5f - s.b.x, B1x = 0.5f + s.b.x;
     const float A0y = 0.5f - s.a.y, A1y = 0.5f + s.a.y, B0y = 0.5f - s.b.y, B1y = 0.5f + s.b.y;
     const float jx = qnx * ( s.b.x - s.a.x );
     const float jy = qny * ( s.b.y - s.a.y );
     const float wy0 = 0.5f * ( A0y + B0y ), wy1 = 0.5f * ( A1y + B1y );
     const float wx0 = 0.5f * ( A0x + B0x ), wx1 = 0.5f * ( A1x + B1x );
-    const int k = s.c.x + ys * s.c.y;
     const float q = s.qvz;
-    lds_add<ATOMIC>( &Jx[k], jx * wy0 );
-    lds_add<ATOMIC>( &Jy[k], jy * wx0 );
-    lds_add<ATOMIC>( &Jz[k], q * ( A0x * A0y + B0x * B0y + ( A0x * B0y - B0x * A0y ) / 2.0f ) );
-    lds_add<ATOMIC>( &Jy[k + 1], jy * wx1 );
-    lds_add<ATOMIC>( &Jz[k + 1], q * ( A1x * A0y + B1x * B0y + ( A1x * B0y - B1x * A0y ) / 2.0f ) );
-    lds_add<ATOMIC>( &Jx[k + ys], jx * wy1 );
-    lds_add<ATOMIC>( &Jz[k + ys], q * ( A0x * A1y + B0x * B1y + ( A0x * B1y - B0x * A1y ) / 2.0f ) );
-    lds_add<ATOMIC>( &Jz[k + ys + 1], q * ( A1x * A1y + B1x * B1y + ( A1x * B1y - B1x * A1y ) / 2.0f ) );
+    v[0] = jx * wy0;
+    v[1] = jy * wx0;
+    v[2] = q * ( A0x * A0y + B0x * B0y + ( A0x * B0y - B0x * A0y ) / 2.0f );
+    v[3] = jy * wx1;
+    v[4] = q * ( A1x * A0y + B1x * B0y + ( A1x * B0y - B1x * A0y ) / 2.0f );
+    v[5] = jx * wy1;
+    v[6] = q * ( A0x * A1y + B0x * B1y + ( A0x * B1y - B0x * A1y ) / 2.0f );
+    v[7] = q * ( A1x * A1y + B1x * B1y + ( A1x * B1y - B1x * A1y ) / 2.0f );
+}
+
+// LDS (address space 3) element types: volatile accesses through generic
+// pointers would be lowered to flat instructions.
+typedef __attribute__( ( address_space( 3 ) ) ) float lds_f32;
+typedef __attribute__( ( address_space( 3 ) ) ) unsigned char lds_u8;
+
+// Read-add-write of the 8 contributions.  The accesses are volatile so that
+// they are issued in program order: another lane's contribution to the same
+// element through a different slot (Jx(k+ys) of one cell is Jx(k) of the cell
+// above) then always sees the previous write (LDS executes one wave's
+// instructions in order).
+__device__ inline void add_current( volatile lds_f32 * Jx, volatile lds_f32 * Jy, volatile lds_f32 * Jz, int k,
+                                    int ys, const float v[8] ) {
+    Jx[k] = Jx[k] + v[0];
+    Jy[k] = Jy[k] + v[1];
+    Jz[k] = Jz[k] + v[2];
+    Jy[k + 1] = Jy[k + 1] + v[3];
+    Jz[k + 1] = Jz[k + 1] + v[4];
+    Jx[k + ys] = Jx[k + ys] + v[5];
+    Jz[k + ys] = Jz[k + ys] + v[6];
+    Jz[k + ys + 1] = Jz[k + ys + 1] + v[7];
+}
+
+// Fast-mode deposit of one segment per pending lane into the wave's private J
+// planes, without atomics: each round, every pending lane writes its lane id
+// into the tag byte of its cell and reads it back; the lane whose id survived
+// owns the cell for this round, so the owners of a round hold distinct cells
+// and their read-add-writes cannot collide.  Rounds repeat while lanes are
+// pending (as many rounds as the most repeated cell in the wave).
+__device__ inline void deposit_elected( bool pend, const Seg & s, float qnx, float qny, volatile lds_f32 * Jx,
+                                        volatile lds_f32 * Jy, volatile lds_f32 * Jz, volatile lds_u8 * tag, int ys,
+                                        unsigned lane ) {
+    float v[8];
+    int k = 0;
+    if ( pend ) {
+        segment_current( s, qnx, qny, v );
+        k = s.c.x + ys * s.c.y;
+    }
+    while ( __ballot( pend ) ) {
+        if ( pend ) tag[k] = (unsigned char) lane;
+        if ( pend && tag[k] == (unsigned char) lane ) {
+            add_current( Jx, Jy, Jz, k, ys, v );
+            pend = false;
+        }
+    }
 }
 
 struct MoveArgs {
@@ -220,6 +257,14 @@ struct MoveArgs {
 // tile's particles, so that the leaver indices it appends to the tile's list
 // (idx[offset + w*chunk ...]) are in increasing order; seg[16 t + w] keeps the
 // count of each wave's segment, seg[16 t + 15] the chunk.
+//
+// LDS: E and B planes (6*vol floats, PUSH), then W private J plane triples
+// (3*vol floats each, MOVE), then W tag byte arrays (vol bytes each, MOVE).
+__host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move ) {
+    return sizeof( float ) * (size_t) vol * ( ( push ? 6 : 0 ) + ( move ? 3 * W : 0 ) ) +
+           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) : 0 );
+}
+
 template <bool PUSH, bool MOVE, bool DET>
 __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, const float3 * __restrict__ E,
                                                           const float3 * __restrict__ B, float3 * __restrict__ J,
@@ -231,6 +276,9 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, 
     const int nthr = blockDim.x;
     const int vol = g.vol;
     const int ys = g.ext_x;
+    const int W = nthr / WAVE;
+    const int w = tid / WAVE;
+    const int lane = tid % WAVE;
 
     float * Ex = smem;
     float * Ey = Ex + vol;
@@ -238,9 +286,11 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, 
     float * Bx = Ez + vol;
     float * By = Bx + vol;
     float * Bz = By + vol;
-    float * Jx = PUSH ? Bz + vol : smem;
+    float * Jall = PUSH ? Bz + vol : smem;        // W triples of planes
+    float * Jx = Jall + 3 * vol * w;              // this wave's planes
     float * Jy = Jx + vol;
     float * Jz = Jy + vol;
+    unsigned char * tags = reinterpret_cast<unsigned char *>( Jall + 3 * vol * W ) + (size_t) vol * w;
     __shared__ int s_cnt[9];
     __shared__ double s_en[FAST_BLOCK / WAVE];
     __shared__ int s_nl[FAST_BLOCK / WAVE];
@@ -255,23 +305,28 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, 
         }
     }
     if constexpr ( MOVE ) {
-        for ( int k = tid; k < 3 * vol; k += nthr ) Jx[k] = 0.0f;
+        for ( int k = tid; k < 3 * vol * W; k += nthr ) Jall[k] = 0.0f;
         if ( tid < 9 ) s_cnt[tid] = 0;
     }
     __syncthreads();
 
     const int off = P.off[t];
     const int np = P.np[t];
-    const int W = nthr / WAVE;
-    const int w = tid / WAVE;
-    const int lane = tid % WAVE;
     const int chunk = ( ( np + W - 1 ) / W + WAVE - 1 ) & ~( WAVE - 1 );
     const int beg = w * chunk;
     const int end = min( np, beg + chunk );
 
+    // tile base pointers (uniform) + 32-bit particle index
+    int2 * __restrict__ pix = P.ix + off;
+    float2 * __restrict__ px = P.x + off;
+    float3 * __restrict__ pu = P.u + off;
+
     const float * e0x = Ex + g.off; const float * e0y = Ey + g.off; const float * e0z = Ez + g.off;
     const float * b0x = Bx + g.off; const float * b0y = By + g.off; const float * b0z = Bz + g.off;
-    float * j0x = Jx + g.off; float * j0y = Jy + g.off; float * j0z = Jz + g.off;
+    volatile lds_f32 * j0x = (volatile lds_f32 *) ( Jx + g.off );
+    volatile lds_f32 * j0y = (volatile lds_f32 *) ( Jy + g.off );
+    volatile lds_f32 * j0z = (volatile lds_f32 *) ( Jz + g.off );
+    volatile lds_u8 * tag0 = (volatile lds_u8 *) ( tags + g.off );
 
     double en = 0.0;
     int nl = 0;
@@ -283,9 +338,9 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, 
         float2 x = make_float2( 0.f, 0.f );
         float3 u = make_float3( 0.f, 0.f, 0.f );
         if ( act ) {
-            c = P.ix[off + i];
-            x = P.x[off + i];
-            u = P.u[off + i];
+            c = pix[(unsigned) i];
+            x = px[(unsigned) i];
+            u = pu[(unsigned) i];
         }
         if constexpr ( PUSH ) {
             float ene = 0.0f;
@@ -293,7 +348,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, 
                 float3 e, b;
                 interpolate( e0x, e0y, e0z, b0x, b0y, b0z, ys, c, x, e, b );
                 u = boris( alpha, e, b, u, ene );
-                P.u[off + i] = u;
+                pu[(unsigned) i] = u;
             }
             if constexpr ( DET ) {
                 // energy += term, in particle order
@@ -316,28 +371,38 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, 
                 float2 x1 = make_float2( x.x + d.x, x.y + d.y );
                 int2 di;
                 ns = split_path( c, x, x1, d, qvz, di, sg );
-                if constexpr ( !DET ) {
-                    deposit<true>( sg[0], m.qnx, m.qny, j0x, j0y, j0z, ys );
-                    if ( ns > 1 ) deposit<true>( sg[1], m.qnx, m.qny, j0x, j0y, j0z, ys );
-                    if ( ns > 2 ) deposit<true>( sg[2], m.qnx, m.qny, j0x, j0y, j0z, ys );
-                }
                 x1.x -= di.x;
                 x1.y -= di.y;
                 const int2 c1 = make_int2( c.x + di.x, c.y + di.y );
-                P.x[off + i] = x1;
-                P.ix[off + i] = c1;
+                px[(unsigned) i] = x1;
+                pix[(unsigned) i] = c1;
                 const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
                 const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
                 dir = ( yc + 1 ) * 3 + ( xc + 1 );
             }
             if constexpr ( DET ) {
+                // particle order: lane l deposits all its segments before lane l+1
                 for ( int l = 0; l < WAVE; l++ ) {
                     if ( lane == l && ns > 0 ) {
-                        deposit<false>( sg[0], m.qnx, m.qny, j0x, j0y, j0z, ys );
-                        if ( ns > 1 ) deposit<false>( sg[1], m.qnx, m.qny, j0x, j0y, j0z, ys );
-                        if ( ns > 2 ) deposit<false>( sg[2], m.qnx, m.qny, j0x, j0y, j0z, ys );
+                        float v[8];
+                        segment_current( sg[0], m.qnx, m.qny, v );
+                        add_current( j0x, j0y, j0z, sg[0].c.x + ys * sg[0].c.y, ys, v );
+                        if ( ns > 1 ) {
+                            segment_current( sg[1], m.qnx, m.qny, v );
+                            add_current( j0x, j0y, j0z, sg[1].c.x + ys * sg[1].c.y, ys, v );
+                        }
+                        if ( ns > 2 ) {
+                            segment_current( sg[2], m.qnx, m.qny, v );
+                            add_current( j0x, j0y, j0z, sg[2].c.x + ys * sg[2].c.y, ys, v );
+                        }
                     }
                 }
+            } else {
+                deposit_elected( ns > 0, sg[0], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
+                if ( __ballot( ns > 1 ) )
+                    deposit_elected( ns > 1, sg[1], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
+                if ( __ballot( ns > 2 ) )
+                    deposit_elected( ns > 2, sg[2], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
             }
             const bool leave = act && dir != 4;
             const unsigned long long mask = __ballot( leave );
@@ -362,11 +427,14 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, 
     __syncthreads();
 
     if constexpr ( MOVE ) {
-        // flush: J(tile) += J(LDS), guard cells included (species.cpp:1488-1492)
+        // flush: J(tile) += sum of the waves' J (LDS), guard cells included
+        // (species.cpp:1488-1492)
         float * Jt = reinterpret_cast<float *>( J + (size_t) t * vol );
         for ( int k = tid; k < 3 * g.ext_x * g.ext_y; k += nthr ) {
             const int el = k / 3, c = k - 3 * el;
-            Jt[k] += Jx[c * vol + el];
+            float s = Jall[c * vol + el];
+            for ( int v = 1; v < W; v++ ) s += Jall[3 * vol * v + c * vol + el];
+            Jt[k] += s;
         }
         if ( tid == 0 ) {
             int tot = 0;
@@ -413,9 +481,22 @@ __device__ inline int source_of( int t, int d, const SortGeo & q ) {
     return wrap_tile( t % q.ntx - ( d % 3 - 1 ), t / q.ntx - ( d / 3 - 1 ), q.ntx, q.nty, q.px, q.py );
 }
 
-// New particle count per tile and its exclusive scan (bnd_check's new_np and
-// update_tile_info, particles.cpp:212-357).  One workgroup; each thread owns a
-// contiguous run of tiles.
+// New particle count per tile (bnd_check's new_np, particles.cpp:212-303):
+// the tile gathers the direction counts its 9 source tiles send to it.
+__global__ void k_sort_newnp( SortView S, SortGeo q, int ntiles ) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( t >= ntiles ) return;
+    int n = 0;
+    for ( int d = 0; d < 9; d++ ) {
+        const int s = source_of( t, d, q );
+        if ( s >= 0 ) n += S.npt[9 * s + d];
+    }
+    S.new_np[t] = n;
+}
+
+// Exclusive scan of the new counts into the new tile offsets
+// (update_tile_info, particles.cpp:314-357).  One workgroup; each thread owns
+// a contiguous run of tiles.
 constexpr int SCAN_BLOCK = 1024;
 __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortView S, int * __restrict__ new_off, SortGeo q,
                                                               int ntiles, unsigned max_part ) {
@@ -424,15 +505,7 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortView S, int * 
     const int per = ( ntiles + SCAN_BLOCK - 1 ) / SCAN_BLOCK;
     const int t0 = min( ntiles, tid * per ), t1 = min( ntiles, t0 + per );
     int local = 0;
-    for ( int t = t0; t < t1; t++ ) {
-        int n = 0;
-        for ( int d = 0; d < 9; d++ ) {
-            const int s = source_of( t, d, q );
-            if ( s >= 0 ) n += S.npt[9 * s + d];
-        }
-        S.new_np[t] = n;
-        local += n;
-    }
+    for ( int t = t0; t < t1; t++ ) local += S.new_np[t];
     s_sum[tid] = local;
     __syncthreads();
     // Hillis-Steele inclusive scan of the per-thread sums
@@ -516,13 +589,47 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( PartView P, PartView
     const int o = s_hdr[0], np = s_hdr[1], n0 = s_hdr[2], chunk = s_hdr[3];
     const int nl = np - n0;
 
-    if ( tid < WAVE ) {
-        const int lane = tid;
-        // leavers to staging, rank within each direction = order in the list
-        int run[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
-        for ( int jb = 0; jb < nl; jb += WAVE ) {
+    // The leaver list and the tail [n0, np) (both nl long) are split into W
+    // contiguous quarters, one per wave.  Pass 1 counts, per quarter, the
+    // leavers of each direction and the stayers; the exclusive sums over the
+    // preceding quarters give each wave its starting ranks.
+    const int W = blockDim.x / WAVE, w = tid / WAVE, lane = tid % WAVE;
+    const int qlen = ( ( nl + W - 1 ) / W + WAVE - 1 ) & ~( WAVE - 1 );
+    const int j0 = min( nl, w * qlen ), j1 = min( nl, j0 + qlen );
+    __shared__ int s_dir[SORT_BLOCK / WAVE][9];
+    __shared__ int s_stay[SORT_BLOCK / WAVE];
+    {
+        int cnt[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+        int st = 0;
+        for ( int jb = j0; jb < j1; jb += WAVE ) {
             const int j = jb + lane;
-            const bool valid = j < nl;
+            const bool valid = j < j1;
+            const int d = valid ? ( leaver_at( j, s_pre, chunk, S.idx, o ) & 15 ) : 4;
+            for ( int dd = 0; dd < 9; dd++ ) cnt[dd] += __popcll( __ballot( d == dd && valid ) );
+            bool stay = false;
+            if ( valid ) {
+                const int2 c = P.ix[o + n0 + j];
+                stay = c.x >= 0 && c.x < q.nx && c.y >= 0 && c.y < q.ny;
+            }
+            st += __popcll( __ballot( stay ) );
+        }
+        if ( lane == 0 ) {
+            for ( int dd = 0; dd < 9; dd++ ) s_dir[w][dd] = cnt[dd];
+            s_stay[w] = st;
+        }
+    }
+    __syncthreads();
+
+    // Pass 2: leavers to staging, rank within each direction = order in the list
+    {
+        int run[9];
+        for ( int dd = 0; dd < 9; dd++ ) {
+            run[dd] = 0;
+            for ( int v = 0; v < w; v++ ) run[dd] += s_dir[v][dd];
+        }
+        for ( int jb = j0; jb < j1; jb += WAVE ) {
+            const int j = jb + lane;
+            const bool valid = j < j1;
             const int ent = valid ? leaver_at( j, s_pre, chunk, S.idx, o ) : 0;
             const int d = ent & 15;
             int rank = 0;
@@ -543,22 +650,28 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( PartView P, PartView
                 T.u[dst] = P.u[k];
             }
         }
-        // holes below n0 <- stayers above n0, both in increasing order
+    }
+    // every leaver is read before any hole (a leaver slot below n0) is overwritten
+    __syncthreads();
+
+    // Pass 3: holes below n0 <- stayers above n0, both in increasing order
+    {
         int r0 = 0;
-        for ( int pb = n0; pb < np; pb += WAVE ) {
-            const int p = pb + lane;
+        for ( int v = 0; v < w; v++ ) r0 += s_stay[v];
+        for ( int jb = j0; jb < j1; jb += WAVE ) {
+            const int j = jb + lane;
             bool stay = false;
-            if ( p < np ) {
-                const int2 c = P.ix[o + p];
+            if ( j < j1 ) {
+                const int2 c = P.ix[o + n0 + j];
                 stay = c.x >= 0 && c.x < q.nx && c.y >= 0 && c.y < q.ny;
             }
             const unsigned long long mk = __ballot( stay );
             if ( stay ) {
                 const int r = r0 + __popcll( mk & lanes_below() );
                 const int hole = o + ( leaver_at( r, s_pre, chunk, S.idx, o ) >> 4 );
-                P.ix[hole] = P.ix[o + p];
-                P.x[hole] = P.x[o + p];
-                P.u[hole] = P.u[o + p];
+                P.ix[hole] = P.ix[o + n0 + j];
+                P.x[hole] = P.x[o + n0 + j];
+                P.u[hole] = P.u[o + n0 + j];
             }
             r0 += __popcll( mk );
         }
@@ -638,10 +751,11 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_charge( PartView P, Geo g, flo
                 }
             }
         } else if ( act ) {
-            lds_add<true>( &c0[k], w00 );
-            lds_add<true>( &c0[k + 1], w10 );
-            lds_add<true>( &c0[k + ys], w01 );
-            lds_add<true>( &c0[k + ys + 1], w11 );
+            // diagnostic path: LDS float atomics are adequate here
+            __hip_atomic_fetch_add( &c0[k], w00, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+            __hip_atomic_fetch_add( &c0[k + 1], w10, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+            __hip_atomic_fetch_add( &c0[k + ys], w01, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
+            __hip_atomic_fetch_add( &c0[k + ys + 1], w11, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP );
         }
     }
     __syncthreads();
@@ -686,7 +800,7 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
     if ( !p || ( PUSH && ( !E || !B || !tile_energy ) ) || ( MOVE && ( !J || !s ) ) )
         return set_error( ZPIC_EINVAL, "null buffer" );
     const int nt = fld->ntiles_x * fld->ntiles_y;
-    const size_t lds = sizeof( float ) * fld->tile_vol * ( ( PUSH ? 6 : 0 ) + ( MOVE ? 3 : 0 ) );
+    const size_t lds = particles_lds_bytes( fld->tile_vol, det ? 1 : FAST_BLOCK / WAVE, PUSH, MOVE );
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     const Geo g = make_geo( fld );
     hipStream_t st = (hipStream_t) stream;
@@ -735,6 +849,8 @@ int zpic_tile_sort( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int
     hipStream_t st = (hipStream_t) stream;
     const PartView P = view( p ), T = view( tmp );
     const SortView S = view( s );
+    hipLaunchKernelGGL( k_sort_newnp, dim3( ( nt + 255 ) / 256 ), dim3( 256 ), 0, st, S, q, nt );
+    ZPIC_LAUNCH_CHECK( "k_sort_newnp" );
     hipLaunchKernelGGL( k_sort_count, dim3( 1 ), dim3( SCAN_BLOCK ), 0, st, S, T.off, q, nt, p->max_part );
     ZPIC_LAUNCH_CHECK( "k_sort_count" );
     hipLaunchKernelGGL( k_sort_out, dim3( nt ), dim3( SORT_BLOCK ), 0, st, P, T, S, q );
