@@ -158,6 +158,62 @@ int zpic_gather_component(const float *buf, const zpic_geom *g, int ncomp, int f
 /* Fills n floats with a value (grid::zero / set). */
 int zpic_fill(float *buf, size_t n, float value, void *stream);
 
+/* Threads per tile of the fast particle pass: a multiple of 64 in [64, 512]
+ * (default 512).  Library-wide; the deterministic mode always uses 64. */
+int zpic_set_threads_per_tile(int threads);
+int zpic_get_threads_per_tile(void);
+
+/* --------------------------------------------------- batched stages
+ * Many independent simulations advanced with one launch per stage over all
+ * their tiles (a parameter scan).  No reference counterpart: the reference
+ * runs one simulation per process; each instance of a batch computes what its
+ * own single-instance run computes (same kernels, same per-tile work). */
+
+/* One species of one instance. */
+typedef struct {
+    zpic_particles p, tmp;  /* particle and staging buffers              */
+    zpic_sort s;            /* sort scratch                              */
+    zpic_geom fld;          /* field geometry of the instance (gc {1,2}) */
+    const float *E, *B;     /* fields of the instance                    */
+    float *J;               /* current of the instance                   */
+    double *tile_energy;    /* [ntiles] scratch                          */
+    double *energy_out;     /* [1] device: sum of the push energies      */
+    float alpha, dt_dx, dt_dy, q, qnx, qny;
+    int periodic_x, periodic_y;
+} zpic_batch_species;
+
+typedef struct zpic_batch_particles zpic_batch_particles;
+
+/* Builds the device tables for advancing `n` species (at most one per
+ * instance: species of one instance share J and must be advanced one after
+ * the other, so use one plan per species slot). */
+int zpic_batch_particles_create(zpic_batch_particles **plan, const zpic_batch_species *items, int n,
+                                int threads_per_tile);
+/* Species::advance(emf, current) of every item: fused push + move + deposit,
+ * tile sort, energy sums (fast mode). */
+int zpic_batch_particles_advance(zpic_batch_particles *plan, void *stream);
+void zpic_batch_particles_free(zpic_batch_particles *plan);
+
+/* Fields of one instance. */
+typedef struct {
+    float *E, *B, *J;
+    zpic_geom g;                  /* gc {1,2}                                */
+    float dt_dx, dt_dy, dt;       /* as EMF::advance narrows them            */
+    int filter_kind, filter_dir, filter_order;   /* as zpic_current_advance  */
+    int periodic_x, periodic_y;
+} zpic_batch_fields_item;
+
+typedef struct zpic_batch_fields zpic_batch_fields;
+
+int zpic_batch_fields_create(zpic_batch_fields **plan, const zpic_batch_fields_item *items, int n);
+/* Current::zero of every instance. */
+int zpic_batch_zero_current(zpic_batch_fields *plan, void *stream);
+/* Current::advance of every instance (guard add, guard copy, filter). */
+int zpic_batch_current_advance(zpic_batch_fields *plan, void *stream);
+/* EMF::advance(current) of every instance (Yee + guard copy of E, B). */
+int zpic_batch_yee(zpic_batch_fields *plan, void *stream);
+void zpic_batch_fields_free(zpic_batch_fields *plan);
+
 /* --------------------------------------------- object entry points (2) */
 typedef struct zpic_simulation zpic_simulation;
 typedef struct zpic_species zpic_species;
@@ -235,6 +291,17 @@ int zpic_simulation_get_timing(zpic_simulation *sim, double out[5]);
 
 /* Device synchronisation of the simulation's stream. */
 int zpic_simulation_sync(zpic_simulation *sim);
+
+/* Batch of independent simulations advanced together (one launch per stage
+ * over all instances, optionally one hipGraph launch per step).  No reference
+ * counterpart.  Simulations are added by reference and must outlive the
+ * batch; each must hold the same number of species. */
+typedef struct zpic_batch zpic_batch;
+zpic_batch *zpic_batch_new(int threads_per_tile);
+void zpic_batch_free(zpic_batch *b);
+int zpic_batch_add(zpic_batch *b, zpic_simulation *sim);
+int zpic_batch_set_graph(zpic_batch *b, int on);
+int zpic_batch_advance(zpic_batch *b, int n);
 
 #ifdef __cplusplus
 }

@@ -283,3 +283,46 @@ int zpic_simulation_sync( zpic_simulation * s ) {
 }
 
 }  // extern "C"
+
+struct zpic_batch {
+    std::unique_ptr<Batch> b;
+};
+
+extern "C" {
+
+zpic_batch * zpic_batch_new( int threads_per_tile ) {
+    if ( threads_per_tile < 64 || threads_per_tile > 512 || threads_per_tile % 64 ) {
+        zpic::set_error( ZPIC_EINVAL, "threads per tile must be a multiple of 64 in [64, 512]" );
+        return nullptr;
+    }
+    auto * h = new zpic_batch();
+    h->b.reset( new Batch( threads_per_tile ) );
+    return h;
+}
+
+void zpic_batch_free( zpic_batch * b ) { delete b; }
+
+int zpic_batch_add( zpic_batch * b, zpic_simulation * s ) {
+    return guarded( [&] {
+        if ( !b || !s ) return zpic::set_error( ZPIC_EINVAL, "null handle" );
+        if ( s->deterministic ) return zpic::set_error( ZPIC_EINVAL, "batches run the fast mode only" );
+        b->b->add( *s->sim );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_batch_set_graph( zpic_batch * b, int on ) {
+    if ( !b ) return zpic::set_error( ZPIC_EINVAL, "null batch" );
+    b->b->set_graph( on != 0 );
+    return ZPIC_OK;
+}
+
+int zpic_batch_advance( zpic_batch * b, int n ) {
+    return guarded( [&] {
+        if ( !b ) return zpic::set_error( ZPIC_EINVAL, "null batch" );
+        for ( int i = 0; i < n; i++ ) b->b->advance();
+        return ZPIC_OK;
+    } );
+}
+
+}  // extern "C"

@@ -6,7 +6,9 @@
 // reference's.
 #include "common.h"
 
-#include <mutex>
+#include <algorithm>
+#include <memory>
+#include <vector>
 
 namespace zpic {
 
@@ -38,14 +40,51 @@ __device__ inline int nbr( int tx, int ty, int dx, int dy, const G & g, int px, 
     return y * g.ntx + x;
 }
 
+// Arguments of one grid operand; batched launches pass a table of them and a
+// map from block to (entry, tile), single launches pass nullptr for both.
+struct GcArgs {
+    float * buf;
+    G g;
+    int px, py;
+};
+
+struct K3Args {
+    float * buf;
+    G g;
+    int dir;
+    float a, b, c;
+};
+
+struct YeeArgs {
+    float * E;
+    float * B;
+    const float * J;
+    G g;
+    float dtdx, dtdy, dt;
+};
+
+template <class A>
+__device__ inline int select_instance( A & a, const A * tab, const int2 * map ) {
+    if ( map ) {
+        const int2 r = map[blockIdx.x];
+        a = tab[r.x];
+        return r.y;
+    }
+    return blockIdx.x;
+}
+
 constexpr int GC_BLOCK = 128;
 
 // add_from_gc, x pass (grid.h:367-401): lower neighbour's upper guards into
 // the first interior columns, then upper neighbour's lower guards into the
 // last interior columns.
 template <int NC>
-__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( float * __restrict__ buf, G g, int px, int py ) {
-    const int t = blockIdx.x, tx = t % g.ntx, ty = t / g.ntx;
+__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( GcArgs a, const GcArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    float * __restrict__ buf = a.buf;
+    const G g = a.g;
+    const int px = a.px, py = a.py;
+    const int tx = t % g.ntx, ty = t / g.ntx;
     float * loc = buf + (size_t) t * g.vol * NC;
     const int lo = nbr( tx, ty, -1, 0, g, px, py ), hi = nbr( tx, ty, +1, 0, g, px, py );
     const int E = g.ext_x;
@@ -68,8 +107,12 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( float * __restrict__ b
 
 // add_from_gc, y pass (grid.h:406-444): full rows of the y neighbours' guards.
 template <int NC>
-__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( float * __restrict__ buf, G g, int px, int py ) {
-    const int t = blockIdx.x, tx = t % g.ntx, ty = t / g.ntx;
+__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( GcArgs a, const GcArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    float * __restrict__ buf = a.buf;
+    const G g = a.g;
+    const int px = a.px, py = a.py;
+    const int tx = t % g.ntx, ty = t / g.ntx;
     float * loc = buf + (size_t) t * g.vol * NC;
     const int lo = nbr( tx, ty, 0, -1, g, px, py ), hi = nbr( tx, ty, 0, +1, g, px, py );
     const int E = g.ext_x;
@@ -92,8 +135,12 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( float * __restrict__ b
 
 // copy_to_gc_x (grid.h:253-297): x guard columns from the neighbours' edges.
 template <int NC>
-__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( float * __restrict__ buf, G g, int px, int py ) {
-    const int t = blockIdx.x, tx = t % g.ntx, ty = t / g.ntx;
+__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( GcArgs a, const GcArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    float * __restrict__ buf = a.buf;
+    const G g = a.g;
+    const int px = a.px, py = a.py;
+    const int tx = t % g.ntx, ty = t / g.ntx;
     float * loc = buf + (size_t) t * g.vol * NC;
     const int lo = nbr( tx, ty, -1, 0, g, px, py ), hi = nbr( tx, ty, +1, 0, g, px, py );
     const int E = g.ext_x;
@@ -115,8 +162,12 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( float * __restrict__ 
 
 // copy_to_gc_y (grid.h:303-340): y guard rows (full width) from the neighbours.
 template <int NC>
-__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_y( float * __restrict__ buf, G g, int px, int py ) {
-    const int t = blockIdx.x, tx = t % g.ntx, ty = t / g.ntx;
+__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_y( GcArgs a, const GcArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    float * __restrict__ buf = a.buf;
+    const G g = a.g;
+    const int px = a.px, py = a.py;
+    const int tx = t % g.ntx, ty = t / g.ntx;
     float * loc = buf + (size_t) t * g.vol * NC;
     const int lo = nbr( tx, ty, 0, -1, g, px, py ), hi = nbr( tx, ty, 0, +1, g, px, py );
     const int E = g.ext_x;
@@ -140,10 +191,13 @@ constexpr int TILE_BLOCK = 256;
 
 // kernel3_x / kernel3_y (grid.h:495-593): B = A[-1]*a + A[0]*b + A[+1]*c per
 // component over all rows (x) or all columns (y) of the tile, A in LDS.
-__global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( float * __restrict__ buf, G g, int dir, float a, float b,
-                                                         float c ) {
+__global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( K3Args args, const K3Args * tab, const int2 * map ) {
     extern __shared__ float A[];
-    const int t = blockIdx.x;
+    const int t = select_instance( args, tab, map );
+    float * __restrict__ buf = args.buf;
+    const G g = args.g;
+    const int dir = args.dir;
+    const float a = args.a, b = args.b, c = args.c;
     float * tb = buf + (size_t) t * g.vol * 3;
     for ( int k = threadIdx.x; k < 3 * g.vol; k += blockDim.x ) A[k] = tb[k];
     __syncthreads();
@@ -161,11 +215,14 @@ __global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( float * __restrict__ 
 
 // EMF::advance(Current&) per tile (emf.cpp:513-566, yee_b :67-81,
 // yeeJ_e :486-506): E and B of the tile in LDS planes, J from global.
-__global__ void __launch_bounds__( TILE_BLOCK ) k_yee( float * __restrict__ Eg, float * __restrict__ Bg,
-                                                     const float * __restrict__ Jg, G g, float dtdx, float dtdy,
-                                                     float dt ) {
+__global__ void __launch_bounds__( TILE_BLOCK ) k_yee( YeeArgs a, const YeeArgs * tab, const int2 * map ) {
     extern __shared__ float sm[];
-    const int t = blockIdx.x;
+    const int t = select_instance( a, tab, map );
+    float * __restrict__ Eg = a.E;
+    float * __restrict__ Bg = a.B;
+    const float * __restrict__ Jg = a.J;
+    const G g = a.g;
+    const float dtdx = a.dtdx, dtdy = a.dtdy, dt = a.dt;
     const int vol = g.vol, ys = g.ext_x;
     float * Ex = sm; float * Ey = Ex + vol; float * Ez = Ey + vol;
     float * Bx = Ez + vol; float * By = Bx + vol; float * Bz = By + vol;
@@ -295,11 +352,14 @@ template <int NC>
 int gc_pass( int which, float * buf, const zpic_geom * gg, int px, int py, hipStream_t st ) {
     const G g = make( gg );
     const int nt = g.ntx * g.nty;
+    const GcArgs a{ buf, g, px, py };
+    const GcArgs * tab = nullptr;
+    const int2 * map = nullptr;
     switch ( which ) {
-    case 0: hipLaunchKernelGGL( k_add_gc_x<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, buf, g, px, py ); break;
-    case 1: hipLaunchKernelGGL( k_add_gc_y<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, buf, g, px, py ); break;
-    case 2: hipLaunchKernelGGL( k_copy_gc_x<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, buf, g, px, py ); break;
-    case 3: hipLaunchKernelGGL( k_copy_gc_y<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, buf, g, px, py ); break;
+    case 0: hipLaunchKernelGGL( k_add_gc_x<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, a, tab, map ); break;
+    case 1: hipLaunchKernelGGL( k_add_gc_y<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, a, tab, map ); break;
+    case 2: hipLaunchKernelGGL( k_copy_gc_x<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, a, tab, map ); break;
+    case 3: hipLaunchKernelGGL( k_copy_gc_y<NC>, dim3( nt ), dim3( GC_BLOCK ), 0, st, a, tab, map ); break;
     }
     ZPIC_LAUNCH_CHECK( "guard cell kernel" );
     return ZPIC_OK;
@@ -314,10 +374,169 @@ int gc( int which, float * buf, const zpic_geom * g, int ncomp, int px, int py, 
     return set_error( ZPIC_EINVAL, "ncomp must be 1 or 3" );
 }
 
+// ------------------------------------------------------------------ batches
+// Fill of whole buffers, one row of blocks (blockIdx.y) per entry.
+struct FillArgs {
+    float * buf;
+    size_t n;
+};
+
+__global__ void k_fill_batch( const FillArgs * tab ) {
+    const FillArgs a = tab[blockIdx.y];
+    for ( size_t k = blockIdx.x * (size_t) blockDim.x + threadIdx.x; k < a.n; k += (size_t) gridDim.x * blockDim.x )
+        a.buf[k] = 0.0f;
+}
+
+// One batched launch: a device table of per-entry arguments and the
+// block -> (entry, tile) map.
+struct Group {
+    int kind;  // 0 add_gc_x, 1 add_gc_y, 2 copy_gc_x, 3 copy_gc_y, 4 kernel3, 5 yee
+    void * tab;
+    int2 * map;
+    int nblocks;
+    size_t lds;
+};
+
+template <class A>
+int make_group( Group & grp, int kind, const std::vector<A> & entries, const std::vector<int> & tiles, size_t lds ) {
+    std::vector<int2> map;
+    for ( size_t e = 0; e < entries.size(); e++ )
+        for ( int t = 0; t < tiles[e]; t++ ) map.push_back( make_int2( (int) e, t ) );
+    grp.kind = kind;
+    grp.nblocks = (int) map.size();
+    grp.lds = lds;
+    grp.tab = nullptr;
+    grp.map = nullptr;
+    if ( map.empty() ) return ZPIC_OK;
+    ZPIC_HIP_TRY( hipMalloc( &grp.tab, sizeof( A ) * entries.size() ) );
+    ZPIC_HIP_TRY( hipMalloc( (void **) &grp.map, sizeof( int2 ) * map.size() ) );
+    ZPIC_HIP_TRY( hipMemcpy( grp.tab, entries.data(), sizeof( A ) * entries.size(), hipMemcpyHostToDevice ) );
+    ZPIC_HIP_TRY( hipMemcpy( grp.map, map.data(), sizeof( int2 ) * map.size(), hipMemcpyHostToDevice ) );
+    return ZPIC_OK;
+}
+
+int run_group( const Group & grp, hipStream_t st ) {
+    if ( grp.nblocks == 0 ) return ZPIC_OK;
+    const dim3 nb( grp.nblocks );
+    switch ( grp.kind ) {
+    case 0: hipLaunchKernelGGL( k_add_gc_x<3>, nb, dim3( GC_BLOCK ), 0, st, GcArgs{}, (const GcArgs *) grp.tab, grp.map ); break;
+    case 1: hipLaunchKernelGGL( k_add_gc_y<3>, nb, dim3( GC_BLOCK ), 0, st, GcArgs{}, (const GcArgs *) grp.tab, grp.map ); break;
+    case 2: hipLaunchKernelGGL( k_copy_gc_x<3>, nb, dim3( GC_BLOCK ), 0, st, GcArgs{}, (const GcArgs *) grp.tab, grp.map ); break;
+    case 3: hipLaunchKernelGGL( k_copy_gc_y<3>, nb, dim3( GC_BLOCK ), 0, st, GcArgs{}, (const GcArgs *) grp.tab, grp.map ); break;
+    case 4: hipLaunchKernelGGL( k_kernel3, nb, dim3( TILE_BLOCK ), grp.lds, st, K3Args{}, (const K3Args *) grp.tab, grp.map ); break;
+    case 5: hipLaunchKernelGGL( k_yee, nb, dim3( TILE_BLOCK ), grp.lds, st, YeeArgs{}, (const YeeArgs *) grp.tab, grp.map ); break;
+    }
+    ZPIC_LAUNCH_CHECK( "batched grid kernel" );
+    return ZPIC_OK;
+}
+
 }  // namespace
+
+struct BatchFieldPlan {
+    int n = 0;
+    FillArgs * d_fill = nullptr;
+    size_t max_fill = 0;
+    std::vector<Group> current;  // add, copy, filter passes
+    std::vector<Group> yee;      // Yee, then E and B guard copies
+    ~BatchFieldPlan() {
+        (void) hipFree( d_fill );
+        for ( auto * v : { &current, &yee } )
+            for ( auto & g : *v ) { (void) hipFree( g.tab ); (void) hipFree( g.map ); }
+    }
+};
+
+int batch_fields_create( BatchFieldPlan ** out, const zpic_batch_fields_item * items, int n ) {
+    if ( !out || !items || n < 1 ) return set_error( ZPIC_EINVAL, "empty batch" );
+    auto plan = std::make_unique<BatchFieldPlan>();
+    plan->n = n;
+    std::vector<FillArgs> fills( n );
+    std::vector<GcArgs> jgc( n ), ebgc;
+    std::vector<YeeArgs> yees( n );
+    std::vector<int> tiles( n ), tiles2;
+    int max_vol = 0;
+    // filter passes of each instance: (dir, a, b, c)
+    struct Pass { int dir; float a, b, c; };
+    std::vector<std::vector<Pass>> passes( n );
+    size_t max_passes = 0;
+    for ( int i = 0; i < n; i++ ) {
+        const zpic_batch_fields_item & it = items[i];
+        if ( int rc = check( &it.g ) ) return rc;
+        if ( it.g.gc_lower < 1 || it.g.gc_upper < 2 ) return set_error( ZPIC_EINVAL, "batch fields need guard cells {1,2}" );
+        const G g = make( &it.g );
+        const int px = it.periodic_x ? 1 : 0, py = it.periodic_y ? 1 : 0;
+        tiles[i] = g.ntx * g.nty;
+        max_vol = std::max( max_vol, g.vol );
+        fills[i] = FillArgs{ it.J, (size_t) g.vol * tiles[i] * 3 };
+        plan->max_fill = std::max( plan->max_fill, fills[i].n );
+        jgc[i] = GcArgs{ it.J, g, px, py };
+        ebgc.push_back( GcArgs{ it.E, g, px, py } );
+        ebgc.push_back( GcArgs{ it.B, g, px, py } );
+        tiles2.push_back( tiles[i] );
+        tiles2.push_back( tiles[i] );
+        yees[i] = YeeArgs{ it.E, it.B, it.J, g, it.dt_dx, it.dt_dy, it.dt };
+        if ( it.filter_kind ) {
+            const unsigned order = it.filter_order > 0 ? it.filter_order : 1;
+            for ( unsigned k = 0; k < order; k++ ) passes[i].push_back( Pass{ it.filter_dir, 0.25f, 0.5f, 0.25f } );
+            if ( it.filter_kind == 2 ) {
+                // Filter::Compensated (filter.h:58-76)
+                const float a = -1.0f;
+                const float b = ( 4.0 + 2.0 * order ) / order;
+                const float norm = 2 * a + b;
+                passes[i].push_back( Pass{ it.filter_dir, a / norm, b / norm, a / norm } );
+            }
+        }
+        max_passes = std::max( max_passes, passes[i].size() );
+    }
+    if ( sizeof( float ) * 6 * max_vol > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    ZPIC_HIP_TRY( hipMalloc( (void **) &plan->d_fill, sizeof( FillArgs ) * n ) );
+    ZPIC_HIP_TRY( hipMemcpy( plan->d_fill, fills.data(), sizeof( FillArgs ) * n, hipMemcpyHostToDevice ) );
+
+    for ( int kind = 0; kind < 4; kind++ ) {
+        Group grp;
+        if ( int rc = make_group( grp, kind, jgc, tiles, 0 ) ) return rc;
+        plan->current.push_back( grp );
+    }
+    for ( size_t p = 0; p < max_passes; p++ ) {
+        for ( int dir = 0; dir < 2; dir++ ) {
+            std::vector<K3Args> k3;
+            std::vector<GcArgs> cg;
+            std::vector<int> nt;
+            for ( int i = 0; i < n; i++ ) {
+                if ( p < passes[i].size() && passes[i][p].dir == dir ) {
+                    const Pass & q = passes[i][p];
+                    k3.push_back( K3Args{ jgc[i].buf, jgc[i].g, dir, q.a, q.b, q.c } );
+                    cg.push_back( jgc[i] );
+                    nt.push_back( tiles[i] );
+                }
+            }
+            if ( k3.empty() ) continue;
+            Group g1, g2;
+            if ( int rc = make_group( g1, 4, k3, nt, sizeof( float ) * 3 * max_vol ) ) return rc;
+            if ( int rc = make_group( g2, dir == 0 ? 2 : 3, cg, nt, 0 ) ) return rc;
+            plan->current.push_back( g1 );
+            plan->current.push_back( g2 );
+        }
+    }
+    {
+        Group gy, gx, gyy;
+        if ( int rc = make_group( gy, 5, yees, tiles, sizeof( float ) * 6 * max_vol ) ) return rc;
+        if ( int rc = make_group( gx, 2, ebgc, tiles2, 0 ) ) return rc;
+        if ( int rc = make_group( gyy, 3, ebgc, tiles2, 0 ) ) return rc;
+        plan->yee.push_back( gy );
+        plan->yee.push_back( gx );
+        plan->yee.push_back( gyy );
+    }
+    *out = plan.release();
+    return ZPIC_OK;
+}
+
 }  // namespace zpic
 
 using namespace zpic;
+
+struct zpic_batch_fields {
+    BatchFieldPlan * plan;
+};
 
 extern "C" {
 
@@ -354,7 +573,9 @@ int zpic_kernel3( float * buf, const zpic_geom * gg, int dir, float a, float b, 
     const size_t lds = sizeof( float ) * 3 * g.vol;
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     hipStream_t st = (hipStream_t) stream;
-    hipLaunchKernelGGL( k_kernel3, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, g, dir, a, b, c );
+    const K3Args args{ buf, g, dir, a, b, c };
+    hipLaunchKernelGGL( k_kernel3, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, args, (const K3Args *) nullptr,
+                        (const int2 *) nullptr );
     ZPIC_LAUNCH_CHECK( "k_kernel3" );
     return gc( dir == 0 ? 2 : 3, buf, gg, 3, px, py, stream );
 }
@@ -386,7 +607,9 @@ int zpic_yee( float * E, float * B, const float * J, const zpic_geom * gg, float
     const size_t lds = sizeof( float ) * 6 * g.vol;
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     hipStream_t st = (hipStream_t) stream;
-    hipLaunchKernelGGL( k_yee, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, E, B, J, g, dtdx, dtdy, dt );
+    const YeeArgs args{ E, B, J, g, dtdx, dtdy, dt };
+    hipLaunchKernelGGL( k_yee, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, args, (const YeeArgs *) nullptr,
+                        (const int2 *) nullptr );
     ZPIC_LAUNCH_CHECK( "k_yee" );
     if ( int rc = zpic_copy_to_gc( E, gg, 3, px, py, stream ) ) return rc;
     return zpic_copy_to_gc( B, gg, 3, px, py, stream );
@@ -436,6 +659,48 @@ int zpic_fill( float * buf, size_t n, float v, void * stream ) {
     hipLaunchKernelGGL( k_fill, dim3( blocks ), dim3( 256 ), 0, (hipStream_t) stream, buf, n, v );
     ZPIC_LAUNCH_CHECK( "k_fill" );
     return ZPIC_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int zpic_batch_fields_create( zpic_batch_fields ** out, const zpic_batch_fields_item * items, int n ) {
+    if ( !out ) return set_error( ZPIC_EINVAL, "null output" );
+    BatchFieldPlan * plan = nullptr;
+    if ( int rc = batch_fields_create( &plan, items, n ) ) return rc;
+    *out = new zpic_batch_fields{ plan };
+    return ZPIC_OK;
+}
+
+int zpic_batch_zero_current( zpic_batch_fields * b, void * stream ) {
+    if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch" );
+    size_t blocks = ( b->plan->max_fill + 255 ) / 256;
+    if ( blocks > 1024 ) blocks = 1024;
+    hipLaunchKernelGGL( k_fill_batch, dim3( blocks, b->plan->n ), dim3( 256 ), 0, (hipStream_t) stream,
+                        (const FillArgs *) b->plan->d_fill );
+    ZPIC_LAUNCH_CHECK( "k_fill_batch" );
+    return ZPIC_OK;
+}
+
+int zpic_batch_current_advance( zpic_batch_fields * b, void * stream ) {
+    if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch" );
+    for ( const Group & g : b->plan->current )
+        if ( int rc = run_group( g, (hipStream_t) stream ) ) return rc;
+    return ZPIC_OK;
+}
+
+int zpic_batch_yee( zpic_batch_fields * b, void * stream ) {
+    if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch" );
+    for ( const Group & g : b->plan->yee )
+        if ( int rc = run_group( g, (hipStream_t) stream ) ) return rc;
+    return ZPIC_OK;
+}
+
+void zpic_batch_fields_free( zpic_batch_fields * b ) {
+    if ( !b ) return;
+    delete b->plan;
+    delete b;
 }
 
 }  // extern "C"

@@ -615,3 +615,135 @@ void Simulation::energy_info() {
     std::cout << "(*info*) Magnetic field = " << eB.x + eB.y + eB.z << '\n';
     std::cout << "(*info*) total = " << part_ene + eE.x + eE.y + eE.z + eB.x + eB.y + eB.z << '\n';
 }
+
+// ------------------------------------------------------------------- Batch
+zpic_batch_species Species::batch_item( EMF const & emf, Current & current ) const {
+    zpic_batch_species it;
+    it.p = particles->view();
+    it.tmp = tmp->view();
+    it.s = sort->buf;
+    it.fld = emf.E->geom();
+    it.E = emf.E->fbuf();
+    it.B = emf.B->fbuf();
+    it.J = current.J->fbuf();
+    it.tile_energy = d_tile_energy;
+    it.energy_out = d_energy_dev;
+    it.alpha = 0.5 * dt / m_q;
+    it.dt_dx = dt / dx.x;
+    it.dt_dy = dt / dx.y;
+    it.q = q;
+    it.qnx = q * dx.x / dt;
+    it.qny = q * dx.y / dt;
+    it.periodic_x = particles->periodic.x;
+    it.periodic_y = particles->periodic.y;
+    return it;
+}
+
+void Simulation::note_step() {
+    for ( auto * s : species ) s->note_step();
+    current.note_step();
+    emf.note_step();
+    iter++;
+}
+
+zpic_batch_fields_item Simulation::batch_fields() const {
+    zpic_batch_fields_item it;
+    it.E = emf.E->fbuf();
+    it.B = emf.B->fbuf();
+    it.J = current.J->fbuf();
+    it.g = emf.E->geom();
+    const float2 d = emf.get_dx();
+    it.dt_dx = dt / d.x;
+    it.dt_dy = dt / d.y;
+    it.dt = (float) dt;
+    it.filter_kind = current.filter->kind();
+    it.filter_dir = 0;
+    it.filter_order = 1;
+    if ( auto * b = dynamic_cast<const Filter::Binomial *>( current.filter ) ) {
+        it.filter_dir = b->get_dir() == coord::x ? 0 : 1;
+        it.filter_order = b->get_order();
+    }
+    const int2 p = emf.E->get_periodic();
+    it.periodic_x = p.x;
+    it.periodic_y = p.y;
+    return it;
+}
+
+Batch::Batch( int threads_per_tile )
+    : fields( nullptr ), threads( threads_per_tile ), use_graph( false ), graph( nullptr ), exec( nullptr ) {}
+
+Batch::~Batch() { release(); }
+
+void Batch::release() {
+    if ( exec ) (void) hipGraphExecDestroy( exec );
+    if ( graph ) (void) hipGraphDestroy( graph );
+    exec = nullptr;
+    graph = nullptr;
+    for ( auto * p : slots ) zpic_batch_particles_free( p );
+    slots.clear();
+    zpic_batch_fields_free( fields );
+    fields = nullptr;
+}
+
+void Batch::add( Simulation & sim ) {
+    if ( !sims.empty() && sim.species.size() != sims[0]->species.size() )
+        throw std::runtime_error( "(*error*) all instances of a batch must hold the same number of species" );
+    release();
+    sims.push_back( &sim );
+}
+
+void Batch::set_graph( bool on ) {
+    if ( exec ) (void) hipGraphExecDestroy( exec );
+    if ( graph ) (void) hipGraphDestroy( graph );
+    exec = nullptr;
+    graph = nullptr;
+    use_graph = on;
+}
+
+void Batch::build() {
+    if ( sims.empty() ) throw std::runtime_error( "(*error*) empty batch" );
+    release();
+    const size_t nsp = sims[0]->species.size();
+    for ( size_t k = 0; k < nsp; k++ ) {
+        std::vector<zpic_batch_species> items;
+        for ( auto * s : sims ) items.push_back( s->species[k]->batch_item( s->emf, s->current ) );
+        zpic_batch_particles * plan = nullptr;
+        check( zpic_batch_particles_create( &plan, items.data(), (int) items.size(), threads ) );
+        slots.push_back( plan );
+    }
+    std::vector<zpic_batch_fields_item> fitems;
+    for ( auto * s : sims ) fitems.push_back( s->batch_fields() );
+    check( zpic_batch_fields_create( &fields, fitems.data(), (int) fitems.size() ) );
+}
+
+// Simulation::advance (simulation.h:90-107) for every instance: each stage is
+// one launch over all instances.
+void Batch::step_kernels() {
+    check( zpic_batch_zero_current( fields, stream() ) );
+    for ( auto * plan : slots ) check( zpic_batch_particles_advance( plan, stream() ) );
+    check( zpic_batch_current_advance( fields, stream() ) );
+    check( zpic_batch_yee( fields, stream() ) );
+}
+
+void Batch::advance() {
+    if ( !fields ) build();
+    if ( use_graph ) {
+        if ( !exec ) {
+            hip_check( hipStreamBeginCapture( stream(), hipStreamCaptureModeThreadLocal ), "hipStreamBeginCapture" );
+            try {
+                step_kernels();
+            } catch ( ... ) {
+                hipGraph_t g = nullptr;
+                (void) hipStreamEndCapture( stream(), &g );
+                if ( g ) (void) hipGraphDestroy( g );
+                throw;
+            }
+            hip_check( hipStreamEndCapture( stream(), &graph ), "hipStreamEndCapture" );
+            hip_check( hipGraphInstantiate( &exec, graph, nullptr, nullptr, 0 ), "hipGraphInstantiate" );
+        }
+        hip_check( hipGraphLaunch( exec, stream() ), "hipGraphLaunch" );
+    } else {
+        step_kernels();
+    }
+    for ( auto * s : sims ) s->note_step();
+}

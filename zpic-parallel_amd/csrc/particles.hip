@@ -2,10 +2,12 @@
 // push, split-segment charge-conserving current deposit, and the tile sort.
 //
 // One workgroup per tile.  The tile's E and B (with guard cells) are staged in
-// LDS as six float planes; the tile's current J is accumulated in LDS with
-// ds_add_f32 and flushed once, with plain adds, into the tile's own region of
-// the global J grid (each tile owns its guard cells, so the flush needs no
-// global atomics).
+// LDS as six float planes.  Each wavefront accumulates its particles' current
+// in its own three J planes in LDS with plain read-add-writes, made
+// conflict-free by a per-round owner election (LDS float atomics run at about
+// one lane per clock on this part); the waves' planes are summed and flushed
+// once, with plain adds, into the tile's own region of the global J grid (each
+// tile owns its guard cells, so the flush needs no global atomics).
 //
 // Arithmetic is the reference's, operation for operation: no contraction
 // (built with -ffp-contract=off), fmaf exactly where the reference calls
@@ -17,11 +19,14 @@
 // order, which reproduces the reference's single-thread results exactly.
 #include "common.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace zpic {
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int FAST_BLOCK = 256;  // 4 wavefronts per tile in the fast mode
+constexpr int FAST_BLOCK = 512;  // 8 wavefronts per tile in the fast mode
 constexpr int MAX_SEG = 15;      // wave segments recorded per tile (slot 15 = chunk size)
 
 struct PartView {
@@ -220,6 +225,33 @@ __device__ inline void add_current( volatile lds_f32 * Jx, volatile lds_f32 * Jy
     Jz[k + ys + 1] = Jz[k + ys + 1] + v[7];
 }
 
+// Read-add-write of the 8 contributions of lanes that own distinct cells, in
+// four batches by node offset (0, +1, +ys, +ys+1): inside a batch the lanes'
+// addresses are distinct, so each batch issues its reads together; a later
+// batch may touch what an earlier one wrote (Jx(k+ys) of one cell is Jx(k)
+// of the cell above) and, the accesses being volatile and LDS in order per
+// wave, sees those writes.
+__device__ inline void add_current_batched( volatile lds_f32 * Jx, volatile lds_f32 * Jy, volatile lds_f32 * Jz,
+                                            int k, int ys, const float v[8] ) {
+    {
+        const float a = Jx[k], b = Jy[k], c = Jz[k];
+        Jx[k] = a + v[0];
+        Jy[k] = b + v[1];
+        Jz[k] = c + v[2];
+    }
+    {
+        const float a = Jy[k + 1], b = Jz[k + 1];
+        Jy[k + 1] = a + v[3];
+        Jz[k + 1] = b + v[4];
+    }
+    {
+        const float a = Jx[k + ys], b = Jz[k + ys];
+        Jx[k + ys] = a + v[5];
+        Jz[k + ys] = b + v[6];
+    }
+    Jz[k + ys + 1] = Jz[k + ys + 1] + v[7];
+}
+
 // Fast-mode deposit of one segment per pending lane into the wave's private J
 // planes, without atomics: each round, every pending lane writes its lane id
 // into the tag byte of its cell and reads it back; the lane whose id survived
@@ -238,7 +270,7 @@ __device__ inline void deposit_elected( bool pend, const Seg & s, float qnx, flo
     while ( __ballot( pend ) ) {
         if ( pend ) tag[k] = (unsigned char) lane;
         if ( pend && tag[k] == (unsigned char) lane ) {
-            add_current( Jx, Jy, Jz, k, ys, v );
+            add_current_batched( Jx, Jy, Jz, k, ys, v );
             pend = false;
         }
     }
@@ -265,13 +297,45 @@ __host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push
            ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) : 0 );
 }
 
+// Arguments of one instance of the particle pass.
+struct PartArgs {
+    PartView P;
+    Geo g;
+    const float3 * E;
+    const float3 * B;
+    float3 * J;
+    float alpha;
+    MoveArgs m;
+    double * tile_energy;
+    SortView S;
+};
+
+// Single-instance launches pass map == nullptr and use `a` with tile =
+// blockIdx.x; batched launches pass a table of per-instance arguments and a
+// map from block to (instance, tile).
+template <class A>
+__device__ inline int select_instance( A & a, const A * tab, const int2 * map ) {
+    if ( map ) {
+        const int2 r = map[blockIdx.x];
+        a = tab[r.x];
+        return r.y;
+    }
+    return blockIdx.x;
+}
+
 template <bool PUSH, bool MOVE, bool DET>
-__global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, const float3 * __restrict__ E,
-                                                          const float3 * __restrict__ B, float3 * __restrict__ J,
-                                                          float alpha, MoveArgs m, double * __restrict__ tile_energy,
-                                                          SortView S ) {
+__global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const PartArgs * tab, const int2 * map ) {
     extern __shared__ float smem[];
-    const int t = blockIdx.x;
+    const int t = select_instance( a, tab, map );
+    const PartView P = a.P;
+    const Geo g = a.g;
+    const float3 * __restrict__ E = a.E;
+    const float3 * __restrict__ B = a.B;
+    float3 * __restrict__ J = a.J;
+    const float alpha = a.alpha;
+    const MoveArgs m = a.m;
+    double * __restrict__ tile_energy = a.tile_energy;
+    const SortView S = a.S;
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int vol = g.vol;
@@ -331,16 +395,26 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartView P, Geo g, 
     double en = 0.0;
     int nl = 0;
 
+    // the next 64 particles are loaded while the current ones are processed
+    int2 cn = make_int2( 0, 0 );
+    float2 xn = make_float2( 0.f, 0.f );
+    float3 un = make_float3( 0.f, 0.f, 0.f );
+    if ( beg + lane < end ) {
+        cn = pix[(unsigned) ( beg + lane )];
+        xn = px[(unsigned) ( beg + lane )];
+        un = pu[(unsigned) ( beg + lane )];
+    }
+
     for ( int i0 = beg; i0 < end; i0 += WAVE ) {
         const int i = i0 + lane;
         const bool act = i < end;
-        int2 c = make_int2( 0, 0 );
-        float2 x = make_float2( 0.f, 0.f );
-        float3 u = make_float3( 0.f, 0.f, 0.f );
-        if ( act ) {
-            c = pix[(unsigned) i];
-            x = px[(unsigned) i];
-            u = pu[(unsigned) i];
+        int2 c = cn;
+        float2 x = xn;
+        float3 u = un;
+        if ( i + WAVE < end ) {
+            cn = pix[(unsigned) ( i + WAVE )];
+            xn = px[(unsigned) ( i + WAVE )];
+            un = pu[(unsigned) ( i + WAVE )];
         }
         if constexpr ( PUSH ) {
             float ene = 0.0f;
@@ -481,25 +555,39 @@ __device__ inline int source_of( int t, int d, const SortGeo & q ) {
     return wrap_tile( t % q.ntx - ( d % 3 - 1 ), t / q.ntx - ( d / 3 - 1 ), q.ntx, q.nty, q.px, q.py );
 }
 
+// Arguments of one instance of the tile sort.
+struct SortArgs {
+    PartView P;  // particles (offsets are the old ones until k_sort_in)
+    PartView T;  // staging buffer; T.off receives the new offsets
+    SortView S;
+    SortGeo q;
+    int ntiles;
+};
+
 // New particle count per tile (bnd_check's new_np, particles.cpp:212-303):
 // the tile gathers the direction counts its 9 source tiles send to it.
-__global__ void k_sort_newnp( SortView S, SortGeo q, int ntiles ) {
+// Batched launches index the instance with blockIdx.y.
+__global__ void k_sort_newnp( SortArgs a, const SortArgs * tab ) {
+    if ( tab ) a = tab[blockIdx.y];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( t >= ntiles ) return;
+    if ( t >= a.ntiles ) return;
     int n = 0;
     for ( int d = 0; d < 9; d++ ) {
-        const int s = source_of( t, d, q );
-        if ( s >= 0 ) n += S.npt[9 * s + d];
+        const int s = source_of( t, d, a.q );
+        if ( s >= 0 ) n += a.S.npt[9 * s + d];
     }
-    S.new_np[t] = n;
+    a.S.new_np[t] = n;
 }
 
 // Exclusive scan of the new counts into the new tile offsets
-// (update_tile_info, particles.cpp:314-357).  One workgroup; each thread owns
-// a contiguous run of tiles.
+// (update_tile_info, particles.cpp:314-357).  One workgroup per instance;
+// each thread owns a contiguous run of tiles.
 constexpr int SCAN_BLOCK = 1024;
-__global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortView S, int * __restrict__ new_off, SortGeo q,
-                                                              int ntiles, unsigned max_part ) {
+__global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const SortArgs * tab ) {
+    if ( tab ) a = tab[blockIdx.x];
+    const SortView S = a.S;
+    int * __restrict__ new_off = a.T.off;
+    const int ntiles = a.ntiles;
     __shared__ int s_sum[SCAN_BLOCK];
     const int tid = threadIdx.x;
     const int per = ( ntiles + SCAN_BLOCK - 1 ) / SCAN_BLOCK;
@@ -521,7 +609,6 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortView S, int * 
         base += S.new_np[t];
     }
     if ( tid == SCAN_BLOCK - 1 ) S.total[0] = s_sum[tid];
-    (void) max_part;
 }
 
 // Stayers that move to the staging buffer because the tile's range moved
@@ -548,11 +635,14 @@ constexpr int SORT_BLOCK = 256;
 // staging slots of each target tile are filled in the reference's
 // single-thread reservation order: by source tile, the source's own shifted
 // stayers first, then directions 0..8.
-__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( PartView P, PartView T, SortView S, SortGeo q ) {
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const SortArgs * tab, const int2 * map ) {
     __shared__ int s_start[9];
     __shared__ int s_pre[MAX_SEG + 1];
     __shared__ int s_hdr[4];
-    const int s = blockIdx.x;
+    const int s = select_instance( a, tab, map );
+    const PartView P = a.P, T = a.T;
+    const SortView S = a.S;
+    const SortGeo q = a.q;
     const int tid = threadIdx.x;
 
     if ( tid < 9 ) {
@@ -693,9 +783,11 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( PartView P, PartView
 
 // copy_in (particles.cpp:559-614): staged particles appended after the
 // in-place stayers, or first into the gap when the range moved down.
-__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( PartView P, PartView T, SortView S ) {
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( SortArgs a, const SortArgs * tab, const int2 * map ) {
     __shared__ int s_hdr[4];
-    const int t = blockIdx.x;
+    const int t = select_instance( a, tab, map );
+    const PartView P = a.P, T = a.T;
+    const SortView S = a.S;
     if ( threadIdx.x == 0 ) {
         s_hdr[0] = P.off[t];
         s_hdr[1] = P.np[t];
@@ -793,6 +885,16 @@ int check_geom( const zpic_geom * g ) {
     return ZPIC_OK;
 }
 
+// Threads per tile of the fast particle pass (a multiple of 64, at most
+// FAST_BLOCK); the deterministic mode always uses one wavefront.
+int g_threads = FAST_BLOCK;  // library-wide setting
+
+int check_threads( int threads ) {
+    if ( threads < WAVE || threads > FAST_BLOCK || threads % WAVE )
+        return set_error( ZPIC_EINVAL, "threads per tile must be a multiple of 64 in [64, 512]" );
+    return ZPIC_OK;
+}
+
 template <bool PUSH, bool MOVE>
 int launch_particles( const zpic_particles * p, const zpic_geom * fld, const float * E, const float * B, float * J,
                       float alpha, MoveArgs m, double * tile_energy, zpic_sort * s, int det, void * stream ) {
@@ -800,21 +902,158 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
     if ( !p || ( PUSH && ( !E || !B || !tile_energy ) ) || ( MOVE && ( !J || !s ) ) )
         return set_error( ZPIC_EINVAL, "null buffer" );
     const int nt = fld->ntiles_x * fld->ntiles_y;
-    const size_t lds = particles_lds_bytes( fld->tile_vol, det ? 1 : FAST_BLOCK / WAVE, PUSH, MOVE );
+    const int threads = det ? WAVE : g_threads;
+    const size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE );
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
-    const Geo g = make_geo( fld );
+    PartArgs a;
+    a.P = view( p ); a.g = make_geo( fld );
+    a.E = (const float3 *) E; a.B = (const float3 *) B; a.J = (float3 *) J;
+    a.alpha = alpha; a.m = m; a.tile_energy = tile_energy; a.S = view( s );
     hipStream_t st = (hipStream_t) stream;
     if ( det )
-        hipLaunchKernelGGL( ( k_particles<PUSH, MOVE, true> ), dim3( nt ), dim3( WAVE ), lds, st, view( p ), g,
-                            (const float3 *) E, (const float3 *) B, (float3 *) J, alpha, m, tile_energy, view( s ) );
+        hipLaunchKernelGGL( ( k_particles<PUSH, MOVE, true> ), dim3( nt ), dim3( threads ), lds, st, a,
+                            (const PartArgs *) nullptr, (const int2 *) nullptr );
     else
-        hipLaunchKernelGGL( ( k_particles<PUSH, MOVE, false> ), dim3( nt ), dim3( FAST_BLOCK ), lds, st, view( p ), g,
-                            (const float3 *) E, (const float3 *) B, (float3 *) J, alpha, m, tile_energy, view( s ) );
+        hipLaunchKernelGGL( ( k_particles<PUSH, MOVE, false> ), dim3( nt ), dim3( threads ), lds, st, a,
+                            (const PartArgs *) nullptr, (const int2 *) nullptr );
     ZPIC_LAUNCH_CHECK( "k_particles" );
     return ZPIC_OK;
 }
 
+SortArgs sort_args( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int ntx, int nty, int nx, int ny, int px,
+                    int py ) {
+    SortArgs a;
+    a.P = view( p ); a.T = view( tmp ); a.S = view( s );
+    a.q = SortGeo{ ntx, nty, nx, ny, px ? 1 : 0, py ? 1 : 0 };
+    a.ntiles = ntx * nty;
+    return a;
+}
+
 }  // namespace
+
+// ------------------------------------------------------------------ batches
+// Per entry: sum of its per-tile push energies (parallel tree, fast mode),
+// written to the entry's energy output.  One workgroup per entry.
+struct EnergyArgs {
+    const double * tile;
+    int n;
+    double * out;
+};
+
+// Device tables for one launch per stage over many independent instances:
+// entry e of a table holds the arguments of one (instance, species) pair,
+// map[b] = (entry, tile) for block b.
+struct BatchParticlePlan {
+    int nentries = 0;
+    int nblocks = 0;
+    int max_tiles = 0;
+    int max_vol = 0;
+    int threads = FAST_BLOCK;
+    PartArgs * d_part = nullptr;
+    SortArgs * d_sort = nullptr;
+    int2 * d_map = nullptr;
+    EnergyArgs * d_energy = nullptr;  // [nentries]
+    std::vector<PartArgs> h_part;
+};
+
+int batch_particles_create( BatchParticlePlan ** out, const zpic_batch_species * items, int n, int threads ) {
+    if ( !out || !items || n < 1 ) return set_error( ZPIC_EINVAL, "empty batch" );
+    if ( int rc = check_threads( threads ) ) return rc;
+    auto * plan = new BatchParticlePlan();
+    plan->threads = threads;
+    plan->nentries = n;
+    std::vector<SortArgs> sorts( n );
+    std::vector<EnergyArgs> energies( n );
+    std::vector<int2> map;
+    for ( int e = 0; e < n; e++ ) {
+        const zpic_batch_species & it = items[e];
+        if ( int rc = check_geom( &it.fld ) ) { delete plan; return rc; }
+        PartArgs a;
+        a.P = view( &it.p ); a.g = make_geo( &it.fld );
+        a.E = (const float3 *) it.E; a.B = (const float3 *) it.B; a.J = (float3 *) it.J;
+        a.alpha = it.alpha; a.m = MoveArgs{ it.dt_dx, it.dt_dy, it.q, it.qnx, it.qny };
+        a.tile_energy = it.tile_energy;
+        zpic_sort s = it.s;
+        a.S = view( &s );
+        plan->h_part.push_back( a );
+        zpic_particles p = it.p, t = it.tmp;
+        sorts[e] = sort_args( &p, &t, &s, it.fld.ntiles_x, it.fld.ntiles_y, it.fld.nx, it.fld.ny, it.periodic_x,
+                              it.periodic_y );
+        const int nt = it.fld.ntiles_x * it.fld.ntiles_y;
+        energies[e] = EnergyArgs{ it.tile_energy, nt, it.energy_out };
+        for ( int t2 = 0; t2 < nt; t2++ ) map.push_back( make_int2( e, t2 ) );
+        plan->max_tiles = std::max( plan->max_tiles, nt );
+        plan->max_vol = std::max( plan->max_vol, it.fld.tile_vol );
+    }
+    plan->nblocks = (int) map.size();
+    if ( particles_lds_bytes( plan->max_vol, threads / WAVE, true, true ) > 65536 ) {
+        delete plan;
+        return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    }
+    hipError_t e1 = hipMalloc( (void **) &plan->d_part, sizeof( PartArgs ) * n );
+    hipError_t e2 = hipMalloc( (void **) &plan->d_sort, sizeof( SortArgs ) * n );
+    hipError_t e3 = hipMalloc( (void **) &plan->d_map, sizeof( int2 ) * map.size() );
+    hipError_t e4 = hipMalloc( (void **) &plan->d_energy, sizeof( EnergyArgs ) * n );
+    if ( e1 || e2 || e3 || e4 ) {
+        (void) hipFree( plan->d_part ); (void) hipFree( plan->d_sort ); (void) hipFree( plan->d_map );
+        (void) hipFree( plan->d_energy );
+        delete plan;
+        return set_error( ZPIC_EHIP, "hipMalloc failed for batch tables" );
+    }
+    ZPIC_HIP_TRY( hipMemcpy( plan->d_part, plan->h_part.data(), sizeof( PartArgs ) * n, hipMemcpyHostToDevice ) );
+    ZPIC_HIP_TRY( hipMemcpy( plan->d_sort, sorts.data(), sizeof( SortArgs ) * n, hipMemcpyHostToDevice ) );
+    ZPIC_HIP_TRY( hipMemcpy( plan->d_map, map.data(), sizeof( int2 ) * map.size(), hipMemcpyHostToDevice ) );
+    ZPIC_HIP_TRY( hipMemcpy( plan->d_energy, energies.data(), sizeof( EnergyArgs ) * n, hipMemcpyHostToDevice ) );
+    *out = plan;
+    return ZPIC_OK;
+}
+
+void batch_particles_destroy( BatchParticlePlan * plan ) {
+    if ( !plan ) return;
+    (void) hipFree( plan->d_part ); (void) hipFree( plan->d_sort ); (void) hipFree( plan->d_map );
+    (void) hipFree( plan->d_energy );
+    delete plan;
+}
+
+__global__ void __launch_bounds__( 256 ) k_batch_energy( const EnergyArgs * tab ) {
+    __shared__ double buf[256];
+    const EnergyArgs a = tab[blockIdx.x];
+    double acc = 0;
+    for ( int t = threadIdx.x; t < a.n; t += blockDim.x ) acc += a.tile[t];
+    buf[threadIdx.x] = acc;
+    __syncthreads();
+    for ( int o = 128; o > 0; o >>= 1 ) {
+        if ( (int) threadIdx.x < o ) buf[threadIdx.x] += buf[threadIdx.x + o];
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 ) *a.out = buf[0];
+}
+
+int batch_particles_step( BatchParticlePlan * plan, void * stream ) {
+    hipStream_t st = (hipStream_t) stream;
+    const size_t lds = particles_lds_bytes( plan->max_vol, plan->threads / WAVE, true, true );
+    PartArgs dummy{};
+    hipLaunchKernelGGL( ( k_particles<true, true, false> ), dim3( plan->nblocks ), dim3( plan->threads ), lds, st,
+                        dummy, (const PartArgs *) plan->d_part, (const int2 *) plan->d_map );
+    ZPIC_LAUNCH_CHECK( "k_particles (batch)" );
+    SortArgs sd{};
+    hipLaunchKernelGGL( k_sort_newnp, dim3( ( plan->max_tiles + 255 ) / 256, plan->nentries ), dim3( 256 ), 0, st, sd,
+                        (const SortArgs *) plan->d_sort );
+    ZPIC_LAUNCH_CHECK( "k_sort_newnp (batch)" );
+    hipLaunchKernelGGL( k_sort_count, dim3( plan->nentries ), dim3( SCAN_BLOCK ), 0, st, sd,
+                        (const SortArgs *) plan->d_sort );
+    ZPIC_LAUNCH_CHECK( "k_sort_count (batch)" );
+    hipLaunchKernelGGL( k_sort_out, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
+                        (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
+    ZPIC_LAUNCH_CHECK( "k_sort_out (batch)" );
+    hipLaunchKernelGGL( k_sort_in, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
+                        (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
+    ZPIC_LAUNCH_CHECK( "k_sort_in (batch)" );
+    hipLaunchKernelGGL( k_batch_energy, dim3( plan->nentries ), dim3( 256 ), 0, st, (const EnergyArgs *) plan->d_energy );
+    ZPIC_LAUNCH_CHECK( "k_batch_energy" );
+    return ZPIC_OK;
+}
+
 }  // namespace zpic
 
 using namespace zpic;
@@ -845,19 +1084,51 @@ int zpic_tile_sort( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int
     if ( !p || !tmp || !s || ntiles_x < 1 || ntiles_y < 1 || nx < 1 || ny < 1 )
         return set_error( ZPIC_EINVAL, "invalid tile_sort arguments" );
     const int nt = ntiles_x * ntiles_y;
-    SortGeo q{ ntiles_x, ntiles_y, nx, ny, periodic_x ? 1 : 0, periodic_y ? 1 : 0 };
+    const SortArgs a = sort_args( p, tmp, s, ntiles_x, ntiles_y, nx, ny, periodic_x, periodic_y );
+    const SortArgs * tab = nullptr;
+    const int2 * map = nullptr;
     hipStream_t st = (hipStream_t) stream;
-    const PartView P = view( p ), T = view( tmp );
-    const SortView S = view( s );
-    hipLaunchKernelGGL( k_sort_newnp, dim3( ( nt + 255 ) / 256 ), dim3( 256 ), 0, st, S, q, nt );
+    hipLaunchKernelGGL( k_sort_newnp, dim3( ( nt + 255 ) / 256 ), dim3( 256 ), 0, st, a, tab );
     ZPIC_LAUNCH_CHECK( "k_sort_newnp" );
-    hipLaunchKernelGGL( k_sort_count, dim3( 1 ), dim3( SCAN_BLOCK ), 0, st, S, T.off, q, nt, p->max_part );
+    hipLaunchKernelGGL( k_sort_count, dim3( 1 ), dim3( SCAN_BLOCK ), 0, st, a, tab );
     ZPIC_LAUNCH_CHECK( "k_sort_count" );
-    hipLaunchKernelGGL( k_sort_out, dim3( nt ), dim3( SORT_BLOCK ), 0, st, P, T, S, q );
+    hipLaunchKernelGGL( k_sort_out, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
     ZPIC_LAUNCH_CHECK( "k_sort_out" );
-    hipLaunchKernelGGL( k_sort_in, dim3( nt ), dim3( SORT_BLOCK ), 0, st, P, T, S );
+    hipLaunchKernelGGL( k_sort_in, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
     ZPIC_LAUNCH_CHECK( "k_sort_in" );
     return ZPIC_OK;
+}
+
+int zpic_set_threads_per_tile( int threads ) {
+    if ( int rc = check_threads( threads ) ) return rc;
+    g_threads = threads;
+    return ZPIC_OK;
+}
+
+int zpic_get_threads_per_tile( void ) { return g_threads; }
+
+struct zpic_batch_particles {
+    BatchParticlePlan * plan;
+};
+
+int zpic_batch_particles_create( zpic_batch_particles ** out, const zpic_batch_species * items, int n,
+                                 int threads_per_tile ) {
+    if ( !out ) return set_error( ZPIC_EINVAL, "null output" );
+    BatchParticlePlan * plan = nullptr;
+    if ( int rc = batch_particles_create( &plan, items, n, threads_per_tile ) ) return rc;
+    *out = new zpic_batch_particles{ plan };
+    return ZPIC_OK;
+}
+
+int zpic_batch_particles_advance( zpic_batch_particles * b, void * stream ) {
+    if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch" );
+    return batch_particles_step( b->plan, stream );
+}
+
+void zpic_batch_particles_free( zpic_batch_particles * b ) {
+    if ( !b ) return;
+    batch_particles_destroy( b->plan );
+    delete b;
 }
 
 int zpic_deposit_charge( const zpic_particles * p, const zpic_geom * g, float * rho, float q, int deterministic,

@@ -362,6 +362,8 @@ class Current {
     }
     void advance();
     void zero();
+    void note_step() { iter++; }
+    float2 get_dx() const { return dx; }
 };
 
 // ---------------------------------------------------------------------- EMF
@@ -386,6 +388,8 @@ class EMF {
     void advance();
     void advance( Current & current );
     void get_energy( double3 & ene_E, double3 & ene_B );
+    void note_step() { iter++; }
+    float2 get_dx() const { return dx; }
 };
 
 // ------------------------------------------------------------------ Species
@@ -459,6 +463,11 @@ class Species {
 
     // Device access for callers that drive the stages themselves
     Particles * get_particles_buffer() { return particles; }
+
+    // Arguments of this species for a batched advance with the given fields
+    zpic_batch_species batch_item( EMF const & emf, Current & current ) const;
+    // Host-side bookkeeping of one step advanced by a batch (iteration, nmove)
+    void note_step() { d_nmove += np_count; iter++; }
 };
 
 // --------------------------------------------------------------- Simulation
@@ -502,4 +511,42 @@ class Simulation {
     // HIP-event timing of the stages (push+move, sort, current, yee, step)
     void set_timing( bool on );
     void get_timing( double out[5] );
+    // Host-side bookkeeping of one step advanced by a batch
+    void note_step();
+    // Arguments of this simulation's fields for a batched advance
+    zpic_batch_fields_item batch_fields() const;
+};
+
+// ------------------------------------------------------------------- Batch
+// Many independent simulations advanced together: every stage of
+// Simulation::advance is one launch over the tiles of all instances, and the
+// whole step can be captured once in a hipGraph and replayed.  Instances keep
+// their own buffers; each computes exactly what its own Simulation::advance
+// computes with the same threads per tile (fast mode).  No reference
+// counterpart (the reference advances one simulation per process).
+class Batch {
+    std::vector<Simulation *> sims;
+    std::vector<zpic_batch_particles *> slots;  // one plan per species slot
+    zpic_batch_fields * fields;
+    int threads;
+    bool use_graph;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    void release();
+    void step_kernels();
+
+  public:
+    explicit Batch( int threads_per_tile = 512 );
+    ~Batch();
+    Batch( const Batch & ) = delete;
+    Batch & operator=( const Batch & ) = delete;
+
+    // Non-owning; all instances must hold the same number of species.
+    void add( Simulation & sim );
+    // Builds the launch tables (called by the first advance if needed).
+    void build();
+    void advance();
+    // Capture the step in a hipGraph on the first advance and replay it.
+    void set_graph( bool on );
+    size_t size() const { return sims.size(); }
 };

@@ -73,6 +73,13 @@ def lib():
         "zpic_simulation_set_timing": (i, [vp, i]),
         "zpic_simulation_get_timing": (i, [vp, vp]),
         "zpic_simulation_sync": (i, [vp]),
+        "zpic_batch_new": (vp, [i]),
+        "zpic_batch_free": (None, [vp]),
+        "zpic_batch_add": (i, [vp, vp]),
+        "zpic_batch_set_graph": (i, [vp, i]),
+        "zpic_batch_advance": (i, [vp, i]),
+        "zpic_set_threads_per_tile": (i, [i]),
+        "zpic_get_threads_per_tile": (i, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -246,6 +253,52 @@ class Simulation:
         out = np.zeros(5)
         _chk(lib().zpic_simulation_get_timing(self.h, _p(out)))
         return dict(zip(("push_move", "sort", "current", "yee", "step"), out.tolist()))
+
+
+class Batch:
+    """Independent simulations advanced together: one launch per stage over
+    every instance, and one hipGraph launch per step when graph=True.
+
+    The simulations are held by reference and must stay open while the batch
+    lives; each must hold the same number of species."""
+
+    def __init__(self, sims=(), threads_per_tile=256, graph=False):
+        L = lib()
+        h = L.zpic_batch_new(threads_per_tile)
+        if not h:
+            raise ZpicError(L.zpic_last_error().decode())
+        self.h = h
+        self.sims = []
+        for s in sims:
+            self.add(s)
+        self.set_graph(graph)
+
+    def add(self, sim):
+        _chk(lib().zpic_batch_add(self.h, sim.h))
+        self.sims.append(sim)
+
+    def set_graph(self, on=True):
+        _chk(lib().zpic_batch_set_graph(self.h, 1 if on else 0))
+
+    def advance(self, n=1):
+        _chk(lib().zpic_batch_advance(self.h, n))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zpic_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def set_threads_per_tile(n):
+    """Threads per tile of the fast particle pass (multiple of 64 in [64, 512])."""
+    _chk(lib().zpic_set_threads_per_tile(n))
+
+
+def get_threads_per_tile():
+    return int(lib().zpic_get_threads_per_tile())
 
 
 def weibel(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), box=None, dt=0.07, uth=(0.1, 0.1, 0.1),
