@@ -151,6 +151,16 @@ int zpic_sum_tiles(const double *tile_values, int ntiles, double *out, int deter
 int zpic_deposit_charge(const zpic_particles *p, const zpic_geom *g, float *rho, float q,
                         int deterministic, void *stream);
 
+/* Species::dep_phasespace (species.cpp:2579-2710 1-D, :2786-3020 2-D): phase-
+ * space density of the particles into out[size0] (1-D, size1 = 0) or
+ * out[size1][size0] (2-D, quant1 > quant0).  Quantities 0..4 = x, y, ux, uy,
+ * uz; the ranges of x and y are in cells (the reference divides them by dx
+ * first); norm is the caller's normalisation.  out is cleared first.  The
+ * deterministic mode adds in the reference's particle order (one wavefront). */
+int zpic_phasespace(const zpic_particles *p, int ntiles_x, int ntiles_y, int nx, int ny, int quant0,
+                    float min0, float max0, int size0, int quant1, float min1, float max1, int size1,
+                    float norm, float *out, int deterministic, void *stream);
+
 /* vec3grid::gather (vec3grid.h:76-158): component fc (0,1,2) of the interior
  * cells into a contiguous [ny*ntiles_y][nx*ntiles_x] device array. */
 int zpic_gather_component(const float *buf, const zpic_geom *g, int ncomp, int fc, float *out, void *stream);
@@ -288,6 +298,49 @@ int zpic_species_gather(zpic_species *sp, int quant, float *host_out);
  * out[2] current, out[3] yee, out[4] whole step. */
 int zpic_simulation_set_timing(zpic_simulation *sim, int on);
 int zpic_simulation_get_timing(zpic_simulation *sim, double out[5]);
+
+/* ZDF diagnostics, written under the current working directory as the
+ * reference does.  EMF::save(field 0 E / 1 B, fc) (emf.cpp:589-655),
+ * Current::save(fc) (current.cpp:263-313), vec3grid::save(fc, filename)
+ * with minimal metadata (which 0 E, 1 B, 2 J; vec3grid.h:422-437),
+ * Species::save (species.cpp:2417-2501), save_charge (:2508-2560) and
+ * save_phasespace 1-D (:2717-2764) / 2-D (:3026-3100; pass size1 = 0 for
+ * 1-D). */
+int zpic_emf_save(zpic_simulation *sim, int field, int fc);
+int zpic_current_save(zpic_simulation *sim, int fc);
+int zpic_field_save_file(zpic_simulation *sim, int which, int fc, const char *filename);
+int zpic_species_save(zpic_species *sp);
+int zpic_species_save_charge(zpic_species *sp);
+int zpic_species_save_phasespace(zpic_species *sp, int quant0, float min0, float max0, int size0, int quant1,
+                                 float min1, float max1, int size1);
+
+/* zdf_save_grid (zdf.c:1515-1585) on host data: writes
+ * "<path>/<info.name>-<it.n:06>.zdf" holding TYPE "grid", the grid info, the
+ * iteration and one dataset of `data_type` (9 = float32, 10 = float64, ...;
+ * the zdf_data_type numbering).  Pure host code: needs no GPU. */
+typedef struct {
+    const char *name;
+    int type;              /* 0 linear, 1 log10, 2 log2 */
+    double min, max;
+    const char *label;
+    const char *units;
+} zpic_zdf_axis;
+typedef struct {
+    const char *name;
+    uint32_t ndims;
+    uint64_t count[3];
+    const char *label;
+    const char *units;
+    const zpic_zdf_axis *axis;  /* ndims entries, or NULL */
+} zpic_zdf_grid_info;
+typedef struct {
+    const char *name;      /* NULL writes "Iteration" */
+    int32_t n;
+    double t;
+    const char *time_units;
+} zpic_zdf_iteration;
+int zpic_zdf_save_grid(const void *data, int data_type, const zpic_zdf_grid_info *info,
+                       const zpic_zdf_iteration *it, const char *path);
 
 /* Device synchronisation of the simulation's stream. */
 int zpic_simulation_sync(zpic_simulation *sim);

@@ -189,3 +189,79 @@ def read_records(path):
         out[name] = np.frombuffer(buf, t, cnt, pos).copy()
         pos += cnt * t.itemsize
     return out
+
+
+def phasespace(np_tile, ix, x, u, ntiles, nx, dx, q, axes):
+    """Phase-space density of one species, restating Species::dep_phasespace
+    (species.cpp:2631-2710 for one axis, :2856-3020 for two) and its kernels
+    dep_pha1_kernel / dep_pha2_kernel (:2579-2620, :2786-2850).
+
+    np_tile: particles per tile (tile order); ix [n,2] int32, x [n,2] and
+    u [n,3] float32 in tile order; dx: float32 cell size (x, y); q: float32
+    charge; axes: [(quant, (min, max), size)] with one or two entries,
+    quant in "x", "y", "ux", "uy", "uz" (ranges of x, y in simulation units).
+
+    Arithmetic follows the reference: the normalisation is evaluated in double
+    (fabs(q) resolves to the double overload there) and rounded to float once;
+    bin coordinates and weights are float; contributions are added in
+    particle order (tile by tile), four per particle in 2-D, and never to the
+    last bin of an axis.
+    """
+    f32 = np.float32
+    qn = {"x": 0, "y": 1, "ux": 2, "uy": 3, "uz": 4}
+    dxx, dxy = f32(dx[0]), f32(dx[1])
+    cell = float(f32(dxx * dxy))
+    if len(axes) == 1:
+        (_, (a, b), size), = axes
+        norm = f32(abs(float(q)) * cell * size / float(f32(f32(b) - f32(a))))
+    else:
+        (_, (a0, b0), s0), (_, (a1, b1), s1) = axes
+        norm = f32(abs(float(q)) * cell * float(f32(f32(s0) / f32(f32(b0) - f32(a0))))
+                   * float(f32(f32(s1) / f32(f32(b1) - f32(a1)))))
+    ntx = ntiles[0]
+    tile = np.repeat(np.arange(len(np_tile)), np_tile)
+    shift = np.stack([(tile % ntx) * nx[0], (tile // ntx) * nx[1]], axis=1)
+
+    def coord(name, lo, hi, size):
+        k = qn[name]
+        lo, hi = f32(lo), f32(hi)
+        if k < 2:
+            d = dxx if k == 0 else dxy
+            lo, hi = f32(lo / d), f32(hi / d)
+            val = (shift[:, k] + ix[:, k]).astype(f32) + (x[:, k].astype(f32) + f32(0.5))
+        else:
+            val = u[:, k - 2].astype(f32)
+        rdx = f32(f32(size) / f32(hi - lo))
+        n = (val - lo) * rdx - f32(0.5)
+        kk = (n + f32(1)).astype(np.int32) - 1
+        w = (n - kk.astype(f32)).astype(f32)
+        return kk, w
+
+    if len(axes) == 1:
+        (name, (a, b), size), = axes
+        k, w = coord(name, a, b, size)
+        out = np.zeros(size, f32)
+        for kk, ww in zip(k.tolist(), w.tolist()):
+            ww = f32(ww)
+            if 0 <= kk < size - 1:
+                out[kk] += f32(f32(1) - ww) * norm
+            if 0 <= kk + 1 < size - 1:
+                out[kk + 1] += ww * norm
+        return out
+    (n0, r0, s0), (n1, r1, s1) = axes
+    k0, w0 = coord(n0, r0[0], r0[1], s0)
+    k1, w1 = coord(n1, r1[0], r1[1], s1)
+    out = np.zeros(s0 * s1, f32)
+    for a, wa, b, wb in zip(k0.tolist(), w0.tolist(), k1.tolist(), w1.tolist()):
+        wa, wb = f32(wa), f32(wb)
+        ia, ia1 = 0 <= a < s0 - 1, 0 <= a + 1 < s0 - 1
+        ib, ib1 = 0 <= b < s1 - 1, 0 <= b + 1 < s1 - 1
+        if ia and ib:
+            out[b * s0 + a] += f32(f32(1) - wa) * f32(f32(1) - wb) * norm
+        if ia1 and ib:
+            out[b * s0 + a + 1] += wa * f32(f32(1) - wb) * norm
+        if ia and ib1:
+            out[(b + 1) * s0 + a] += f32(f32(1) - wa) * wb * norm
+        if ia1 and ib1:
+            out[(b + 1) * s0 + a + 1] += wa * wb * norm
+    return out.reshape(s1, s0)

@@ -15,6 +15,8 @@
 // Usage:
 //   ref_driver dump  <out.bin> key=value ...
 //   ref_driver bench key=value ...
+//   ref_driver zdf   <outdir> key=value ...   (advance `steps` steps, then
+//                    write the diagnostics set below into outdir)
 // keys: ntiles=X,Y nx=X,Y ppc=X,Y box=X,Y dt=F uth=X,Y,Z ufl=X,Y,Z
 //       filter=none|bx<N>|by<N>|cx<N>|cy<N>  snap=s0,s1,...  steps=N warmup=N
 //       energies=1 (record energies after every step up to the last snapshot)
@@ -25,6 +27,7 @@
 #include "filter.h"
 
 #include <chrono>
+#include <unistd.h>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -41,7 +44,69 @@ class ProbeSpecies : public Species {
   public:
     using Species::Species;
     Particles * parts() { return particles; }
+    float2 cell() const { return dx; }
 };
+
+// The diagnostics set written by `zdf` mode with the reference's own writers:
+// EMF::save (all components), Current::save, the minimal-metadata
+// vec3grid::save(fc, filename), and per species save_charge, 1-D and 2-D
+// save_phasespace.  Species::save casts std::string arrays to char**
+// (species.cpp:2445-2447) and faults in zdf_add_part_info; its particle file
+// is written here with the metadata that code intends (names "x" "y" "ux"
+// "uy" "uz", labels "x" "y" "u_x" "u_y" "u_z", units "c/\\omega_n" twice then
+// "c" three times), through the same zdf::open_part_file /
+// add_quant_part_file calls and Particles::gather scales.
+static void write_zdf( Simulation & sim, std::vector<ProbeSpecies*> & sp ) {
+    for ( int f = 0; f < 2; f++ )
+        for ( int c = 0; c < 3; c++ ) sim.emf.save( f ? emf::b : emf::e, (fcomp::cart) c );
+    for ( int c = 0; c < 3; c++ ) sim.current.save( (fcomp::cart) c );
+    sim.emf.E->save( fcomp::y, "E-y-min.zdf" );
+    sim.current.J->save( fcomp::z, "J-z-min.zdf" );
+    for ( auto * s : sp ) {
+        s->save_charge();
+        const float2 box = s->get_box();
+        s->save_phasespace( phasespace::x, make_float2( 0.0f, box.x ), 64 );
+        s->save_phasespace( phasespace::y, make_float2( 0.5f, box.y - 0.3f ), 50 );
+        s->save_phasespace( phasespace::ux, make_float2( -0.3f, 0.3f ), 40 );
+        s->save_phasespace( phasespace::uz, make_float2( -1.0f, 1.0f ), 32 );
+        s->save_phasespace( phasespace::x, make_float2( 0.0f, box.x ), 32, phasespace::uz, make_float2( -1.0f, 1.0f ), 24 );
+        s->save_phasespace( phasespace::ux, make_float2( -0.25f, 0.25f ), 20, phasespace::uy, make_float2( -0.25f, 0.25f ), 20 );
+        s->save_phasespace( phasespace::y, make_float2( 0.0f, box.y ), 16, phasespace::ux, make_float2( -0.4f, 0.4f ), 12 );
+
+        char qn[5][3] = { "x", "y", "ux", "uy", "uz" };
+        char ql[5][4] = { "x", "y", "u_x", "u_y", "u_z" };
+        char qu[5][12] = { "c/\\omega_n", "c/\\omega_n", "c", "c", "c" };
+        char * quants[5], * qlabels[5], * qunits[5];
+        for ( int i = 0; i < 5; i++ ) { quants[i] = qn[i]; qlabels[i] = ql[i]; qunits[i] = qu[i]; }
+        zdf::part_info info = {
+            .name = (char *) s->name.c_str(),
+            .label = (char *) s->name.c_str(),
+            .nquants = 5,
+            .quants = quants,
+            .qlabels = qlabels,
+            .qunits = qunits,
+        };
+        Particles * p = s->parts();
+        const uint32_t np = p->np_total();
+        info.np = np;
+        zdf::iteration it = { .n = s->get_iter(), .t = s->get_iter() * s->get_dt(), .time_units = (char *) "1/\\omega_n" };
+        zdf::file f;
+        zdf::open_part_file( f, info, it, "PARTICLES/" + s->name );
+        std::vector<float> data( np );
+        const float2 d = s->cell();
+        p->gather( part::quant::x, data.data(), make_float2( d.x, 0 ) );
+        zdf::add_quant_part_file( f, "x", data.data(), np );
+        p->gather( part::quant::y, data.data(), make_float2( d.y, 0 ) );
+        zdf::add_quant_part_file( f, "y", data.data(), np );
+        p->gather( part::quant::ux, data.data() );
+        zdf::add_quant_part_file( f, "ux", data.data(), np );
+        p->gather( part::quant::uy, data.data() );
+        zdf::add_quant_part_file( f, "uy", data.data(), np );
+        p->gather( part::quant::uz, data.data() );
+        zdf::add_quant_part_file( f, "uz", data.data(), np );
+        zdf::close_file( f );
+    }
+}
 
 struct Args {
     uint2 ntiles = make_uint2( 8, 8 );
@@ -155,7 +220,8 @@ int main( int argc, char ** argv ) {
     if ( argc < 2 ) { fprintf( stderr, "usage: ref_driver dump|bench ...\n" ); return 2; }
     std::string mode = argv[1];
     bool dump = ( mode == "dump" );
-    Args a = parse( argc, argv, dump ? 3 : 2 );
+    bool zdfmode = ( mode == "zdf" );
+    Args a = parse( argc, argv, ( dump || zdfmode ) ? 3 : 2 );
 
     Simulation sim( a.ntiles, a.nx, a.box, (double) a.dt );
     set_filter( sim, a.filter );
@@ -169,6 +235,13 @@ int main( int argc, char ** argv ) {
     sim.add_species( positrons );
 
     std::vector<ProbeSpecies*> sp = { &electrons, &positrons };
+
+    if ( zdfmode ) {
+        for ( int i = 0; i < a.steps; i++ ) sim.advance();
+        if ( chdir( argv[2] ) ) { perror( "chdir" ); return 1; }
+        write_zdf( sim, sp );
+        return 0;
+    }
 
     if ( dump ) {
         FILE * fp = fopen( argv[2], "wb" );

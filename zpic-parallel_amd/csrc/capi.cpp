@@ -326,3 +326,95 @@ int zpic_batch_advance( zpic_batch * b, int n ) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int zpic_emf_save( zpic_simulation * s, int field, int fc ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        if ( field < 0 || field > 1 || fc < 0 || fc > 2 ) return zpic::set_error( ZPIC_EINVAL, "bad field or component" );
+        s->sim->emf.save( field ? emf::b : emf::e, static_cast<fcomp::cart>( fc ) );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_current_save( zpic_simulation * s, int fc ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        if ( fc < 0 || fc > 2 ) return zpic::set_error( ZPIC_EINVAL, "bad component" );
+        s->sim->current.save( static_cast<fcomp::cart>( fc ) );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_field_save_file( zpic_simulation * s, int which, int fc, const char * filename ) {
+    return guarded( [&] {
+        if ( !s || !filename ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
+        if ( which < 0 || which > 2 || fc < 0 || fc > 2 ) return zpic::set_error( ZPIC_EINVAL, "bad field or component" );
+        vec3grid<float3> * f = which == 0 ? s->sim->emf.E : which == 1 ? s->sim->emf.B : s->sim->current.J;
+        f->save( static_cast<fcomp::cart>( fc ), std::string( filename ) );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_species_save( zpic_species * h ) {
+    return guarded( [&] {
+        if ( !h || !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        h->sp->save();
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_species_save_charge( zpic_species * h ) {
+    return guarded( [&] {
+        if ( !h || !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        h->sp->save_charge();
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_species_save_phasespace( zpic_species * h, int q0, float min0, float max0, int size0, int q1, float min1,
+                                  float max1, int size1 ) {
+    return guarded( [&] {
+        if ( !h || !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        if ( q0 < 0 || q0 > 4 || size0 < 1 || !( max0 > min0 ) ) return zpic::set_error( ZPIC_EINVAL, "bad axis 0" );
+        const auto p0 = static_cast<phasespace::quant>( q0 );
+        if ( size1 <= 0 ) {
+            h->sp->save_phasespace( p0, make_float2( min0, max0 ), size0 );
+            return ZPIC_OK;
+        }
+        if ( q1 <= q0 || q1 > 4 || !( max1 > min1 ) )
+            return zpic::set_error( ZPIC_EINVAL, "bad axis 1 (its quantity must come after axis 0's)" );
+        h->sp->save_phasespace( p0, make_float2( min0, max0 ), size0, static_cast<phasespace::quant>( q1 ),
+                                make_float2( min1, max1 ), size1 );
+        return ZPIC_OK;
+    } );
+}
+
+}  // extern "C"
+
+extern "C" int zpic_zdf_save_grid( const void * data, int data_type, const zpic_zdf_grid_info * info,
+                                   const zpic_zdf_iteration * it, const char * path ) {
+    if ( !data || !info || !it || !path ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
+    if ( info->ndims < 1 || info->ndims > zdf::max_dims || zdf::size_of( static_cast<zdf::data_type>( data_type ) ) == 0 )
+        return zpic::set_error( ZPIC_EINVAL, "bad grid dimensions or data type" );
+    zdf::grid_axis axis[zdf::max_dims];
+    zdf::grid_info g{};
+    g.name = const_cast<char *>( info->name );
+    g.ndims = info->ndims;
+    for ( unsigned i = 0; i < info->ndims; i++ ) g.count[i] = info->count[i];
+    g.label = const_cast<char *>( info->label );
+    g.units = const_cast<char *>( info->units );
+    if ( info->axis ) {
+        for ( unsigned i = 0; i < info->ndims; i++ ) {
+            const zpic_zdf_axis & a = info->axis[i];
+            axis[i] = zdf::grid_axis{ const_cast<char *>( a.name ), static_cast<zdf::axis_type>( a.type ), a.min, a.max,
+                                      const_cast<char *>( a.label ), const_cast<char *>( a.units ) };
+        }
+        g.axis = axis;
+    }
+    zdf::iteration iter{ const_cast<char *>( it->name ), it->n, it->t, const_cast<char *>( it->time_units ) };
+    if ( zdf::save_grid( data, static_cast<zdf::data_type>( data_type ), g, iter, path ) != 1 )
+        return zpic::set_error( ZPIC_EINVAL, std::string( "unable to write ZDF grid file under " ) + path );
+    return ZPIC_OK;
+}

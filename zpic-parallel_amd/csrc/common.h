@@ -16,6 +16,19 @@ inline int hip_error( hipError_t e, const char * where ) {
     return set_error( ZPIC_EHIP, std::string( where ) + ": " + hipGetErrorString( e ) );
 }
 
+// Global-memory (address space 1) view of a buffer.  Pointers that may come
+// from an argument table in memory are generic to the compiler, and generic
+// accesses become flat instructions, which also count against the LDS wait
+// counter: every LDS wait would then wait for the outstanding loads too.
+// The host pass of hipcc also parses device code, for a target without that
+// address space: there the view is a plain pointer.
+#if defined( __HIP_DEVICE_COMPILE__ )
+template <class T> using gptr = __attribute__( ( address_space( 1 ) ) ) T *;
+#else
+template <class T> using gptr = T *;
+#endif
+template <class T> __device__ inline gptr<T> gbl( T * p ) { return (gptr<T>) p; }
+
 }  // namespace zpic
 
 #define ZPIC_HIP_TRY( call )                                               \

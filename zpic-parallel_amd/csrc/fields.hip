@@ -66,8 +66,8 @@ struct YeeArgs {
 template <class A>
 __device__ inline int select_instance( A & a, const A * tab, const int2 * map ) {
     if ( map ) {
-        const int2 r = map[blockIdx.x];
-        a = tab[r.x];
+        const int2 r = gbl( map )[blockIdx.x];
+        a = gbl( tab )[r.x];
         return r.y;
     }
     return blockIdx.x;
@@ -81,15 +81,15 @@ constexpr int GC_BLOCK = 128;
 template <int NC>
 __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( GcArgs a, const GcArgs * tab, const int2 * map ) {
     const int t = select_instance( a, tab, map );
-    float * __restrict__ buf = a.buf;
+    const gptr<float> buf = gbl( a.buf );
     const G g = a.g;
     const int px = a.px, py = a.py;
     const int tx = t % g.ntx, ty = t / g.ntx;
-    float * loc = buf + (size_t) t * g.vol * NC;
+    const gptr<float> loc = buf + (size_t) t * g.vol * NC;
     const int lo = nbr( tx, ty, -1, 0, g, px, py ), hi = nbr( tx, ty, +1, 0, g, px, py );
     const int E = g.ext_x;
     if ( lo >= 0 ) {
-        const float * src = buf + (size_t) lo * g.vol * NC;
+        const gptr<const float> src = buf + (size_t) lo * g.vol * NC;
         for ( int k = threadIdx.x; k < g.ext_y * g.gu * NC; k += blockDim.x ) {
             const int c = k % NC, e = k / NC, i = e % g.gu, j = e / g.gu;
             loc[( g.gl + i + j * E ) * NC + c] += src[( g.gl + g.nx + i + j * E ) * NC + c];
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( GcArgs a, const GcArgs
     }
     __syncthreads();
     if ( hi >= 0 ) {
-        const float * src = buf + (size_t) hi * g.vol * NC;
+        const gptr<const float> src = buf + (size_t) hi * g.vol * NC;
         for ( int k = threadIdx.x; k < g.ext_y * g.gl * NC; k += blockDim.x ) {
             const int c = k % NC, e = k / NC, i = e % g.gl, j = e / g.gl;
             loc[( g.nx + i + j * E ) * NC + c] += src[( i + j * E ) * NC + c];
@@ -109,15 +109,15 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( GcArgs a, const GcArgs
 template <int NC>
 __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( GcArgs a, const GcArgs * tab, const int2 * map ) {
     const int t = select_instance( a, tab, map );
-    float * __restrict__ buf = a.buf;
+    const gptr<float> buf = gbl( a.buf );
     const G g = a.g;
     const int px = a.px, py = a.py;
     const int tx = t % g.ntx, ty = t / g.ntx;
-    float * loc = buf + (size_t) t * g.vol * NC;
+    const gptr<float> loc = buf + (size_t) t * g.vol * NC;
     const int lo = nbr( tx, ty, 0, -1, g, px, py ), hi = nbr( tx, ty, 0, +1, g, px, py );
     const int E = g.ext_x;
     if ( lo >= 0 ) {
-        const float * src = buf + (size_t) lo * g.vol * NC;
+        const gptr<const float> src = buf + (size_t) lo * g.vol * NC;
         for ( int k = threadIdx.x; k < g.gu * E * NC; k += blockDim.x ) {
             const int c = k % NC, e = k / NC, i = e % E, j = e / E;
             loc[( i + ( g.gl + j ) * E ) * NC + c] += src[( i + ( g.gl + g.ny + j ) * E ) * NC + c];
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( GcArgs a, const GcArgs
     }
     __syncthreads();
     if ( hi >= 0 ) {
-        const float * src = buf + (size_t) hi * g.vol * NC;
+        const gptr<const float> src = buf + (size_t) hi * g.vol * NC;
         for ( int k = threadIdx.x; k < g.gl * E * NC; k += blockDim.x ) {
             const int c = k % NC, e = k / NC, i = e % E, j = e / E;
             loc[( i + ( g.ny + j ) * E ) * NC + c] += src[( i + j * E ) * NC + c];
@@ -137,22 +137,22 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( GcArgs a, const GcArgs
 template <int NC>
 __global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( GcArgs a, const GcArgs * tab, const int2 * map ) {
     const int t = select_instance( a, tab, map );
-    float * __restrict__ buf = a.buf;
+    const gptr<float> buf = gbl( a.buf );
     const G g = a.g;
     const int px = a.px, py = a.py;
     const int tx = t % g.ntx, ty = t / g.ntx;
-    float * loc = buf + (size_t) t * g.vol * NC;
+    const gptr<float> loc = buf + (size_t) t * g.vol * NC;
     const int lo = nbr( tx, ty, -1, 0, g, px, py ), hi = nbr( tx, ty, +1, 0, g, px, py );
     const int E = g.ext_x;
     if ( lo >= 0 ) {
-        const float * src = buf + (size_t) lo * g.vol * NC;
+        const gptr<const float> src = buf + (size_t) lo * g.vol * NC;
         for ( int k = threadIdx.x; k < g.ext_y * g.gl * NC; k += blockDim.x ) {
             const int c = k % NC, e = k / NC, i = e % g.gl, j = e / g.gl;
             loc[( i + j * E ) * NC + c] = src[( g.nx + i + j * E ) * NC + c];
         }
     }
     if ( hi >= 0 ) {
-        const float * src = buf + (size_t) hi * g.vol * NC;
+        const gptr<const float> src = buf + (size_t) hi * g.vol * NC;
         for ( int k = threadIdx.x; k < g.ext_y * g.gu * NC; k += blockDim.x ) {
             const int c = k % NC, e = k / NC, i = e % g.gu, j = e / g.gu;
             loc[( g.gl + g.nx + i + j * E ) * NC + c] = src[( g.gl + i + j * E ) * NC + c];
@@ -164,22 +164,22 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( GcArgs a, const GcArg
 template <int NC>
 __global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_y( GcArgs a, const GcArgs * tab, const int2 * map ) {
     const int t = select_instance( a, tab, map );
-    float * __restrict__ buf = a.buf;
+    const gptr<float> buf = gbl( a.buf );
     const G g = a.g;
     const int px = a.px, py = a.py;
     const int tx = t % g.ntx, ty = t / g.ntx;
-    float * loc = buf + (size_t) t * g.vol * NC;
+    const gptr<float> loc = buf + (size_t) t * g.vol * NC;
     const int lo = nbr( tx, ty, 0, -1, g, px, py ), hi = nbr( tx, ty, 0, +1, g, px, py );
     const int E = g.ext_x;
     if ( lo >= 0 ) {
-        const float * src = buf + (size_t) lo * g.vol * NC;
+        const gptr<const float> src = buf + (size_t) lo * g.vol * NC;
         for ( int k = threadIdx.x; k < g.gl * E * NC; k += blockDim.x ) {
             const int c = k % NC, e = k / NC, i = e % E, j = e / E;
             loc[( i + j * E ) * NC + c] = src[( i + ( g.ny + j ) * E ) * NC + c];
         }
     }
     if ( hi >= 0 ) {
-        const float * src = buf + (size_t) hi * g.vol * NC;
+        const gptr<const float> src = buf + (size_t) hi * g.vol * NC;
         for ( int k = threadIdx.x; k < g.gu * E * NC; k += blockDim.x ) {
             const int c = k % NC, e = k / NC, i = e % E, j = e / E;
             loc[( i + ( g.gl + g.ny + j ) * E ) * NC + c] = src[( i + ( g.gl + j ) * E ) * NC + c];
@@ -194,11 +194,11 @@ constexpr int TILE_BLOCK = 256;
 __global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( K3Args args, const K3Args * tab, const int2 * map ) {
     extern __shared__ float A[];
     const int t = select_instance( args, tab, map );
-    float * __restrict__ buf = args.buf;
+    const gptr<float> buf = gbl( args.buf );
     const G g = args.g;
     const int dir = args.dir;
     const float a = args.a, b = args.b, c = args.c;
-    float * tb = buf + (size_t) t * g.vol * 3;
+    const gptr<float> tb = buf + (size_t) t * g.vol * 3;
     for ( int k = threadIdx.x; k < 3 * g.vol; k += blockDim.x ) A[k] = tb[k];
     __syncthreads();
     const int E = g.ext_x;
@@ -218,17 +218,17 @@ __global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( K3Args args, const K3
 __global__ void __launch_bounds__( TILE_BLOCK ) k_yee( YeeArgs a, const YeeArgs * tab, const int2 * map ) {
     extern __shared__ float sm[];
     const int t = select_instance( a, tab, map );
-    float * __restrict__ Eg = a.E;
-    float * __restrict__ Bg = a.B;
-    const float * __restrict__ Jg = a.J;
+    const gptr<float> Eg = gbl( a.E );
+    const gptr<float> Bg = gbl( a.B );
+    const gptr<const float> Jg = gbl( a.J );
     const G g = a.g;
     const float dtdx = a.dtdx, dtdy = a.dtdy, dt = a.dt;
     const int vol = g.vol, ys = g.ext_x;
     float * Ex = sm; float * Ey = Ex + vol; float * Ez = Ey + vol;
     float * Bx = Ez + vol; float * By = Bx + vol; float * Bz = By + vol;
-    float * Et = Eg + (size_t) t * vol * 3;
-    float * Bt = Bg + (size_t) t * vol * 3;
-    const float * Jt = Jg ? Jg + (size_t) t * vol * 3 : nullptr;
+    const gptr<float> Et = Eg + (size_t) t * vol * 3;
+    const gptr<float> Bt = Bg + (size_t) t * vol * 3;
+    const gptr<const float> Jt = a.J ? Jg + (size_t) t * vol * 3 : nullptr;
     for ( int k = threadIdx.x; k < 3 * vol; k += blockDim.x ) {
         const int el = k / 3, c = k - 3 * el;
         Ex[c * vol + el] = Et[k];
@@ -382,9 +382,10 @@ struct FillArgs {
 };
 
 __global__ void k_fill_batch( const FillArgs * tab ) {
-    const FillArgs a = tab[blockIdx.y];
+    const FillArgs a = gbl( tab )[blockIdx.y];
+    const gptr<float> buf = gbl( a.buf );
     for ( size_t k = blockIdx.x * (size_t) blockDim.x + threadIdx.x; k < a.n; k += (size_t) gridDim.x * blockDim.x )
-        a.buf[k] = 0.0f;
+        buf[k] = 0.0f;
 }
 
 // One batched launch: a device table of per-entry arguments and the

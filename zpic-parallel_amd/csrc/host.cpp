@@ -747,3 +747,210 @@ void Batch::advance() {
     }
     for ( auto * s : sims ) s->note_step();
 }
+
+// ------------------------------------------------------------- Diagnostics
+void EMF::save( emf::field const field, fcomp::cart const fc ) {
+    if ( fc < 0 || fc > 2 ) {
+        std::cerr << "(*error*) Invalid field component (fc) selected, returning" << std::endl;
+        return;
+    }
+    static const char comp[] = { 'x', 'y', 'z' };
+    char vfname[16], vflabel[16];
+    vec3grid<float3> * f;
+    switch ( field ) {
+    case emf::e: f = E; break;
+    case emf::b: f = B; break;
+    default:
+        std::cerr << "(*error*) Invalid field type selected, returning..." << std::endl;
+        return;
+    }
+    const char F = ( field == emf::e ) ? 'E' : 'B';
+    std::snprintf( vfname, 16, "%c%c", F, comp[fc] );
+    std::snprintf( vflabel, 16, "%c_%c", F, comp[fc] );
+
+    zdf::grid_axis axis[2];
+    axis[0] = zdf::grid_axis{ (char *) "x", zdf::zdf_linear, 0.0 + moving_window.motion(), box.x + moving_window.motion(),
+                              (char *) "x", (char *) "c/\\omega_n" };
+    axis[1] = zdf::grid_axis{ (char *) "y", zdf::zdf_linear, 0.0, box.y, (char *) "y", (char *) "c/\\omega_n" };
+    zdf::grid_info info{};
+    info.name = vfname;
+    info.ndims = 2;
+    info.label = vflabel;
+    info.units = (char *) "m_e c \\omega_n e^{-1}";
+    info.axis = axis;
+    zdf::iteration iteration{ nullptr, iter, iter * dt, (char *) "1/\\omega_n" };
+    f->save( fc, info, iteration, "EMF" );
+}
+
+void Current::save( fcomp::cart const jc ) {
+    if ( jc < 0 || jc > 2 ) {
+        std::cerr << "(*error*) Invalid current component (jc) selected, returning" << std::endl;
+        return;
+    }
+    static const char comp[] = { 'x', 'y', 'z' };
+    char vfname[16], vflabel[16];
+    std::snprintf( vfname, 16, "J%c", comp[jc] );
+    std::snprintf( vflabel, 16, "J_%c", comp[jc] );
+    // The reference shifts the lower y bound (not the upper x bound) with the
+    // window (current.cpp:282-296); kept as is.
+    zdf::grid_axis axis[2];
+    axis[0] = zdf::grid_axis{ (char *) "x", zdf::zdf_linear, 0.0 + moving_window.motion(), box.x, (char *) "x",
+                              (char *) "c/\\omega_n" };
+    axis[1] = zdf::grid_axis{ (char *) "y", zdf::zdf_linear, 0.0 + moving_window.motion(), box.y, (char *) "y",
+                              (char *) "c/\\omega_n" };
+    zdf::grid_info info{};
+    info.name = vfname;
+    info.ndims = 2;
+    info.label = vflabel;
+    info.units = (char *) "e \\omega_n^2 / c";
+    info.axis = axis;
+    zdf::iteration iteration{ nullptr, iter, iter * dt, (char *) "1/\\omega_n" };
+    J->save( jc, info, iteration, "CURRENT" );
+}
+
+void Species::save() const {
+    const std::string path = "PARTICLES/" + name;
+    static const char * quants[] = { "x", "y", "ux", "uy", "uz" };
+    static const char * qlabels[] = { "x", "y", "u_x", "u_y", "u_z" };
+    static const char * qunits[] = { "c/\\omega_n", "c/\\omega_n", "c", "c", "c" };
+    zdf::iteration iter_info{ nullptr, iter, (double) ( iter * dt ), (char *) "1/\\omega_n" };
+    zdf::part_info info{};
+    info.name = (char *) name.c_str();
+    info.label = (char *) name.c_str();
+    info.nquants = 5;
+    info.quants = (char **) quants;
+    info.qlabels = (char **) qlabels;
+    info.qunits = (char **) qunits;
+    const uint32_t np = np_total();
+    info.np = np;
+    zdf::file f;
+    if ( zdf::open_part_file( f, info, iter_info, path ) != 1 )
+        throw std::runtime_error( "(*error*) Unable to open particle file under " + path );
+    std::vector<float> data( np );
+    for ( int q = 0; q < 5; q++ ) {
+        if ( np > 0 ) gather( static_cast<part::quant>( q ), data.data() );
+        zdf::add_quant_part_file( f, quants[q], data.data(), np );
+    }
+    f.close();
+}
+
+void Species::save_charge() const {
+    bnd<unsigned int> gc;
+    gc.x = { 0, 1 };
+    gc.y = { 0, 1 };
+    grid<float> charge( particles->ntiles, particles->nx, gc );
+    charge.zero();
+    deposit_charge( charge );
+    charge.add_from_gc();
+
+    zdf::grid_axis axis[2];
+    axis[0] = zdf::grid_axis{ (char *) "x", zdf::zdf_linear, 0. + moving_window.motion(), box.x + moving_window.motion(),
+                              (char *) "x", (char *) "c/\\omega_n" };
+    axis[1] = zdf::grid_axis{ (char *) "y", zdf::zdf_linear, 0., box.y, (char *) "y", (char *) "c/\\omega_n" };
+    const std::string grid_name = name + "-charge";
+    const std::string grid_label = name + " \\rho";
+    zdf::grid_info info{};
+    info.name = (char *) grid_name.c_str();
+    info.label = (char *) grid_label.c_str();
+    info.units = (char *) "n_e";
+    info.axis = axis;
+    zdf::iteration iter_info{ (char *) "ITERATION", iter, (double) ( iter * dt ), (char *) "1/\\omega_n" };
+    charge.save( info, iter_info, "CHARGE/" + name );
+}
+
+// The normalisation is evaluated as the reference writes it: fabs(q) is the
+// double-precision overload there, so the products run in double and the
+// result is rounded to float once (species.cpp:2636-2637, 2880-2882).
+void Species::dep_phasespace( float * const d_data, phasespace::quant quant, float2 range, unsigned const size ) const {
+    const float norm = std::fabs( (double) q ) * ( dx.x * dx.y ) * size / ( range.y - range.x );
+    if ( quant == phasespace::x ) { range.y /= dx.x; range.x /= dx.x; }
+    if ( quant == phasespace::y ) { range.y /= dx.y; range.x /= dx.y; }
+    zpic_particles p = particles->view();
+    check( zpic_phasespace( &p, particles->ntiles.x, particles->ntiles.y, particles->nx.x, particles->nx.y, quant, range.x,
+                            range.y, size, -1, 0.f, 0.f, 0, norm, d_data, deterministic, stream() ) );
+}
+
+void Species::dep_phasespace( float * const d_data, phasespace::quant quant0, float2 range0, unsigned const size0,
+                              phasespace::quant quant1, float2 range1, unsigned const size1 ) const {
+    const float norm = std::fabs( (double) q ) * ( dx.x * dx.y ) * ( size0 / ( range0.y - range0.x ) ) *
+                       ( size1 / ( range1.y - range1.x ) );
+    auto scale = [&]( phasespace::quant qn, float2 & r ) {
+        if ( qn == phasespace::x ) { r.y /= dx.x; r.x /= dx.x; }
+        if ( qn == phasespace::y ) { r.y /= dx.y; r.x /= dx.y; }
+    };
+    scale( quant0, range0 );
+    scale( quant1, range1 );
+    zpic_particles p = particles->view();
+    check( zpic_phasespace( &p, particles->ntiles.x, particles->ntiles.y, particles->nx.x, particles->nx.y, quant0,
+                            range0.x, range0.y, size0, quant1, range1.x, range1.y, size1, norm, d_data, deterministic,
+                            stream() ) );
+}
+
+void Species::save_phasespace( phasespace::quant quant, float2 const range, int const size ) const {
+    std::string qname, qlabel, qunits;
+    phasespace::qinfo( quant, qname, qlabel, qunits );
+    zdf::grid_axis axis{ (char *) qname.c_str(), zdf::zdf_linear, range.x, range.y, (char *) qlabel.c_str(),
+                         (char *) qunits.c_str() };
+    if ( quant == phasespace::x ) {
+        axis.min += moving_window.motion();
+        axis.max += moving_window.motion();
+    }
+    const std::string pha_name = name + "-" + qname;
+    const std::string pha_label = name + "\\,(" + qlabel + ")";
+    zdf::grid_info info{};
+    info.name = (char *) pha_name.c_str();
+    info.ndims = 1;
+    info.label = (char *) pha_label.c_str();
+    info.units = (char *) "n_e";
+    info.axis = &axis;
+    info.count[0] = size;
+    zdf::iteration iter_info{ (char *) "ITERATION", iter, (double) ( iter * dt ), (char *) "1/\\omega_n" };
+
+    float * d = zpic::device_alloc<float>( size );
+    std::vector<float> h( size );
+    dep_phasespace( d, quant, range, size );
+    to_host( h.data(), d, size );
+    sync();
+    zpic::device_free( d );
+    zdf::save_grid( h.data(), info, iter_info, "PHASESPACE/" + name );
+}
+
+void Species::save_phasespace( phasespace::quant quant0, float2 const range0, int const size0, phasespace::quant quant1,
+                               float2 const range1, int const size1 ) const {
+    if ( quant0 >= quant1 ) {
+        std::cerr << "(*error*) for 2D phasespaces, the 2nd quantity must be indexed higher than the first one\n";
+        return;
+    }
+    std::string qname0, qlabel0, qunits0, qname1, qlabel1, qunits1;
+    phasespace::qinfo( quant0, qname0, qlabel0, qunits0 );
+    phasespace::qinfo( quant1, qname1, qlabel1, qunits1 );
+    zdf::grid_axis axis[2] = {
+        zdf::grid_axis{ (char *) qname0.c_str(), zdf::zdf_linear, range0.x, range0.y, (char *) qlabel0.c_str(),
+                        (char *) qunits0.c_str() },
+        zdf::grid_axis{ (char *) qname1.c_str(), zdf::zdf_linear, range1.x, range1.y, (char *) qlabel1.c_str(),
+                        (char *) qunits1.c_str() } };
+    if ( quant0 == phasespace::x ) {
+        axis[0].min += moving_window.motion();
+        axis[0].max += moving_window.motion();
+    }
+    const std::string pha_name = name + "-" + qname0 + qname1;
+    const std::string pha_label = name + " \\,(" + qlabel0 + "\\rm{-}" + qlabel1 + ")";
+    zdf::grid_info info{};
+    info.name = (char *) pha_name.c_str();
+    info.ndims = 2;
+    info.count[0] = static_cast<unsigned>( size0 );
+    info.count[1] = static_cast<unsigned>( size1 );
+    info.label = (char *) pha_label.c_str();
+    info.units = (char *) "n_e";
+    info.axis = axis;
+    zdf::iteration iter_info{ (char *) "ITERATION", iter, (double) ( iter * dt ), (char *) "1/\\omega_n" };
+
+    const size_t n = (size_t) size0 * size1;
+    float * d = zpic::device_alloc<float>( n );
+    std::vector<float> h( n );
+    dep_phasespace( d, quant0, range0, size0, quant1, range1, size1 );
+    to_host( h.data(), d, n );
+    sync();
+    zpic::device_free( d );
+    zdf::save_grid( h.data(), info, iter_info, "PHASESPACE/" + name );
+}

@@ -310,14 +310,35 @@ struct PartArgs {
     SortView S;
 };
 
+struct PartG {
+    gptr<int> np;
+    gptr<int> off;
+    gptr<int2> ix;
+    gptr<float2> x;
+    gptr<float3> u;
+};
+__device__ inline PartG gview( const PartView & v ) { return PartG{ gbl( v.np ), gbl( v.off ), gbl( v.ix ), gbl( v.x ), gbl( v.u ) }; }
+
+struct SortG {
+    gptr<int> idx;
+    gptr<int> npt;
+    gptr<int> new_np;
+    gptr<int> seg;
+    gptr<int> tmp_np;
+    gptr<int> total;
+};
+__device__ inline SortG gview( const SortView & v ) {
+    return SortG{ gbl( v.idx ), gbl( v.npt ), gbl( v.new_np ), gbl( v.seg ), gbl( v.tmp_np ), gbl( v.total ) };
+}
+
 // Single-instance launches pass map == nullptr and use `a` with tile =
 // blockIdx.x; batched launches pass a table of per-instance arguments and a
 // map from block to (instance, tile).
 template <class A>
 __device__ inline int select_instance( A & a, const A * tab, const int2 * map ) {
     if ( map ) {
-        const int2 r = map[blockIdx.x];
-        a = tab[r.x];
+        const int2 r = gbl( map )[blockIdx.x];
+        a = gbl( tab )[r.x];
         return r.y;
     }
     return blockIdx.x;
@@ -360,8 +381,8 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
     __shared__ int s_nl[FAST_BLOCK / WAVE];
 
     if constexpr ( PUSH ) {
-        const float * Et = reinterpret_cast<const float *>( E + (size_t) t * vol );
-        const float * Bt = reinterpret_cast<const float *>( B + (size_t) t * vol );
+        const gptr<const float> Et = gbl( reinterpret_cast<const float *>( E + (size_t) t * vol ) );
+        const gptr<const float> Bt = gbl( reinterpret_cast<const float *>( B + (size_t) t * vol ) );
         for ( int k = tid; k < 3 * vol; k += nthr ) {
             const int el = k / 3, c = k - 3 * el;
             Ex[c * vol + el] = Et[k];
@@ -374,16 +395,17 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
     }
     __syncthreads();
 
-    const int off = P.off[t];
-    const int np = P.np[t];
+    const int off = gbl( P.off )[t];
+    const int np = gbl( P.np )[t];
     const int chunk = ( ( np + W - 1 ) / W + WAVE - 1 ) & ~( WAVE - 1 );
     const int beg = w * chunk;
     const int end = min( np, beg + chunk );
 
     // tile base pointers (uniform) + 32-bit particle index
-    int2 * __restrict__ pix = P.ix + off;
-    float2 * __restrict__ px = P.x + off;
-    float3 * __restrict__ pu = P.u + off;
+    const gptr<int2> pix = gbl( P.ix + off );
+    const gptr<float2> px = gbl( P.x + off );
+    const gptr<float3> pu = gbl( P.u + off );
+    const gptr<int> sidx = gbl( S.idx + off );
 
     const float * e0x = Ex + g.off; const float * e0y = Ey + g.off; const float * e0z = Ez + g.off;
     const float * b0x = Bx + g.off; const float * b0y = By + g.off; const float * b0z = Bz + g.off;
@@ -482,7 +504,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
             const unsigned long long mask = __ballot( leave );
             if ( leave ) {
                 const int pos = nl + __popcll( mask & lanes_below() );
-                S.idx[off + beg + pos] = ( i << 4 ) | dir;
+                sidx[beg + pos] = ( i << 4 ) | dir;
                 atomicAdd( &s_cnt[dir], 1 );
             }
             nl += __popcll( mask );
@@ -503,7 +525,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
     if constexpr ( MOVE ) {
         // flush: J(tile) += sum of the waves' J (LDS), guard cells included
         // (species.cpp:1488-1492)
-        float * Jt = reinterpret_cast<float *>( J + (size_t) t * vol );
+        const gptr<float> Jt = gbl( reinterpret_cast<float *>( J + (size_t) t * vol ) );
         for ( int k = tid; k < 3 * g.ext_x * g.ext_y; k += nthr ) {
             const int el = k / 3, c = k - 3 * el;
             float s = Jall[c * vol + el];
@@ -511,22 +533,24 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
             Jt[k] += s;
         }
         if ( tid == 0 ) {
+            const gptr<int> seg = gbl( S.seg + 16 * t );
+            const gptr<int> npt = gbl( S.npt + 9 * t );
             int tot = 0;
             for ( int v = 0; v < W; v++ ) {
-                S.seg[16 * t + v] = s_nl[v];
+                seg[v] = s_nl[v];
                 tot += s_nl[v];
             }
-            for ( int v = W; v < MAX_SEG; v++ ) S.seg[16 * t + v] = 0;
-            S.seg[16 * t + MAX_SEG] = chunk;
-            for ( int d = 0; d < 9; d++ ) S.npt[9 * t + d] = s_cnt[d];
-            S.npt[9 * t + 4] = np - tot;
+            for ( int v = W; v < MAX_SEG; v++ ) seg[v] = 0;
+            seg[MAX_SEG] = chunk;
+            for ( int d = 0; d < 9; d++ ) npt[d] = s_cnt[d];
+            npt[4] = np - tot;
         }
     }
     if constexpr ( PUSH ) {
         if ( tid == 0 ) {
             double s = 0.0;
             for ( int v = 0; v < W; v++ ) s += s_en[v];
-            tile_energy[t] = s;
+            gbl( tile_energy )[t] = s;
         }
     }
 }
@@ -568,15 +592,16 @@ struct SortArgs {
 // the tile gathers the direction counts its 9 source tiles send to it.
 // Batched launches index the instance with blockIdx.y.
 __global__ void k_sort_newnp( SortArgs a, const SortArgs * tab ) {
-    if ( tab ) a = tab[blockIdx.y];
+    if ( tab ) a = gbl( tab )[blockIdx.y];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if ( t >= a.ntiles ) return;
+    const SortG S = gview( a.S );
     int n = 0;
     for ( int d = 0; d < 9; d++ ) {
         const int s = source_of( t, d, a.q );
-        if ( s >= 0 ) n += a.S.npt[9 * s + d];
+        if ( s >= 0 ) n += S.npt[9 * s + d];
     }
-    a.S.new_np[t] = n;
+    S.new_np[t] = n;
 }
 
 // Exclusive scan of the new counts into the new tile offsets
@@ -584,9 +609,9 @@ __global__ void k_sort_newnp( SortArgs a, const SortArgs * tab ) {
 // each thread owns a contiguous run of tiles.
 constexpr int SCAN_BLOCK = 1024;
 __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const SortArgs * tab ) {
-    if ( tab ) a = tab[blockIdx.x];
-    const SortView S = a.S;
-    int * __restrict__ new_off = a.T.off;
+    if ( tab ) a = gbl( tab )[blockIdx.x];
+    const SortG S = gview( a.S );
+    const gptr<int> new_off = gbl( a.T.off );
     const int ntiles = a.ntiles;
     __shared__ int s_sum[SCAN_BLOCK];
     const int tid = threadIdx.x;
@@ -613,7 +638,7 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const 
 
 // Stayers that move to the staging buffer because the tile's range moved
 // (copy_out, particles.cpp:409-423).
-__device__ inline int shifted_stayers( int t, const int * old_off, const int * new_off, const SortView & S ) {
+__device__ inline int shifted_stayers( int t, gptr<int> old_off, gptr<int> new_off, const SortG & S ) {
     const int o = old_off[t], no = new_off[t], n0 = S.npt[9 * t + 4], nn = S.new_np[t];
     int ns;
     if ( no >= o ) ns = no - o;
@@ -622,7 +647,7 @@ __device__ inline int shifted_stayers( int t, const int * old_off, const int * n
 }
 
 // Position j of the tile's ordered leaver list.
-__device__ inline int leaver_at( int j, const int * seg_pre, int chunk, const int * idx, int off ) {
+__device__ inline int leaver_at( int j, const int * seg_pre, int chunk, gptr<int> idx, int off ) {
     int w = 0;
     while ( w + 1 < MAX_SEG && seg_pre[w + 1] <= j ) w++;
     return idx[off + w * chunk + ( j - seg_pre[w] )];
@@ -640,8 +665,8 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
     __shared__ int s_pre[MAX_SEG + 1];
     __shared__ int s_hdr[4];
     const int s = select_instance( a, tab, map );
-    const PartView P = a.P, T = a.T;
-    const SortView S = a.S;
+    const PartG P = gview( a.P ), T = gview( a.T );
+    const SortG S = gview( a.S );
     const SortGeo q = a.q;
     const int tid = threadIdx.x;
 
@@ -786,8 +811,8 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
 __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( SortArgs a, const SortArgs * tab, const int2 * map ) {
     __shared__ int s_hdr[4];
     const int t = select_instance( a, tab, map );
-    const PartView P = a.P, T = a.T;
-    const SortView S = a.S;
+    const PartG P = gview( a.P ), T = gview( a.T );
+    const SortG S = gview( a.S );
     if ( threadIdx.x == 0 ) {
         s_hdr[0] = P.off[t];
         s_hdr[1] = P.np[t];
@@ -1017,16 +1042,17 @@ void batch_particles_destroy( BatchParticlePlan * plan ) {
 
 __global__ void __launch_bounds__( 256 ) k_batch_energy( const EnergyArgs * tab ) {
     __shared__ double buf[256];
-    const EnergyArgs a = tab[blockIdx.x];
+    const EnergyArgs a = gbl( tab )[blockIdx.x];
+    const gptr<const double> tile = gbl( a.tile );
     double acc = 0;
-    for ( int t = threadIdx.x; t < a.n; t += blockDim.x ) acc += a.tile[t];
+    for ( int t = threadIdx.x; t < a.n; t += blockDim.x ) acc += tile[t];
     buf[threadIdx.x] = acc;
     __syncthreads();
     for ( int o = 128; o > 0; o >>= 1 ) {
         if ( (int) threadIdx.x < o ) buf[threadIdx.x] += buf[threadIdx.x + o];
         __syncthreads();
     }
-    if ( threadIdx.x == 0 ) *a.out = buf[0];
+    if ( threadIdx.x == 0 ) gbl( a.out )[0] = buf[0];
 }
 
 int batch_particles_step( BatchParticlePlan * plan, void * stream ) {

@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "zdf.hpp"
 #include "zpic_amd.h"
 
 namespace coord {
@@ -30,6 +31,41 @@ enum cart { x = 0, y, z };
 namespace part {
 enum quant { x = 0, y, ux, uy, uz };
 }
+
+namespace phasespace {
+enum quant { x = 0, y, ux, uy, uz };
+// Name, label and units of a phase-space axis (species.h:20-38)
+inline void qinfo( quant q, std::string & name, std::string & label, std::string & units ) {
+    static const char * n[] = { "x", "y", "ux", "uy", "uz" };
+    static const char * l[] = { "x", "y", "u_x", "u_y", "u_z" };
+    name = n[q];
+    label = l[q];
+    units = ( q <= y ) ? "c/\\omega_n" : "c";
+}
+}  // namespace phasespace
+
+// MovingWindow (moving_window.h): number of cells the window has moved
+class MovingWindow {
+    bool active_;
+    unsigned int n_move_;
+    double dx_;
+
+  public:
+    MovingWindow() : active_( false ), n_move_( 0 ), dx_( 0 ) {}
+    void init( float const dx ) {
+        active_ = true;
+        n_move_ = 0;
+        dx_ = dx;
+    }
+    bool active() const { return active_; }
+    unsigned int n_move() const { return n_move_; }
+    int advance() {
+        if ( active_ ) n_move_++;
+        return n_move_;
+    }
+    double motion() const { return n_move_ * dx_; }
+    bool needs_move( double const t ) const { return active_ && ( t > dx_ * ( n_move_ + 1 ) ); }
+};
 
 // bnd<T> (bnd.h:21-78): per-axis lower / upper values
 template <typename T>
@@ -148,6 +184,21 @@ class grid {
         zpic::device_free( d );
         return dims.x * dims.y;
     }
+    // grid::save (grid.h:601-633): interior of component fc as a 2-D ZDF grid
+    void save( zdf::grid_info & info, zdf::iteration & iter, std::string path, int fc = 0 ) const {
+        info.ndims = 2;
+        info.count[0] = dims.x;
+        info.count[1] = dims.y;
+        std::vector<float> h( (size_t) dims.x * dims.y );
+        gather( h.data(), fc );
+        zdf::save_grid( h.data(), info, iter, path );
+    }
+    void save( std::string filename, int fc = 0 ) const {
+        std::vector<float> h( (size_t) dims.x * dims.y );
+        gather( h.data(), fc );
+        const uint64_t gdims[2] = { dims.x, dims.y };
+        zdf::save_grid( h.data(), zdf::zdf_float32, 2, gdims, name, filename );
+    }
 };
 
 // vec3grid<float3> (vec3grid.h:38-442)
@@ -157,6 +208,17 @@ class vec3grid : public grid<V> {
     using grid<V>::grid;
     // component fc of the interior cells, contiguous (vec3grid.h:76-158)
     unsigned int gather( const fcomp::cart fc, float * host_out ) const { return grid<V>::gather( host_out, fc ); }
+    // vec3grid::save (vec3grid.h:398-437)
+    void save( const fcomp::cart fc, zdf::grid_info & info, zdf::iteration & iter, std::string path ) const {
+        grid<V>::save( info, iter, path, fc );
+    }
+    void save( const fcomp::cart fc, std::string filename ) const {
+        static const char * comp[] = { "x", "y", "z" };
+        std::vector<float> h( (size_t) this->dims.x * this->dims.y );
+        grid<V>::gather( h.data(), fc );
+        const uint64_t gdims[2] = { this->dims.x, this->dims.y };
+        zdf::save_grid( h.data(), zdf::zdf_float32, 2, gdims, this->name + "-" + comp[fc], filename );
+    }
 };
 
 // ----------------------------------------------------------- Filter:: (filter.h)
@@ -349,6 +411,7 @@ class Current {
     vec3grid<float3> * J;
     const float2 box;
     Filter::Digital * filter;
+    MovingWindow moving_window;
 
     Current( uint2 const ntiles, uint2 const nx, float2 const box, double const dt );
     ~Current();
@@ -362,6 +425,8 @@ class Current {
     }
     void advance();
     void zero();
+    // Current::save (current.cpp:263-313): CURRENT/J<c>-<iter>.zdf
+    void save( fcomp::cart const jc );
     void note_step() { iter++; }
     float2 get_dx() const { return dx; }
 };
@@ -378,6 +443,7 @@ class EMF {
     vec3grid<float3> * E;
     vec3grid<float3> * B;
     const float2 box;
+    MovingWindow moving_window;
 
     EMF( uint2 const ntiles, uint2 const nx, float2 const box, double const dt );
     ~EMF();
@@ -388,6 +454,8 @@ class EMF {
     void advance();
     void advance( Current & current );
     void get_energy( double3 & ene_E, double3 & ene_B );
+    // EMF::save (emf.cpp:589-655): EMF/<E|B><c>-<iter>.zdf
+    void save( emf::field const field, fcomp::cart const fc );
     void note_step() { iter++; }
     float2 get_dx() const { return dx; }
 };
@@ -420,6 +488,7 @@ class Species {
     const std::string name;
     const float m_q;
     species::pusher push_type;
+    MovingWindow moving_window;
 
     Species( std::string const name, float const m_q, uint2 const ppc );
     virtual ~Species();
@@ -443,6 +512,17 @@ class Species {
     void set_deterministic( bool on ) { deterministic = on; }
 
     void deposit_charge( grid<float> & charge ) const;
+    // Diagnostics (species.cpp:2417-3100): PARTICLES/<name>/particles-<name>-<iter>.zdf,
+    // CHARGE/<name>/<name>-charge-<iter>.zdf and PHASESPACE/<name>/<name>-<q>-<iter>.zdf
+    void save() const;
+    void save_charge() const;
+    // Phase-space density into the device buffer d_data (size, or size0*size1, floats)
+    void dep_phasespace( float * const d_data, phasespace::quant q, float2 range, unsigned const size ) const;
+    void dep_phasespace( float * const d_data, phasespace::quant quant0, float2 range0, unsigned const size0,
+                         phasespace::quant quant1, float2 range1, unsigned const size1 ) const;
+    void save_phasespace( phasespace::quant quant, float2 const range, int const size ) const;
+    void save_phasespace( phasespace::quant quant0, float2 const range0, int const size0, phasespace::quant quant1,
+                          float2 const range1, int const size1 ) const;
     double get_energy() const;
     uint64_t get_nmove() const { return d_nmove; }
     int get_iter() const { return iter; }

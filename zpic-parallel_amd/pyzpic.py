@@ -80,6 +80,13 @@ def lib():
         "zpic_batch_advance": (i, [vp, i]),
         "zpic_set_threads_per_tile": (i, [i]),
         "zpic_get_threads_per_tile": (i, []),
+        "zpic_emf_save": (i, [vp, i, i]),
+        "zpic_current_save": (i, [vp, i]),
+        "zpic_field_save_file": (i, [vp, i, i, ctypes.c_char_p]),
+        "zpic_species_save": (i, [vp]),
+        "zpic_species_save_charge": (i, [vp]),
+        "zpic_species_save_phasespace": (i, [vp, i, f, f, i, i, f, f, i]),
+        "zpic_zdf_save_grid": (i, [vp, i, vp, vp, ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -97,6 +104,51 @@ def _chk(rc):
     if rc != 0:
         raise ZpicError(lib().zpic_last_error().decode())
     return rc
+
+
+PHASESPACE = {"x": 0, "y": 1, "ux": 2, "uy": 3, "uz": 4}
+
+
+class ZdfAxis(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("type", ctypes.c_int), ("min", ctypes.c_double),
+                ("max", ctypes.c_double), ("label", ctypes.c_char_p), ("units", ctypes.c_char_p)]
+
+
+class ZdfGridInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("ndims", ctypes.c_uint32), ("count", ctypes.c_uint64 * 3),
+                ("label", ctypes.c_char_p), ("units", ctypes.c_char_p), ("axis", ctypes.POINTER(ZdfAxis))]
+
+
+class ZdfIteration(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("n", ctypes.c_int32), ("t", ctypes.c_double),
+                ("time_units", ctypes.c_char_p)]
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+def zdf_save_grid(data, name, path, label=None, units=None, axes=None, iteration=(None, 0, 0.0, None)):
+    """zdf_save_grid on a host array (float32 / float64, C order [ny, nx]):
+    writes <path>/<name>-<n:06>.zdf.  axes: list of (name, min, max, label,
+    units) from the fastest axis; iteration: (name, n, t, time_units)."""
+    data = np.ascontiguousarray(data)
+    dtype = {np.dtype(np.float32): 9, np.dtype(np.float64): 10, np.dtype(np.int32): 5}[data.dtype]
+    info = ZdfGridInfo()
+    info.name = _b(name)
+    info.ndims = data.ndim
+    for k, n in enumerate(reversed(data.shape)):
+        info.count[k] = n
+    info.label, info.units = _b(label), _b(units)
+    keep = []
+    if axes:
+        arr = (ZdfAxis * len(axes))()
+        for k, (an, amin, amax, alab, aun) in enumerate(axes):
+            arr[k] = ZdfAxis(_b(an), 0, amin, amax, _b(alab), _b(aun))
+        info.axis = ctypes.cast(arr, ctypes.POINTER(ZdfAxis))
+        keep.append(arr)
+    it = ZdfIteration(_b(iteration[0]), iteration[1], iteration[2], _b(iteration[3]))
+    _chk(lib().zpic_zdf_save_grid(_p(data), dtype, ctypes.byref(info), ctypes.byref(it), path.encode()))
 
 
 class Species:
@@ -165,6 +217,18 @@ class Species:
         out = np.zeros(self.np_total(), np.float32)
         _chk(lib().zpic_species_gather(self.h, q, _p(out)))
         return out
+
+    # ZDF diagnostics, written under the current working directory
+    def save(self):
+        _chk(lib().zpic_species_save(self.h))
+
+    def save_charge(self):
+        _chk(lib().zpic_species_save_charge(self.h))
+
+    def save_phasespace(self, quant0, range0, size0, quant1=None, range1=(0.0, 1.0), size1=0):
+        q1 = PHASESPACE[quant1] if quant1 else 0
+        _chk(lib().zpic_species_save_phasespace(self.h, PHASESPACE[quant0], range0[0], range0[1], size0, q1,
+                                                range1[0], range1[1], size1 if quant1 else 0))
 
 
 class Simulation:
@@ -245,6 +309,16 @@ class Simulation:
         """Kinetic energy per species, then E.x E.y E.z B.x B.y B.z."""
         e, b = self.emf_energy()
         return np.array([s.get_energy() for s in self.species] + list(e) + list(b))
+
+    # ZDF diagnostics, written under the current working directory
+    def save_emf(self, field, fc):
+        _chk(lib().zpic_emf_save(self.h, "EB".index(field), "xyz".index(fc)))
+
+    def save_current(self, fc):
+        _chk(lib().zpic_current_save(self.h, "xyz".index(fc)))
+
+    def save_field_file(self, which, fc, filename):
+        _chk(lib().zpic_field_save_file(self.h, "EBJ".index(which), "xyz".index(fc), filename.encode()))
 
     def set_timing(self, on=True):
         _chk(lib().zpic_simulation_set_timing(self.h, 1 if on else 0))
