@@ -34,6 +34,11 @@ PEAK_HBM_GBS = 8000.0
 WORKLOADS = {
     "weibel2048": dict(ntiles=(128, 128), nx=(16, 16), ppc=(8, 8), filter=("binomial", 0, 1)),
     "weibel128": dict(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), filter=None),
+    # 64 independent Weibel instances advanced as one batch (one launch per
+    # stage, one hipGraph launch per step): grids drawn from {64, 128, 256,
+    # 512}^2, ppc from {2, 4, 8}^2, drift uz from U[0.2, 0.9], uth from
+    # U[0.02, 0.2], dt 0.07, seed 2024
+    "batch64": dict(instances=64, seed=2024),
 }
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_push_move_deposit_weibel2048.json")
@@ -104,6 +109,9 @@ def main():
         sys.exit(2)
 
     import pyzpic
+
+    if args.workload == "batch64":
+        return run_batch(args, pyzpic)
 
     wl = WORKLOADS[args.workload]
     print(f"[bench] building {args.workload}: ntiles {wl['ntiles']} tile {wl['nx']} ppc {wl['ppc']}",
@@ -185,6 +193,73 @@ def main():
         "host": platform.node(),
     }
     sim.close()
+    print(json.dumps(result), flush=True)
+
+
+def batch_instances(n, seed):
+    rng = np.random.RandomState(seed)
+    out = []
+    for _ in range(n):
+        g = int(rng.choice([64, 128, 256, 512]))
+        p = int(rng.choice([2, 4, 8]))
+        uz = float(rng.uniform(0.2, 0.9))
+        uth = float(rng.uniform(0.02, 0.2))
+        out.append(dict(ntiles=(g // 16, g // 16), nx=(16, 16), ppc=(p, p), uth=(uth, uth, uth), ufl=(0.0, 0.0, uz)))
+    return out
+
+
+def run_batch(args, pyzpic):
+    wl = WORKLOADS["batch64"]
+    inst = batch_instances(wl["instances"], wl["seed"])
+    print(f"[bench] building batch64: {len(inst)} instances", file=sys.stderr, flush=True)
+    t_init = time.perf_counter()
+    sims = [pyzpic.weibel(ntiles=c["ntiles"], nx=c["nx"], ppc=c["ppc"], uth=c["uth"], ufl=c["ufl"]) for c in inst]
+    batch = pyzpic.Batch(sims, threads_per_tile=pyzpic.get_threads_per_tile(), graph=True)
+    for s in sims:
+        s.sync()
+    t_init = time.perf_counter() - t_init
+    npart = sum(sp.np_total() for s in sims for sp in s.species)
+    ncells = sum(c["ntiles"][0] * c["ntiles"][1] * 256 for c in inst)
+    print(f"[bench] {npart} particles, {ncells} cells, init {t_init:.1f} s; warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+    batch.advance(args.warmup)
+    sims[0].sync()
+    n0 = sum(s.get_nmove() for s in sims)
+    t0 = time.perf_counter()
+    batch.advance(args.steps)
+    sims[0].sync()
+    t1 = time.perf_counter()
+    moved = sum(s.get_nmove() for s in sims) - n0
+    sec = t1 - t0
+    ms_step = 1e3 * sec / args.steps
+    print(f"[bench] {args.steps} steps, {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+    step_bytes = 56.0 * npart + 96.0 * ncells
+    achieved = step_bytes / (ms_step * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": moved / (sec * 1e9),
+        "unit": "particles/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch64", "instances": len(inst), "seed": wl["seed"], "particles": npart,
+                   "cells": ncells, "grids": "U{64,128,256,512}^2", "ppc": "U{2,4,8}^2", "tile": [16, 16],
+                   "graph": True, "threads_per_tile": pyzpic.get_threads_per_tile(), "parallelism": "single GPU"},
+        "roofline": {"kernel": "whole step (batched launches)", "bound": "hbm", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "bytes_per_step": step_bytes},
+        "cpu_baseline": None,
+        "host": platform.node(),
+    }
+    batch.close()
+    for s in sims:
+        s.close()
     print(json.dumps(result), flush=True)
 
 

@@ -172,6 +172,27 @@ int zpic_fill(float *buf, size_t n, float value, void *stream);
  * (default 512).  Library-wide; the deterministic mode always uses 64. */
 int zpic_set_threads_per_tile(int threads);
 int zpic_get_threads_per_tile(void);
+/* Threads per tile the fast particle pass actually uses for tiles of
+ * tile_vol elements when `threads` is requested (fewer wavefronts when the
+ * LDS staging of a large tile would not fit); 0 if nothing fits. */
+int zpic_fitted_threads_per_tile(int threads, int tile_vol);
+
+/* Autotuner (no reference counterpart): times short Weibel-like runs of a
+ * grid_x x grid_y grid at ppc for every tile shape that divides the grid
+ * ({8,16,32} x {8,16,32} cells) and every threads-per-tile in {64, 128, 256,
+ * 512} that fits, on a sample of at most max_cells cells (0 = 512*512), and
+ * returns the fastest (particles per ns over push, deposit, sort and field
+ * stages).  Results are cached in the text file cache_path (NULL = none)
+ * keyed by device name, grid and ppc; from_cache reports a hit.  Leaves the
+ * library's threads per tile unchanged. */
+typedef struct {
+    unsigned nx, ny;
+    int threads;
+    double particles_per_ns;
+    int from_cache;
+} zpic_tune_result;
+int zpic_autotune(unsigned grid_x, unsigned grid_y, unsigned ppc_x, unsigned ppc_y, int steps, long long max_cells,
+                  const char *cache_path, zpic_tune_result *out);
 
 /* --------------------------------------------------- batched stages
  * Many independent simulations advanced with one launch per stage over all
@@ -240,6 +261,10 @@ void zpic_simulation_free(zpic_simulation *sim);
 zpic_species *zpic_species_new(const char *name, float m_q, unsigned ppc_x, unsigned ppc_y);
 void zpic_species_free(zpic_species *sp);
 int zpic_species_set_udist(zpic_species *sp, int kind, const float uth[3], const float ufl[3], int npmin);
+/* Offset added to the species id in the momentum RNG seed (0 = the
+ * reference's seeding, udist.cpp:73,94); before add_species.  For seeded
+ * ensembles of otherwise identical runs.  No reference counterpart. */
+int zpic_species_set_seed(zpic_species *sp, unsigned offset);
 int zpic_species_set_density_uniform(zpic_species *sp, float n0);
 
 /* Simulation::add_species (simulation.h:68-71): the simulation keeps a
@@ -341,6 +366,56 @@ typedef struct {
 } zpic_zdf_iteration;
 int zpic_zdf_save_grid(const void *data, int data_type, const zpic_zdf_grid_info *info,
                        const zpic_zdf_iteration *it, const char *path);
+
+/* Learned current denoiser (no reference counterpart): 3 convolution layers
+ * of 5x5 taps, 3 -> 16 -> 16 -> 3 channels with ReLU between them, periodic
+ * padding, BF16 operands on the matrix cores with FP32 accumulation.  The
+ * 8,835 parameters are W1[16][3][5][5], b1[16], W2[16][16][5][5], b2[16],
+ * W3[3][16][5][5], b3[3] (FP32).  The network sees J / scale and its output
+ * is multiplied by scale.  An object is sized for one H x W grid (W a
+ * multiple of 32) and stores up to max_pairs (input, target) J pairs for
+ * training on the GPU (Adam).  Host arrays are [y][x][3] FP32. */
+typedef struct zpic_denoiser zpic_denoiser;
+zpic_denoiser *zpic_denoiser_new(int H, int W, int max_pairs);
+void zpic_denoiser_free(zpic_denoiser *d);
+int zpic_denoiser_set_params(zpic_denoiser *d, const float *host);
+int zpic_denoiser_get_params(zpic_denoiser *d, float *host);
+int zpic_denoiser_set_scale(zpic_denoiser *d, float scale);
+float zpic_denoiser_get_scale(const zpic_denoiser *d);
+int zpic_denoiser_size(const zpic_denoiser *d, int *H, int *W);
+int zpic_denoiser_forward_host(zpic_denoiser *d, const float *in, float *out);
+int zpic_denoiser_gradient_host(zpic_denoiser *d, const float *in, const float *target, float *grad,
+                                double *loss);
+int zpic_denoiser_add_pair(zpic_denoiser *d, const float *J_in, const float *J_target, const zpic_geom *g,
+                           void *stream);
+int zpic_denoiser_num_pairs(const zpic_denoiser *d);
+int zpic_denoiser_clear_pairs(zpic_denoiser *d);
+int zpic_denoiser_get_pair(zpic_denoiser *d, int k, float *in, float *target);
+int zpic_denoiser_target_rms(zpic_denoiser *d, double *rms);
+int zpic_denoiser_train(zpic_denoiser *d, int steps, float lr, unsigned seed, int use, double *loss_hist);
+int zpic_denoiser_eval_pair(zpic_denoiser *d, int k, double *mse_net, double *mse_input);
+int zpic_denoiser_apply(zpic_denoiser *d, float *J, const zpic_geom *g, void *stream);
+int zpic_denoiser_reldiff(zpic_denoiser *d, const float *a, const float *b, size_t n, void *stream, double *out);
+/* Stores (in's current J, target's current J) as a training pair; both
+ * simulations must use the same grid and tiling. */
+int zpic_simulation_add_pair(zpic_denoiser *d, zpic_simulation *in, zpic_simulation *target);
+
+/* Learned-filter hooks: the simulation's current filter becomes the
+ * denoiser (not owned); set_fallback switches it to the binomial x order-1
+ * filter (on = 1) or back (on = 0); get_fallback reports the state (-1 if the
+ * filter is not the learned one). */
+int zpic_current_set_learned_filter(zpic_simulation *sim, zpic_denoiser *d);
+int zpic_current_set_fallback(zpic_simulation *sim, int on);
+int zpic_current_get_fallback(zpic_simulation *sim);
+
+/* Copies E and B of src into dst (same grid). */
+int zpic_simulation_copy_fields(zpic_simulation *dst, const zpic_simulation *src);
+
+/* Every stride-th particle (from index phase) of each tile of src, in tile
+ * order, into dst (same tiling); dst's counts and offsets are rewritten.
+ * Species form: the species' particle buffers. */
+int zpic_subsample(const zpic_particles *src, zpic_particles *dst, int ntiles, int stride, int phase, void *stream);
+int zpic_species_subsample(zpic_species *dst, zpic_species *src, int stride, int phase);
 
 /* Device synchronisation of the simulation's stream. */
 int zpic_simulation_sync(zpic_simulation *sim);

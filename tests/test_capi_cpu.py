@@ -67,3 +67,14 @@ def test_simulation_fails_loudly_without_gpu():
     import pyzpic
     with pytest.raises(pyzpic.ZpicError):
         pyzpic.Simulation((2, 2), (16, 16), (3.2, 3.2), 0.07)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_large_tiles_fit_lds():
+    # host-side sizing: 32x32 tiles cannot hold 8 private J copies in 64 KiB
+    import pyzpic
+    vol32 = (35 * 35 + 3) & ~3
+    t = pyzpic.fitted_threads_per_tile(512, vol32)
+    assert 64 <= t < 512 and t % 64 == 0
+    assert pyzpic.fitted_threads_per_tile(512, 364) == 512
+    assert pyzpic.fitted_threads_per_tile(100, 364) == 0

@@ -79,6 +79,13 @@ int zpic_species_set_udist( zpic_species * h, int kind, const float uth[3], cons
     } );
 }
 
+int zpic_species_set_seed( zpic_species * h, unsigned offset ) {
+    if ( !h ) return zpic::set_error( ZPIC_EINVAL, "null species" );
+    if ( h->added ) return zpic::set_error( ZPIC_ESTATE, "seed must be set before add_species" );
+    h->sp->set_seed( offset );
+    return ZPIC_OK;
+}
+
 int zpic_species_set_density_uniform( zpic_species * h, float n0 ) {
     return guarded( [&] {
         if ( !h ) return zpic::set_error( ZPIC_EINVAL, "null species" );
@@ -417,4 +424,70 @@ extern "C" int zpic_zdf_save_grid( const void * data, int data_type, const zpic_
     if ( zdf::save_grid( data, static_cast<zdf::data_type>( data_type ), g, iter, path ) != 1 )
         return zpic::set_error( ZPIC_EINVAL, std::string( "unable to write ZDF grid file under " ) + path );
     return ZPIC_OK;
+}
+
+extern "C" {
+
+int zpic_current_set_learned_filter( zpic_simulation * s, zpic_denoiser * d ) {
+    return guarded( [&] {
+        if ( !s || !d ) return zpic::set_error( ZPIC_EINVAL, "null handle" );
+        int H = 0, W = 0;
+        zpic_denoiser_size( d, &H, &W );
+        const uint2 dims = s->sim->current.J->get_dims();
+        if ( (int) dims.x != W || (int) dims.y != H ) return zpic::set_error( ZPIC_EINVAL, "denoiser size does not match the grid" );
+        s->sim->current.set_filter( Filter::Learned( d ) );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_current_set_fallback( zpic_simulation * s, int on ) {
+    if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+    auto * f = dynamic_cast<Filter::Learned *>( s->sim->current.filter );
+    if ( !f ) return zpic::set_error( ZPIC_ESTATE, "the current filter is not the learned one" );
+    f->set_fallback( on != 0 );
+    return ZPIC_OK;
+}
+
+int zpic_current_get_fallback( zpic_simulation * s ) {
+    if ( !s ) return -1;
+    auto * f = dynamic_cast<Filter::Learned *>( s->sim->current.filter );
+    return f ? ( f->get_fallback() ? 1 : 0 ) : -1;
+}
+
+int zpic_simulation_copy_fields( zpic_simulation * dst, const zpic_simulation * src ) {
+    return guarded( [&] {
+        if ( !dst || !src ) return zpic::set_error( ZPIC_EINVAL, "null handle" );
+        const auto & a = *dst->sim->emf.E;
+        const auto & b = *src->sim->emf.E;
+        if ( a.buffer_size() != b.buffer_size() || a.get_dims().x != b.get_dims().x || a.get_dims().y != b.get_dims().y ||
+             a.nx.x != b.nx.x || a.nx.y != b.nx.y )
+            return zpic::set_error( ZPIC_EINVAL, "grids differ" );
+        const size_t bytes = a.buffer_size() * sizeof( float3 );
+        zpic::hip_check( hipMemcpyAsync( dst->sim->emf.E->d_buffer, src->sim->emf.E->d_buffer, bytes,
+                                         hipMemcpyDeviceToDevice, zpic::stream() ), "copy E" );
+        zpic::hip_check( hipMemcpyAsync( dst->sim->emf.B->d_buffer, src->sim->emf.B->d_buffer, bytes,
+                                         hipMemcpyDeviceToDevice, zpic::stream() ), "copy B" );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_species_subsample( zpic_species * dst, zpic_species * src, int stride, int phase ) {
+    return guarded( [&] {
+        if ( !dst || !src || !dst->added || !src->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        dst->sp->subsample_from( *src->sp, stride, phase );
+        return ZPIC_OK;
+    } );
+}
+
+}  // extern "C"
+
+extern "C" int zpic_simulation_add_pair( zpic_denoiser * d, zpic_simulation * in, zpic_simulation * target ) {
+    return guarded( [&] {
+        if ( !d || !in || !target ) return zpic::set_error( ZPIC_EINVAL, "null handle" );
+        zpic_geom g = in->sim->current.J->geom(), gt = target->sim->current.J->geom();
+        if ( g.nx != gt.nx || g.ny != gt.ny || g.ntiles_x != gt.ntiles_x || g.ntiles_y != gt.ntiles_y )
+            return zpic::set_error( ZPIC_EINVAL, "pair grids must use the same grid and tiling" );
+        return zpic_denoiser_add_pair( d, in->sim->current.J->fbuf(), target->sim->current.J->fbuf(), &g,
+                                       zpic::stream() );
+    } );
 }

@@ -12,6 +12,8 @@
 // (single-thread) deposit.
 #include "common.h"
 
+#include <vector>
+
 namespace zpic {
 namespace {
 
@@ -167,5 +169,59 @@ extern "C" int zpic_phasespace( const zpic_particles * p, int ntiles_x, int ntil
         hipLaunchKernelGGL( k_phasespace, dim3( nt ), dim3( PHA_BLOCK ), lds, st, a, out, (int) bins );
     }
     ZPIC_LAUNCH_CHECK( "k_phasespace" );
+    return ZPIC_OK;
+}
+
+// --------------------------------------------------------------------------
+// Every stride-th particle (from index `phase`) of each tile of src into dst,
+// in tile order; dst->np / dst->offset receive the counts and their
+// exclusive scan (computed on the host from src's counts).
+namespace zpic {
+namespace {
+
+__global__ void __launch_bounds__( 256 ) k_subsample( const int * __restrict__ snp, const int * __restrict__ soff,
+                                                      const int2 * __restrict__ six, const float2 * __restrict__ sx,
+                                                      const float3 * __restrict__ su, const int * __restrict__ dnp,
+                                                      const int * __restrict__ doff, int2 * __restrict__ dix,
+                                                      float2 * __restrict__ dx, float3 * __restrict__ du, int stride,
+                                                      int phase ) {
+    const int t = blockIdx.x;
+    const int so = soff[t], n = dnp[t], d0 = doff[t];
+    (void) snp;
+    for ( int i = threadIdx.x; i < n; i += blockDim.x ) {
+        const int k = so + phase + i * stride;
+        dix[d0 + i] = six[k];
+        dx[d0 + i] = sx[k];
+        du[d0 + i] = su[k];
+    }
+}
+
+}  // namespace
+}  // namespace zpic
+
+extern "C" int zpic_subsample( const zpic_particles * src, zpic_particles * dst, int ntiles, int stride, int phase,
+                               void * stream ) {
+    if ( !src || !dst || ntiles < 1 || stride < 1 || phase < 0 || phase >= stride )
+        return set_error( ZPIC_EINVAL, "invalid subsample request" );
+    hipStream_t st = (hipStream_t) stream;
+    std::vector<int> np( ntiles ), off( ntiles ), dnp( ntiles ), doff( ntiles );
+    ZPIC_HIP_TRY( hipMemcpyAsync( np.data(), src->np, sizeof( int ) * ntiles, hipMemcpyDeviceToHost, st ) );
+    ZPIC_HIP_TRY( hipMemcpyAsync( off.data(), src->offset, sizeof( int ) * ntiles, hipMemcpyDeviceToHost, st ) );
+    ZPIC_HIP_TRY( hipStreamSynchronize( st ) );
+    long long total = 0;
+    for ( int t = 0; t < ntiles; t++ ) {
+        dnp[t] = np[t] > phase ? ( np[t] - phase + stride - 1 ) / stride : 0;
+        doff[t] = (int) total;
+        total += dnp[t];
+    }
+    if ( total > (long long) dst->max_part ) return set_error( ZPIC_ERANGE, "subsample exceeds the destination's capacity" );
+    ZPIC_HIP_TRY( hipMemcpyAsync( dst->np, dnp.data(), sizeof( int ) * ntiles, hipMemcpyHostToDevice, st ) );
+    ZPIC_HIP_TRY( hipMemcpyAsync( dst->offset, doff.data(), sizeof( int ) * ntiles, hipMemcpyHostToDevice, st ) );
+    hipLaunchKernelGGL( k_subsample, dim3( ntiles ), dim3( 256 ), 0, st, src->np, src->offset, (const int2 *) src->ix,
+                        (const float2 *) src->x, (const float3 *) src->u, dst->np, dst->offset, (int2 *) dst->ix,
+                        (float2 *) dst->x, (float3 *) dst->u, stride, phase );
+    ZPIC_LAUNCH_CHECK( "k_subsample" );
+    // the host count vectors must outlive the copies
+    ZPIC_HIP_TRY( hipStreamSynchronize( st ) );
     return ZPIC_OK;
 }

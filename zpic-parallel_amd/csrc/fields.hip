@@ -463,6 +463,8 @@ int batch_fields_create( BatchFieldPlan ** out, const zpic_batch_fields_item * i
         const zpic_batch_fields_item & it = items[i];
         if ( int rc = check( &it.g ) ) return rc;
         if ( it.g.gc_lower < 1 || it.g.gc_upper < 2 ) return set_error( ZPIC_EINVAL, "batch fields need guard cells {1,2}" );
+        if ( it.filter_kind < 0 || it.filter_kind > 2 )
+            return set_error( ZPIC_EINVAL, "batches run the none, binomial and compensated current filters only" );
         const G g = make( &it.g );
         const int px = it.periodic_x ? 1 : 0, py = it.periodic_y ? 1 : 0;
         tiles[i] = g.ntx * g.nty;

@@ -4,11 +4,13 @@
 // zpic_amd.h.  The initial load (density injection and momentum
 // distribution) runs on the host exactly as the reference does, one worker
 // per tile range, and is uploaded once; every per-step stage runs on the GPU.
+#include "common.h"
 #include "zpic.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <limits>
 #include <mutex>
@@ -348,7 +350,8 @@ void EMF::get_energy( double3 & ene_E, double3 & ene_B ) {
 // ------------------------------------------------------------------ Species
 Species::Species( std::string const name_, float const m_q_, uint2 const ppc_ )
     : ppc( ppc_ ), particles( nullptr ), tmp( nullptr ), sort( nullptr ), d_tile_energy( nullptr ),
-      d_energy_dev( nullptr ), d_nmove( 0 ), np_count( 0 ), deterministic( false ), name( name_ ), m_q( m_q_ ),
+      d_energy_dev( nullptr ), d_nmove( 0 ), np_count( 0 ), deterministic( false ), seed_offset( 0 ), name( name_ ),
+      m_q( m_q_ ),
       push_type( species::boris ) {
     if ( m_q == 0 ) throw std::runtime_error( "(*error*) Invalid m_q value, must be not 0, aborting..." );
     if ( ppc.x < 1 || ppc.y < 1 )
@@ -412,7 +415,7 @@ void Species::initialize( float2 const box_, uint2 const ntiles, uint2 const nx,
     std::vector<float2> hx( total );
     std::vector<float3> hu( total );
     density->inject( ntiles, nx, ppc, off.data(), hix.data(), hx.data() );
-    udist->set( nt, np.data(), off.data(), hix.data(), nx, hu.data(), id );
+    udist->set( nt, np.data(), off.data(), hix.data(), nx, hu.data(), id + seed_offset );
 
     to_device( particles->np, np.data(), nt );
     to_device( particles->offset, off.data(), nt );
@@ -509,6 +512,15 @@ void Species::set_particles( const int * np, const int * ix, const float * x, co
     to_device( particles->u, reinterpret_cast<const float3 *>( u ), n );
     sync();
     np_count = n;
+}
+
+void Species::subsample_from( const Species & src, int stride, int phase ) {
+    if ( src.particles->ntiles.x != particles->ntiles.x || src.particles->ntiles.y != particles->ntiles.y ||
+         src.particles->nx.x != particles->nx.x || src.particles->nx.y != particles->nx.y )
+        throw std::runtime_error( "(*error*) subsample needs species with the same tiling" );
+    zpic_particles s = src.particles->view(), d = particles->view();
+    check( zpic_subsample( &s, &d, particles->ntiles.x * particles->ntiles.y, stride, phase, stream() ) );
+    np_count = particles->np_total();
 }
 
 // Particles::gather (particles.cpp:88-150) with Species::gather's scales
@@ -953,4 +965,106 @@ void Species::save_phasespace( phasespace::quant quant0, float2 const range0, in
     sync();
     zpic::device_free( d );
     zdf::save_grid( h.data(), info, iter_info, "PHASESPACE/" + name );
+}
+
+// --------------------------------------------------------------- Autotuner
+namespace {
+
+std::string tune_key( uint2 grid, uint2 ppc ) {
+    hipDeviceProp_t prop;
+    int dev = 0;
+    (void) hipGetDevice( &dev );
+    std::string arch = "unknown";
+    if ( hipGetDeviceProperties( &prop, dev ) == hipSuccess ) arch = prop.gcnArchName;
+    for ( auto & ch : arch )
+        if ( ch == ' ' ) ch = '_';
+    return arch + ":" + std::to_string( grid.x ) + "x" + std::to_string( grid.y ) + ":" + std::to_string( ppc.x ) + "x" +
+           std::to_string( ppc.y );
+}
+
+bool tune_lookup( const std::string & path, const std::string & key, zpic_tune_result & out ) {
+    std::ifstream in( path );
+    std::string k;
+    zpic_tune_result r{};
+    while ( in >> k >> r.nx >> r.ny >> r.threads >> r.particles_per_ns ) {
+        if ( k == key ) {
+            out = r;
+            out.from_cache = 1;
+            return true;
+        }
+    }
+    return false;
+}
+
+void tune_store( const std::string & path, const std::string & key, const zpic_tune_result & r ) {
+    std::ofstream out( path, std::ios::app );
+    out << key << ' ' << r.nx << ' ' << r.ny << ' ' << r.threads << ' ' << r.particles_per_ns << '\n';
+}
+
+// particles / ns of `steps` steps of a two-species thermal plasma with the
+// given tiling, after two warm-up steps
+double tune_trial( uint2 ntiles, uint2 nx, uint2 ppc, int threads, int steps ) {
+    check( zpic_set_threads_per_tile( threads ) );
+    const float2 box = make_float2( ntiles.x * nx.x * 0.1f, ntiles.y * nx.y * 0.1f );
+    Simulation sim( ntiles, nx, box, 0.07 );
+    Species e( "electrons", -1.0f, ppc ), p( "positrons", 1.0f, ppc );
+    e.set_udist( UDistribution::Thermal( make_float3( 0.1f, 0.1f, 0.1f ), make_float3( 0, 0, 0.6f ) ) );
+    p.set_udist( UDistribution::Thermal( make_float3( 0.1f, 0.1f, 0.1f ), make_float3( 0, 0, -0.6f ) ) );
+    sim.add_species( e );
+    sim.add_species( p );
+    for ( int i = 0; i < 2; i++ ) sim.advance();
+    hipEvent_t t0, t1;
+    hip_check( hipEventCreate( &t0 ), "hipEventCreate" );
+    hip_check( hipEventCreate( &t1 ), "hipEventCreate" );
+    const uint64_t n0 = sim.get_nmove();
+    hip_check( hipEventRecord( t0, stream() ), "hipEventRecord" );
+    for ( int i = 0; i < steps; i++ ) sim.advance();
+    hip_check( hipEventRecord( t1, stream() ), "hipEventRecord" );
+    hip_check( hipEventSynchronize( t1 ), "hipEventSynchronize" );
+    float ms = 0;
+    hip_check( hipEventElapsedTime( &ms, t0, t1 ), "hipEventElapsedTime" );
+    (void) hipEventDestroy( t0 );
+    (void) hipEventDestroy( t1 );
+    return ( sim.get_nmove() - n0 ) / ( ms * 1e6 );
+}
+
+}  // namespace
+
+extern "C" int zpic_autotune( unsigned grid_x, unsigned grid_y, unsigned ppc_x, unsigned ppc_y, int steps,
+                              long long max_cells, const char * cache_path, zpic_tune_result * out ) {
+    if ( !out || grid_x < 8 || grid_y < 8 || ppc_x < 1 || ppc_y < 1 || steps < 1 )
+        return zpic::set_error( ZPIC_EINVAL, "invalid autotune request" );
+    if ( max_cells <= 0 ) max_cells = 512 * 512;
+    const int saved = zpic_get_threads_per_tile();
+    try {
+        const uint2 grid = make_uint2( grid_x, grid_y ), ppc = make_uint2( ppc_x, ppc_y );
+        const std::string key = tune_key( grid, ppc );
+        if ( cache_path && tune_lookup( cache_path, key, *out ) ) return ZPIC_OK;
+        zpic_tune_result best{};
+        const unsigned sizes[] = { 8, 16, 32 };
+        for ( unsigned sy : sizes )
+            for ( unsigned sx : sizes ) {
+                if ( grid_x % sx || grid_y % sy ) continue;
+                // sample: whole tiles, at most max_cells cells, same tile shape
+                uint2 nt = make_uint2( grid_x / sx, grid_y / sy );
+                while ( (long long) nt.x * sx * nt.y * sy > max_cells && ( nt.x > 2 || nt.y > 2 ) ) {
+                    if ( nt.x >= nt.y && nt.x > 2 ) nt.x = ( nt.x + 1 ) / 2;
+                    else nt.y = ( nt.y + 1 ) / 2;
+                }
+                const int vol = ( ( sx + 3 ) * ( sy + 3 ) + 3 ) & ~3;
+                for ( int threads : { 64, 128, 256, 512 } ) {
+                    if ( zpic_fitted_threads_per_tile( threads, vol ) != threads ) continue;
+                    const double pns = tune_trial( nt, make_uint2( sx, sy ), ppc, threads, steps );
+                    if ( pns > best.particles_per_ns ) best = zpic_tune_result{ sx, sy, threads, pns, 0 };
+                }
+            }
+        (void) zpic_set_threads_per_tile( saved );
+        if ( best.threads == 0 ) return zpic::set_error( ZPIC_ERANGE, "no tile shape fits the grid" );
+        *out = best;
+        if ( cache_path ) tune_store( cache_path, key, best );
+        return ZPIC_OK;
+    } catch ( std::exception & e ) {
+        (void) zpic_set_threads_per_tile( saved );
+        return zpic::set_error( ZPIC_EHIP, e.what() );
+    }
 }

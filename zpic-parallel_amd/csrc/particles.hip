@@ -914,6 +914,15 @@ int check_geom( const zpic_geom * g ) {
 // FAST_BLOCK); the deterministic mode always uses one wavefront.
 int g_threads = FAST_BLOCK;  // library-wide setting
 
+constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup
+
+// Threads per tile the particle pass runs with for tiles of `vol` elements
+// when asked for `threads` (reduced until the LDS staging fits).
+int fitted_threads( int threads, int vol ) {
+    while ( threads > WAVE && particles_lds_bytes( vol, threads / WAVE, true, true ) > LDS_LIMIT ) threads -= WAVE;
+    return particles_lds_bytes( vol, threads / WAVE, true, true ) > LDS_LIMIT ? 0 : threads;
+}
+
 int check_threads( int threads ) {
     if ( threads < WAVE || threads > FAST_BLOCK || threads % WAVE )
         return set_error( ZPIC_EINVAL, "threads per tile must be a multiple of 64 in [64, 512]" );
@@ -927,9 +936,12 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
     if ( !p || ( PUSH && ( !E || !B || !tile_energy ) ) || ( MOVE && ( !J || !s ) ) )
         return set_error( ZPIC_EINVAL, "null buffer" );
     const int nt = fld->ntiles_x * fld->ntiles_y;
-    const int threads = det ? WAVE : g_threads;
+    // Large tiles: fewer wavefronts (each holds a private J copy) until LDS fits.
+    int threads = det ? WAVE : g_threads;
+    while ( threads > WAVE && particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE ) > LDS_LIMIT )
+        threads -= WAVE;
     const size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE );
-    if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    if ( lds > LDS_LIMIT ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     PartArgs a;
     a.P = view( p ); a.g = make_geo( fld );
     a.E = (const float3 *) E; a.B = (const float3 *) B; a.J = (float3 *) J;
@@ -1011,7 +1023,8 @@ int batch_particles_create( BatchParticlePlan ** out, const zpic_batch_species *
         plan->max_vol = std::max( plan->max_vol, it.fld.tile_vol );
     }
     plan->nblocks = (int) map.size();
-    if ( particles_lds_bytes( plan->max_vol, threads / WAVE, true, true ) > 65536 ) {
+    plan->threads = fitted_threads( threads, plan->max_vol );
+    if ( plan->threads == 0 ) {
         delete plan;
         return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     }
@@ -1132,6 +1145,11 @@ int zpic_set_threads_per_tile( int threads ) {
 }
 
 int zpic_get_threads_per_tile( void ) { return g_threads; }
+
+int zpic_fitted_threads_per_tile( int threads, int tile_vol ) {
+    if ( threads < WAVE || threads > FAST_BLOCK || threads % WAVE || tile_vol < 1 ) return 0;
+    return fitted_threads( threads, tile_vol );
+}
 
 struct zpic_batch_particles {
     BatchParticlePlan * plan;

@@ -268,6 +268,36 @@ class Compensated : public Binomial {
     }
     int kind() const override { return 2; }
 };
+// Learned denoiser in place of the binomial filter (no reference
+// counterpart): applies a zpic_denoiser (not owned; it must outlive the
+// filter and match the grid) to J's interior, then refreshes the guards.
+// While `fallback` is set (by a failed shadow check) the binomial filter
+// along x of order 1 runs instead.
+class Learned : public Digital {
+    zpic_denoiser * dn;
+    bool fallback;
+
+  public:
+    explicit Learned( zpic_denoiser * d ) : dn( d ), fallback( false ) {}
+    Learned * clone() const override {
+        auto * c = new Learned( dn );
+        c->fallback = fallback;
+        return c;
+    }
+    void apply( vec3grid<float3> & fld ) override {
+        if ( fallback || !dn ) {
+            Binomial( coord::x, 1 ).apply( fld );
+            return;
+        }
+        zpic_geom g = fld.geom();
+        zpic::check( zpic_denoiser_apply( dn, fld.fbuf(), &g, zpic::stream() ) );
+        fld.copy_to_gc();
+    }
+    int kind() const override { return 3; }
+    void set_fallback( bool on ) { fallback = on; }
+    bool get_fallback() const { return fallback; }
+    zpic_denoiser * denoiser() const { return dn; }
+};
 }  // namespace Filter
 
 // -------------------------------------------------------------- Particles
@@ -483,6 +513,7 @@ class Species {
     uint64_t d_nmove;
     uint64_t np_count;       // particles held (constant for periodic boundaries)
     bool deterministic;
+    unsigned seed_offset;
 
   public:
     const std::string name;
@@ -500,6 +531,9 @@ class Species {
     UDistribution::Type & get_udist() { return *udist; }
     void set_bc( species::bc_type new_bc );
     species::bc_type get_bc() const { return bc; }
+    // Offset added to the species id in the momentum RNG seed (0: the
+    // reference's seeding, udist.cpp:73,94); must be set before initialize
+    void set_seed( unsigned offset ) { seed_offset = offset; }
 
     void push( vec3grid<float3> * const E, vec3grid<float3> * const B );
     void move( vec3grid<float3> * const J );
@@ -538,6 +572,8 @@ class Species {
     // Host copies in tile order (compact)
     void get_particles( int * np, int * offset, int * ix, float * x, float * u ) const;
     void set_particles( const int * np, const int * ix, const float * x, const float * u );
+    // Every stride-th particle of each of src's tiles (same tiling), from index phase
+    void subsample_from( const Species & src, int stride, int phase );
     // Species::gather (species.h:515): x, y in simulation units, or ux, uy, uz
     void gather( part::quant quant, float * host_out ) const;
 

@@ -1,0 +1,169 @@
+"""Learned current denoiser: training data from this engine, training on the
+GPU, the shadow check with binomial fallback, and the growth-rate judge.
+
+Training pairs.  A high-ppc Weibel run (32x32 particles per cell) is the
+source of truth.  Before each sampled step, every 256th particle of each of
+its tiles is copied into a 2x2-ppc simulation on the same grid (whose charge
+per particle is 256 times larger), together with the current E and B; both
+then deposit the current of the same step.  The pair (J of the subsample, J
+of all particles) is one noisy/clean sample of the same physical state.
+
+Shadow check.  A small high-ppc Weibel box runs beside the production run;
+every `every` steps a pair is drawn from it the same way, and the network's
+error on the subsample current is compared with the binomial filter's.  If
+the network is worse than the filter by more than `tol`, the production
+run's filter falls back to the binomial filter for good.
+
+Judge.  The growth rate of the magnetic-field energy during the linear
+phase of the instability, fitted per run, compared with the plain high-ppc
+run's.
+"""
+import time
+
+import numpy as np
+
+import pyzpic as zp
+
+HI_PPC = (32, 32)
+LO_PPC = (2, 2)
+STRIDE = (HI_PPC[0] * HI_PPC[1]) // (LO_PPC[0] * LO_PPC[1])  # 256
+
+
+def weibel_grid(n, ppc, seed=0, filter=None, uth=(0.1, 0.1, 0.1), ufl=(0.0, 0.0, 0.6)):
+    """The Weibel case on an n x n grid (dx = 0.1, dt = 0.07, 16x16 tiles)."""
+    return zp.weibel(ntiles=(n // 16, n // 16), nx=(16, 16), ppc=ppc, uth=uth, ufl=ufl, filter=filter, seed=seed)
+
+
+def sample_pair(hi, lo, dn, phase):
+    """Stores (J of every STRIDE-th particle, J of all particles) of hi's next
+    step in dn's pair store and advances hi by that step."""
+    lo.copy_fields_from(hi)
+    for sl, sh in zip(lo.species, hi.species):
+        sl.subsample_from(sh, STRIDE, phase)
+    lo.stage("zero_j")
+    for k in range(len(lo.species)):
+        lo.stage("species", k)
+    lo.stage("current")  # guard add + guard copy (no filter)
+    hi.advance()
+    lo.add_pair_to(dn, hi)
+
+
+def generate_pairs(dn, n, seeds, steps, every, first=0, log=None):
+    """Pairs from len(seeds) high-ppc runs of `steps` steps, one pair every
+    `every` steps from step `first`."""
+    for s in seeds:
+        t0 = time.perf_counter()
+        hi = weibel_grid(n, HI_PPC, seed=s)
+        lo = weibel_grid(n, LO_PPC)
+        rng = np.random.RandomState(1000 + s)
+        for step in range(steps):
+            if step >= first and (step - first) % every == 0:
+                sample_pair(hi, lo, dn, int(rng.randint(STRIDE)))
+            else:
+                hi.advance()
+        hi.close()
+        lo.close()
+        if log:
+            log(f"seed {s}: {dn.num_pairs()} pairs, {time.perf_counter() - t0:.1f} s")
+
+
+def binomial_xy(J, passes=1):
+    """[1/4, 1/2, 1/4] periodic smoothing along x then y, `passes` times
+    each, of a [H, W, 3] array (host, for evaluation)."""
+    J = np.asarray(J, np.float64)
+    for ax in (1, 0):
+        for _ in range(passes):
+            J = 0.25 * np.roll(J, 1, axis=ax) + 0.5 * J + 0.25 * np.roll(J, -1, axis=ax)
+    return J
+
+
+def binomial_x(J):
+    """The production fallback: one x pass (Filter::Binomial(x, 1))."""
+    J = np.asarray(J, np.float64)
+    return 0.25 * np.roll(J, 1, axis=1) + 0.5 * J + 0.25 * np.roll(J, -1, axis=1)
+
+
+def evaluate(dn, pairs):
+    """MSE against the target of the network, of the raw input and of the
+    binomial filters, over stored pairs `pairs`."""
+    out = {"net": 0.0, "raw": 0.0, "binomial_x": 0.0, "binomial_xy": 0.0}
+    for k in pairs:
+        net, raw = dn.eval_pair(k)
+        a, t = dn.pair(k)
+        out["net"] += net
+        out["raw"] += raw
+        out["binomial_x"] += float(np.mean((binomial_x(a) - t) ** 2))
+        out["binomial_xy"] += float(np.mean((binomial_xy(a) - t) ** 2))
+    return {k: v / max(len(pairs), 1) for k, v in out.items()}
+
+
+class ShadowCheck:
+    """High-ppc shadow box that periodically checks the network against the
+    binomial filter and switches `sim` to the fallback when it loses."""
+
+    def __init__(self, sim, params, scale, n=64, every=25, tol=0.05, seed=7):
+        self.sim, self.every, self.tol = sim, every, tol
+        self.hi = weibel_grid(n, HI_PPC, seed=seed)
+        self.lo = weibel_grid(n, LO_PPC)
+        self.dn = zp.Denoiser(n, n, max_pairs=1, params=params, scale=scale)
+        self.rng = np.random.RandomState(seed)
+        self.step = 0
+        self.history = []
+
+    def advance(self):
+        """One step of the shadow box (checking on every `every`-th)."""
+        if self.step % self.every == 0 and not self.sim.get_fallback():
+            self.dn.clear_pairs()
+            sample_pair(self.hi, self.lo, self.dn, int(self.rng.randint(STRIDE)))
+            net, _ = self.dn.eval_pair(0)
+            a, t = self.dn.pair(0)
+            ref = float(np.mean((binomial_x(a) - t) ** 2))
+            ok = net <= (1.0 + self.tol) * ref
+            self.history.append((self.step, net, ref, ok))
+            if not ok:
+                self.sim.set_fallback(True)
+        else:
+            self.hi.advance()
+        self.step += 1
+
+    def close(self):
+        for s in (self.hi, self.lo):
+            s.close()
+        self.dn.close()
+
+
+def growth_rate(eb, dt, lo_factor=30.0, hi_factor=30.0):
+    """Growth rate of the field amplitude (half the slope of ln E_B) fitted
+    over the linear phase: from the first step where E_B exceeds lo_factor
+    times its initial level to the first step where it reaches 1/hi_factor of
+    its maximum.  Returns (gamma, first, last) or (nan, -1, -1)."""
+    eb = np.asarray(eb, np.float64)
+    e0 = max(np.median(eb[:5]), 1e-300)
+    top = eb.max()
+    a = int(np.argmax(eb > lo_factor * e0))
+    b = int(np.argmax(eb >= top / hi_factor))
+    if not (eb[a] > lo_factor * e0) or b - a < 5:
+        return float("nan"), -1, -1
+    t = np.arange(a, b + 1) * dt
+    slope = np.polyfit(t, np.log(eb[a:b + 1]), 1)[0]
+    return 0.5 * slope, a, b
+
+
+def field_b_energy(sim):
+    e = sim.energies()
+    ns = len(sim.species)
+    return float(np.sum(e[ns + 3:ns + 6]))
+
+
+def run_series(sim, steps, shadow=None):
+    """B-field energy after every step and the wall time of the run."""
+    sim.sync()
+    out = np.zeros(steps)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        sim.advance()
+        if shadow is not None:
+            shadow.advance()
+        out[k] = field_b_energy(sim)
+    sim.sync()
+    return out, time.perf_counter() - t0

@@ -51,6 +51,7 @@ def lib():
         "zpic_species_free": (None, [vp]),
         "zpic_species_set_udist": (i, [vp, i, vp, vp, i]),
         "zpic_species_set_density_uniform": (i, [vp, f]),
+        "zpic_species_set_seed": (i, [vp, u]),
         "zpic_simulation_add_species": (i, [vp, vp]),
         "zpic_simulation_advance": (i, [vp, i]),
         "zpic_simulation_stage": (i, [vp, i, i]),
@@ -87,6 +88,29 @@ def lib():
         "zpic_species_save_charge": (i, [vp]),
         "zpic_species_save_phasespace": (i, [vp, i, f, f, i, i, f, f, i]),
         "zpic_zdf_save_grid": (i, [vp, i, vp, vp, ctypes.c_char_p]),
+        "zpic_fitted_threads_per_tile": (i, [i, i]),
+        "zpic_denoiser_new": (vp, [i, i, i]),
+        "zpic_denoiser_free": (None, [vp]),
+        "zpic_denoiser_set_params": (i, [vp, vp]),
+        "zpic_denoiser_get_params": (i, [vp, vp]),
+        "zpic_denoiser_set_scale": (i, [vp, f]),
+        "zpic_denoiser_get_scale": (f, [vp]),
+        "zpic_denoiser_size": (i, [vp, vp, vp]),
+        "zpic_denoiser_forward_host": (i, [vp, vp, vp]),
+        "zpic_denoiser_gradient_host": (i, [vp, vp, vp, vp, vp]),
+        "zpic_denoiser_num_pairs": (i, [vp]),
+        "zpic_denoiser_clear_pairs": (i, [vp]),
+        "zpic_denoiser_get_pair": (i, [vp, i, vp, vp]),
+        "zpic_denoiser_target_rms": (i, [vp, vp]),
+        "zpic_denoiser_train": (i, [vp, i, f, u, i, vp]),
+        "zpic_denoiser_eval_pair": (i, [vp, i, vp, vp]),
+        "zpic_simulation_add_pair": (i, [vp, vp, vp]),
+        "zpic_current_set_learned_filter": (i, [vp, vp]),
+        "zpic_current_set_fallback": (i, [vp, i]),
+        "zpic_current_get_fallback": (i, [vp]),
+        "zpic_simulation_copy_fields": (i, [vp, vp]),
+        "zpic_species_subsample": (i, [vp, vp, i, i]),
+        "zpic_autotune": (i, [u, u, u, u, i, ctypes.c_longlong, ctypes.c_char_p, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -154,7 +178,7 @@ def zdf_save_grid(data, name, path, label=None, units=None, axes=None, iteration
 class Species:
     """Species(name, m_q, ppc) with the reference's udist / density setters."""
 
-    def __init__(self, name, m_q, ppc, udist="none", uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2, n0=None):
+    def __init__(self, name, m_q, ppc, udist="none", uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2, n0=None, seed=0):
         L = lib()
         h = L.zpic_species_new(name.encode(), m_q, ppc[0], ppc[1])
         if not h:
@@ -164,6 +188,8 @@ class Species:
         self.sim = None
         if n0 is not None:
             _chk(L.zpic_species_set_density_uniform(h, n0))
+        if seed:
+            _chk(L.zpic_species_set_seed(h, seed))
         self.set_udist(udist, uth, ufl, npmin)
 
     def set_udist(self, kind, uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2):
@@ -219,6 +245,10 @@ class Species:
         return out
 
     # ZDF diagnostics, written under the current working directory
+    def subsample_from(self, src, stride, phase=0):
+        """Every stride-th particle of each of src's tiles (same tiling)."""
+        _chk(lib().zpic_species_subsample(self.h, src.h, stride, phase))
+
     def save(self):
         _chk(lib().zpic_species_save(self.h))
 
@@ -320,6 +350,24 @@ class Simulation:
     def save_field_file(self, which, fc, filename):
         _chk(lib().zpic_field_save_file(self.h, "EBJ".index(which), "xyz".index(fc), filename.encode()))
 
+    # learned current filter (see Denoiser)
+    def set_learned_filter(self, dn):
+        _chk(lib().zpic_current_set_learned_filter(self.h, dn.h))
+        self._dn = dn  # keep it alive while the filter uses it
+
+    def set_fallback(self, on=True):
+        _chk(lib().zpic_current_set_fallback(self.h, 1 if on else 0))
+
+    def get_fallback(self):
+        return int(lib().zpic_current_get_fallback(self.h))
+
+    def copy_fields_from(self, other):
+        _chk(lib().zpic_simulation_copy_fields(self.h, other.h))
+
+    def add_pair_to(self, dn, target_sim):
+        """Stores (this simulation's J, target_sim's J) in dn's pair store."""
+        _chk(lib().zpic_simulation_add_pair(dn.h, self.h, target_sim.h))
+
     def set_timing(self, on=True):
         _chk(lib().zpic_simulation_set_timing(self.h, 1 if on else 0))
 
@@ -366,6 +414,122 @@ class Batch:
         self.close()
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_uint), ("ny", ctypes.c_uint), ("threads", ctypes.c_int),
+                ("particles_per_ns", ctypes.c_double), ("from_cache", ctypes.c_int)]
+
+
+def autotune(grid, ppc, steps=3, max_cells=0, cache=None):
+    """Fastest tile shape and threads per tile for a grid and ppc (timed on
+    the GPU; see zpic_autotune).  Returns a dict; cache: optional file path."""
+    r = TuneResult()
+    _chk(lib().zpic_autotune(grid[0], grid[1], ppc[0], ppc[1], steps, max_cells,
+                             cache.encode() if cache else None, ctypes.byref(r)))
+    return {"nx": (r.nx, r.ny), "threads": r.threads, "particles_per_ns": r.particles_per_ns,
+            "from_cache": bool(r.from_cache)}
+
+
+def fitted_threads_per_tile(threads, tile_vol):
+    return int(lib().zpic_fitted_threads_per_tile(threads, tile_vol))
+
+
+class Denoiser:
+    """Learned current denoiser for an H x W grid (see zpic_amd.h): 8,835
+    FP32 parameters, BF16 matrix-core kernels, training on stored pairs."""
+
+    NPARAM = 8835
+
+    def __init__(self, H, W, max_pairs=0, params=None, scale=None):
+        L = lib()
+        h = L.zpic_denoiser_new(H, W, max_pairs)
+        if not h:
+            raise ZpicError(L.zpic_last_error().decode())
+        self.h = h
+        self.H, self.W = H, W
+        if params is not None:
+            self.set_params(params)
+        if scale is not None:
+            self.set_scale(scale)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zpic_denoiser_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.NPARAM
+        _chk(lib().zpic_denoiser_set_params(self.h, _p(p)))
+
+    def params(self):
+        out = np.zeros(self.NPARAM, np.float32)
+        _chk(lib().zpic_denoiser_get_params(self.h, _p(out)))
+        return out
+
+    def set_scale(self, s):
+        _chk(lib().zpic_denoiser_set_scale(self.h, float(s)))
+
+    @property
+    def scale(self):
+        return float(lib().zpic_denoiser_get_scale(self.h))
+
+    def forward(self, x):
+        """x: [H, W, 3] float32 (physical units) -> network output, same shape."""
+        x = np.ascontiguousarray(x, np.float32)
+        assert x.shape == (self.H, self.W, 3)
+        out = np.zeros_like(x)
+        _chk(lib().zpic_denoiser_forward_host(self.h, _p(x), _p(out)))
+        return out
+
+    def gradient(self, x, target):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(self.NPARAM, np.float32)
+        loss = ctypes.c_double()
+        _chk(lib().zpic_denoiser_gradient_host(self.h, _p(x), _p(t), _p(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def num_pairs(self):
+        return int(lib().zpic_denoiser_num_pairs(self.h))
+
+    def clear_pairs(self):
+        _chk(lib().zpic_denoiser_clear_pairs(self.h))
+
+    def pair(self, k):
+        a = np.zeros((self.H, self.W, 3), np.float32)
+        b = np.zeros_like(a)
+        _chk(lib().zpic_denoiser_get_pair(self.h, k, _p(a), _p(b)))
+        return a, b
+
+    def target_rms(self):
+        r = ctypes.c_double()
+        _chk(lib().zpic_denoiser_target_rms(self.h, ctypes.byref(r)))
+        return r.value
+
+    def train(self, steps, lr=1e-3, seed=0, use=0):
+        """Adam steps on the first `use` stored pairs (all if 0); returns the
+        per-step losses."""
+        hist = np.zeros(max(steps, 1), np.float64)
+        _chk(lib().zpic_denoiser_train(self.h, steps, float(lr), seed, use, _p(hist)))
+        return hist[:steps]
+
+    def eval_pair(self, k):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _chk(lib().zpic_denoiser_eval_pair(self.h, k, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def save(self, path):
+        np.savez(path, params=self.params(), scale=np.float32(self.scale))
+
+    @classmethod
+    def load(cls, path, H, W, max_pairs=0):
+        d = np.load(path, allow_pickle=False)
+        return cls(H, W, max_pairs, params=d["params"], scale=float(d["scale"]))
+
+
 def set_threads_per_tile(n):
     """Threads per tile of the fast particle pass (multiple of 64 in [64, 512])."""
     _chk(lib().zpic_set_threads_per_tile(n))
@@ -376,11 +540,12 @@ def get_threads_per_tile():
 
 
 def weibel(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), box=None, dt=0.07, uth=(0.1, 0.1, 0.1),
-           ufl=(0.0, 0.0, 0.6), filter=None, deterministic=False):
-    """The reference Weibel setup (openmp/em2d/main.cpp:179-259) on the GPU."""
+           ufl=(0.0, 0.0, 0.6), filter=None, deterministic=False, seed=0):
+    """The reference Weibel setup (openmp/em2d/main.cpp:179-259) on the GPU.
+    seed != 0 offsets the momentum RNG seeds (an ensemble member)."""
     if box is None:
         box = (np.float32(ntiles[0] * nx[0] / 10.0), np.float32(ntiles[1] * nx[1] / 10.0))
     sim = Simulation(ntiles, nx, box, float(np.float32(dt)), filter=filter, deterministic=deterministic)
-    sim.add_species(Species("electrons", -1.0, ppc, "thermal_corr", uth, ufl))
-    sim.add_species(Species("positrons", +1.0, ppc, "thermal_corr", uth, tuple(-v for v in ufl)))
+    sim.add_species(Species("electrons", -1.0, ppc, "thermal_corr", uth, ufl, seed=16 * seed))
+    sim.add_species(Species("positrons", +1.0, ppc, "thermal_corr", uth, tuple(-v for v in ufl), seed=16 * seed))
     return sim
