@@ -82,15 +82,41 @@ __device__ inline void interpolate( const float * __restrict__ Ex, const float *
     b.z = ( Bz[khh] * s0xh + Bz[khh + 1] * s1xh ) * s0yh + ( Bz[khh + ys] * s0xh + Bz[khh + ys + 1] * s1xh ) * s1yh;
 }
 
+// Correctly rounded square root of x >= 1: the hardware square root moved by
+// one ulp down or up from the signs of the two residuals x - s(s -+ ulp),
+// which is the compiler's correctly rounded sequence without its rescaling
+// of inputs below 2^-96 and its +-0 / inf / NaN passthrough, neither of which
+// applies to x >= 1.
+__device__ inline float sqrt_ge1( float x ) {
+    const float s = __builtin_amdgcn_sqrtf( x );
+    const float dn = __int_as_float( __float_as_int( s ) - 1 );
+    const float up = __int_as_float( __float_as_int( s ) + 1 );
+    const float lo = ( fmaf( -dn, s, x ) <= 0.0f ) ? dn : s;
+    return ( fmaf( -up, s, x ) > 0.0f ) ? up : lo;
+}
+
+// Correctly rounded a / b for |b| >= 1 and normal a / b: the hardware
+// reciprocal refined once, then the quotient corrected twice with fused
+// residuals, which is the compiler's division sequence when its operand
+// scaling and special-case fixup leave the values unchanged (they only act
+// on denormal, huge or special operands and results).
+__device__ inline float div_ge1( float a, float b ) {
+    float r = __builtin_amdgcn_rcpf( b );
+    r = fmaf( fmaf( -b, r, 1.0f ), r, r );
+    float q = a * r;
+    q = fmaf( fmaf( -b, q, a ), r, q );
+    return fmaf( fmaf( -b, q, a ), r, q );
+}
+
 // ---- dudt_boris (species.cpp:117-178); returns u and the energy term
 __device__ inline float3 boris( float alpha, float3 e, float3 b, float3 u, float & ene ) {
     e.x *= alpha; e.y *= alpha; e.z *= alpha;
     float3 ut = make_float3( u.x + e.x, u.y + e.y, u.z + e.z );
     {
         const float usq = fmaf( ut.z, ut.z, fmaf( ut.y, ut.y, ut.x * ut.x ) );
-        const float g = sqrtf( 1.0f + usq );
-        ene = usq / ( g + 1.0f );
-        const float ag = alpha / g;
+        const float g = sqrt_ge1( 1.0f + usq );
+        ene = usq / ( g + 1.0f );  // usq may be 0 or tiny: general division
+        const float ag = div_ge1( alpha, g );
         b.x *= ag; b.y *= ag; b.z *= ag;
     }
     float3 v;
@@ -101,7 +127,7 @@ __device__ inline float3 boris( float alpha, float3 e, float3 b, float3 u, float
     v.y = fmaf( -b.z, ut.x, v.y );
     v.z = fmaf( -b.x, ut.y, v.z );
     {
-        const float s = 2.0f / fmaf( b.z, b.z, fmaf( b.y, b.y, fmaf( b.x, b.x, 1.0f ) ) );
+        const float s = div_ge1( 2.0f, fmaf( b.z, b.z, fmaf( b.y, b.y, fmaf( b.x, b.x, 1.0f ) ) ) );
         b.x *= s; b.y *= s; b.z *= s;
     }
     ut.x = fmaf( b.z, v.y, ut.x );
@@ -116,7 +142,7 @@ __device__ inline float3 boris( float alpha, float3 e, float3 b, float3 u, float
 
 // ---- rgamma (species.cpp:22-26): true division by a rounded sqrt, not rsqrt
 __device__ inline float inv_gamma( float3 u ) {
-    return 1.0f / sqrtf( fmaf( u.z, u.z, fmaf( u.y, u.y, fmaf( u.x, u.x, 1.0f ) ) ) );
+    return div_ge1( 1.0f, sqrt_ge1( fmaf( u.z, u.z, fmaf( u.y, u.y, fmaf( u.x, u.x, 1.0f ) ) ) ) );
 }
 
 struct Seg {
@@ -291,10 +317,20 @@ struct MoveArgs {
 // count of each wave's segment, seg[16 t + 15] the chunk.
 //
 // LDS: E and B planes (6*vol floats, PUSH), then W private J plane triples
-// (3*vol floats each, MOVE), then W tag byte arrays (vol bytes each, MOVE).
+// (3*vol floats each, MOVE), then W tag byte arrays (vol bytes each, MOVE),
+// then W crossing queues (QCAP entries of QWORDS words, MOVE).
+//
+// Fast mode: a particle that stays in its cell deposits its single segment
+// in the iteration that moves it; one that crosses a cell edge (about one in
+// ten) is queued with its path, and the wave splits and deposits 64 queued
+// paths at a time with every lane busy, so the rarely needed split and the
+// second and third segments do not occupy the whole wave every iteration.
+constexpr int QCAP = WAVE;  // crossing-queue entries per wave
+constexpr int QWORDS = 6;   // cell, x0.x, x0.y, d.x, d.y, qvz
+
 __host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move ) {
     return sizeof( float ) * (size_t) vol * ( ( push ? 6 : 0 ) + ( move ? 3 * W : 0 ) ) +
-           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) : 0 );
+           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) + sizeof( float ) * (size_t) W * QCAP * QWORDS : 0 );
 }
 
 // Arguments of one instance of the particle pass.
@@ -330,6 +366,16 @@ struct SortG {
 __device__ inline SortG gview( const SortView & v ) {
     return SortG{ gbl( v.idx ), gbl( v.npt ), gbl( v.new_np ), gbl( v.seg ), gbl( v.tmp_np ), gbl( v.total ) };
 }
+
+// Element i of one tile's particle array through the tile's (uniform) base
+// address and a 32-bit byte offset: the global load / store form with a
+// scalar base and a vector offset, no 64-bit address arithmetic per lane.
+template <class T>
+struct TileArray {
+    gptr<char> base;
+    __device__ T get( unsigned i ) const { return *(gptr<T>) ( base + i * (unsigned) sizeof( T ) ); }
+    __device__ void set( unsigned i, const T & v ) const { *(gptr<T>) ( base + i * (unsigned) sizeof( T ) ) = v; }
+};
 
 // Single-instance launches pass map == nullptr and use `a` with tile =
 // blockIdx.x; batched launches pass a table of per-instance arguments and a
@@ -376,6 +422,9 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
     float * Jy = Jx + vol;
     float * Jz = Jy + vol;
     unsigned char * tags = reinterpret_cast<unsigned char *>( Jall + 3 * vol * W ) + (size_t) vol * w;
+    float * queue = reinterpret_cast<float *>( reinterpret_cast<unsigned char *>( Jall + 3 * vol * W ) +
+                                               ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) ) +
+                    (size_t) w * QCAP * QWORDS;
     __shared__ int s_cnt[9];
     __shared__ double s_en[FAST_BLOCK / WAVE];
     __shared__ int s_nl[FAST_BLOCK / WAVE];
@@ -402,9 +451,9 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
     const int end = min( np, beg + chunk );
 
     // tile base pointers (uniform) + 32-bit particle index
-    const gptr<int2> pix = gbl( P.ix + off );
-    const gptr<float2> px = gbl( P.x + off );
-    const gptr<float3> pu = gbl( P.u + off );
+    const TileArray<int2> pix{ (gptr<char>) gbl( P.ix + off ) };
+    const TileArray<float2> px{ (gptr<char>) gbl( P.x + off ) };
+    const TileArray<float3> pu{ (gptr<char>) gbl( P.u + off ) };
     const gptr<int> sidx = gbl( S.idx + off );
 
     const float * e0x = Ex + g.off; const float * e0y = Ey + g.off; const float * e0z = Ez + g.off;
@@ -416,15 +465,36 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
 
     double en = 0.0;
     int nl = 0;
+    int qn = 0;  // queued crossing paths of this wave
+    volatile lds_f32 * q = (volatile lds_f32 *) queue;
+
+    // split + deposit of the first `count` queued paths, one per lane
+    auto flush_queue = [&]( int count ) {
+        Seg sg[3];
+        int ns = 0;
+        if ( lane < count ) {
+            const int cp = __float_as_int( q[lane] );
+            const int2 c0 = make_int2( cp & 0xffff, cp >> 16 );
+            const float2 x0 = make_float2( q[QCAP + lane], q[2 * QCAP + lane] );
+            const float2 d = make_float2( q[3 * QCAP + lane], q[4 * QCAP + lane] );
+            const float qvz = q[5 * QCAP + lane];
+            const float2 x1 = make_float2( x0.x + d.x, x0.y + d.y );
+            int2 di;
+            ns = split_path( c0, x0, x1, d, qvz, di, sg );
+        }
+        deposit_elected( ns > 0, sg[0], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
+        if ( __ballot( ns > 1 ) ) deposit_elected( ns > 1, sg[1], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
+        if ( __ballot( ns > 2 ) ) deposit_elected( ns > 2, sg[2], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
+    };
 
     // the next 64 particles are loaded while the current ones are processed
     int2 cn = make_int2( 0, 0 );
     float2 xn = make_float2( 0.f, 0.f );
     float3 un = make_float3( 0.f, 0.f, 0.f );
     if ( beg + lane < end ) {
-        cn = pix[(unsigned) ( beg + lane )];
-        xn = px[(unsigned) ( beg + lane )];
-        un = pu[(unsigned) ( beg + lane )];
+        cn = pix.get( beg + lane );
+        xn = px.get( beg + lane );
+        un = pu.get( beg + lane );
     }
 
     for ( int i0 = beg; i0 < end; i0 += WAVE ) {
@@ -434,9 +504,9 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
         float2 x = xn;
         float3 u = un;
         if ( i + WAVE < end ) {
-            cn = pix[(unsigned) ( i + WAVE )];
-            xn = px[(unsigned) ( i + WAVE )];
-            un = pu[(unsigned) ( i + WAVE )];
+            cn = pix.get( i + WAVE );
+            xn = px.get( i + WAVE );
+            un = pu.get( i + WAVE );
         }
         if constexpr ( PUSH ) {
             float ene = 0.0f;
@@ -444,7 +514,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 float3 e, b;
                 interpolate( e0x, e0y, e0z, b0x, b0y, b0z, ys, c, x, e, b );
                 u = boris( alpha, e, b, u, ene );
-                pu[(unsigned) i] = u;
+                pu.set( i, u );
             }
             if constexpr ( DET ) {
                 // energy += term, in particle order
@@ -456,7 +526,62 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 en += ene;
             }
         }
-        if constexpr ( MOVE ) {
+        if constexpr ( MOVE && !DET ) {
+            int dir = 4;
+            bool cross = false;
+            Seg s0;
+            float2 d = make_float2( 0.f, 0.f );
+            if ( act ) {
+                const float rg = inv_gamma( u );
+                d = make_float2( m.dtdx * rg * u.x, m.dtdy * rg * u.y );
+                s0.c = c;
+                s0.a = x;
+                s0.b = make_float2( x.x + d.x, x.y + d.y );
+                s0.qvz = m.q * u.z * rg * 0.5f;
+                // deltai (species.cpp:342-343)
+                const int2 di = make_int2( ( s0.b.x >= 0.5f ) - ( s0.b.x < -0.5f ), ( s0.b.y >= 0.5f ) - ( s0.b.y < -0.5f ) );
+                cross = di.x != 0 || di.y != 0;
+                const int2 c1 = make_int2( c.x + di.x, c.y + di.y );
+                px.set( i, make_float2( s0.b.x - di.x, s0.b.y - di.y ) );
+                pix.set( i, c1 );
+                const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
+                const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
+                dir = ( yc + 1 ) * 3 + ( xc + 1 );
+            }
+            // paths inside one cell: their single segment now
+            deposit_elected( act && !cross, s0, m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
+            // paths that cross: queued, split and deposited 64 at a time
+            const unsigned long long cm = __ballot( cross );
+            if ( cm ) {
+                const int nc = __popcll( cm );
+                const int pos = qn + __popcll( cm & lanes_below() );
+                auto put = [&]( int k ) {
+                    q[k] = __int_as_float( ( s0.c.y << 16 ) | s0.c.x );
+                    q[QCAP + k] = s0.a.x;
+                    q[2 * QCAP + k] = s0.a.y;
+                    q[3 * QCAP + k] = d.x;
+                    q[4 * QCAP + k] = d.y;
+                    q[5 * QCAP + k] = s0.qvz;
+                };
+                if ( cross && pos < QCAP ) put( pos );
+                if ( qn + nc >= QCAP ) {
+                    flush_queue( QCAP );
+                    if ( cross && pos >= QCAP ) put( pos - QCAP );
+                    qn += nc - QCAP;
+                } else {
+                    qn += nc;
+                }
+            }
+            const bool leave = act && dir != 4;
+            const unsigned long long mask = __ballot( leave );
+            if ( leave ) {
+                const int pos = nl + __popcll( mask & lanes_below() );
+                sidx[beg + pos] = ( i << 4 ) | dir;
+                atomicAdd( &s_cnt[dir], 1 );
+            }
+            nl += __popcll( mask );
+        }
+        if constexpr ( MOVE && DET ) {
             int dir = 4;
             int ns = 0;
             Seg sg[3];
@@ -470,13 +595,13 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 x1.x -= di.x;
                 x1.y -= di.y;
                 const int2 c1 = make_int2( c.x + di.x, c.y + di.y );
-                px[(unsigned) i] = x1;
-                pix[(unsigned) i] = c1;
+                px.set( i, x1 );
+                pix.set( i, c1 );
                 const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
                 const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
                 dir = ( yc + 1 ) * 3 + ( xc + 1 );
             }
-            if constexpr ( DET ) {
+            {
                 // particle order: lane l deposits all its segments before lane l+1
                 for ( int l = 0; l < WAVE; l++ ) {
                     if ( lane == l && ns > 0 ) {
@@ -493,12 +618,6 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                         }
                     }
                 }
-            } else {
-                deposit_elected( ns > 0, sg[0], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
-                if ( __ballot( ns > 1 ) )
-                    deposit_elected( ns > 1, sg[1], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
-                if ( __ballot( ns > 2 ) )
-                    deposit_elected( ns > 2, sg[2], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
             }
             const bool leave = act && dir != 4;
             const unsigned long long mask = __ballot( leave );
@@ -509,6 +628,9 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
             }
             nl += __popcll( mask );
         }
+    }
+    if constexpr ( MOVE && !DET ) {
+        if ( qn > 0 ) flush_queue( qn );
     }
 
     if constexpr ( PUSH ) {

@@ -29,17 +29,18 @@ LO_PPC = (2, 2)
 STRIDE = (HI_PPC[0] * HI_PPC[1]) // (LO_PPC[0] * LO_PPC[1])  # 256
 
 
-def weibel_grid(n, ppc, seed=0, filter=None, uth=(0.1, 0.1, 0.1), ufl=(0.0, 0.0, 0.6)):
-    """The Weibel case on an n x n grid (dx = 0.1, dt = 0.07, 16x16 tiles)."""
-    return zp.weibel(ntiles=(n // 16, n // 16), nx=(16, 16), ppc=ppc, uth=uth, ufl=ufl, filter=filter, seed=seed)
+def weibel_grid(n, ppc, seed=0, filter=None, uth=(0.1, 0.1, 0.1), ufl=(0.0, 0.0, 0.6), tile=16):
+    """The Weibel case on an n x n grid (dx = 0.1, dt = 0.07, tile x tile tiles)."""
+    return zp.weibel(ntiles=(n // tile, n // tile), nx=(tile, tile), ppc=ppc, uth=uth, ufl=ufl, filter=filter,
+                     seed=seed)
 
 
-def sample_pair(hi, lo, dn, phase):
-    """Stores (J of every STRIDE-th particle, J of all particles) of hi's next
+def sample_pair(hi, lo, dn, phase, stride=STRIDE):
+    """Stores (J of every stride-th particle, J of all particles) of hi's next
     step in dn's pair store and advances hi by that step."""
     lo.copy_fields_from(hi)
     for sl, sh in zip(lo.species, hi.species):
-        sl.subsample_from(sh, STRIDE, phase)
+        sl.subsample_from(sh, stride, phase)
     lo.stage("zero_j")
     for k in range(len(lo.species)):
         lo.stage("species", k)
@@ -99,12 +100,17 @@ def evaluate(dn, pairs):
 
 class ShadowCheck:
     """High-ppc shadow box that periodically checks the network against the
-    binomial filter and switches `sim` to the fallback when it loses."""
+    binomial filter and switches `sim` to the fallback when it loses.
 
-    def __init__(self, sim, params, scale, n=64, every=25, tol=0.05, seed=7):
+    The box is small and cut into small tiles so that it is cheap beside the
+    production run (default: 64x64 cells in 8x8 tiles, 16x16 ppc, checked
+    against subsamples of every 64th particle, i.e. 2x2 ppc)."""
+
+    def __init__(self, sim, params, scale, n=64, every=25, tol=0.05, seed=7, hi_ppc=(16, 16), tile=8):
         self.sim, self.every, self.tol = sim, every, tol
-        self.hi = weibel_grid(n, HI_PPC, seed=seed)
-        self.lo = weibel_grid(n, LO_PPC)
+        self.stride = (hi_ppc[0] * hi_ppc[1]) // (LO_PPC[0] * LO_PPC[1])
+        self.hi = weibel_grid(n, hi_ppc, seed=seed, tile=tile)
+        self.lo = weibel_grid(n, LO_PPC, tile=tile)
         self.dn = zp.Denoiser(n, n, max_pairs=1, params=params, scale=scale)
         self.rng = np.random.RandomState(seed)
         self.step = 0
@@ -114,7 +120,7 @@ class ShadowCheck:
         """One step of the shadow box (checking on every `every`-th)."""
         if self.step % self.every == 0 and not self.sim.get_fallback():
             self.dn.clear_pairs()
-            sample_pair(self.hi, self.lo, self.dn, int(self.rng.randint(STRIDE)))
+            sample_pair(self.hi, self.lo, self.dn, int(self.rng.randint(self.stride)), self.stride)
             net, _ = self.dn.eval_pair(0)
             a, t = self.dn.pair(0)
             ref = float(np.mean((binomial_x(a) - t) ** 2))
@@ -132,17 +138,23 @@ class ShadowCheck:
         self.dn.close()
 
 
-def growth_rate(eb, dt, lo_factor=30.0, hi_factor=30.0):
+def growth_rate(eb, dt, lo_factor=30.0, hi_factor=30.0, band=None):
     """Growth rate of the field amplitude (half the slope of ln E_B) fitted
-    over the linear phase: from the first step where E_B exceeds lo_factor
-    times its initial level to the first step where it reaches 1/hi_factor of
-    its maximum.  Returns (gamma, first, last) or (nan, -1, -1)."""
+    over the linear phase.  Default window: from the first step where E_B
+    exceeds lo_factor times its initial level to the first step where it
+    reaches 1/hi_factor of its maximum.  With band=(lo, hi): from the first
+    step where E_B exceeds lo to the first where it reaches hi (the same
+    amplitude band for runs with different noise floors).  Returns (gamma,
+    first, last) or (nan, -1, -1)."""
     eb = np.asarray(eb, np.float64)
-    e0 = max(np.median(eb[:5]), 1e-300)
-    top = eb.max()
-    a = int(np.argmax(eb > lo_factor * e0))
-    b = int(np.argmax(eb >= top / hi_factor))
-    if not (eb[a] > lo_factor * e0) or b - a < 5:
+    if band is None:
+        lo = lo_factor * max(np.median(eb[:5]), 1e-300)
+        hi = eb.max() / hi_factor
+    else:
+        lo, hi = band
+    a = int(np.argmax(eb > lo))
+    b = int(np.argmax(eb >= hi))
+    if not (eb[a] > lo) or not (eb[b] >= hi) or b - a < 5:
         return float("nan"), -1, -1
     t = np.arange(a, b + 1) * dt
     slope = np.polyfit(t, np.log(eb[a:b + 1]), 1)[0]

@@ -34,65 +34,76 @@ def log(msg):
     print(f"[train] {msg}", flush=True)
 
 
-def timed_steps(sim, steps):
-    """Wall seconds per step of `steps` steps (after one warm-up), no diagnostics."""
-    sim.advance(1)
+def wall_per_step(make, steps, shadow_args=None):
+    """Wall seconds per step of a fresh run of `steps` steps (no
+    diagnostics); with shadow_args, the shadow box advances beside it."""
+    sim = make()
+    sh = denoise.ShadowCheck(sim, **shadow_args) if shadow_args else None
     sim.sync()
     t0 = time.perf_counter()
-    sim.advance(steps)
+    if sh is None:
+        sim.advance(steps)
+    else:
+        for _ in range(steps):
+            sim.advance(1)
+            sh.advance()
     sim.sync()
-    return (time.perf_counter() - t0) / steps
+    sec = (time.perf_counter() - t0) / steps
+    if sh is not None:
+        sh.close()
+    sim.close()
+    return sec
 
 
 def judge(params, scale, n, steps, shadow_every, shadow_n):
     dt = float(np.float32(0.07))
     res = {"grid": n, "steps": steps}
-    runs = {}
-    # plain high-ppc run: the judge
-    hi = denoise.weibel_grid(n, denoise.HI_PPC)
-    eb, wall = denoise.run_series(hi, steps)
-    runs["plain_32x32"] = (eb, timed_steps(hi, 20))
-    hi.close()
-    log(f"plain 32x32: {wall:.1f} s")
-    for name, filt in (("lo_none", None), ("lo_binomial_x", ("binomial", 0, 1))):
-        lo = denoise.weibel_grid(n, denoise.LO_PPC, filter=filt)
-        eb, wall = denoise.run_series(lo, steps)
-        runs[name] = (eb, timed_steps(lo, 50))
-        lo.close()
-        log(f"{name}: {wall:.1f} s")
     dn = zp.Denoiser(n, n, params=params, scale=scale)
-    lo = denoise.weibel_grid(n, denoise.LO_PPC)
-    lo.set_learned_filter(dn)
-    sh = denoise.ShadowCheck(lo, params, scale, n=shadow_n, every=shadow_every)
-    eb, wall = denoise.run_series(lo, steps, shadow=sh)
-    res["shadow"] = {"checks": len(sh.history), "fallback": bool(lo.get_fallback()),
-                     "first": [list(map(float, h[:3])) for h in sh.history[:3]]}
-    # wall time of the denoised step with its shadow box
-    lo2 = denoise.weibel_grid(n, denoise.LO_PPC)
-    lo2.set_learned_filter(dn)
-    sh2 = denoise.ShadowCheck(lo2, params, scale, n=shadow_n, every=shadow_every)
-    lo2.advance(1)
-    lo2.sync()
-    t0 = time.perf_counter()
-    for _ in range(50):
-        lo2.advance(1)
-        sh2.advance()
-    lo2.sync()
-    runs["lo_learned"] = (eb, (time.perf_counter() - t0) / 50)
-    sh2.close()
-    lo2.close()
-    sh.close()
-    lo.close()
-    dn.close()
-    log(f"lo_learned: {wall:.1f} s")
-    g_ref = denoise.growth_rate(runs["plain_32x32"][0], dt)
-    res["growth_rate_plain"] = g_ref[0]
-    res["window_plain"] = g_ref[1:]
-    for name, (eb, sps) in runs.items():
+
+    def learned():
+        s = denoise.weibel_grid(n, denoise.LO_PPC)
+        s.set_learned_filter(dn)
+        return s
+
+    shadow_args = {"params": params, "scale": scale, "n": shadow_n, "every": shadow_every}
+    makers = {
+        "plain_32x32": lambda: denoise.weibel_grid(n, denoise.HI_PPC),
+        "lo_none": lambda: denoise.weibel_grid(n, denoise.LO_PPC),
+        "lo_binomial_x": lambda: denoise.weibel_grid(n, denoise.LO_PPC, filter=("binomial", 0, 1)),
+        "lo_learned": learned,
+    }
+    series = {}
+    for name, make in makers.items():
+        sim = make()
+        sh = denoise.ShadowCheck(sim, **shadow_args) if name == "lo_learned" else None
+        eb, wall = denoise.run_series(sim, steps, shadow=sh)
+        series[name] = eb
+        if sh is not None:
+            res["shadow"] = {"checks": [[int(h[0]), float(h[1]), float(h[2]), bool(h[3])] for h in sh.history],
+                             "fallback": bool(sim.get_fallback())}
+            sh.close()
+        sim.close()
+        log(f"{name}: {steps} steps with diagnostics in {wall:.1f} s")
+    walls = {name: wall_per_step(make, steps) for name, make in makers.items()}
+    walls["lo_learned_with_shadow"] = wall_per_step(learned, steps, shadow_args)
+    log("ms/step " + ", ".join(f"{k} {1e3 * v:.3f}" for k, v in walls.items()))
+    top = float(series["plain_32x32"].max())
+    band = (top * 1e-4, top / 30.0)
+    ref = denoise.growth_rate(series["plain_32x32"], dt)
+    ref_band = denoise.growth_rate(series["plain_32x32"], dt, band=band)
+    res.update({"growth_rate_plain": ref[0], "window_plain": ref[1:], "band": band,
+                "growth_rate_plain_band": ref_band[0], "window_plain_band": ref_band[1:]})
+    for name, eb in series.items():
         g = denoise.growth_rate(eb, dt)
-        res[name] = {"growth_rate": g[0], "window": g[1:], "rel_err": abs(g[0] - g_ref[0]) / g_ref[0],
-                     "ms_per_step": 1e3 * sps, "eb_first": float(eb[0]), "eb_max": float(eb.max())}
-    res["speedup_learned_vs_plain"] = runs["plain_32x32"][1] / runs["lo_learned"][1]
+        gb = denoise.growth_rate(eb, dt, band=band)
+        res[name] = {"growth_rate": g[0], "window": g[1:], "rel_err": abs(g[0] - ref[0]) / ref[0],
+                     "growth_rate_band": gb[0], "window_band": gb[1:],
+                     "rel_err_band": abs(gb[0] - ref_band[0]) / ref_band[0],
+                     "ms_per_step": 1e3 * walls[name], "eb_first": float(eb[0]), "eb_max": float(eb.max())}
+    res["lo_learned"]["ms_per_step_with_shadow"] = 1e3 * walls["lo_learned_with_shadow"]
+    res["speedup_learned_vs_plain"] = walls["plain_32x32"] / walls["lo_learned"]
+    res["speedup_learned_with_shadow_vs_plain"] = walls["plain_32x32"] / walls["lo_learned_with_shadow"]
+    dn.close()
     return res
 
 
@@ -100,9 +111,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--seeds", type=int, default=32)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--every", type=int, default=25)
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-steps", type=int, default=12000)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default="denoiser.npz")
     ap.add_argument("--judge", action="store_true")
