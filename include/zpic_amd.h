@@ -78,6 +78,8 @@ typedef struct {
     int *seg;       /* [16*ntiles]  leavers found by each wave segment              */
     int *tmp_np;    /* [ntiles]     staged particles per tile                       */
     int *total;     /* [1]          particles after the sort                        */
+    unsigned long long *nmove; /* [1] running sum of the particles each sort's
+                                  pass moved (Species::get_nmove); may be NULL */
 } zpic_sort;
 
 /* ----------------------------------------------- stage entry points (1) */
@@ -109,6 +111,14 @@ int zpic_push_move_deposit(const zpic_particles *p, const zpic_geom *fld, const 
  * classification left in `s` by zpic_move_deposit / zpic_push_move_deposit.
  * `tmp` supplies the staging buffers and receives the new offsets.  The
  * resulting within-tile order is the reference's single-thread order. */
+/* As zpic_push_move_deposit, then Species::process_bc (species.cpp:1854-1989)
+ * on the moved particles: reflect is a mask of reflecting walls (1 x lower,
+ * 2 x upper, 4 y lower, 8 y upper); particles crossing an open wall are
+ * dropped by the next zpic_tile_sort with that axis not periodic. */
+int zpic_push_move_deposit_bc(const zpic_particles *p, const zpic_geom *fld, const float *E, const float *B,
+                              float *J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
+                              double *tile_energy, zpic_sort *s, int reflect, int deterministic, void *stream);
+
 int zpic_tile_sort(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, int ntiles_x, int ntiles_y,
                    int nx, int ny, int periodic_x, int periodic_y, void *stream);
 
@@ -160,6 +170,14 @@ int zpic_deposit_charge(const zpic_particles *p, const zpic_geom *g, float *rho,
 int zpic_phasespace(const zpic_particles *p, int ntiles_x, int ntiles_y, int nx, int ny, int quant0,
                     float min0, float max0, int size0, int quant1, float min1, float max1, int size1,
                     float norm, float *out, int deterministic, void *stream);
+
+/* Physical boundaries of the fields, after their guard cells were filled
+ * with the axis' periodic flag off: EMF::process_bc (emf.cpp:207-471; bc
+ * values 2 pec, 3 pmc) on E and B, Current::process_bc (current.cpp:53-218;
+ * 2 reflecting) on J.  bc[4] = {x lower, x upper, y lower, y upper} (0 none,
+ * 1 periodic: nothing to do). */
+int zpic_emf_bc(float *E, float *B, const zpic_geom *g, const int bc[4], void *stream);
+int zpic_current_bc(float *J, const zpic_geom *g, const int bc[4], void *stream);
 
 /* vec3grid::gather (vec3grid.h:76-158): component fc (0,1,2) of the interior
  * cells into a contiguous [ny*ntiles_y][nx*ntiles_x] device array. */
@@ -399,6 +417,15 @@ int zpic_denoiser_reldiff(zpic_denoiser *d, const float *a, const float *b, size
 /* Stores (in's current J, target's current J) as a training pair; both
  * simulations must use the same grid and tiling. */
 int zpic_simulation_add_pair(zpic_denoiser *d, zpic_simulation *in, zpic_simulation *target);
+
+/* Boundary conditions (Species::set_bc species.h:210-237, EMF::set_bc
+ * emf.h:112-143, Current::set_bc current.h:84-114): bc[4] = {x lower,
+ * x upper, y lower, y upper}; species 0 open, 1 periodic, 2 reflecting; EMF
+ * 0 none, 1 periodic, 2 pec, 3 pmc; current 0 none, 1 periodic, 2
+ * reflecting.  Periodic must be set on both sides of an axis. */
+int zpic_species_set_bc(zpic_species *sp, const int bc[4]);
+int zpic_emf_set_bc(zpic_simulation *sim, const int bc[4]);
+int zpic_current_set_bc(zpic_simulation *sim, const int bc[4]);
 
 /* Learned-filter hooks: the simulation's current filter becomes the
  * denoiser (not owned); set_fallback switches it to the binomial x order-1

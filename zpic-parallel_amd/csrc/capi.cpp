@@ -491,3 +491,52 @@ extern "C" int zpic_simulation_add_pair( zpic_denoiser * d, zpic_simulation * in
                                        zpic::stream() );
     } );
 }
+
+extern "C" {
+
+int zpic_species_set_bc( zpic_species * h, const int bc[4] ) {
+    return guarded( [&] {
+        if ( !h || !bc ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
+        for ( int k = 0; k < 4; k++ )
+            if ( bc[k] < 0 || bc[k] > 2 ) return zpic::set_error( ZPIC_EINVAL, "species bc must be 0 open, 1 periodic, 2 reflecting" );
+        species::bc_type b;
+        b.x.lower = (species::bc::type) bc[0];
+        b.x.upper = (species::bc::type) bc[1];
+        b.y.lower = (species::bc::type) bc[2];
+        b.y.upper = (species::bc::type) bc[3];
+        h->sp->set_bc( b );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_emf_set_bc( zpic_simulation * s, const int bc[4] ) {
+    return guarded( [&] {
+        if ( !s || !bc ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
+        for ( int k = 0; k < 4; k++ )
+            if ( bc[k] < 0 || bc[k] > 3 ) return zpic::set_error( ZPIC_EINVAL, "emf bc must be 0 none, 1 periodic, 2 pec, 3 pmc" );
+        emf::bc_type b;
+        b.x.lower = (emf::bc::type) bc[0];
+        b.x.upper = (emf::bc::type) bc[1];
+        b.y.lower = (emf::bc::type) bc[2];
+        b.y.upper = (emf::bc::type) bc[3];
+        s->sim->emf.set_bc( b );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_current_set_bc( zpic_simulation * s, const int bc[4] ) {
+    return guarded( [&] {
+        if ( !s || !bc ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
+        for ( int k = 0; k < 4; k++ )
+            if ( bc[k] < 0 || bc[k] > 2 ) return zpic::set_error( ZPIC_EINVAL, "current bc must be 0 none, 1 periodic, 2 reflecting" );
+        current::bc_type b;
+        b.x.lower = (current::bc::type) bc[0];
+        b.x.upper = (current::bc::type) bc[1];
+        b.y.lower = (current::bc::type) bc[2];
+        b.y.upper = (current::bc::type) bc[3];
+        s->sim->current.set_bc( b );
+        return ZPIC_OK;
+    } );
+}
+
+}  // extern "C"

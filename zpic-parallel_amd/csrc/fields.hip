@@ -340,6 +340,208 @@ __global__ void k_fill( float * __restrict__ buf, size_t n, float v ) {
         buf[k] = v;
 }
 
+// ------------------------------------------------------ physical boundaries
+// Field values of a tile through its (x = gc lower) or (y = gc lower) origin,
+// as the reference's emf_bcx / emf_bcy / current_bcx / current_bcy address
+// them (emf.cpp:207-471, current.cpp:53-176).  bc[4] = {x lower, x upper,
+// y lower, y upper} with the reference's enum values (emf: 2 pec, 3 pmc;
+// current: 2 reflecting).  One workgroup per boundary tile; the x pass runs
+// before the y pass (the y pass reads the guard columns the x pass wrote).
+// A single tile along an axis takes only the lower branch, as the
+// reference's `for tx : {0, ntiles.x - 1}` with `tile_idx.x == 0` does.
+struct BcArgs {
+    float * E;  // E, or J for the current
+    float * B;  // null for the current
+    G g;
+    int lo, hi;  // bc of the lower / upper side of this axis
+};
+
+__device__ inline float3 ld3( gptr<float> p, int k ) { return make_float3( p[3 * k], p[3 * k + 1], p[3 * k + 2] ); }
+__device__ inline void st3( gptr<float> p, int k, float3 v ) {
+    p[3 * k] = v.x;
+    p[3 * k + 1] = v.y;
+    p[3 * k + 2] = v.z;
+}
+
+// x pass: blockIdx.y = tile row, blockIdx.x = 0 lower tile, 1 upper tile
+__global__ void k_emf_bcx( BcArgs a ) {
+    const G g = a.g;
+    const int tx = blockIdx.x == 0 ? 0 : g.ntx - 1;
+    const int t = blockIdx.y * g.ntx + tx;
+    const int ys = g.ext_x, nx = g.nx;
+    const gptr<float> E = gbl( a.E ) + ( (size_t) t * g.vol + g.gl ) * 3;
+    const gptr<float> B = gbl( a.B ) + ( (size_t) t * g.vol + g.gl ) * 3;
+    for ( int idx = threadIdx.x; idx < g.ext_y; idx += blockDim.x ) {
+        const int iy = idx * ys;
+        if ( tx == 0 ) {
+            if ( a.lo == 3 ) {  // pmc
+                float3 e = ld3( E, -1 + iy ), b = ld3( B, -1 + iy );
+                e.x = -E[3 * ( 0 + iy )];
+                e.y = E[3 * ( 1 + iy ) + 1];
+                e.z = E[3 * ( 1 + iy ) + 2];
+                b.x = B[3 * ( 1 + iy )];
+                b.y = -B[3 * ( 0 + iy ) + 1];
+                b.z = -B[3 * ( 0 + iy ) + 2];
+                st3( E, -1 + iy, e );
+                st3( B, -1 + iy, b );
+            } else if ( a.lo == 2 ) {  // pec
+                float3 e = ld3( E, -1 + iy ), b = ld3( B, -1 + iy );
+                e.x = E[3 * ( 0 + iy )];
+                e.y = -E[3 * ( 1 + iy ) + 1];
+                e.z = -E[3 * ( 1 + iy ) + 2];
+                b.x = -B[3 * ( 1 + iy )];
+                b.y = B[3 * ( 0 + iy ) + 1];
+                b.z = B[3 * ( 0 + iy ) + 2];
+                st3( E, -1 + iy, e );
+                st3( B, -1 + iy, b );
+                E[3 * ( 0 + iy ) + 1] = 0.0f;
+                E[3 * ( 0 + iy ) + 2] = 0.0f;
+                B[3 * ( 0 + iy )] = 0.0f;
+            }
+        } else {
+            if ( a.hi == 3 ) {  // pmc
+                E[3 * ( nx + iy )] = -E[3 * ( nx - 1 + iy )];
+                E[3 * ( nx + 1 + iy )] = -E[3 * ( nx - 2 + iy )];
+                E[3 * ( nx + 1 + iy ) + 1] = E[3 * ( nx - 1 + iy ) + 1];
+                E[3 * ( nx + 1 + iy ) + 2] = E[3 * ( nx - 1 + iy ) + 2];
+                B[3 * ( nx + iy ) + 1] = -B[3 * ( nx - 1 + iy ) + 1];
+                B[3 * ( nx + iy ) + 2] = -B[3 * ( nx - 1 + iy ) + 2];
+                B[3 * ( nx + 1 + iy )] = B[3 * ( nx - 1 + iy )];
+                B[3 * ( nx + 1 + iy ) + 1] = -B[3 * ( nx - 2 + iy ) + 1];
+                B[3 * ( nx + 1 + iy ) + 2] = -B[3 * ( nx - 2 + iy ) + 2];
+            } else if ( a.hi == 2 ) {  // pec
+                E[3 * ( nx + iy )] = E[3 * ( nx - 1 + iy )];
+                E[3 * ( nx + iy ) + 1] = 0.0f;
+                E[3 * ( nx + iy ) + 2] = 0.0f;
+                E[3 * ( nx + 1 + iy )] = E[3 * ( nx - 2 + iy )];
+                E[3 * ( nx + 1 + iy ) + 1] = -E[3 * ( nx - 1 + iy ) + 1];
+                E[3 * ( nx + 1 + iy ) + 2] = -E[3 * ( nx - 1 + iy ) + 2];
+                B[3 * ( nx + iy )] = 0.0f;
+                B[3 * ( nx + iy ) + 1] = B[3 * ( nx - 1 + iy ) + 1];
+                B[3 * ( nx + iy ) + 2] = B[3 * ( nx - 1 + iy ) + 2];
+                B[3 * ( nx + 1 + iy )] = -B[3 * ( nx - 1 + iy )];
+                B[3 * ( nx + 1 + iy ) + 1] = B[3 * ( nx - 2 + iy ) + 1];
+                B[3 * ( nx + 1 + iy ) + 2] = B[3 * ( nx - 2 + iy ) + 2];
+            }
+        }
+    }
+}
+
+// y pass: blockIdx.x = tile column, blockIdx.y = 0 lower tile, 1 upper tile.
+// The upper formulas keep the reference's rows (some are indexed with nx.x
+// where nx.y is meant, emf.cpp:400-402 and 420-422; equal for square tiles).
+__global__ void k_emf_bcy( BcArgs a ) {
+    const G g = a.g;
+    const int ty = blockIdx.y == 0 ? 0 : g.nty - 1;
+    const int t = ty * g.ntx + blockIdx.x;
+    const int ys = g.ext_x, ny = g.ny, nxx = g.nx;
+    const gptr<float> E = gbl( a.E ) + ( (size_t) t * g.vol + (size_t) g.gl * ys ) * 3;
+    const gptr<float> B = gbl( a.B ) + ( (size_t) t * g.vol + (size_t) g.gl * ys ) * 3;
+    for ( int ix = threadIdx.x; ix < g.ext_x; ix += blockDim.x ) {
+        if ( ty == 0 ) {
+            if ( a.lo == 3 ) {  // pmc
+                E[3 * ( ix - ys )] = E[3 * ( ix + ys )];
+                E[3 * ( ix - ys ) + 1] = -E[3 * ix + 1];
+                E[3 * ( ix - ys ) + 2] = E[3 * ( ix + ys ) + 2];
+                B[3 * ( ix - ys )] = -B[3 * ix];
+                B[3 * ( ix - ys ) + 1] = B[3 * ( ix + ys ) + 1];
+                B[3 * ( ix - ys ) + 2] = -B[3 * ix + 2];
+            } else if ( a.lo == 2 ) {  // pec
+                E[3 * ( ix - ys )] = -E[3 * ( ix + ys )];
+                E[3 * ( ix - ys ) + 1] = E[3 * ix + 1];
+                E[3 * ( ix - ys ) + 2] = -E[3 * ( ix + ys ) + 2];
+                E[3 * ix] = 0.0f;
+                E[3 * ix + 2] = 0.0f;
+                B[3 * ( ix - ys )] = B[3 * ix];
+                B[3 * ( ix - ys ) + 1] = -B[3 * ( ix + ys ) + 1];
+                B[3 * ( ix - ys ) + 2] = B[3 * ix + 2];
+                B[3 * ix + 1] = 0.0f;
+            }
+        } else {
+            if ( a.hi == 3 ) {  // pmc
+                E[3 * ( ix + ny * ys ) + 1] = -E[3 * ( ix + ( ny - 1 ) * ys ) + 1];
+                E[3 * ( ix + ( ny + 1 ) * ys )] = E[3 * ( ix + ( ny - 1 ) * ys )];
+                E[3 * ( ix + ( ny + 1 ) * ys ) + 1] = -E[3 * ( ix + ( ny - 2 ) * ys ) + 1];
+                E[3 * ( ix + ( ny + 1 ) * ys ) + 2] = E[3 * ( ix + ( ny - 1 ) * ys ) + 2];
+                B[3 * ( ix + ny * ys )] = -B[3 * ( ix + ( ny - 1 ) * ys )];
+                B[3 * ( ix + ny * ys ) + 2] = -B[3 * ( ix + ( ny - 1 ) * ys ) + 2];
+                B[3 * ( ix + ( ny + 1 ) * ys )] = -B[3 * ( ix + ( nxx - 2 ) * ys )];
+                B[3 * ( ix + ( ny + 1 ) * ys ) + 1] = B[3 * ( ix + ( nxx - 1 ) * ys ) + 1];
+                B[3 * ( ix + ( ny + 1 ) * ys ) + 2] = -B[3 * ( ix + ( nxx - 2 ) * ys ) + 2];
+            } else if ( a.hi == 2 ) {  // pec
+                E[3 * ( ix + ny * ys )] = 0.0f;
+                E[3 * ( ix + ny * ys ) + 1] = E[3 * ( ix + ( ny - 1 ) * ys ) + 1];
+                E[3 * ( ix + ny * ys ) + 2] = 0.0f;
+                E[3 * ( ix + ( ny + 1 ) * ys )] = -E[3 * ( ix + ( nxx - 1 ) * ys )];
+                E[3 * ( ix + ( ny + 1 ) * ys ) + 1] = E[3 * ( ix + ( nxx - 2 ) * ys ) + 1];
+                E[3 * ( ix + ( ny + 1 ) * ys ) + 2] = -E[3 * ( ix + ( nxx - 1 ) * ys ) + 2];
+                B[3 * ( ix + ny * ys )] = B[3 * ( ix + ( ny - 1 ) * ys )];
+                B[3 * ( ix + ny * ys ) + 1] = 0.0f;
+                B[3 * ( ix + ny * ys ) + 2] = B[3 * ( ix + ( ny - 1 ) * ys ) + 2];
+                B[3 * ( ix + ( ny + 1 ) * ys )] = B[3 * ( ix + ( ny - 2 ) * ys )];
+                B[3 * ( ix + ( ny + 1 ) * ys ) + 1] = -B[3 * ( ix + ( ny - 1 ) * ys ) + 1];
+                B[3 * ( ix + ( ny + 1 ) * ys ) + 2] = B[3 * ( ix + ( ny - 2 ) * ys ) + 2];
+            }
+        }
+    }
+}
+
+// current reflecting boundaries (current.cpp:53-124 x, 127-176 y)
+__global__ void k_current_bcx( BcArgs a ) {
+    const G g = a.g;
+    const int tx = blockIdx.x == 0 ? 0 : g.ntx - 1;
+    const int t = blockIdx.y * g.ntx + tx;
+    const int ys = g.ext_x, nx = g.nx;
+    const gptr<float> J = gbl( a.E ) + ( (size_t) t * g.vol + g.gl ) * 3;
+    for ( int idx = threadIdx.x; idx < g.ext_y; idx += blockDim.x ) {
+        const int iy = idx * ys;
+        if ( tx == 0 ) {
+            if ( a.lo == 2 ) {
+                const float jx0 = -J[3 * ( -1 + iy )] + J[3 * ( 0 + iy )];
+                const float jy1 = J[3 * ( -1 + iy ) + 1] + J[3 * ( 1 + iy ) + 1];
+                const float jz1 = J[3 * ( -1 + iy ) + 2] + J[3 * ( 1 + iy ) + 2];
+                J[3 * ( -1 + iy )] = J[3 * ( 0 + iy )] = jx0;
+                J[3 * ( -1 + iy ) + 1] = J[3 * ( 1 + iy ) + 1] = jy1;
+                J[3 * ( -1 + iy ) + 2] = J[3 * ( 1 + iy ) + 2] = jz1;
+            }
+        } else if ( a.hi == 2 ) {
+            const float jx0 = J[3 * ( nx - 1 + iy )] - J[3 * ( nx + iy )];
+            const float jy1 = J[3 * ( nx - 1 + iy ) + 1] + J[3 * ( nx + 1 + iy ) + 1];
+            const float jz1 = J[3 * ( nx - 1 + iy ) + 2] + J[3 * ( nx + 1 + iy ) + 2];
+            J[3 * ( nx - 1 + iy )] = J[3 * ( nx + iy )] = jx0;
+            J[3 * ( nx - 1 + iy ) + 1] = J[3 * ( nx + 1 + iy ) + 1] = jy1;
+            J[3 * ( nx - 1 + iy ) + 2] = J[3 * ( nx + 1 + iy ) + 2] = jz1;
+        }
+    }
+}
+
+__global__ void k_current_bcy( BcArgs a ) {
+    const G g = a.g;
+    const int ty = blockIdx.y == 0 ? 0 : g.nty - 1;
+    const int t = ty * g.ntx + blockIdx.x;
+    const int ys = g.ext_x, ny = g.ny;
+    const gptr<float> J = gbl( a.E ) + ( (size_t) t * g.vol + (size_t) g.gl * ys ) * 3;
+    for ( int ix = threadIdx.x; ix < g.ext_x; ix += blockDim.x ) {
+        if ( ty == 0 ) {
+            if ( a.lo == 2 ) {
+                const float jx1 = J[3 * ( ix - ys )] + J[3 * ( ix + ys )];
+                const float jy0 = -J[3 * ( ix - ys ) + 1] + J[3 * ix + 1];
+                const float jz1 = J[3 * ( ix - ys ) + 2] + J[3 * ( ix + ys ) + 2];
+                J[3 * ( ix - ys )] = J[3 * ( ix + ys )] = jx1;
+                J[3 * ( ix - ys ) + 1] = J[3 * ix + 1] = jy0;
+                J[3 * ( ix - ys ) + 2] = J[3 * ( ix + ys ) + 2] = jz1;
+            }
+        } else if ( a.hi == 2 ) {
+            const float jx1 = J[3 * ( ix + ( ny - 1 ) * ys )] + J[3 * ( ix + ( ny + 1 ) * ys )];
+            const float jy0 = J[3 * ( ix + ( ny - 1 ) * ys ) + 1] - J[3 * ( ix + ny * ys ) + 1];
+            const float jz1 = J[3 * ( ix + ( ny - 1 ) * ys ) + 2] + J[3 * ( ix + ( ny + 1 ) * ys ) + 2];
+            J[3 * ( ix + ( ny - 1 ) * ys )] = J[3 * ( ix + ( ny + 1 ) * ys )] = jx1;
+            J[3 * ( ix + ( ny - 1 ) * ys ) + 1] = J[3 * ( ix + ny * ys ) + 1] = jy0;
+            J[3 * ( ix + ( ny - 1 ) * ys ) + 2] = J[3 * ( ix + ( ny + 1 ) * ys ) + 2] = jz1;
+        }
+    }
+}
+
 int check( const zpic_geom * g ) {
     if ( !g || g->ntiles_x < 1 || g->ntiles_y < 1 || g->nx < 1 || g->ny < 1 || g->gc_lower < 0 || g->gc_upper < 0 )
         return set_error( ZPIC_EINVAL, "invalid grid geometry" );
@@ -616,6 +818,44 @@ int zpic_yee( float * E, float * B, const float * J, const zpic_geom * gg, float
     ZPIC_LAUNCH_CHECK( "k_yee" );
     if ( int rc = zpic_copy_to_gc( E, gg, 3, px, py, stream ) ) return rc;
     return zpic_copy_to_gc( B, gg, 3, px, py, stream );
+}
+
+// EMF::process_bc / Current::process_bc (emf.cpp:431-471, current.cpp:178-218).
+static int physical_bc( float * F, float * Bf, const zpic_geom * gg, const int bc[4], bool emf, void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !F || ( emf && !Bf ) || !bc ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( gg->gc_lower < 1 || gg->gc_upper < 2 ) return set_error( ZPIC_EINVAL, "boundaries need guard cells {1,2}" );
+    if ( gg->nx < 2 || gg->ny < 2 ) return set_error( ZPIC_EINVAL, "boundaries need tiles of at least 2x2 cells" );
+    // the reference's upper-y EMF formulas read rows nx.x - 1, nx.x - 2, which
+    // must stay inside the tile (rows up to ny + 2)
+    if ( emf && ( bc[3] == 2 || bc[3] == 3 ) && gg->nx > gg->ny + 2 )
+        return set_error( ZPIC_EINVAL, "upper y pec/pmc needs tile nx <= ny + 2 (reference row indexing)" );
+    const G g = make( gg );
+    hipStream_t st = (hipStream_t) stream;
+    // bc values: 0 none, 1 periodic, >1 physical
+    if ( bc[0] > 1 || bc[1] > 1 ) {
+        const BcArgs a{ F, Bf, g, bc[0], bc[1] };
+        const dim3 grid( g.ntx > 1 ? 2 : 1, g.nty );
+        if ( emf ) hipLaunchKernelGGL( k_emf_bcx, grid, dim3( 64 ), 0, st, a );
+        else hipLaunchKernelGGL( k_current_bcx, grid, dim3( 64 ), 0, st, a );
+        ZPIC_LAUNCH_CHECK( "boundary x" );
+    }
+    if ( bc[2] > 1 || bc[3] > 1 ) {
+        const BcArgs a{ F, Bf, g, bc[2], bc[3] };
+        const dim3 grid( g.ntx, g.nty > 1 ? 2 : 1 );
+        if ( emf ) hipLaunchKernelGGL( k_emf_bcy, grid, dim3( 64 ), 0, st, a );
+        else hipLaunchKernelGGL( k_current_bcy, grid, dim3( 64 ), 0, st, a );
+        ZPIC_LAUNCH_CHECK( "boundary y" );
+    }
+    return ZPIC_OK;
+}
+
+int zpic_emf_bc( float * E, float * B, const zpic_geom * g, const int bc[4], void * stream ) {
+    return physical_bc( E, B, g, bc, true, stream );
+}
+
+int zpic_current_bc( float * J, const zpic_geom * g, const int bc[4], void * stream ) {
+    return physical_bc( J, nullptr, g, bc, false, stream );
 }
 
 int zpic_field_energy( const float * E, const float * B, const zpic_geom * gg, double * scratch, double * out6,

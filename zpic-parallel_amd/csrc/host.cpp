@@ -157,8 +157,11 @@ ParticleSort::ParticleSort( uint2 const nt2, uint32_t const max_part ) : ntiles(
     buf.seg = device_alloc<int>( 16 * (size_t) ntiles );
     buf.tmp_np = device_alloc<int>( ntiles );
     buf.total = device_alloc<int>( 1 );
+    buf.nmove = device_alloc<unsigned long long>( 1 );
+    hip_check( hipMemsetAsync( buf.nmove, 0, sizeof( unsigned long long ), stream() ), "hipMemsetAsync" );
 }
 ParticleSort::~ParticleSort() {
+    device_free( buf.nmove );
     device_free( buf.total ); device_free( buf.tmp_np ); device_free( buf.seg );
     device_free( buf.new_np ); device_free( buf.npt ); device_free( buf.idx );
 }
@@ -266,18 +269,28 @@ Current::~Current() {
     delete filter;
 }
 
+// Current::set_bc (current.h:84-114)
 void Current::set_bc( current::bc_type new_bc ) {
-    if ( new_bc.x.lower != current::bc::periodic || new_bc.x.upper != current::bc::periodic ||
-         new_bc.y.lower != current::bc::periodic || new_bc.y.upper != current::bc::periodic )
-        throw std::runtime_error( "(*error*) only periodic current boundaries are available in this build" );
+    if ( ( new_bc.x.lower == current::bc::periodic || new_bc.x.upper == current::bc::periodic ) &&
+         new_bc.x.lower != new_bc.x.upper )
+        throw std::runtime_error( "(*error*) Current boundary type mismatch along x." );
+    if ( ( new_bc.y.lower == current::bc::periodic || new_bc.y.upper == current::bc::periodic ) &&
+         new_bc.y.lower != new_bc.y.upper )
+        throw std::runtime_error( "(*error*) Current boundary type mismatch along y." );
     bc = new_bc;
-    J->set_periodic( make_int2( 1, 1 ) );
+    J->set_periodic( make_int2( bc.x.lower == current::bc::periodic, bc.y.lower == current::bc::periodic ) );
 }
 
 // Current::advance (current.cpp:227-246)
 void Current::advance() {
     J->add_from_gc();
     J->copy_to_gc();
+    if ( bc.x.lower > current::bc::periodic || bc.x.upper > current::bc::periodic ||
+         bc.y.lower > current::bc::periodic || bc.y.upper > current::bc::periodic ) {
+        const int b[4] = { bc.x.lower, bc.x.upper, bc.y.lower, bc.y.upper };
+        zpic_geom g = J->geom();
+        check( zpic_current_bc( J->fbuf(), &g, b, stream() ) );
+    }
     filter->apply( *J );
     iter++;
 }
@@ -311,11 +324,28 @@ EMF::~EMF() {
     delete B;
 }
 
+// EMF::set_bc (emf.h:112-143)
 void EMF::set_bc( emf::bc_type new_bc ) {
-    if ( new_bc.x.lower != emf::bc::periodic || new_bc.x.upper != emf::bc::periodic ||
-         new_bc.y.lower != emf::bc::periodic || new_bc.y.upper != emf::bc::periodic )
-        throw std::runtime_error( "(*error*) only periodic EMF boundaries are available in this build" );
+    if ( ( new_bc.x.lower == emf::bc::periodic || new_bc.x.upper == emf::bc::periodic ) &&
+         new_bc.x.lower != new_bc.x.upper )
+        throw std::runtime_error( "(*error*) EMF boundary type mismatch along x." );
+    if ( ( new_bc.y.lower == emf::bc::periodic || new_bc.y.upper == emf::bc::periodic ) &&
+         new_bc.y.lower != new_bc.y.upper )
+        throw std::runtime_error( "(*error*) EMF boundary type mismatch along y." );
     bc = new_bc;
+    const int2 p = make_int2( bc.x.lower == emf::bc::periodic, bc.y.lower == emf::bc::periodic );
+    E->set_periodic( p );
+    B->set_periodic( p );
+}
+
+// EMF::process_bc (emf.cpp:431-471)
+void EMF::process_bc() {
+    if ( bc.x.lower > emf::bc::periodic || bc.x.upper > emf::bc::periodic || bc.y.lower > emf::bc::periodic ||
+         bc.y.upper > emf::bc::periodic ) {
+        const int b[4] = { bc.x.lower, bc.x.upper, bc.y.lower, bc.y.upper };
+        zpic_geom g = E->geom();
+        check( zpic_emf_bc( E->fbuf(), B->fbuf(), &g, b, stream() ) );
+    }
 }
 
 // EMF::advance(Current&) (emf.cpp:513-580): dt/dx is evaluated in double and
@@ -325,6 +355,7 @@ void EMF::advance( Current & current ) {
     zpic_geom g = E->geom();
     const int2 p = E->get_periodic();
     check( zpic_yee( E->fbuf(), B->fbuf(), current.J->fbuf(), &g, dtdx, dtdy, (float) dt, p.x, p.y, stream() ) );
+    process_bc();
     iter += 1;
 }
 
@@ -374,11 +405,32 @@ Species::~Species() {
     delete udist;
 }
 
+// Species::set_bc (species.h:210-237)
 void Species::set_bc( species::bc_type new_bc ) {
-    if ( new_bc.x.lower != species::bc::periodic || new_bc.x.upper != species::bc::periodic ||
-         new_bc.y.lower != species::bc::periodic || new_bc.y.upper != species::bc::periodic )
-        throw std::runtime_error( "(*error*) only periodic species boundaries are available in this build" );
+    if ( ( new_bc.x.lower == species::bc::periodic || new_bc.x.upper == species::bc::periodic ) &&
+         new_bc.x.lower != new_bc.x.upper )
+        throw std::runtime_error( "(*error*) Species boundary type mismatch along x." );
+    if ( ( new_bc.y.lower == species::bc::periodic || new_bc.y.upper == species::bc::periodic ) &&
+         new_bc.y.lower != new_bc.y.upper )
+        throw std::runtime_error( "(*error*) Species boundary type mismatch along y." );
     bc = new_bc;
+    if ( particles ) {
+        particles->periodic.x = ( bc.x.lower == species::bc::periodic );
+        particles->periodic.y = ( bc.y.lower == species::bc::periodic );
+    }
+}
+
+int Species::reflect_mask() const {
+    return ( bc.x.lower == species::bc::reflecting ? 1 : 0 ) | ( bc.x.upper == species::bc::reflecting ? 2 : 0 ) |
+           ( bc.y.lower == species::bc::reflecting ? 4 : 0 ) | ( bc.y.upper == species::bc::reflecting ? 8 : 0 );
+}
+
+uint64_t Species::get_nmove() const {
+    if ( !sort ) return 0;
+    unsigned long long n = 0;
+    to_host( &n, sort->buf.nmove, 1 );
+    sync();
+    return n;
 }
 
 // Species::initialize (species.cpp:1730-1788)
@@ -443,7 +495,6 @@ void Species::move( vec3grid<float3> * const J ) {
     zpic_particles p = particles->view();
     check( zpic_move_deposit( &p, &g, J->fbuf(), dt_dx.x, dt_dx.y, q, qnx.x, qnx.y, &sort->buf, deterministic,
                               stream() ) );
-    d_nmove += np_count;
 }
 
 void Species::tile_sort() {
@@ -462,10 +513,10 @@ void Species::advance_timed( EMF const & emf, Current & current, hipEvent_t afte
     const float2 qnx = make_float2( q * dx.x / dt, q * dx.y / dt );
     zpic_geom g = emf.E->geom();
     zpic_particles p = particles->view();
-    check( zpic_push_move_deposit( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha, dt_dx.x, dt_dx.y,
-                                   q, qnx.x, qnx.y, d_tile_energy, &sort->buf, deterministic, stream() ) );
+    check( zpic_push_move_deposit_bc( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha, dt_dx.x,
+                                      dt_dx.y, q, qnx.x, qnx.y, d_tile_energy, &sort->buf, reflect_mask(),
+                                      deterministic, stream() ) );
     if ( after_push ) hip_check( hipEventRecord( after_push, stream() ), "hipEventRecord" );
-    d_nmove += np_count;
     iter++;
     tile_sort();
     check( zpic_sum_tiles( d_tile_energy, g.ntiles_x * g.ntiles_y, d_energy_dev, deterministic, stream() ) );
@@ -700,6 +751,16 @@ void Batch::release() {
 void Batch::add( Simulation & sim ) {
     if ( !sims.empty() && sim.species.size() != sims[0]->species.size() )
         throw std::runtime_error( "(*error*) all instances of a batch must hold the same number of species" );
+    auto periodic = []( int lo, int hi ) { return lo == 1 && hi == 1; };
+    const auto eb = sim.emf.get_bc();
+    const auto cb = sim.current.get_bc();
+    bool ok = periodic( eb.x.lower, eb.x.upper ) && periodic( eb.y.lower, eb.y.upper ) &&
+              periodic( cb.x.lower, cb.x.upper ) && periodic( cb.y.lower, cb.y.upper );
+    for ( auto * s : sim.species ) {
+        const auto sb = s->get_bc();
+        ok = ok && periodic( sb.x.lower, sb.x.upper ) && periodic( sb.y.lower, sb.y.upper );
+    }
+    if ( !ok ) throw std::runtime_error( "(*error*) batches run periodic boundaries only" );
     release();
     sims.push_back( &sim );
 }

@@ -44,10 +44,11 @@ struct SortView {
     int * seg;
     int * tmp_np;
     int * total;
+    unsigned long long * nmove;
 };
 
 struct Geo {
-    int nx, ny, ext_x, ext_y, vol, off;
+    int nx, ny, ext_x, ext_y, vol, off, ntx, nty;
 };
 
 __device__ inline unsigned long long lanes_below() {
@@ -304,7 +305,28 @@ __device__ inline void deposit_elected( bool pend, const Seg & s, float qnx, flo
 
 struct MoveArgs {
     float dtdx, dtdy, q, qnx, qny;
+    int refl;  // reflecting walls: 1 x lower, 2 x upper, 4 y lower, 8 y upper
 };
+
+// Species::process_bc (species.cpp:1854-1989) for one moved particle of
+// tile (tx, ty): a reflecting wall mirrors the cell index, the in-cell
+// position and the momentum component.  As in the reference's loops over
+// tiles {0, ntiles - 1}, a single tile along an axis only sees its lower wall.
+__device__ inline bool reflect_walls( int refl, int tx, int ty, int ntx, int nty, int nx, int ny, int2 & c,
+                                      float2 & x, float3 & u ) {
+    bool r = false;
+    if ( tx == 0 ) {
+        if ( ( refl & 1 ) && c.x < 0 ) { c.x += 1; x.x = -x.x; u.x = -u.x; r = true; }
+    } else if ( tx == ntx - 1 ) {
+        if ( ( refl & 2 ) && c.x >= nx ) { c.x -= 1; x.x = -x.x; u.x = -u.x; r = true; }
+    }
+    if ( ty == 0 ) {
+        if ( ( refl & 4 ) && c.y < 0 ) { c.y += 1; x.y = -x.y; u.y = -u.y; r = true; }
+    } else if ( ty == nty - 1 ) {
+        if ( ( refl & 8 ) && c.y >= ny ) { c.y -= 1; x.y = -x.y; u.y = -u.y; r = true; }
+    }
+    return r;
+}
 
 // ---------------------------------------------------------------------------
 // Particle pass.  PUSH: interpolation + Boris.  MOVE: position advance,
@@ -362,9 +384,11 @@ struct SortG {
     gptr<int> seg;
     gptr<int> tmp_np;
     gptr<int> total;
+    gptr<unsigned long long> nmove;
 };
 __device__ inline SortG gview( const SortView & v ) {
-    return SortG{ gbl( v.idx ), gbl( v.npt ), gbl( v.new_np ), gbl( v.seg ), gbl( v.tmp_np ), gbl( v.total ) };
+    return SortG{ gbl( v.idx ), gbl( v.npt ), gbl( v.new_np ), gbl( v.seg ), gbl( v.tmp_np ), gbl( v.total ),
+                  gbl( v.nmove ) };
 }
 
 // Element i of one tile's particle array through the tile's (uniform) base
@@ -541,8 +565,11 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 // deltai (species.cpp:342-343)
                 const int2 di = make_int2( ( s0.b.x >= 0.5f ) - ( s0.b.x < -0.5f ), ( s0.b.y >= 0.5f ) - ( s0.b.y < -0.5f ) );
                 cross = di.x != 0 || di.y != 0;
-                const int2 c1 = make_int2( c.x + di.x, c.y + di.y );
-                px.set( i, make_float2( s0.b.x - di.x, s0.b.y - di.y ) );
+                int2 c1 = make_int2( c.x + di.x, c.y + di.y );
+                float2 x1 = make_float2( s0.b.x - di.x, s0.b.y - di.y );
+                if ( m.refl && reflect_walls( m.refl, t % g.ntx, t / g.ntx, g.ntx, g.nty, g.nx, g.ny, c1, x1, u ) )
+                    pu.set( i, u );
+                px.set( i, x1 );
                 pix.set( i, c1 );
                 const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
                 const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
@@ -594,7 +621,9 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 ns = split_path( c, x, x1, d, qvz, di, sg );
                 x1.x -= di.x;
                 x1.y -= di.y;
-                const int2 c1 = make_int2( c.x + di.x, c.y + di.y );
+                int2 c1 = make_int2( c.x + di.x, c.y + di.y );
+                if ( m.refl && reflect_walls( m.refl, t % g.ntx, t / g.ntx, g.ntx, g.nty, g.nx, g.ny, c1, x1, u ) )
+                    pu.set( i, u );
                 px.set( i, x1 );
                 pix.set( i, c1 );
                 const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
@@ -756,6 +785,19 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const 
         base += S.new_np[t];
     }
     if ( tid == SCAN_BLOCK - 1 ) S.total[0] = s_sum[tid];
+    // Species::move's d_nmove += np (species.cpp:2162): the counts the
+    // particle pass just moved, i.e. before this sort
+    if ( a.S.nmove ) {
+        __shared__ unsigned long long s_moved;
+        if ( tid == 0 ) s_moved = 0;
+        __syncthreads();
+        const gptr<const int> np_old = gbl( (const int *) a.P.np );
+        unsigned long long mine = 0;
+        for ( int t = t0; t < t1; t++ ) mine += (unsigned) np_old[t];
+        if ( mine ) atomicAdd( &s_moved, mine );
+        __syncthreads();
+        if ( tid == 0 ) S.nmove[0] += s_moved;
+    }
 }
 
 // Stayers that move to the staging buffer because the tile's range moved
@@ -1005,6 +1047,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_charge( PartView P, Geo g, flo
 Geo make_geo( const zpic_geom * f ) {
     Geo g;
     g.nx = f->nx; g.ny = f->ny; g.ext_x = f->ext_x; g.ext_y = f->ext_y; g.vol = f->tile_vol; g.off = f->offset;
+    g.ntx = f->ntiles_x; g.nty = f->ntiles_y;
     return g;
 }
 
@@ -1019,8 +1062,13 @@ PartView view( const zpic_particles * p ) {
 
 SortView view( const zpic_sort * s ) {
     SortView v;
-    if ( s ) { v.idx = s->idx; v.npt = s->npt; v.new_np = s->new_np; v.seg = s->seg; v.tmp_np = s->tmp_np; v.total = s->total; }
-    else { v.idx = v.npt = v.new_np = v.seg = v.tmp_np = v.total = nullptr; }
+    if ( s ) {
+        v.idx = s->idx; v.npt = s->npt; v.new_np = s->new_np; v.seg = s->seg; v.tmp_np = s->tmp_np; v.total = s->total;
+        v.nmove = s->nmove;
+    } else {
+        v.idx = v.npt = v.new_np = v.seg = v.tmp_np = v.total = nullptr;
+        v.nmove = nullptr;
+    }
     return v;
 }
 
@@ -1237,6 +1285,14 @@ int zpic_push_move_deposit( const zpic_particles * p, const zpic_geom * fld, con
                             float * J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
                             double * tile_energy, zpic_sort * s, int deterministic, void * stream ) {
     MoveArgs m{ dt_dx, dt_dy, q, qnx, qny };
+    return launch_particles<true, true>( p, fld, E, B, J, alpha, m, tile_energy, s, deterministic, stream );
+}
+
+int zpic_push_move_deposit_bc( const zpic_particles * p, const zpic_geom * fld, const float * E, const float * B,
+                               float * J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
+                               double * tile_energy, zpic_sort * s, int reflect, int deterministic, void * stream ) {
+    if ( reflect < 0 || reflect > 15 ) return set_error( ZPIC_EINVAL, "reflect mask must be in [0, 15]" );
+    MoveArgs m{ dt_dx, dt_dy, q, qnx, qny, reflect };
     return launch_particles<true, true>( p, fld, E, B, J, alpha, m, tile_energy, s, deterministic, stream );
 }
 

@@ -484,6 +484,8 @@ class EMF {
     void advance();
     void advance( Current & current );
     void get_energy( double3 & ene_E, double3 & ene_B );
+    // EMF::process_bc (emf.cpp:431-471)
+    void process_bc();
     // EMF::save (emf.cpp:589-655): EMF/<E|B><c>-<iter>.zdf
     void save( emf::field const field, fcomp::cart const fc );
     void note_step() { iter++; }
@@ -558,7 +560,10 @@ class Species {
     void save_phasespace( phasespace::quant quant0, float2 const range0, int const size0, phasespace::quant quant1,
                           float2 const range1, int const size1 ) const;
     double get_energy() const;
-    uint64_t get_nmove() const { return d_nmove; }
+    // particles moved so far (device counter; synchronises)
+    uint64_t get_nmove() const;
+    // reflecting walls of set_bc as the particle pass's mask
+    int reflect_mask() const;
     int get_iter() const { return iter; }
     float get_dt() const { return dt; }
     uint2 get_ntiles() const { return particles->ntiles; }
@@ -583,7 +588,7 @@ class Species {
     // Arguments of this species for a batched advance with the given fields
     zpic_batch_species batch_item( EMF const & emf, Current & current ) const;
     // Host-side bookkeeping of one step advanced by a batch (iteration, nmove)
-    void note_step() { d_nmove += np_count; iter++; }
+    void note_step() { iter++; }
 };
 
 // --------------------------------------------------------------- Simulation

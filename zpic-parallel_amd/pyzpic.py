@@ -52,6 +52,9 @@ def lib():
         "zpic_species_set_udist": (i, [vp, i, vp, vp, i]),
         "zpic_species_set_density_uniform": (i, [vp, f]),
         "zpic_species_set_seed": (i, [vp, u]),
+        "zpic_species_set_bc": (i, [vp, vp]),
+        "zpic_emf_set_bc": (i, [vp, vp]),
+        "zpic_current_set_bc": (i, [vp, vp]),
         "zpic_simulation_add_species": (i, [vp, vp]),
         "zpic_simulation_advance": (i, [vp, i]),
         "zpic_simulation_stage": (i, [vp, i, i]),
@@ -245,6 +248,12 @@ class Species:
         return out
 
     # ZDF diagnostics, written under the current working directory
+    def set_bc(self, x=("periodic", "periodic"), y=("periodic", "periodic")):
+        """Species::set_bc: each side "open", "periodic" or "reflecting"."""
+        names = {"open": 0, "periodic": 1, "reflecting": 2}
+        b = np.array([names[x[0]], names[x[1]], names[y[0]], names[y[1]]], np.int32)
+        _chk(lib().zpic_species_set_bc(self.h, _p(b)))
+
     def subsample_from(self, src, stride, phase=0):
         """Every stride-th particle of each of src's tiles (same tiling)."""
         _chk(lib().zpic_species_subsample(self.h, src.h, stride, phase))
@@ -349,6 +358,18 @@ class Simulation:
 
     def save_field_file(self, which, fc, filename):
         _chk(lib().zpic_field_save_file(self.h, "EBJ".index(which), "xyz".index(fc), filename.encode()))
+
+    def set_emf_bc(self, x=("periodic", "periodic"), y=("periodic", "periodic")):
+        """EMF::set_bc: each side "none", "periodic", "pec" or "pmc"."""
+        names = {"none": 0, "periodic": 1, "pec": 2, "pmc": 3}
+        b = np.array([names[x[0]], names[x[1]], names[y[0]], names[y[1]]], np.int32)
+        _chk(lib().zpic_emf_set_bc(self.h, _p(b)))
+
+    def set_current_bc(self, x=("periodic", "periodic"), y=("periodic", "periodic")):
+        """Current::set_bc: each side "none", "periodic" or "reflecting"."""
+        names = {"none": 0, "periodic": 1, "reflecting": 2}
+        b = np.array([names[x[0]], names[x[1]], names[y[0]], names[y[1]]], np.int32)
+        _chk(lib().zpic_current_set_bc(self.h, _p(b)))
 
     # learned current filter (see Denoiser)
     def set_learned_filter(self, dn):
