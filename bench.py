@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="weibel2048", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="threads per tile of the particle pass (multiple of 64 in [64, 512]; 0 = library default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 2 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -110,6 +112,8 @@ def main():
 
     import pyzpic
 
+    if args.threads:
+        pyzpic.set_threads_per_tile(args.threads)
     if args.workload == "batch64":
         return run_batch(args, pyzpic)
 
@@ -173,7 +177,7 @@ def main():
             "workload": args.workload, "grid": [wl["ntiles"][0] * wl["nx"][0], wl["ntiles"][1] * wl["nx"][1]],
             "tiles": list(wl["ntiles"]), "tile": list(wl["nx"]), "ppc": list(wl["ppc"]), "species": nsp,
             "particles": npart, "dt": 0.07, "dx": 0.1, "filter": "binomial x order 1" if wl["filter"] else "none",
-            "parallelism": "single GPU",
+            "threads_per_tile": pyzpic.get_threads_per_tile(), "parallelism": "single GPU",
         },
         "push_deposit_sort_particles_per_ns": moved / args.steps / (pds_ms * 1e6) if pds_ms > 0 else None,
         "stage_ms": {k: round(v, 4) for k, v in tm.items()},

@@ -20,6 +20,9 @@
 // keys: ntiles=X,Y nx=X,Y ppc=X,Y box=X,Y dt=F uth=X,Y,Z ufl=X,Y,Z
 //       filter=none|bx<N>|by<N>|cx<N>|cy<N>  snap=s0,s1,...  steps=N warmup=N
 //       energies=1 (record energies after every step up to the last snapshot)
+//       sbc=xl,xu,yl,yu  ebc=...  cbc=...  boundary conditions of the species
+//       (0 open, 1 periodic, 2 reflecting), the EMF (0 none, 1 periodic, 2 pec,
+//       3 pmc) and the current (0 none, 1 periodic, 2 reflecting)
 
 #include "simulation.h"
 #include "species.h"
@@ -121,6 +124,9 @@ struct Args {
     int steps = 10;
     int warmup = 1;
     int energies = 0;
+    int sbc[4] = { 1, 1, 1, 1 };
+    int ebc[4] = { 1, 1, 1, 1 };
+    int cbc[4] = { 1, 1, 1, 1 };
 };
 
 static std::vector<std::string> split( const std::string & s, char c ) {
@@ -150,6 +156,10 @@ static Args parse( int argc, char ** argv, int first ) {
         else if ( k == "steps" ) a.steps = atoi( v.c_str() );
         else if ( k == "warmup" ) a.warmup = atoi( v.c_str() );
         else if ( k == "energies" ) a.energies = atoi( v.c_str() );
+        else if ( k == "sbc" || k == "ebc" || k == "cbc" ) {
+            int * dst = ( k == "sbc" ) ? a.sbc : ( k == "ebc" ) ? a.ebc : a.cbc;
+            for ( int i = 0; i < 4; i++ ) dst[i] = atoi( f[i].c_str() );
+        }
         else { fprintf( stderr, "unknown key %s\n", k.c_str() ); exit( 2 ); }
     }
     return a;
@@ -235,6 +245,21 @@ int main( int argc, char ** argv ) {
     sim.add_species( positrons );
 
     std::vector<ProbeSpecies*> sp = { &electrons, &positrons };
+
+    {
+        species::bc_type sb;
+        sb.x = { (species::bc::type) a.sbc[0], (species::bc::type) a.sbc[1] };
+        sb.y = { (species::bc::type) a.sbc[2], (species::bc::type) a.sbc[3] };
+        for ( auto * s : sp ) s->set_bc( sb );
+        emf::bc_type eb;
+        eb.x = { (emf::bc::type) a.ebc[0], (emf::bc::type) a.ebc[1] };
+        eb.y = { (emf::bc::type) a.ebc[2], (emf::bc::type) a.ebc[3] };
+        sim.emf.set_bc( eb );
+        current::bc_type cb;
+        cb.x = { (current::bc::type) a.cbc[0], (current::bc::type) a.cbc[1] };
+        cb.y = { (current::bc::type) a.cbc[2], (current::bc::type) a.cbc[3] };
+        sim.current.set_bc( cb );
+    }
 
     if ( zdfmode ) {
         for ( int i = 0; i < a.steps; i++ ) sim.advance();

@@ -34,4 +34,25 @@ def driver_args_to_config(args):
             if v != "none":
                 kind = {"b": "binomial", "c": "compensated"}[v[0]]
                 cfg["filter"] = (kind, 0 if v[1] == "x" else 1, int(v[2:]))
+        elif k in ("sbc", "ebc", "cbc"):
+            cfg[k] = tuple(int(t) for t in f)
     return cfg
+
+
+SPECIES_BC = {0: "open", 1: "periodic", 2: "reflecting"}
+EMF_BC = {0: "none", 1: "periodic", 2: "pec", 3: "pmc"}
+CURRENT_BC = {0: "none", 1: "periodic", 2: "reflecting"}
+
+
+def apply_bc(sim, cfg):
+    """Boundary conditions of a driver configuration on a pyzpic simulation."""
+    if "sbc" in cfg:
+        b = [SPECIES_BC[v] for v in cfg["sbc"]]
+        for s in sim.species:
+            s.set_bc((b[0], b[1]), (b[2], b[3]))
+    if "ebc" in cfg:
+        b = [EMF_BC[v] for v in cfg["ebc"]]
+        sim.set_emf_bc((b[0], b[1]), (b[2], b[3]))
+    if "cbc" in cfg:
+        b = [CURRENT_BC[v] for v in cfg["cbc"]]
+        sim.set_current_bc((b[0], b[1]), (b[2], b[3]))

@@ -11,6 +11,10 @@ deterministic serial ones.  Outputs:
   digests.json              SHA-256 of every state array (raw little-endian
                              bytes in tile order) and exact energies for the
                              larger cases listed in CASES.
+  digests_bc.json           the same for BC_CASES: non-periodic boundary
+                             conditions of the species, the EMF and the
+                             current (checked on the GPU against the reference
+                             directly; the CPU oracle restates periodic runs).
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -54,6 +58,26 @@ CASES = {
 }
 
 
+# boundary conditions: sbc (0 open, 1 periodic, 2 reflecting), ebc (0 none,
+# 1 periodic, 2 pec, 3 pmc), cbc (0 none, 1 periodic, 2 reflecting)
+HOT = ["uth=0.2,0.2,0.2", "dt=0.07"]
+BC_CASES = {
+    # reflecting particle walls and pec / reflecting fields along y
+    "walls_y_pec": (["ntiles=3,2", "nx=16,16", "ppc=2,2", "box=4.8,3.2", "ufl=0.1,0.3,0.6", "filter=none",
+                     "sbc=1,1,2,2", "ebc=1,1,2,2", "cbc=1,1,2,2"] + HOT, [1, 10, 25]),
+    # open particle boundaries (particles leave) with pmc fields along x
+    "open_x_pmc": (["ntiles=2,3", "nx=16,16", "ppc=2,2", "box=3.2,4.8", "ufl=0.4,0.1,0.6", "filter=bx1",
+                    "sbc=0,0,1,1", "ebc=3,3,1,1", "cbc=2,2,1,1"] + HOT, [1, 10, 25]),
+    # every wall reflecting, mixed pec / pmc fields
+    "closed_box": (["ntiles=2,2", "nx=16,16", "ppc=2,2", "box=3.2,3.2", "ufl=0.3,-0.2,0.6", "filter=none",
+                    "sbc=2,2,2,2", "ebc=2,3,3,2", "cbc=2,2,2,2"] + HOT, [1, 10, 30]),
+    # a single tile along x: only its lower wall is processed (as in the
+    # reference's tile loop); open y
+    "single_tile_x": (["ntiles=1,3", "nx=16,16", "ppc=2,2", "box=1.6,4.8", "ufl=0.5,0.2,0.6", "filter=none",
+                       "sbc=2,2,0,0", "ebc=2,2,3,3", "cbc=2,2,0,0"] + HOT, [1, 10, 20]),
+}
+
+
 def run_driver(args, snap, path):
     env = dict(os.environ, OMP_NUM_THREADS="1")
     cmd = [pyoracle.REF_DRIVER, "dump", path] + args + ["snap=" + ",".join(map(str, snap)), "energies=1"]
@@ -85,6 +109,19 @@ def main():
                     entry["sha256"][k] = digest(v)
             dig[name] = entry
         with open(os.path.join(OUT, "digests.json"), "w") as fp:
+            json.dump(dig, fp, indent=1, sort_keys=True)
+
+        dig = {}
+        for name, (args, snap) in BC_CASES.items():
+            rec = run_driver(args, snap, os.path.join(td, name + ".bin"))
+            entry = {"args": args, "snap": snap, "sha256": {}, "values": {}}
+            for k, v in rec.items():
+                if k.endswith("energy") or k.endswith("nmove") or k == "energies" or k.endswith("field_energy"):
+                    entry["values"][k] = [float(x) if v.dtype.kind == "f" else int(x) for x in v]
+                if k != "energies":
+                    entry["sha256"][k] = digest(v)
+            dig[name] = entry
+        with open(os.path.join(OUT, "digests_bc.json"), "w") as fp:
             json.dump(dig, fp, indent=1, sort_keys=True)
 
 

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from conftest import driver_args_to_config
+from conftest import apply_bc, driver_args_to_config
 from gridutil import gather, global_positions
 
 pytestmark = pytest.mark.gpu
@@ -128,6 +128,56 @@ def test_deterministic_digests_match_reference(zp, case):
                 h = hashlib.sha256(np.ascontiguousarray(val).tobytes()).hexdigest()
                 assert h == entry["sha256"][key], f"{case}: {key} differs"
     assert energies == entry["values"]["energies"]
+
+
+DIGESTS_BC = json.load(open(os.path.join(GOLD, "digests_bc.json")))
+
+
+@pytest.mark.parametrize("case", sorted(DIGESTS_BC))
+def test_deterministic_boundaries_match_reference(zp, case):
+    """Open / reflecting particle walls, pec / pmc fields and reflecting
+    current (SURVEY next rank 3): every array bit-exact with the reference's
+    single-thread run at every snapshot, energies and nmove exact."""
+    entry = DIGESTS_BC[case]
+    cfg = driver_args_to_config(entry["args"])
+    say(f"deterministic boundary case {case} {entry['args']}")
+    sim = gpu_weibel(zp, cfg, det=True)
+    apply_bc(sim, cfg)
+    energies = []
+    for step in range(0, max(entry["snap"]) + 1):
+        if step > 0:
+            sim.advance(1)
+            energies.extend(sim.energies().tolist())
+        if step in entry["snap"]:
+            for key, val in gpu_state(sim, step).items():
+                h = hashlib.sha256(np.ascontiguousarray(val).tobytes()).hexdigest()
+                assert h == entry["sha256"][key], f"{case}: {key} differs"
+    assert energies == entry["values"]["energies"]
+    sim.close()
+
+
+@pytest.mark.parametrize("case", ["walls_y_pec", "open_x_pmc"])
+def test_fast_boundaries_close_to_reference(zp, case):
+    """Fast mode with boundaries: particle counts per tile and nmove exact
+    after one step; after the last snapshot kinetic energies within 1e-4 and
+    the total field energy within 1e-2 (summation-order drift)."""
+    entry = DIGESTS_BC[case]
+    cfg = driver_args_to_config(entry["args"])
+    say(f"fast boundary case {case}")
+    sim = gpu_weibel(zp, cfg, det=False)
+    apply_bc(sim, cfg)
+    sim.advance(1)
+    for k, s in enumerate(sim.species):
+        h = hashlib.sha256(np.ascontiguousarray(s.particles()["np"]).tobytes()).hexdigest()
+        assert h == entry["sha256"][f"s1/sp{k}/np"]
+        assert s.get_nmove() == entry["values"][f"s1/sp{k}/nmove"][0]
+    last = max(entry["snap"])
+    sim.advance(last - 1)
+    ref = np.array(entry["values"]["energies"]).reshape(last, -1)[-1]
+    got = sim.energies()
+    assert np.all(np.abs(got[:2] - ref[:2]) <= 1e-4 * np.abs(ref[:2])), (got, ref)
+    assert abs(got[2:].sum() - ref[2:].sum()) <= 1e-2 * ref[2:].sum(), (got, ref)
+    sim.close()
 
 
 def test_fast_first_step(zp):
