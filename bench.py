@@ -107,7 +107,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the step would shard).", file=sys.stderr)
+              "(the step would shard by tile rows, one slab of tiles per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import pyzpic

@@ -106,11 +106,6 @@ int zpic_push_move_deposit(const zpic_particles *p, const zpic_geom *fld, const 
                            float q, float qnx, float qny, double *tile_energy, zpic_sort *s,
                            int deterministic, void *stream);
 
-/* Particles::tile_sort (particles.cpp:855-904: bnd_check, update_tile_info,
- * copy_out, copy_in) as per-tile counting + prefix scan.  Uses the
- * classification left in `s` by zpic_move_deposit / zpic_push_move_deposit.
- * `tmp` supplies the staging buffers and receives the new offsets.  The
- * resulting within-tile order is the reference's single-thread order. */
 /* As zpic_push_move_deposit, then Species::process_bc (species.cpp:1854-1989)
  * on the moved particles: reflect is a mask of reflecting walls (1 x lower,
  * 2 x upper, 4 y lower, 8 y upper); particles crossing an open wall are
@@ -119,8 +114,74 @@ int zpic_push_move_deposit_bc(const zpic_particles *p, const zpic_geom *fld, con
                               float *J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
                               double *tile_energy, zpic_sort *s, int reflect, int deterministic, void *stream);
 
+/* As zpic_push_move_deposit_bc with the cell shift of
+ * move_deposit_shift_kernel (species.cpp:1508-1597): (shift_x, shift_y) is
+ * added to every particle's new cell before the boundary processing.  The
+ * moving window of Species::advance_mov_window (species.cpp:2083-2128)
+ * shifts by (-1, 0) on the steps where the window moves. */
+int zpic_push_move_deposit_shift(const zpic_particles *p, const zpic_geom *fld, const float *E, const float *B,
+                                 float *J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
+                                 double *tile_energy, zpic_sort *s, int reflect, int shift_x, int shift_y,
+                                 int deterministic, void *stream);
+
+/* Particles::tile_sort (particles.cpp:855-904: bnd_check, update_tile_info,
+ * copy_out, copy_in) as per-tile counting + prefix scan.  Uses the
+ * classification left in `s` by zpic_move_deposit / zpic_push_move_deposit.
+ * `tmp` supplies the staging buffers and receives the new offsets.  The
+ * resulting within-tile order is the reference's single-thread order. */
 int zpic_tile_sort(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, int ntiles_x, int ntiles_y,
                    int nx, int ny, int periodic_x, int periodic_y, void *stream);
+
+/* Particles::tile_sort(tmp, sort, extra) (particles.cpp:855-904), the sort
+ * that leaves room for particles injected next, in two calls so that the
+ * caller can pick the path from the total between them:
+ *  zpic_tile_sort_count: new particle counts, and new offsets in tmp->offset
+ *    that reserve extra[t] free slots after tile t's particles
+ *    (update_tile_info with extra, :336-357; extra may be NULL); the total,
+ *    reserved slots included, goes to s->total[0].
+ *  zpic_tile_sort_copy, mode 0: the particles move inside p (copy_out +
+ *    copy_in, :379-614; zpic_tile_sort is count + copy mode 0).  Mode 1: every
+ *    particle is copied into tmp at the new offsets and tmp->np is set
+ *    (copy_sorted, :628-763), the path the reference takes when the total
+ *    exceeds p->max_part; the caller then swaps the two buffers
+ *    (swap_buffers, particles.h:195-206).  In mode 1 tmp must hold at least
+ *    s->total[0] particles. */
+int zpic_tile_sort_count(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, int ntiles_x, int ntiles_y,
+                         int nx, int ny, int periodic_x, int periodic_y, const int *extra, void *stream);
+int zpic_tile_sort_copy(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, int ntiles_x, int ntiles_y,
+                        int nx, int ny, int periodic_x, int periodic_y, int mode, void *stream);
+
+/* Density profiles of particle injection (Density::Uniform / Step / Slab /
+ * Sphere, density.h:30-132) as plain data, lengths relative to the window
+ * origin: a step keeps the lattice points past `start` (in cells) along
+ * `dir`, a slab those in [start, end), a sphere those within `radius` of the
+ * centre (simulation units, cell size dx, dy). */
+#define ZPIC_DENSITY_NONE 0
+#define ZPIC_DENSITY_UNIFORM 1
+#define ZPIC_DENSITY_STEP 2
+#define ZPIC_DENSITY_SLAB 3
+#define ZPIC_DENSITY_SPHERE 4
+typedef struct {
+    int kind;                          /* ZPIC_DENSITY_*                      */
+    int dir;                           /* step / slab axis: 0 = x, 1 = y      */
+    float start, end;                  /* step / slab bounds, in cells        */
+    float center_x, center_y, radius;  /* sphere                              */
+    float dx, dy;                      /* cell size                           */
+} zpic_inject_profile;
+
+/* Density::*::np_inject (density.cpp:231-298, 444-573, 720-854, 969-1067):
+ * number of particles the profile places in each tile's part of the global
+ * cell range [range[0], range[1]] x [range[2], range[3]] (ppc_x * ppc_y
+ * lattice points per cell), into np[ntiles]. */
+int zpic_np_inject(const zpic_inject_profile *d, int ntiles_x, int ntiles_y, int nx, int ny, int ppc_x,
+                   int ppc_y, const int range[4], int *np, void *stream);
+
+/* Density::*::inject (density.cpp:103-218, 315-442, 575-718, 856-967): those
+ * particles (u = 0) appended after each tile's p->np[t] particles, in the
+ * reference's order, and p->np[t] increased.  The slots must be free (the
+ * extra of zpic_tile_sort_count). */
+int zpic_inject(const zpic_inject_profile *d, const zpic_particles *p, int ntiles_x, int ntiles_y, int nx,
+                int ny, int ppc_x, int ppc_y, const int range[4], void *stream);
 
 /* grid<T>::add_from_gc / copy_to_gc (grid.h:253-445) on a tile-major grid
  * of `ncomp` floats per element (3 for vec3grid<float3>, 1 for grid<float>). */
@@ -186,8 +247,24 @@ int zpic_gather_component(const float *buf, const zpic_geom *g, int ncomp, int f
 /* Fills n floats with a value (grid::zero / set). */
 int zpic_fill(float *buf, size_t n, float value, void *stream);
 
+/* grid<T>::add (grid.h:208-212): dst += src over n floats (whole buffers,
+ * guard cells included). */
+int zpic_grid_add(float *dst, const float *src, size_t n, void *stream);
+
+/* grid<T>::x_shift_left (grid.h:455-485): every row of every tile, guard
+ * cells included, shifted left by `shift` cells with zeros entering at the
+ * right, then the x guard copy.  Requires 0 < shift < gc_upper. */
+int zpic_x_shift_left(float *buf, const zpic_geom *g, int ncomp, int shift, int periodic_x, int periodic_y,
+                      void *stream);
+
+/* div_corr_x (laser.cpp:20-70): Ex and Bx of the interior cells set so that
+ * the discrete divergence of E and B vanishes, accumulated in double from
+ * the right edge of each row of cells leftwards (zero field assumed beyond
+ * it), dx/dy the cell size.  Guard cells are left as they are. */
+int zpic_div_corr_x(float *E, float *B, const zpic_geom *g, float dx, float dy, void *stream);
+
 /* Threads per tile of the fast particle pass: a multiple of 64 in [64, 512]
- * (default 512).  Library-wide; the deterministic mode always uses 64. */
+ * (default 256).  Library-wide; the deterministic mode always uses 64. */
 int zpic_set_threads_per_tile(int threads);
 int zpic_get_threads_per_tile(void);
 /* Threads per tile the fast particle pass actually uses for tiles of
@@ -285,12 +362,63 @@ int zpic_species_set_udist(zpic_species *sp, int kind, const float uth[3], const
 int zpic_species_set_seed(zpic_species *sp, unsigned offset);
 int zpic_species_set_density_uniform(zpic_species *sp, float n0);
 
+/* Species::set_density (species.h:172-175) with any profile of
+ * density.h:30-132 (the Cython classes openmp/python/em2d/density/defs.pxd
+ * bind None, Uniform, Step and Slab): kind ZPIC_DENSITY_*, density n0; a
+ * step starts at `pos` along dir (0 x, 1 y), a slab spans [begin, end), a
+ * sphere has centre and radius.  Before add_species. */
+typedef struct {
+    int kind;
+    int dir;
+    float n0;
+    float pos;
+    float begin, end;
+    float center[2];
+    float radius;
+} zpic_density;
+int zpic_species_set_density(zpic_species *sp, const zpic_density *d);
+
 /* Simulation::add_species (simulation.h:68-71): the simulation keeps a
  * non-owning reference; the species must outlive it. */
 int zpic_simulation_add_species(zpic_simulation *sim, zpic_species *sp);
 
 /* Simulation::advance (simulation.h:90-107), n times. */
 int zpic_simulation_advance(zpic_simulation *sim, int n);
+
+/* Simulation::set_moving_window (simulation.h:56-62; Cython:
+ * Simulation(..., moving_window=True), em2d.pyx:1478-1480): fields and
+ * current lose their x periodicity, species get open x walls, and the window
+ * moves one cell whenever the time passes the next cell.  ZPIC_ESTATE after
+ * the first step. */
+int zpic_simulation_set_moving_window(zpic_simulation *sim);
+
+/* Simulation::advance_mov_window (simulation.h:113-127), n times: on the
+ * steps where the window moves, particles shift one cell left, the species'
+ * density profile is injected into the rightmost column of cells, and E and
+ * B shift one cell left after the field update. */
+int zpic_simulation_advance_mov_window(zpic_simulation *sim, int n);
+
+/* Cells the window has moved (EMF's MovingWindow::n_move, moving_window.h). */
+int zpic_simulation_window_moves(const zpic_simulation *sim, unsigned *n_move);
+
+/* Laser pulses (Laser::PlaneWave / Laser::Gaussian, laser.h; Cython
+ * openmp/python/em2d/laser/defs.pxd): kind 0 plane wave, 1 Gaussian beam;
+ * fwhm > 0 overrides rise / flat / fall; cos_pol = sin_pol = 0 takes the
+ * polarisation angle; filter = compensated x filter passes (reference
+ * default 1); W0 / focus / axis for the Gaussian beam. */
+typedef struct {
+    int kind;
+    float start, fwhm, rise, flat, fall;
+    float a0, omega0;
+    float polarization, cos_pol, sin_pol;
+    unsigned filter;
+    float W0, focus, axis;
+} zpic_laser;
+
+/* Pulse::add (laser.h:67-81, launch laser.cpp:224-437): adds the pulse to the
+ * simulation's E and B.  ZPIC_EINVAL for invalid pulse parameters (the
+ * reference's -1). */
+int zpic_laser_add(zpic_simulation *sim, const zpic_laser *laser);
 
 /* Stages of one Simulation::advance, for stage-level parity checks:
  * 0 push(k), 1 move(k), 2 tile_sort(k), 3 Species::advance(k),

@@ -23,11 +23,24 @@
 //       sbc=xl,xu,yl,yu  ebc=...  cbc=...  boundary conditions of the species
 //       (0 open, 1 periodic, 2 reflecting), the EMF (0 none, 1 periodic, 2 pec,
 //       3 pmc) and the current (0 none, 1 periodic, 2 reflecting)
+//       plasma=weibel|electrons  two ThermalCorr species (default), or one
+//       electron species (q/m = -1) with udist cold (ufl) or thermal (uth > 0)
+//       dens=uniform | step:D,n0,pos | slab:D,n0,begin,end | sphere:n0,cx,cy,r
+//       (D = x or y) density profile of every species
+//       laser=none | plane:start,fwhm,a0,omega0,cos_pol,sin_pol
+//       | gauss:start,fwhm,a0,omega0,W0,focus,axis,cos_pol,sin_pol
+//       window=1  Simulation::set_moving_window, advance_mov_window steps;
+//       wsbc=xl,xu,yl,yu  species boundaries set after set_moving_window
+// Dumped grids have their alignment padding (elements ext_x*ext_y ..
+// tile_vol of every tile, never read) zeroed: the laser's temporary grids
+// leave uninitialised values there.
 
 #include "simulation.h"
 #include "species.h"
 #include "udist.h"
 #include "filter.h"
+#include "density.h"
+#include "laser.h"
 
 #include <chrono>
 #include <unistd.h>
@@ -127,6 +140,11 @@ struct Args {
     int sbc[4] = { 1, 1, 1, 1 };
     int ebc[4] = { 1, 1, 1, 1 };
     int cbc[4] = { 1, 1, 1, 1 };
+    std::string plasma = "weibel";
+    std::string dens = "uniform";
+    std::string laser = "none";
+    int window = 0;
+    int wsbc[4] = { -1, -1, -1, -1 };
 };
 
 static std::vector<std::string> split( const std::string & s, char c ) {
@@ -156,10 +174,14 @@ static Args parse( int argc, char ** argv, int first ) {
         else if ( k == "steps" ) a.steps = atoi( v.c_str() );
         else if ( k == "warmup" ) a.warmup = atoi( v.c_str() );
         else if ( k == "energies" ) a.energies = atoi( v.c_str() );
-        else if ( k == "sbc" || k == "ebc" || k == "cbc" ) {
-            int * dst = ( k == "sbc" ) ? a.sbc : ( k == "ebc" ) ? a.ebc : a.cbc;
+        else if ( k == "sbc" || k == "ebc" || k == "cbc" || k == "wsbc" ) {
+            int * dst = ( k == "sbc" ) ? a.sbc : ( k == "ebc" ) ? a.ebc : ( k == "cbc" ) ? a.cbc : a.wsbc;
             for ( int i = 0; i < 4; i++ ) dst[i] = atoi( f[i].c_str() );
         }
+        else if ( k == "plasma" ) a.plasma = v;
+        else if ( k == "dens" ) a.dens = v;
+        else if ( k == "laser" ) a.laser = v;
+        else if ( k == "window" ) a.window = atoi( v.c_str() );
         else { fprintf( stderr, "unknown key %s\n", k.c_str() ); exit( 2 ); }
     }
     return a;
@@ -181,6 +203,58 @@ static void set_filter( Simulation & sim, const std::string & f ) {
     if ( f[0] == 'b' ) sim.current.set_filter( Filter::Binomial( dir, order ) );
     else if ( f[0] == 'c' ) sim.current.set_filter( Filter::Compensated( dir, order ) );
     else { fprintf( stderr, "bad filter %s\n", f.c_str() ); exit( 2 ); }
+}
+
+// Copy of a tile-major grid with the alignment padding of every tile zeroed.
+template <class T>
+static std::vector<T> unpadded( grid<T> & g ) {
+    std::vector<T> out( g.d_buffer, g.d_buffer + g.buffer_size() );
+    const size_t used = (size_t) g.ext_nx.x * g.ext_nx.y;
+    const size_t nt = g.buffer_size() / g.tile_vol;
+    T zero;
+    std::memset( (void *) &zero, 0, sizeof( T ) );
+    for ( size_t t = 0; t < nt; t++ )
+        for ( size_t i = used; i < g.tile_vol; i++ ) out[t * g.tile_vol + i] = zero;
+    return out;
+}
+
+static Density::Profile * make_density( const std::string & d ) {
+    if ( d == "uniform" ) return new Density::Uniform( 1.0f );
+    auto c = d.find( ':' );
+    if ( c == std::string::npos ) { fprintf( stderr, "bad dens %s\n", d.c_str() ); exit( 2 ); }
+    std::string kind = d.substr( 0, c );
+    auto f = split( d.substr( c + 1 ), ',' );
+    auto fl = [&]( size_t i ) { return strtof( f.at( i ).c_str(), 0 ); };
+    if ( kind == "step" ) return new Density::Step( f.at( 0 ) == "y" ? coord::y : coord::x, fl( 1 ), fl( 2 ) );
+    if ( kind == "slab" ) return new Density::Slab( f.at( 0 ) == "y" ? coord::y : coord::x, fl( 1 ), fl( 2 ), fl( 3 ) );
+    if ( kind == "sphere" ) return new Density::Sphere( fl( 0 ), make_float2( fl( 1 ), fl( 2 ) ), fl( 3 ) );
+    fprintf( stderr, "bad dens %s\n", d.c_str() );
+    exit( 2 );
+}
+
+static void add_laser( Simulation & sim, const std::string & l ) {
+    if ( l == "none" ) return;
+    auto c = l.find( ':' );
+    std::string kind = l.substr( 0, c );
+    auto f = split( l.substr( c + 1 ), ',' );
+    auto fl = [&]( size_t i ) { return strtof( f.at( i ).c_str(), 0 ); };
+    int rc;
+    if ( kind == "plane" ) {
+        Laser::PlaneWave w;
+        w.start = fl( 0 ); w.fwhm = fl( 1 ); w.a0 = fl( 2 ); w.omega0 = fl( 3 );
+        w.cos_pol = fl( 4 ); w.sin_pol = fl( 5 );
+        rc = w.add( sim.emf );
+    } else if ( kind == "gauss" ) {
+        Laser::Gaussian w;
+        w.start = fl( 0 ); w.fwhm = fl( 1 ); w.a0 = fl( 2 ); w.omega0 = fl( 3 );
+        w.W0 = fl( 4 ); w.focus = fl( 5 ); w.axis = fl( 6 );
+        w.cos_pol = fl( 7 ); w.sin_pol = fl( 8 );
+        rc = w.add( sim.emf );
+    } else {
+        fprintf( stderr, "bad laser %s\n", l.c_str() );
+        exit( 2 );
+    }
+    if ( rc ) { fprintf( stderr, "laser add failed\n" ); exit( 1 ); }
 }
 
 static void snapshot( FILE * fp, int step, Simulation & sim, std::vector<ProbeSpecies*> & sp ) {
@@ -214,12 +288,14 @@ static void snapshot( FILE * fp, int step, Simulation & sim, std::vector<ProbeSp
         charge.zero();
         sp[k]->deposit_charge( charge );
         charge.add_from_gc();
-        put( fp, sn + "charge", 'f', charge.buffer_size(), charge.d_buffer );
+        auto ch = unpadded( charge );
+        put( fp, sn + "charge", 'f', ch.size(), ch.data() );
     }
     size_t fsz = sim.emf.E->buffer_size() * 3;
-    put( fp, pre + "E", 'f', fsz, sim.emf.E->d_buffer );
-    put( fp, pre + "B", 'f', fsz, sim.emf.B->d_buffer );
-    put( fp, pre + "J", 'f', fsz, sim.current.J->d_buffer );
+    auto vE = unpadded( *sim.emf.E ), vB = unpadded( *sim.emf.B ), vJ = unpadded( *sim.current.J );
+    put( fp, pre + "E", 'f', fsz, vE.data() );
+    put( fp, pre + "B", 'f', fsz, vB.data() );
+    put( fp, pre + "J", 'f', fsz, vJ.data() );
     double3 eE, eB;
     sim.emf.get_energy( eE, eB );
     double fe[6] = { eE.x, eE.y, eE.z, eB.x, eB.y, eB.z };
@@ -236,15 +312,26 @@ int main( int argc, char ** argv ) {
     Simulation sim( a.ntiles, a.nx, a.box, (double) a.dt );
     set_filter( sim, a.filter );
 
+    const bool weibel = ( a.plasma == "weibel" );
+    if ( !weibel && a.plasma != "electrons" ) { fprintf( stderr, "bad plasma %s\n", a.plasma.c_str() ); return 2; }
+    Density::Profile * dens = make_density( a.dens );
+
     ProbeSpecies electrons( "electrons", -1.0f, a.ppc );
-    electrons.set_udist( UDistribution::ThermalCorr( a.uth, a.ufl ) );
+    electrons.set_density( *dens );
+    if ( weibel ) electrons.set_udist( UDistribution::ThermalCorr( a.uth, a.ufl ) );
+    else if ( a.uth.x > 0 || a.uth.y > 0 || a.uth.z > 0 ) electrons.set_udist( UDistribution::Thermal( a.uth, a.ufl ) );
+    else electrons.set_udist( UDistribution::Cold( a.ufl ) );
     sim.add_species( electrons );
 
     ProbeSpecies positrons( "positrons", +1.0f, a.ppc );
-    positrons.set_udist( UDistribution::ThermalCorr( a.uth, make_float3( -a.ufl.x, -a.ufl.y, -a.ufl.z ) ) );
-    sim.add_species( positrons );
-
-    std::vector<ProbeSpecies*> sp = { &electrons, &positrons };
+    std::vector<ProbeSpecies*> sp = { &electrons };
+    if ( weibel ) {
+        positrons.set_density( *dens );
+        positrons.set_udist( UDistribution::ThermalCorr( a.uth, make_float3( -a.ufl.x, -a.ufl.y, -a.ufl.z ) ) );
+        sim.add_species( positrons );
+        sp.push_back( &positrons );
+    }
+    delete dens;
 
     {
         species::bc_type sb;
@@ -261,8 +348,23 @@ int main( int argc, char ** argv ) {
         sim.current.set_bc( cb );
     }
 
+    add_laser( sim, a.laser );
+    if ( a.window ) {
+        sim.set_moving_window();
+        if ( a.wsbc[0] >= 0 ) {
+            species::bc_type sb;
+            sb.x = { (species::bc::type) a.wsbc[0], (species::bc::type) a.wsbc[1] };
+            sb.y = { (species::bc::type) a.wsbc[2], (species::bc::type) a.wsbc[3] };
+            for ( auto * s : sp ) s->set_bc( sb );
+        }
+    }
+    auto advance = [&]() {
+        if ( a.window ) sim.advance_mov_window();
+        else sim.advance();
+    };
+
     if ( zdfmode ) {
-        for ( int i = 0; i < a.steps; i++ ) sim.advance();
+        for ( int i = 0; i < a.steps; i++ ) advance();
         if ( chdir( argv[2] ) ) { perror( "chdir" ); return 1; }
         write_zdf( sim, sp );
         return 0;
@@ -275,13 +377,12 @@ int main( int argc, char ** argv ) {
         for ( int s : a.snap ) last = std::max( last, s );
         std::vector<double> ene;
         for ( int step = 0; step <= last; step++ ) {
-            if ( step > 0 ) sim.advance();
+            if ( step > 0 ) advance();
             for ( int s : a.snap ) if ( s == step ) snapshot( fp, step, sim, sp );
             if ( a.energies && step > 0 ) {
                 double3 eE, eB;
                 sim.emf.get_energy( eE, eB );
-                ene.push_back( electrons.get_energy() );
-                ene.push_back( positrons.get_energy() );
+                for ( auto * s : sp ) ene.push_back( s->get_energy() );
                 ene.push_back( eE.x ); ene.push_back( eE.y ); ene.push_back( eE.z );
                 ene.push_back( eB.x ); ene.push_back( eB.y ); ene.push_back( eB.z );
             }
@@ -292,10 +393,10 @@ int main( int argc, char ** argv ) {
     }
 
     // bench: warmup steps, then `steps` timed steps of Simulation::advance()
-    for ( int i = 0; i < a.warmup; i++ ) sim.advance();
+    for ( int i = 0; i < a.warmup; i++ ) advance();
     uint64_t n0 = sim.get_nmove();
     auto t0 = std::chrono::steady_clock::now();
-    for ( int i = 0; i < a.steps; i++ ) sim.advance();
+    for ( int i = 0; i < a.steps; i++ ) advance();
     auto t1 = std::chrono::steady_clock::now();
     uint64_t moved = sim.get_nmove() - n0;
     double sec = std::chrono::duration<double>( t1 - t0 ).count();
@@ -306,6 +407,6 @@ int main( int argc, char ** argv ) {
     printf( "{\"particles_per_ns\": %.6g, \"ms_per_step\": %.6g, \"steps\": %d, \"threads\": %d, "
             "\"particles\": %llu}\n",
             moved / sec / 1e9, 1e3 * sec / a.steps, a.steps, threads,
-            (unsigned long long) ( electrons.np_total() + positrons.np_total() ) );
+            (unsigned long long) ( electrons.np_total() + ( weibel ? positrons.np_total() : 0 ) ) );
     return 0;
 }

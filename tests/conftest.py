@@ -56,3 +56,68 @@ def apply_bc(sim, cfg):
     if "cbc" in cfg:
         b = [CURRENT_BC[v] for v in cfg["cbc"]]
         sim.set_current_bc((b[0], b[1]), (b[2], b[3]))
+
+
+def _f32(v):
+    import numpy as np
+    return float(np.float32(float(v)))
+
+
+def build_sim(zp, args, det=False):
+    """A pyzpic simulation set up as oracle/ref_driver.cpp sets up the
+    reference for the same key=value arguments (plasma, density profile,
+    udist, boundaries, laser, moving window), in the same order."""
+    kv = dict(a.split("=", 1) for a in args)
+    ints = lambda v: tuple(int(t) for t in v.split(","))  # noqa: E731
+    flts = lambda v: tuple(_f32(t) for t in v.split(","))  # noqa: E731
+    ntiles, nx, ppc = ints(kv["ntiles"]), ints(kv["nx"]), ints(kv["ppc"])
+    box = flts(kv["box"])
+    dt = _f32(kv.get("dt", "0.07"))
+    uth = flts(kv.get("uth", "0.1,0.1,0.1"))
+    ufl = flts(kv.get("ufl", "0,0,0.6"))
+    filt = None
+    f = kv.get("filter", "none")
+    if f != "none":
+        filt = ({"b": "binomial", "c": "compensated"}[f[0]], 0 if f[1] == "x" else 1, int(f[2:]))
+    sim = zp.Simulation(ntiles, nx, box, dt, filter=filt, deterministic=det)
+
+    def density():
+        d = kv.get("dens", "uniform")
+        if d == "uniform":
+            return zp.Density.uniform(1.0)
+        kind, rest = d.split(":")
+        p = rest.split(",")
+        if kind == "step":
+            return zp.Density.step(p[0], _f32(p[1]), _f32(p[2]))
+        if kind == "slab":
+            return zp.Density.slab(p[0], _f32(p[1]), _f32(p[2]), _f32(p[3]))
+        if kind == "sphere":
+            return zp.Density.sphere(_f32(p[0]), (_f32(p[1]), _f32(p[2])), _f32(p[3]))
+        raise ValueError(d)
+
+    weibel = kv.get("plasma", "weibel") == "weibel"
+    if weibel:
+        sim.add_species(zp.Species("electrons", -1.0, ppc, "thermal_corr", uth, ufl, density=density()))
+        sim.add_species(zp.Species("positrons", +1.0, ppc, "thermal_corr", uth, tuple(-v for v in ufl),
+                                   density=density()))
+    else:
+        kind = "thermal" if any(v > 0 for v in uth) else "cold"
+        sim.add_species(zp.Species("electrons", -1.0, ppc, kind, uth, ufl, density=density()))
+    cfg = driver_args_to_config(args)
+    apply_bc(sim, cfg)
+    las = kv.get("laser", "none")
+    if las != "none":
+        kind, rest = las.split(":")
+        p = [_f32(t) for t in rest.split(",")]
+        if kind == "plane":
+            sim.add_laser(zp.PlaneWave(start=p[0], fwhm=p[1], a0=p[2], omega0=p[3], cos_pol=p[4], sin_pol=p[5]))
+        else:
+            sim.add_laser(zp.Gaussian(start=p[0], fwhm=p[1], a0=p[2], omega0=p[3], W0=p[4], focus=p[5], axis=p[6],
+                                      cos_pol=p[7], sin_pol=p[8]))
+    if int(kv.get("window", "0")):
+        sim.set_moving_window()
+        if "wsbc" in kv:
+            b = [SPECIES_BC[v] for v in ints(kv["wsbc"])]
+            for s in sim.species:
+                s.set_bc((b[0], b[1]), (b[2], b[3]))
+    return sim

@@ -15,6 +15,11 @@ deterministic serial ones.  Outputs:
                              conditions of the species, the EMF and the
                              current (checked on the GPU against the reference
                              directly; the CPU oracle restates periodic runs).
+  digests_window.json       the same for WINDOW_CASES: moving window with
+                             injection of step / slab / sphere / uniform
+                             density profiles, Gaussian and plane-wave laser
+                             pulses, the compensated filter, and the sort path
+                             that grows the particle buffer (copy_sorted).
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -78,6 +83,37 @@ BC_CASES = {
 }
 
 
+# Moving-window cases (the LWFA set-up of
+# openmp/em2d/main.cpp:540-600 at small size).  The last one ends on the step
+# whose sort first exceeds the particle buffer (1.2 x the initial load), where
+# the reference takes the copy_sorted path and grows the buffer; it stops
+# there because the reference's later sorts write past its (ungrown) staging
+# buffer.
+WINDOW_CASES = {
+    # Gaussian pulse into a plasma step at the right edge, compensated x filter
+    "lwfa_gauss_step": (["ntiles=8,2", "nx=16,16", "ppc=2,2", "box=2.56,6.4", "dt=0.0179", "plasma=electrons",
+                         "uth=0,0,0", "ufl=0,0,0", "dens=step:x,1.0,2.56",
+                         "laser=gauss:2.0,0.5,1.0,10.0,1.5,2.5,3.2,0,1", "window=1", "filter=cx2"], [0, 1, 30, 80]),
+    # plane wave, slab along x of thermal electrons, pec / reflecting y walls
+    "plane_slab_walls": (["ntiles=6,3", "nx=8,8", "ppc=3,2", "box=2.4,2.4", "dt=0.04", "plasma=electrons",
+                          "uth=0.05,0.05,0.05", "ufl=0,0,0", "dens=slab:x,1.0,1.0,2.2",
+                          "laser=plane:1.5,0.6,0.5,8.0,1,0", "sbc=1,1,2,2", "ebc=1,1,2,2", "cbc=1,1,2,2",
+                          "window=1", "filter=bx1"], [0, 1, 25, 60]),
+    # a sphere of warm plasma drifting while the window moves over it
+    "sphere_window": (["ntiles=4,4", "nx=8,8", "ppc=2,2", "box=1.6,3.2", "dt=0.04", "plasma=electrons",
+                       "uth=0.02,0.02,0.02", "ufl=0.3,0,0", "dens=sphere:1.0,1.2,1.6,0.6", "window=1",
+                       "filter=none"], [0, 1, 20, 50]),
+    # step along y, no window: the reference places it with dx.x
+    "step_y": (["ntiles=3,3", "nx=8,8", "ppc=2,3", "box=1.2,2.4", "dt=0.04", "plasma=electrons",
+                "uth=0.05,0.05,0.05", "ufl=0,0,0.2", "dens=step:y,1.0,0.7", "filter=none"], [0, 1, 20]),
+    # uniform plasma piling up against a reflecting lower x wall as the window
+    # moves: the particle count grows until the sort takes the copy_sorted path
+    "snowplow_copy_sorted": (["ntiles=4,2", "nx=16,8", "ppc=2,2", "box=3.2,1.6", "dt=0.04", "plasma=electrons",
+                              "uth=0.05,0.05,0.05", "ufl=0,0,0", "window=1", "wsbc=2,0,1,1", "filter=none"],
+                             [1, 10, 18, 19]),
+}
+
+
 def run_driver(args, snap, path):
     env = dict(os.environ, OMP_NUM_THREADS="1")
     cmd = [pyoracle.REF_DRIVER, "dump", path] + args + ["snap=" + ",".join(map(str, snap)), "energies=1"]
@@ -122,6 +158,21 @@ def main():
                     entry["sha256"][k] = digest(v)
             dig[name] = entry
         with open(os.path.join(OUT, "digests_bc.json"), "w") as fp:
+            json.dump(dig, fp, indent=1, sort_keys=True)
+
+        dig = {}
+        for name, (args, snap) in WINDOW_CASES.items():
+            rec = run_driver(args, snap, os.path.join(td, name + ".bin"))
+            entry = {"args": args, "snap": snap, "sha256": {}, "values": {}}
+            for k, v in rec.items():
+                if k.endswith("energy") or k.endswith("nmove") or k == "energies" or k.endswith("field_energy"):
+                    entry["values"][k] = [float(x) if v.dtype.kind == "f" else int(x) for x in v]
+                if k.endswith("/np"):
+                    entry["values"][k] = [int(x) for x in v]
+                if k != "energies":
+                    entry["sha256"][k] = digest(v)
+            dig[name] = entry
+        with open(os.path.join(OUT, "digests_window.json"), "w") as fp:
             json.dump(dig, fp, indent=1, sort_keys=True)
 
 

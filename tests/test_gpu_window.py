@@ -1,0 +1,130 @@
+"""Moving window, particle injection by density profile, laser pulses and
+the growing sort path on the GPU, against the compiled reference's runs of
+the same set-ups (tests/golden/digests_window.json, written by
+tests/golden/make_golden.py with oracle/_ref/ref_driver).
+
+Bars: deterministic mode (lane-serialised deposit) bit-exact with the
+reference's single-thread run in every array at every snapshot, energies and
+nmove exact; fast mode bit-exact after the first step in the particles and
+the injection counts, fields within norm-wise 1e-5 after one step, energies
+within 1e-3 at the last snapshot.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import build_sim
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "digests_window.json")) as fp:
+    DIGESTS = json.load(fp)
+
+
+def say(msg):
+    print(f"[gpu-window] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def zp():
+    import pyzpic
+    pyzpic.lib()
+    return pyzpic
+
+
+def state(sim, step):
+    out = {}
+    e = sim.energies()
+    ns = len(sim.species)
+    for k, s in enumerate(sim.species):
+        p = s.particles()
+        pre = f"s{step}/sp{k}/"
+        for name in ("np", "offset", "ix", "x", "u"):
+            out[pre + name] = p[name].ravel()
+        out[pre + "energy"] = np.array([e[k]])
+        out[pre + "nmove"] = np.array([s.get_nmove()], np.uint64)
+        out[pre + "charge"] = s.charge()
+    for f in "EBJ":
+        out[f"s{step}/{f}"] = sim.field(f).ravel()
+    out[f"s{step}/field_energy"] = e[ns:]
+    return out
+
+
+def relnorm(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+@pytest.mark.parametrize("case", sorted(DIGESTS))
+def test_deterministic_window_matches_reference(zp, case):
+    entry = DIGESTS[case]
+    say(f"deterministic {case} {entry['args']}")
+    sim = build_sim(zp, entry["args"], det=True)
+    energies = []
+    for step in range(0, max(entry["snap"]) + 1):
+        if step > 0:
+            sim.advance(1)
+            energies.extend(sim.energies().tolist())
+        if step in entry["snap"]:
+            st = state(sim, step)
+            for key in sorted(st):
+                if key.endswith("/np"):
+                    assert st[key].tolist() == entry["values"][key], f"{case}: {key} counts differ"
+            for key, val in st.items():
+                h = hashlib.sha256(np.ascontiguousarray(val).tobytes()).hexdigest()
+                assert h == entry["sha256"][key], f"{case}: {key} differs"
+    assert energies == entry["values"]["energies"]
+    if "window=1" in entry["args"]:
+        assert sim.window_moves() > 0
+    sim.close()
+
+
+@pytest.mark.parametrize("case", sorted(DIGESTS))
+def test_fast_window_close_to_reference(zp, case):
+    entry = DIGESTS[case]
+    say(f"fast {case}")
+    sim = build_sim(zp, entry["args"], det=False)
+    ns = len(sim.species)
+    last = max(entry["snap"])
+    for step in range(1, last + 1):
+        sim.advance(1)
+        if step == 1:
+            st = state(sim, 1)
+            for k in range(ns):
+                for name in ("np", "offset", "ix", "x", "u"):
+                    key = f"s1/sp{k}/{name}"
+                    h = hashlib.sha256(np.ascontiguousarray(st[key]).tobytes()).hexdigest()
+                    assert h == entry["sha256"][key], f"{case}: {key} differs after one step"
+            assert np.allclose(st["s1/field_energy"], entry["values"]["s1/field_energy"], rtol=1e-5, atol=0)
+    ref = np.array(entry["values"]["energies"]).reshape(last, ns + 6)[-1]
+    got = sim.energies()
+    tot_ref, tot_got = ref.sum(), got.sum()
+    say(f"{case}: total energy {tot_got:.9e} vs reference {tot_ref:.9e}")
+    assert abs(tot_got - tot_ref) <= 1e-3 * abs(tot_ref)
+    npk = f"s{last}/sp0/np"
+    if npk in entry["values"]:
+        n_ref = sum(entry["values"][npk])
+        n_got = sim.species[0].np_total()
+        assert abs(n_got - n_ref) <= max(2, 0.005 * n_ref)
+    sim.close()
+
+
+def test_window_api_errors(zp):
+    sim = build_sim(zp, ["ntiles=2,2", "nx=8,8", "ppc=1,1", "box=0.8,0.8", "dt=0.04", "plasma=electrons",
+                         "uth=0,0,0", "ufl=0,0,0"])
+    sim.advance(1)
+    with pytest.raises(zp.ZpicError):
+        sim.set_moving_window()    # only before the first step
+    with pytest.raises(zp.ZpicError):
+        sim.add_laser(zp.Gaussian(start=0.5, fwhm=0.2, a0=1.0, omega0=10.0, W0=0.0))   # W0 must be > 0
+    with pytest.raises(zp.ZpicError):
+        sim.add_laser(zp.PlaneWave(start=0.5, fwhm=0.2, a0=0.0, omega0=10.0))   # a0 must be > 0
+    sim.close()

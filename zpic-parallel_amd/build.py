@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libzpic_amd.so")
 
-SOURCES = ["particles.hip", "fields.hip", "diag.hip", "denoise.hip", "zdf.cpp", "host.cpp", "capi.cpp"]
+SOURCES = ["particles.hip", "fields.hip", "inject.hip", "diag.hip", "denoise.hip", "zdf.cpp", "host.cpp", "capi.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -95,6 +95,27 @@ int zpic_species_set_density_uniform( zpic_species * h, float n0 ) {
     } );
 }
 
+int zpic_species_set_density( zpic_species * h, const zpic_density * d ) {
+    return guarded( [&] {
+        if ( !h || !d ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
+        if ( h->added ) return zpic::set_error( ZPIC_ESTATE, "density must be set before add_species" );
+        if ( ( d->kind == ZPIC_DENSITY_STEP || d->kind == ZPIC_DENSITY_SLAB ) && d->dir != 0 && d->dir != 1 )
+            return zpic::set_error( ZPIC_EINVAL, "profile direction must be 0 (x) or 1 (y)" );
+        const coord::cart dir = d->dir == 1 ? coord::y : coord::x;
+        switch ( d->kind ) {
+        case ZPIC_DENSITY_NONE: h->sp->set_density( Density::None() ); break;
+        case ZPIC_DENSITY_UNIFORM: h->sp->set_density( Density::Uniform( d->n0 ) ); break;
+        case ZPIC_DENSITY_STEP: h->sp->set_density( Density::Step( dir, d->n0, d->pos ) ); break;
+        case ZPIC_DENSITY_SLAB: h->sp->set_density( Density::Slab( dir, d->n0, d->begin, d->end ) ); break;
+        case ZPIC_DENSITY_SPHERE:
+            h->sp->set_density( Density::Sphere( d->n0, make_float2( d->center[0], d->center[1] ), d->radius ) );
+            break;
+        default: return zpic::set_error( ZPIC_EINVAL, "unknown density profile" );
+        }
+        return ZPIC_OK;
+    } );
+}
+
 int zpic_simulation_add_species( zpic_simulation * s, zpic_species * h ) {
     return guarded( [&] {
         if ( !s || !h ) return zpic::set_error( ZPIC_EINVAL, "null handle" );
@@ -110,6 +131,61 @@ int zpic_simulation_advance( zpic_simulation * s, int n ) {
     return guarded( [&] {
         if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
         for ( int i = 0; i < n; i++ ) s->sim->advance();
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_set_moving_window( zpic_simulation * s ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        if ( s->sim->get_iter() != 0 )
+            return zpic::set_error( ZPIC_ESTATE, "(*error*) set_moving_window() called with iter != 0" );
+        s->sim->set_moving_window();
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_advance_mov_window( zpic_simulation * s, int n ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        for ( int i = 0; i < n; i++ ) s->sim->advance_mov_window();
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_window_moves( const zpic_simulation * s, unsigned * n_move ) {
+    if ( !s || !n_move ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
+    *n_move = s->sim->emf.moving_window.n_move();
+    return ZPIC_OK;
+}
+
+int zpic_laser_add( zpic_simulation * s, const zpic_laser * l ) {
+    return guarded( [&] {
+        if ( !s || !l ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
+        std::unique_ptr<Laser::Pulse> p;
+        if ( l->kind == 0 ) {
+            p.reset( new Laser::PlaneWave() );
+        } else if ( l->kind == 1 ) {
+            auto * g = new Laser::Gaussian();
+            g->W0 = l->W0;
+            g->focus = l->focus;
+            g->axis = l->axis;
+            p.reset( g );
+        } else {
+            return zpic::set_error( ZPIC_EINVAL, "unknown laser kind (0 plane wave, 1 Gaussian)" );
+        }
+        p->start = l->start;
+        p->fwhm = l->fwhm;
+        p->rise = l->rise;
+        p->flat = l->flat;
+        p->fall = l->fall;
+        p->a0 = l->a0;
+        p->omega0 = l->omega0;
+        p->polarization = l->polarization;
+        p->cos_pol = l->cos_pol;
+        p->sin_pol = l->sin_pol;
+        p->filter = l->filter;
+        if ( p->add( s->sim->emf ) != 0 ) return zpic::set_error( ZPIC_EINVAL, "(*error*) invalid laser parameters" );
         return ZPIC_OK;
     } );
 }

@@ -340,6 +340,50 @@ __global__ void k_fill( float * __restrict__ buf, size_t n, float v ) {
         buf[k] = v;
 }
 
+__global__ void k_grid_add( float * __restrict__ dst, const float * __restrict__ src, size_t n ) {
+    for ( size_t k = blockIdx.x * (size_t) blockDim.x + threadIdx.x; k < n; k += (size_t) gridDim.x * blockDim.x )
+        dst[k] += src[k];
+}
+
+// x_shift_left (grid.h:455-485) of one tile: rows of ext_x elements,
+// guard rows and columns included, staged in LDS and written back shifted.
+template <int NC>
+__global__ void __launch_bounds__( TILE_BLOCK ) k_x_shift_left( float * buf, G g, int shift ) {
+    extern __shared__ float A[];
+    const gptr<float> tb = gbl( buf ) + (size_t) blockIdx.x * g.vol * NC;
+    const int n = g.ext_x * g.ext_y * NC;
+    for ( int k = threadIdx.x; k < n; k += blockDim.x ) A[k] = tb[k];
+    __syncthreads();
+    const int keep = ( g.ext_x - shift ) * NC;
+    for ( int k = threadIdx.x; k < n; k += blockDim.x ) {
+        const int r = k % ( g.ext_x * NC );
+        tb[k] = ( r < keep ) ? A[k + shift * NC] : 0.0f;
+    }
+}
+
+// div_corr_x (laser.cpp:20-70): one thread per row of cells, tiles right to
+// left and cells right to left, the divergence accumulated in double.
+__global__ void k_div_corr_x( float * E, float * B, G g, double dx_dy ) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( row >= g.nty * g.ny ) return;
+    const int ty = row / g.ny, iy = row % g.ny;
+    const gptr<float> Eg = gbl( E ), Bg = gbl( B );
+    const int ys = g.ext_x;
+    double dE = 0.0, dB = 0.0;
+    for ( int tx = g.ntx - 1; tx >= 0; tx-- ) {
+        const size_t base = ( (size_t) ( ty * g.ntx + tx ) * g.vol + g.off ) * 3;
+        const gptr<float> te = Eg + base, tbb = Bg + base;
+        for ( int ix = g.nx - 1; ix >= 0; ix-- ) {
+            const float de = te[3 * ( ix + 1 + iy * ys ) + 1] - te[3 * ( ix + 1 + ( iy - 1 ) * ys ) + 1];
+            dE += dx_dy * de;
+            te[3 * ( ix + iy * ys )] = (float) dE;
+            const float db = tbb[3 * ( ix + ( iy + 1 ) * ys ) + 1] - tbb[3 * ( ix + iy * ys ) + 1];
+            dB += dx_dy * db;
+            tbb[3 * ( ix + iy * ys )] = (float) dB;
+        }
+    }
+}
+
 // ------------------------------------------------------ physical boundaries
 // Field values of a tile through its (x = gc lower) or (y = gc lower) origin,
 // as the reference's emf_bcx / emf_bcy / current_bcx / current_bcy address
@@ -347,13 +391,17 @@ __global__ void k_fill( float * __restrict__ buf, size_t n, float v ) {
 // y lower, y upper} with the reference's enum values (emf: 2 pec, 3 pmc;
 // current: 2 reflecting).  One workgroup per boundary tile; the x pass runs
 // before the y pass (the y pass reads the guard columns the x pass wrote).
-// A single tile along an axis takes only the lower branch, as the
-// reference's `for tx : {0, ntiles.x - 1}` with `tile_idx.x == 0` does.
+// A single tile along an axis takes only the lower branch, and takes it
+// twice, as the reference's `for tx : {0, ntiles.x - 1}` with
+// `tile_idx.x == 0` does (twice matters for the current's reflecting wall,
+// whose update is not idempotent); every row (x pass) or column (y pass)
+// only touches its own cells, so the thread of a row repeats it in order.
 struct BcArgs {
     float * E;  // E, or J for the current
     float * B;  // null for the current
     G g;
     int lo, hi;  // bc of the lower / upper side of this axis
+    int reps;    // 2 for a single tile along the axis, else 1
 };
 
 __device__ inline float3 ld3( gptr<float> p, int k ) { return make_float3( p[3 * k], p[3 * k + 1], p[3 * k + 2] ); }
@@ -371,7 +419,8 @@ __global__ void k_emf_bcx( BcArgs a ) {
     const int ys = g.ext_x, nx = g.nx;
     const gptr<float> E = gbl( a.E ) + ( (size_t) t * g.vol + g.gl ) * 3;
     const gptr<float> B = gbl( a.B ) + ( (size_t) t * g.vol + g.gl ) * 3;
-    for ( int idx = threadIdx.x; idx < g.ext_y; idx += blockDim.x ) {
+    for ( int idx = threadIdx.x; idx < g.ext_y; idx += blockDim.x )
+        for ( int r = 0; r < a.reps; r++ ) {
         const int iy = idx * ys;
         if ( tx == 0 ) {
             if ( a.lo == 3 ) {  // pmc
@@ -437,7 +486,8 @@ __global__ void k_emf_bcy( BcArgs a ) {
     const int ys = g.ext_x, ny = g.ny, nxx = g.nx;
     const gptr<float> E = gbl( a.E ) + ( (size_t) t * g.vol + (size_t) g.gl * ys ) * 3;
     const gptr<float> B = gbl( a.B ) + ( (size_t) t * g.vol + (size_t) g.gl * ys ) * 3;
-    for ( int ix = threadIdx.x; ix < g.ext_x; ix += blockDim.x ) {
+    for ( int ix = threadIdx.x; ix < g.ext_x; ix += blockDim.x )
+        for ( int r = 0; r < a.reps; r++ ) {
         if ( ty == 0 ) {
             if ( a.lo == 3 ) {  // pmc
                 E[3 * ( ix - ys )] = E[3 * ( ix + ys )];
@@ -493,7 +543,8 @@ __global__ void k_current_bcx( BcArgs a ) {
     const int t = blockIdx.y * g.ntx + tx;
     const int ys = g.ext_x, nx = g.nx;
     const gptr<float> J = gbl( a.E ) + ( (size_t) t * g.vol + g.gl ) * 3;
-    for ( int idx = threadIdx.x; idx < g.ext_y; idx += blockDim.x ) {
+    for ( int idx = threadIdx.x; idx < g.ext_y; idx += blockDim.x )
+        for ( int r = 0; r < a.reps; r++ ) {
         const int iy = idx * ys;
         if ( tx == 0 ) {
             if ( a.lo == 2 ) {
@@ -521,7 +572,8 @@ __global__ void k_current_bcy( BcArgs a ) {
     const int t = ty * g.ntx + blockIdx.x;
     const int ys = g.ext_x, ny = g.ny;
     const gptr<float> J = gbl( a.E ) + ( (size_t) t * g.vol + (size_t) g.gl * ys ) * 3;
-    for ( int ix = threadIdx.x; ix < g.ext_x; ix += blockDim.x ) {
+    for ( int ix = threadIdx.x; ix < g.ext_x; ix += blockDim.x )
+        for ( int r = 0; r < a.reps; r++ ) {
         if ( ty == 0 ) {
             if ( a.lo == 2 ) {
                 const float jx1 = J[3 * ( ix - ys )] + J[3 * ( ix + ys )];
@@ -834,14 +886,14 @@ static int physical_bc( float * F, float * Bf, const zpic_geom * gg, const int b
     hipStream_t st = (hipStream_t) stream;
     // bc values: 0 none, 1 periodic, >1 physical
     if ( bc[0] > 1 || bc[1] > 1 ) {
-        const BcArgs a{ F, Bf, g, bc[0], bc[1] };
+        const BcArgs a{ F, Bf, g, bc[0], bc[1], g.ntx > 1 ? 1 : 2 };
         const dim3 grid( g.ntx > 1 ? 2 : 1, g.nty );
         if ( emf ) hipLaunchKernelGGL( k_emf_bcx, grid, dim3( 64 ), 0, st, a );
         else hipLaunchKernelGGL( k_current_bcx, grid, dim3( 64 ), 0, st, a );
         ZPIC_LAUNCH_CHECK( "boundary x" );
     }
     if ( bc[2] > 1 || bc[3] > 1 ) {
-        const BcArgs a{ F, Bf, g, bc[2], bc[3] };
+        const BcArgs a{ F, Bf, g, bc[2], bc[3], g.nty > 1 ? 1 : 2 };
         const dim3 grid( g.ntx, g.nty > 1 ? 2 : 1 );
         if ( emf ) hipLaunchKernelGGL( k_emf_bcy, grid, dim3( 64 ), 0, st, a );
         else hipLaunchKernelGGL( k_current_bcy, grid, dim3( 64 ), 0, st, a );
@@ -901,6 +953,48 @@ int zpic_fill( float * buf, size_t n, float v, void * stream ) {
     if ( blocks > 8192 ) blocks = 8192;
     hipLaunchKernelGGL( k_fill, dim3( blocks ), dim3( 256 ), 0, (hipStream_t) stream, buf, n, v );
     ZPIC_LAUNCH_CHECK( "k_fill" );
+    return ZPIC_OK;
+}
+
+int zpic_grid_add( float * dst, const float * src, size_t n, void * stream ) {
+    if ( ( !dst || !src ) && n ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( n == 0 ) return ZPIC_OK;
+    size_t blocks = ( n + 255 ) / 256;
+    if ( blocks > 8192 ) blocks = 8192;
+    hipLaunchKernelGGL( k_grid_add, dim3( blocks ), dim3( 256 ), 0, (hipStream_t) stream, dst, src, n );
+    ZPIC_LAUNCH_CHECK( "k_grid_add" );
+    return ZPIC_OK;
+}
+
+int zpic_x_shift_left( float * buf, const zpic_geom * gg, int ncomp, int shift, int px, int py, void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !buf ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( ncomp != 1 && ncomp != 3 ) return set_error( ZPIC_EINVAL, "ncomp must be 1 or 3" );
+    if ( !( shift > 0 && shift < gg->gc_upper ) )
+        return set_error( ZPIC_EINVAL, "x_shift_left(), invalid shift value, must be 0 < shift < gc.x.upper" );
+    const G g = make( gg );
+    const size_t lds = sizeof( float ) * ncomp * g.ext_x * g.ext_y;
+    if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    hipStream_t st = (hipStream_t) stream;
+    if ( ncomp == 3 )
+        hipLaunchKernelGGL( k_x_shift_left<3>, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, g, shift );
+    else
+        hipLaunchKernelGGL( k_x_shift_left<1>, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, g, shift );
+    ZPIC_LAUNCH_CHECK( "k_x_shift_left" );
+    return gc( 2, buf, gg, ncomp, px, py, stream );
+}
+
+int zpic_div_corr_x( float * E, float * B, const zpic_geom * gg, float dx, float dy, void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !E || !B ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( gg->gc_lower < 1 || gg->gc_upper < 1 ) return set_error( ZPIC_EINVAL, "div_corr_x needs guard cells" );
+    if ( !( dx > 0 ) || !( dy > 0 ) ) return set_error( ZPIC_EINVAL, "cell size must be positive" );
+    const G g = make( gg );
+    const int rows = g.nty * g.ny;
+    const double dx_dy = ( (double) dx ) / ( (double) dy );
+    hipLaunchKernelGGL( k_div_corr_x, dim3( ( rows + 63 ) / 64 ), dim3( 64 ), 0, (hipStream_t) stream, E, B, g,
+                        dx_dy );
+    ZPIC_LAUNCH_CHECK( "k_div_corr_x" );
     return ZPIC_OK;
 }
 

@@ -6,6 +6,7 @@
 // per tile range, and is uploaded once; every per-step stage runs on the GPU.
 #include "common.h"
 #include "zpic.h"
+#include "density.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -150,7 +151,27 @@ uint64_t Particles::np_total() const {
     return s;
 }
 
-ParticleSort::ParticleSort( uint2 const nt2, uint32_t const max_part ) : ntiles( nt2.x * nt2.y ) {
+void Particles::grow_buffer( uint32_t new_max ) {
+    if ( new_max <= max_part ) return;
+    sync();  // no kernel may still use the old arrays
+    device_free( u ); device_free( x ); device_free( ix );
+    u = nullptr; x = nullptr; ix = nullptr;
+    max_part = ( new_max + 65535u ) & ~65535u;
+    ix = device_alloc<int2>( max_part );
+    x = device_alloc<float2>( max_part );
+    u = device_alloc<float3>( max_part );
+}
+
+void swap_buffers( Particles & a, Particles & b ) {
+    std::swap( a.ix, b.ix );
+    std::swap( a.x, b.x );
+    std::swap( a.u, b.u );
+    std::swap( a.max_part, b.max_part );
+    std::swap( a.np, b.np );
+    std::swap( a.offset, b.offset );
+}
+
+ParticleSort::ParticleSort( uint2 const nt2, uint32_t const max_part ) : ntiles( nt2.x * nt2.y ), idx_size( max_part ) {
     buf.idx = device_alloc<int>( max_part );
     buf.npt = device_alloc<int>( 9 * (size_t) ntiles );
     buf.new_np = device_alloc<int>( ntiles );
@@ -160,31 +181,106 @@ ParticleSort::ParticleSort( uint2 const nt2, uint32_t const max_part ) : ntiles(
     buf.nmove = device_alloc<unsigned long long>( 1 );
     hip_check( hipMemsetAsync( buf.nmove, 0, sizeof( unsigned long long ), stream() ), "hipMemsetAsync" );
 }
+void ParticleSort::grow( uint32_t max_part ) {
+    if ( max_part <= idx_size ) return;
+    sync();
+    device_free( buf.idx );
+    buf.idx = nullptr;
+    buf.idx = device_alloc<int>( max_part );
+    idx_size = max_part;
+}
+
 ParticleSort::~ParticleSort() {
     device_free( buf.nmove );
     device_free( buf.total ); device_free( buf.tmp_np ); device_free( buf.seg );
     device_free( buf.new_np ); device_free( buf.npt ); device_free( buf.idx );
 }
 
-// ------------------------------------------------------------ Density::Uniform
-// Whole-box injection: every cell of every tile gets ppc.x*ppc.y particles on
-// a regular lattice, in the "mk2" order of density.cpp:91-186 (all cells for
-// lattice point 0, then all cells for lattice point 1, ...).
-void Density::Uniform::np_inject( uint2 ntiles, uint2 nx, uint2 ppc, int * np ) const {
-    for ( unsigned t = 0; t < ntiles.x * ntiles.y; t++ ) np[t] = nx.x * nx.y * ppc.x * ppc.y;
+// ----------------------------------------------------------------- Density
+zpic_inject_profile Density::None::desc( float2 const dx, float2 const ) const {
+    zpic_inject_profile d{};
+    d.kind = ZPIC_DENSITY_NONE;
+    d.dx = dx.x; d.dy = dx.y;
+    return d;
 }
 
-void Density::Uniform::inject( uint2 ntiles, uint2 nx, uint2 ppc, const int * offset, int2 * ix, float2 * x ) const {
-    const int cells = nx.x * nx.y;
-    const double sx = 1.0 / ppc.x, sy = 1.0 / ppc.y;
+zpic_inject_profile Density::Uniform::desc( float2 const dx, float2 const ) const {
+    zpic_inject_profile d{};
+    d.kind = ZPIC_DENSITY_UNIFORM;
+    d.dx = dx.x; d.dy = dx.y;
+    return d;
+}
+
+// Density::Step::inject / np_inject (density.cpp:415-442, 528-573):
+// step_pos = (pos - ref.x) / dx.x
+zpic_inject_profile Density::Step::desc( float2 const dx, float2 const ref ) const {
+    zpic_inject_profile d{};
+    d.kind = ZPIC_DENSITY_STEP;
+    d.dir = ( dir == coord::x ) ? 0 : 1;
+    d.start = ( pos - ref.x ) / dx.x;
+    d.dx = dx.x; d.dy = dx.y;
+    return d;
+}
+
+// Density::Slab::inject / np_inject (density.cpp:674-718, 807-854)
+zpic_inject_profile Density::Slab::desc( float2 const dx, float2 const ref ) const {
+    zpic_inject_profile d{};
+    d.kind = ZPIC_DENSITY_SLAB;
+    d.dir = ( dir == coord::x ) ? 0 : 1;
+    d.start = ( begin - ref.x ) / dx.x;
+    d.end = ( end - ref.x ) / dx.x;
+    d.dx = dx.x; d.dy = dx.y;
+    return d;
+}
+
+// Density::Sphere::inject / np_inject (density.cpp:950-967, 1050-1067)
+zpic_inject_profile Density::Sphere::desc( float2 const dx, float2 const ref ) const {
+    zpic_inject_profile d{};
+    d.kind = ZPIC_DENSITY_SPHERE;
+    d.center_x = center.x - ref.x;
+    d.center_y = center.y - ref.y;
+    d.radius = radius;
+    d.dx = dx.x; d.dy = dx.y;
+    return d;
+}
+
+// The lattice points of the range kept by the profile, per tile, in the
+// reference's order (lattice point i1, i0, then cell in the clipped range).
+void Density::np_inject_host( zpic_inject_profile const & d, uint2 ntiles, uint2 nx, uint2 ppc, const int range[4],
+                              int * np ) {
     parallel_tiles( ntiles.x * ntiles.y, [&]( int t ) {
+        const int tx = t % ntiles.x, ty = t / ntiles.x;
+        TileRange r;
+        int n = 0;
+        if ( d.kind != ZPIC_DENSITY_NONE && tile_range( range, tx, ty, nx.x, nx.y, r ) ) {
+            for ( unsigned i1 = 0; i1 < ppc.y; i1++ )
+                for ( unsigned i0 = 0; i0 < ppc.x; i0++ ) {
+                    const float px = lattice_pos( i0, ppc.x ), py = lattice_pos( i1, ppc.y );
+                    for ( int c = 0; c < r.vol; c++ )
+                        n += profile_keeps( d, tx * nx.x + c % r.row + r.i0, ty * nx.y + c / r.row + r.j0, px, py );
+                }
+        }
+        np[t] = n;
+    } );
+}
+
+void Density::inject_host( zpic_inject_profile const & d, uint2 ntiles, uint2 nx, uint2 ppc, const int range[4],
+                           const int * offset, int2 * ix, float2 * x ) {
+    parallel_tiles( ntiles.x * ntiles.y, [&]( int t ) {
+        const int tx = t % ntiles.x, ty = t / ntiles.x;
+        TileRange r;
+        if ( d.kind == ZPIC_DENSITY_NONE || !tile_range( range, tx, ty, nx.x, nx.y, r ) ) return;
         int n = offset[t];
-        for ( unsigned ly = 0; ly < ppc.y; ly++ )
-            for ( unsigned lx = 0; lx < ppc.x; lx++ ) {
-                const float2 pos = make_float2( (float) ( sx * ( lx + 0.5 ) - 0.5 ), (float) ( sy * ( ly + 0.5 ) - 0.5 ) );
-                for ( int c = 0; c < cells; c++, n++ ) {
-                    ix[n] = make_int2( c % (int) nx.x, c / (int) nx.x );
-                    x[n] = pos;
+        for ( unsigned i1 = 0; i1 < ppc.y; i1++ )
+            for ( unsigned i0 = 0; i0 < ppc.x; i0++ ) {
+                const float2 pos = make_float2( lattice_pos( i0, ppc.x ), lattice_pos( i1, ppc.y ) );
+                for ( int c = 0; c < r.vol; c++ ) {
+                    const int2 cell = make_int2( c % r.row + r.i0, c / r.row + r.j0 );
+                    if ( profile_keeps( d, tx * nx.x + cell.x, ty * nx.y + cell.y, pos.x, pos.y ) ) {
+                        ix[n] = cell;
+                        x[n] = pos;
+                        n++;
+                    }
                 }
             }
     } );
@@ -293,6 +389,20 @@ void Current::advance() {
     }
     filter->apply( *J );
     iter++;
+    // only the axis range of the current's diagnostics follows this window
+    // (current.cpp:240-245)
+    if ( moving_window.needs_move( iter * dt ) ) moving_window.advance();
+}
+
+int Current::set_moving_window() {
+    if ( iter == 0 ) {
+        moving_window.init( dx.x );
+        bc.x.lower = bc.x.upper = current::bc::none;
+        J->set_periodic( make_int2( 0, J->get_periodic().y ) );
+        return 0;
+    }
+    std::cerr << "(*error*) set_moving_window() called with iter != 0\n";
+    return -1;
 }
 
 void Current::zero() { J->zero(); }
@@ -357,14 +467,41 @@ void EMF::advance( Current & current ) {
     check( zpic_yee( E->fbuf(), B->fbuf(), current.J->fbuf(), &g, dtdx, dtdy, (float) dt, p.x, p.y, stream() ) );
     process_bc();
     iter += 1;
+    if ( moving_window.active() ) move_window();
 }
 
+// EMF::advance() (emf.cpp:135-193): no current
 void EMF::advance() {
     const float dtdx = dt / dx.x, dtdy = dt / dx.y;
     zpic_geom g = E->geom();
     const int2 p = E->get_periodic();
     check( zpic_yee( E->fbuf(), B->fbuf(), nullptr, &g, dtdx, dtdy, (float) dt, p.x, p.y, stream() ) );
+    process_bc();
     iter += 1;
+    if ( moving_window.active() ) move_window();
+}
+
+int EMF::set_moving_window() {
+    if ( iter == 0 ) {
+        moving_window.init( dx.x );
+        bc.x.lower = bc.x.upper = emf::bc::none;
+        const int2 p = make_int2( 0, E->get_periodic().y );
+        E->set_periodic( p );
+        B->set_periodic( p );
+        return 0;
+    }
+    std::cerr << "(*error*) set_moving_window() called with iter != 0\n";
+    return -1;
+}
+
+void EMF::move_window() {
+    if ( moving_window.needs_move( iter * dt ) ) {
+        zpic_geom g = E->geom();
+        const int2 p = E->get_periodic();
+        check( zpic_x_shift_left( E->fbuf(), &g, 3, 1, p.x, p.y, stream() ) );
+        check( zpic_x_shift_left( B->fbuf(), &g, 3, 1, p.x, p.y, stream() ) );
+        moving_window.advance();
+    }
 }
 
 void EMF::get_energy( double3 & ene_E, double3 & ene_B ) {
@@ -381,7 +518,7 @@ void EMF::get_energy( double3 & ene_E, double3 & ene_B ) {
 // ------------------------------------------------------------------ Species
 Species::Species( std::string const name_, float const m_q_, uint2 const ppc_ )
     : ppc( ppc_ ), particles( nullptr ), tmp( nullptr ), sort( nullptr ), d_tile_energy( nullptr ),
-      d_energy_dev( nullptr ), d_nmove( 0 ), np_count( 0 ), deterministic( false ), seed_offset( 0 ), name( name_ ),
+      d_energy_dev( nullptr ), d_nmove( 0 ), np_bound( 0 ), d_np_inj( nullptr ), deterministic( false ), seed_offset( 0 ), name( name_ ),
       m_q( m_q_ ),
       push_type( species::boris ) {
     if ( m_q == 0 ) throw std::runtime_error( "(*error*) Invalid m_q value, must be not 0, aborting..." );
@@ -396,6 +533,7 @@ Species::Species( std::string const name_, float const m_q_, uint2 const ppc_ )
 }
 
 Species::~Species() {
+    device_free( d_np_inj );
     device_free( d_energy_dev );
     device_free( d_tile_energy );
     delete sort;
@@ -453,20 +591,25 @@ void Species::initialize( float2 const box_, uint2 const ntiles, uint2 const nx,
     const int nt = ntiles.x * ntiles.y;
     d_tile_energy = device_alloc<double>( nt );
     d_energy_dev = device_alloc<double>( 1 );
+    d_np_inj = device_alloc<int>( nt );
     hip_check( hipMemsetAsync( d_tile_energy, 0, nt * sizeof( double ), stream() ), "memset" );
     hip_check( hipMemsetAsync( d_energy_dev, 0, sizeof( double ), stream() ), "memset" );
     d_nmove = 0;
     iter = 0;
 
+    // Species::inject over the whole box (species.cpp:1808-1813), window origin
+    // at its current motion
+    const int range[4] = { 0, (int) ( nx.x * ntiles.x ) - 1, 0, (int) ( nx.y * ntiles.y ) - 1 };
+    const zpic_inject_profile prof = density->desc( dx, make_float2( moving_window.motion(), 0 ) );
     std::vector<int> np( nt ), off( nt );
-    density->np_inject( ntiles, nx, ppc, np.data() );
+    Density::np_inject_host( prof, ntiles, nx, ppc, range, np.data() );
     size_t total = 0;
     for ( int t = 0; t < nt; t++ ) { off[t] = (int) total; total += np[t]; }
     if ( total > max_part ) throw std::runtime_error( "(*error*) particle buffer too small for the injection" );
     std::vector<int2> hix( total );
     std::vector<float2> hx( total );
     std::vector<float3> hu( total );
-    density->inject( ntiles, nx, ppc, off.data(), hix.data(), hx.data() );
+    Density::inject_host( prof, ntiles, nx, ppc, range, off.data(), hix.data(), hx.data() );
     udist->set( nt, np.data(), off.data(), hix.data(), nx, hu.data(), id + seed_offset );
 
     to_device( particles->np, np.data(), nt );
@@ -475,7 +618,7 @@ void Species::initialize( float2 const box_, uint2 const ntiles, uint2 const nx,
     to_device( particles->x, hx.data(), total );
     to_device( particles->u, hu.data(), total );
     sync();
-    np_count = total;
+    np_bound = total;
 }
 
 // Species::push (species.cpp:2302-2335)
@@ -497,28 +640,100 @@ void Species::move( vec3grid<float3> * const J ) {
                               stream() ) );
 }
 
-void Species::tile_sort() {
+void Species::tile_sort() { tile_sort( nullptr, 0 ); }
+
+void Species::tile_sort( const int * extra, uint64_t max_extra ) {
     zpic_particles p = particles->view(), t = tmp->view();
-    check( zpic_tile_sort( &p, &t, &sort->buf, particles->ntiles.x, particles->ntiles.y, particles->nx.x,
-                           particles->nx.y, particles->periodic.x, particles->periodic.y, stream() ) );
+    const uint2 nt = particles->ntiles, nx = particles->nx;
+    const int2 per = particles->periodic;
+    check( zpic_tile_sort_count( &p, &t, &sort->buf, nt.x, nt.y, nx.x, nx.y, per.x, per.y, extra, stream() ) );
+    int mode = 0;
+    int total = 0;
+    if ( extra ) {
+        // the reference compares the total (reserved slots included) with
+        // the buffer size; it is read back only when the bound could exceed it
+        np_bound += max_extra;
+        if ( np_bound > particles->max_part ) {
+            to_host( &total, sort->buf.total, 1 );
+            sync();
+            np_bound = (uint32_t) total;
+            if ( (uint32_t) total > particles->max_part ) mode = 1;
+        }
+    }
+    if ( mode == 0 ) {
+        check( zpic_tile_sort_copy( &p, &t, &sort->buf, nt.x, nt.y, nx.x, nx.y, per.x, per.y, 0, stream() ) );
+        return;
+    }
+    // copy_sorted path (particles.cpp:868-881): grow the staging buffer, copy
+    // every particle into it in order, swap.  The staging buffer and the
+    // leaver list are then grown to the new size, so that later sorts (whose
+    // staging writes reach the new total) stay inside them.
+    tmp->grow_buffer( (uint32_t) total );
+    t = tmp->view();
+    check( zpic_tile_sort_copy( &p, &t, &sort->buf, nt.x, nt.y, nx.x, nx.y, per.x, per.y, 1, stream() ) );
+    swap_buffers( *particles, *tmp );
+    tmp->grow_buffer( particles->max_part );
+    sort->grow( particles->max_part );
+}
+
+void Species::np_inject( const int range[4], int * d_np ) {
+    const zpic_inject_profile prof = density->desc( dx, make_float2( moving_window.motion(), 0 ) );
+    const uint2 nt = particles->ntiles, nx = particles->nx;
+    check( zpic_np_inject( &prof, nt.x, nt.y, nx.x, nx.y, ppc.x, ppc.y, range, d_np, stream() ) );
+}
+
+void Species::inject( const int range[4] ) {
+    const zpic_inject_profile prof = density->desc( dx, make_float2( moving_window.motion(), 0 ) );
+    const uint2 nt = particles->ntiles, nx = particles->nx;
+    zpic_particles p = particles->view();
+    check( zpic_inject( &prof, &p, nt.x, nt.y, nx.x, nx.y, ppc.x, ppc.y, range, stream() ) );
+}
+
+int Species::set_moving_window() {
+    if ( iter == 0 ) {
+        moving_window.init( dx.x );
+        bc.x.lower = bc.x.upper = species::bc::open;
+        particles->periodic.x = false;
+        return 0;
+    }
+    std::cerr << "(*error*) set_moving_window() called with iter != 0\n";
+    return -1;
 }
 
 // Species::advance(emf, current) (species.cpp:2051-2068): the push and the
 // move run as one pass over the particles.
-void Species::advance( EMF const & emf, Current & current ) { advance_timed( emf, current, nullptr ); }
+void Species::advance( EMF const & emf, Current & current ) { step( emf, current, nullptr, false ); }
 
-void Species::advance_timed( EMF const & emf, Current & current, hipEvent_t after_push ) {
+void Species::advance_timed( EMF const & emf, Current & current, hipEvent_t after_push, bool window ) {
+    step( emf, current, after_push, window );
+}
+
+void Species::advance_mov_window( EMF const & emf, Current & current ) { step( emf, current, nullptr, true ); }
+
+void Species::step( EMF const & emf, Current & current, hipEvent_t after_push, bool window ) {
     const float alpha = 0.5 * dt / m_q;
     const float2 dt_dx = make_float2( dt / dx.x, dt / dx.y );
     const float2 qnx = make_float2( q * dx.x / dt, q * dx.y / dt );
+    // species.cpp:2088: the window time is (iter + 1) * dt in float
+    const bool shift = window && moving_window.needs_move( ( iter + 1 ) * dt );
     zpic_geom g = emf.E->geom();
     zpic_particles p = particles->view();
-    check( zpic_push_move_deposit_bc( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha, dt_dx.x,
-                                      dt_dx.y, q, qnx.x, qnx.y, d_tile_energy, &sort->buf, reflect_mask(),
-                                      deterministic, stream() ) );
+    check( zpic_push_move_deposit_shift( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha, dt_dx.x,
+                                         dt_dx.y, q, qnx.x, qnx.y, d_tile_energy, &sort->buf, reflect_mask(),
+                                         shift ? -1 : 0, 0, deterministic, stream() ) );
     if ( after_push ) hip_check( hipEventRecord( after_push, stream() ), "hipEventRecord" );
+    if ( shift ) {
+        // inject into the rightmost column of cells (species.cpp:2094-2112)
+        const uint2 gd = particles->dims;
+        const int range[4] = { (int) gd.x - 1, (int) gd.x - 1, 0, (int) gd.y - 1 };
+        np_inject( range, d_np_inj );
+        tile_sort( d_np_inj, (uint64_t) gd.y * ppc.x * ppc.y );
+        inject( range );
+        moving_window.advance();
+    } else {
+        tile_sort();
+    }
     iter++;
-    tile_sort();
     check( zpic_sum_tiles( d_tile_energy, g.ntiles_x * g.ntiles_y, d_energy_dev, deterministic, stream() ) );
 }
 
@@ -562,7 +777,7 @@ void Species::set_particles( const int * np, const int * ix, const float * x, co
     to_device( particles->x, reinterpret_cast<const float2 *>( x ), n );
     to_device( particles->u, reinterpret_cast<const float3 *>( u ), n );
     sync();
-    np_count = n;
+    np_bound = n;
 }
 
 void Species::subsample_from( const Species & src, int stride, int phase ) {
@@ -571,7 +786,7 @@ void Species::subsample_from( const Species & src, int stride, int phase ) {
         throw std::runtime_error( "(*error*) subsample needs species with the same tiling" );
     zpic_particles s = src.particles->view(), d = particles->view();
     check( zpic_subsample( &s, &d, particles->ntiles.x * particles->ntiles.y, stride, phase, stream() ) );
-    np_count = particles->np_total();
+    np_bound = particles->np_total();
 }
 
 // Particles::gather (particles.cpp:88-150) with Species::gather's scales
@@ -615,8 +830,18 @@ void Simulation::set_timing( bool on ) {
     t_steps = 0;
 }
 
-// Simulation::advance (simulation.h:90-107)
-void Simulation::advance() {
+// Simulation::advance (simulation.h:90-107) and advance_mov_window
+// (simulation.h:113-127)
+void Simulation::advance() { step( false ); }
+void Simulation::advance_mov_window() { step( true ); }
+
+void Simulation::set_moving_window() {
+    emf.set_moving_window();
+    current.set_moving_window();
+    for ( auto * s : species ) s->set_moving_window();
+}
+
+void Simulation::step( bool window ) {
     const size_t nsp = species.size();
     const size_t nev = 2 * nsp + 3;
     if ( timing && ev.size() < nev ) {
@@ -629,7 +854,7 @@ void Simulation::advance() {
     current.zero();
     if ( timing ) hip_check( hipEventRecord( ev[0], stream() ), "hipEventRecord" );
     for ( size_t k = 0; k < nsp; k++ ) {
-        species[k]->advance_timed( emf, current, timing ? ev[1 + 2 * k] : nullptr );
+        species[k]->advance_timed( emf, current, timing ? ev[1 + 2 * k] : nullptr, window );
         if ( timing ) hip_check( hipEventRecord( ev[2 + 2 * k], stream() ), "hipEventRecord" );
     }
     current.advance();
@@ -1128,4 +1353,186 @@ extern "C" int zpic_autotune( unsigned grid_x, unsigned grid_y, unsigned ppc_x, 
         (void) zpic_set_threads_per_tile( saved );
         return zpic::set_error( ZPIC_EHIP, e.what() );
     }
+}
+
+// -------------------------------------------------------------------- Laser
+// Field values are computed on the host with the reference's types: the
+// envelope's sine in double (M_PI_2 * csi / rise), the Gaussian beam's Gouy
+// phase with the double atan2 that laser.cpp's unqualified call resolves
+// to, every other function in float.
+float Laser::Pulse::lon_env( const float z ) const {
+    if ( z > start ) return 0.0;
+    if ( z > start - rise ) {
+        const float csi = z - start;
+        const float e = std::sin( M_PI_2 * csi / rise );
+        return e * e;
+    }
+    if ( z > start - ( rise + flat ) ) return 1.0;
+    if ( z > start - ( rise + flat + fall ) ) {
+        const float csi = z - ( start - rise - flat - fall );
+        const float e = std::sin( M_PI_2 * csi / fall );
+        return e * e;
+    }
+    return 0.0;
+}
+
+int Laser::Pulse::validate() {
+    if ( a0 <= 0 ) {
+        std::cerr << "(*error*) Invalid laser a0, must be > 0\n";
+        return -1;
+    }
+    if ( omega0 <= 0 ) {
+        std::cerr << "(*error*) Invalid laser OMEGA0, must be > 0\n";
+        return -1;
+    }
+    if ( fwhm > 0 ) {
+        rise = fwhm;
+        fall = fwhm;
+        flat = 0.;
+    } else {
+        if ( rise <= 0 ) {
+            std::cerr << "(*error*) Invalid laser RISE, must be > 0\n";
+            return -1;
+        }
+        if ( flat < 0 ) {
+            std::cerr << "(*error*) Invalid laser FLAT, must be >= 0\n";
+            return -1;
+        }
+        if ( fall <= 0 ) {
+            std::cerr << "(*error*) Invalid laser FALL, must be > 0\n";
+            return -1;
+        }
+    }
+    return 0;
+}
+
+int Laser::Pulse::add( EMF & emf ) {
+    vec3grid<float3> tmp_E( emf.E->get_ntiles(), emf.E->nx, emf.E->gc );
+    vec3grid<float3> tmp_B( emf.E->get_ntiles(), emf.B->nx, emf.B->gc );
+    const int ierr = launch( tmp_E, tmp_B, emf.box );
+    if ( !ierr ) {
+        check( zpic_grid_add( emf.E->fbuf(), tmp_E.fbuf(), emf.E->buffer_size() * 3, stream() ) );
+        check( zpic_grid_add( emf.B->fbuf(), tmp_B.fbuf(), emf.B->buffer_size() * 3, stream() ) );
+        sync();
+    }
+    return ierr;
+}
+
+namespace {
+
+// Guard copy, compensated x filter and upload-side steps shared by the
+// pulses (laser.cpp:276-286, 420-432).
+void finish_launch( vec3grid<float3> & E, vec3grid<float3> & B, std::vector<float3> & hE, std::vector<float3> & hB,
+                    unsigned filter ) {
+    E.upload( hE.data() );
+    B.upload( hB.data() );
+    E.copy_to_gc();
+    B.copy_to_gc();
+    if ( filter > 0 ) {
+        Filter::Compensated fcomp( coord::x, filter );
+        fcomp.apply( E );
+        fcomp.apply( B );
+    }
+}
+
+// gauss_phase (laser.cpp:303-316)
+float gauss_phase( const float omega0, const float W0, const float z, const float r ) {
+    const float z0 = omega0 * ( W0 * W0 ) / 2;
+    const float rho2 = r * r;
+    const float curv = rho2 * z / ( z0 * z0 + z * z );
+    const float rWl2 = ( z0 * z0 ) / ( z0 * z0 + z * z );
+    const float gouy_shift = (float) std::atan2( (double) z, (double) z0 );
+    return std::sqrt( std::sqrt( rWl2 ) ) * std::exp( -rho2 * rWl2 / ( W0 * W0 ) ) *
+           std::cos( omega0 * ( z + curv ) - gouy_shift );
+}
+
+}  // namespace
+
+// Laser::PlaneWave::launch (laser.cpp:224-289)
+int Laser::PlaneWave::launch( vec3grid<float3> & E, vec3grid<float3> & B, float2 box ) {
+    if ( validate() < 0 ) return -1;
+    if ( ( cos_pol == 0 ) && ( sin_pol == 0 ) ) {
+        cos_pol = std::cos( polarization );
+        sin_pol = std::sin( polarization );
+    }
+    const uint2 g_nx = E.get_dims();
+    const float2 dx = make_float2( box.x / g_nx.x, box.y / g_nx.y );
+    const uint2 ntiles = E.get_ntiles();
+    const uint2 nx = E.nx;
+    const int ystride = E.ext_nx.x;
+    const float k = omega0;
+    const float amp = omega0 * a0;
+    std::vector<float3> hE( E.buffer_size(), make_float3( 0, 0, 0 ) ), hB( B.buffer_size(), make_float3( 0, 0, 0 ) );
+    for ( unsigned ty = 0; ty < ntiles.y; ty++ )
+        for ( unsigned tx = 0; tx < ntiles.x; tx++ ) {
+            const unsigned tid = ty * ntiles.x + tx;
+            float3 * tile_E = &hE[(size_t) tid * E.tile_vol + E.offset];
+            float3 * tile_B = &hB[(size_t) tid * B.tile_vol + B.offset];
+            const int ix0 = tx * nx.x;
+            for ( unsigned iy = 0; iy < nx.y; iy++ )
+                for ( unsigned ix = 0; ix < nx.x; ix++ ) {
+                    const float z = ( ix0 + ix ) * dx.x;
+                    const float z_2 = ( ix0 + ix + 0.5 ) * dx.x;
+                    const float lenv = amp * lon_env( z );
+                    const float lenv_2 = amp * lon_env( z_2 );
+                    tile_E[ix + iy * ystride] = make_float3( 0, +lenv * std::cos( k * z ) * cos_pol,
+                                                             +lenv * std::cos( k * z ) * sin_pol );
+                    tile_B[ix + iy * ystride] = make_float3( 0, -lenv_2 * std::cos( k * z_2 ) * sin_pol,
+                                                             +lenv_2 * std::cos( k * z_2 ) * cos_pol );
+                }
+        }
+    finish_launch( E, B, hE, hB, filter );
+    return 0;
+}
+
+int Laser::Gaussian::validate() {
+    if ( Laser::Pulse::validate() < 0 ) return -1;
+    if ( W0 <= 0 ) {
+        std::cerr << "(*error*) Invalid laser W0, must be > 0\n";
+        return -1;
+    }
+    return 0;
+}
+
+// Laser::Gaussian::launch (laser.cpp:361-437)
+int Laser::Gaussian::launch( vec3grid<float3> & E, vec3grid<float3> & B, float2 box ) {
+    if ( validate() < 0 ) return -1;
+    if ( ( cos_pol == 0 ) && ( sin_pol == 0 ) ) {
+        cos_pol = std::cos( polarization );
+        sin_pol = std::sin( polarization );
+    }
+    const uint2 g_nx = E.get_dims();
+    const float2 dx = make_float2( box.x / g_nx.x, box.y / g_nx.y );
+    const uint2 ntiles = E.get_ntiles();
+    const uint2 nx = E.nx;
+    const int ystride = E.ext_nx.x;
+    const float amp = omega0 * a0;
+    std::vector<float3> hE( E.buffer_size(), make_float3( 0, 0, 0 ) ), hB( B.buffer_size(), make_float3( 0, 0, 0 ) );
+    for ( unsigned ty = 0; ty < ntiles.y; ty++ )
+        for ( unsigned tx = 0; tx < ntiles.x; tx++ ) {
+            const unsigned tid = ty * ntiles.x + tx;
+            float3 * tile_E = &hE[(size_t) tid * E.tile_vol + E.offset];
+            float3 * tile_B = &hB[(size_t) tid * B.tile_vol + B.offset];
+            const int ix0 = tx * nx.x;
+            const int iy0 = ty * nx.y;
+            for ( unsigned iy = 0; iy < nx.y; iy++ )
+                for ( unsigned ix = 0; ix < nx.x; ix++ ) {
+                    const float z = ( ix0 + ix ) * dx.x;
+                    const float z_2 = ( ix0 + ix + 0.5 ) * dx.x;
+                    const float r = ( iy0 + iy ) * dx.y - axis;
+                    const float r_2 = ( iy0 + iy + 0.5 ) * dx.y - axis;
+                    const float lenv = amp * lon_env( z );
+                    const float lenv_2 = amp * lon_env( z_2 );
+                    tile_E[ix + iy * ystride] =
+                        make_float3( 0, +lenv * gauss_phase( omega0, W0, z - focus, r_2 ) * cos_pol,
+                                     +lenv * gauss_phase( omega0, W0, z - focus, r ) * sin_pol );
+                    tile_B[ix + iy * ystride] =
+                        make_float3( 0, -lenv_2 * gauss_phase( omega0, W0, z_2 - focus, r ) * sin_pol,
+                                     +lenv_2 * gauss_phase( omega0, W0, z_2 - focus, r_2 ) * cos_pol );
+                }
+        }
+    finish_launch( E, B, hE, hB, filter );
+    zpic_geom g = E.geom();
+    check( zpic_div_corr_x( E.fbuf(), B.fbuf(), &g, dx.x, dx.y, stream() ) );
+    return 0;
 }

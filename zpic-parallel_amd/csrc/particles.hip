@@ -305,7 +305,8 @@ __device__ inline void deposit_elected( bool pend, const Seg & s, float qnx, flo
 
 struct MoveArgs {
     float dtdx, dtdy, q, qnx, qny;
-    int refl;  // reflecting walls: 1 x lower, 2 x upper, 4 y lower, 8 y upper
+    int refl;      // reflecting walls: 1 x lower, 2 x upper, 4 y lower, 8 y upper
+    int shx, shy;  // cell shift added to every moved particle (moving window)
 };
 
 // Species::process_bc (species.cpp:1854-1989) for one moved particle of
@@ -565,7 +566,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 // deltai (species.cpp:342-343)
                 const int2 di = make_int2( ( s0.b.x >= 0.5f ) - ( s0.b.x < -0.5f ), ( s0.b.y >= 0.5f ) - ( s0.b.y < -0.5f ) );
                 cross = di.x != 0 || di.y != 0;
-                int2 c1 = make_int2( c.x + di.x, c.y + di.y );
+                int2 c1 = make_int2( c.x + di.x + m.shx, c.y + di.y + m.shy );
                 float2 x1 = make_float2( s0.b.x - di.x, s0.b.y - di.y );
                 if ( m.refl && reflect_walls( m.refl, t % g.ntx, t / g.ntx, g.ntx, g.nty, g.nx, g.ny, c1, x1, u ) )
                     pu.set( i, u );
@@ -621,7 +622,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 ns = split_path( c, x, x1, d, qvz, di, sg );
                 x1.x -= di.x;
                 x1.y -= di.y;
-                int2 c1 = make_int2( c.x + di.x, c.y + di.y );
+                int2 c1 = make_int2( c.x + di.x + m.shx, c.y + di.y + m.shy );
                 if ( m.refl && reflect_walls( m.refl, t % g.ntx, t / g.ntx, g.ntx, g.nty, g.nx, g.ny, c1, x1, u ) )
                     pu.set( i, u );
                 px.set( i, x1 );
@@ -737,6 +738,7 @@ struct SortArgs {
     SortView S;
     SortGeo q;
     int ntiles;
+    const int * extra;  // free slots reserved after each tile (may be null)
 };
 
 // New particle count per tile (bnd_check's new_np, particles.cpp:212-303):
@@ -768,8 +770,9 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const 
     const int tid = threadIdx.x;
     const int per = ( ntiles + SCAN_BLOCK - 1 ) / SCAN_BLOCK;
     const int t0 = min( ntiles, tid * per ), t1 = min( ntiles, t0 + per );
+    const gptr<const int> extra = gbl( a.extra );
     int local = 0;
-    for ( int t = t0; t < t1; t++ ) local += S.new_np[t];
+    for ( int t = t0; t < t1; t++ ) local += S.new_np[t] + ( a.extra ? extra[t] : 0 );
     s_sum[tid] = local;
     __syncthreads();
     // Hillis-Steele inclusive scan of the per-thread sums
@@ -782,7 +785,7 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const 
     int base = s_sum[tid] - local;
     for ( int t = t0; t < t1; t++ ) {
         new_off[t] = base;
-        base += S.new_np[t];
+        base += S.new_np[t] + ( a.extra ? extra[t] : 0 );
     }
     if ( tid == SCAN_BLOCK - 1 ) S.total[0] = s_sum[tid];
     // Species::move's d_nmove += np (species.cpp:2162): the counts the
@@ -824,6 +827,10 @@ constexpr int SORT_BLOCK = 256;
 // staging slots of each target tile are filled in the reference's
 // single-thread reservation order: by source tile, the source's own shifted
 // stayers first, then directions 0..8.
+// SORTED: copy_sorted (particles.cpp:628-763) instead: every particle goes to
+// T at the new offsets, the n0 stayers (after the hole fill) reserved as
+// direction 4 in the same by-source, by-direction order, and T.np is set.
+template <bool SORTED>
 __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const SortArgs * tab, const int2 * map ) {
     __shared__ int s_start[9];
     __shared__ int s_pre[MAX_SEG + 1];
@@ -839,18 +846,21 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
         const int tg = ( d == 4 ) ? s : target_of( s, d, q );
         int start = -1;
         if ( tg >= 0 ) {
-            const int mine = ( d == 4 ) ? -1 : d;
+            const int mine = ( d == 4 && !SORTED ) ? -1 : d;
             int before = 0, total = 0;
             for ( int e = 0; e < 9; e++ ) {
                 const int se = ( e == 4 ) ? tg : source_of( tg, e, q );
                 if ( se < 0 ) continue;
-                const int cnt = ( e == 4 ) ? shifted_stayers( tg, P.off, T.off, S ) : S.npt[9 * se + e];
-                const int rank = ( e == 4 ) ? -1 : e;
+                int cnt;
+                if ( e != 4 ) cnt = S.npt[9 * se + e];
+                else if ( SORTED ) cnt = S.npt[9 * tg + 4];
+                else cnt = shifted_stayers( tg, P.off, T.off, S );
+                const int rank = ( e == 4 && !SORTED ) ? -1 : e;
                 if ( se < s || ( se == s && rank < mine ) ) before += cnt;
                 total += cnt;
             }
             start = T.off[tg] + before;
-            if ( d == 4 ) S.tmp_np[s] = total;
+            if ( d == 4 && !SORTED ) S.tmp_np[s] = total;
         }
         s_start[d] = start;
     }
@@ -957,17 +967,28 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
     }
     __syncthreads();
 
-    // shifted stayers (after the hole fill)
-    const int ns = shifted_stayers( s, P.off, T.off, S );
-    const int first = ( T.off[s] >= o ) ? 0 : n0 - ns;
-    const int dst0 = s_start[4];
-    for ( int i = tid; i < ns; i += blockDim.x ) {
-        T.ix[dst0 + i] = P.ix[o + first + i];
-        T.x[dst0 + i] = P.x[o + first + i];
-        T.u[dst0 + i] = P.u[o + first + i];
+    if constexpr ( SORTED ) {
+        // all stayers (after the hole fill)
+        const int dst0 = s_start[4];
+        for ( int i = tid; i < n0; i += blockDim.x ) {
+            T.ix[dst0 + i] = P.ix[o + i];
+            T.x[dst0 + i] = P.x[o + i];
+            T.u[dst0 + i] = P.u[o + i];
+        }
+        if ( tid == 0 ) T.np[s] = S.new_np[s];
+    } else {
+        // shifted stayers (after the hole fill)
+        const int ns = shifted_stayers( s, P.off, T.off, S );
+        const int first = ( T.off[s] >= o ) ? 0 : n0 - ns;
+        const int dst0 = s_start[4];
+        for ( int i = tid; i < ns; i += blockDim.x ) {
+            T.ix[dst0 + i] = P.ix[o + first + i];
+            T.x[dst0 + i] = P.x[o + first + i];
+            T.u[dst0 + i] = P.u[o + first + i];
+        }
+        __syncthreads();
+        if ( tid == 0 ) P.np[s] = n0 - ns;
     }
-    __syncthreads();
-    if ( tid == 0 ) P.np[s] = n0 - ns;
 }
 
 // copy_in (particles.cpp:559-614): staged particles appended after the
@@ -1081,8 +1102,10 @@ int check_geom( const zpic_geom * g ) {
 }
 
 // Threads per tile of the fast particle pass (a multiple of 64, at most
-// FAST_BLOCK); the deterministic mode always uses one wavefront.
-int g_threads = FAST_BLOCK;  // library-wide setting
+// FAST_BLOCK); the deterministic mode always uses one wavefront.  Default 256:
+// on the 2048^2 Weibel step the particle pass took 7.08 ms with 4 waves per
+// tile against 7.36 ms with 8 (fewer private J planes to clear and reduce).
+int g_threads = 256;  // library-wide setting
 
 constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup
 
@@ -1133,6 +1156,7 @@ SortArgs sort_args( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int
     a.P = view( p ); a.T = view( tmp ); a.S = view( s );
     a.q = SortGeo{ ntx, nty, nx, ny, px ? 1 : 0, py ? 1 : 0 };
     a.ntiles = ntx * nty;
+    a.extra = nullptr;
     return a;
 }
 
@@ -1252,7 +1276,7 @@ int batch_particles_step( BatchParticlePlan * plan, void * stream ) {
     hipLaunchKernelGGL( k_sort_count, dim3( plan->nentries ), dim3( SCAN_BLOCK ), 0, st, sd,
                         (const SortArgs *) plan->d_sort );
     ZPIC_LAUNCH_CHECK( "k_sort_count (batch)" );
-    hipLaunchKernelGGL( k_sort_out, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
+    hipLaunchKernelGGL( k_sort_out<false>, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
                         (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
     ZPIC_LAUNCH_CHECK( "k_sort_out (batch)" );
     hipLaunchKernelGGL( k_sort_in, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
@@ -1296,24 +1320,61 @@ int zpic_push_move_deposit_bc( const zpic_particles * p, const zpic_geom * fld, 
     return launch_particles<true, true>( p, fld, E, B, J, alpha, m, tile_energy, s, deterministic, stream );
 }
 
-int zpic_tile_sort( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int ntiles_x, int ntiles_y, int nx,
-                    int ny, int periodic_x, int periodic_y, void * stream ) {
+int zpic_push_move_deposit_shift( const zpic_particles * p, const zpic_geom * fld, const float * E, const float * B,
+                                  float * J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
+                                  double * tile_energy, zpic_sort * s, int reflect, int shift_x, int shift_y,
+                                  int deterministic, void * stream ) {
+    if ( reflect < 0 || reflect > 15 ) return set_error( ZPIC_EINVAL, "reflect mask must be in [0, 15]" );
+    if ( shift_x < -1 || shift_x > 1 || shift_y < -1 || shift_y > 1 )
+        return set_error( ZPIC_EINVAL, "cell shift must be -1, 0 or 1 per axis" );
+    MoveArgs m{ dt_dx, dt_dy, q, qnx, qny, reflect, shift_x, shift_y };
+    return launch_particles<true, true>( p, fld, E, B, J, alpha, m, tile_energy, s, deterministic, stream );
+}
+
+int zpic_tile_sort_count( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int ntiles_x, int ntiles_y, int nx,
+                          int ny, int periodic_x, int periodic_y, const int * extra, void * stream ) {
     if ( !p || !tmp || !s || ntiles_x < 1 || ntiles_y < 1 || nx < 1 || ny < 1 )
         return set_error( ZPIC_EINVAL, "invalid tile_sort arguments" );
     const int nt = ntiles_x * ntiles_y;
-    const SortArgs a = sort_args( p, tmp, s, ntiles_x, ntiles_y, nx, ny, periodic_x, periodic_y );
+    SortArgs a = sort_args( p, tmp, s, ntiles_x, ntiles_y, nx, ny, periodic_x, periodic_y );
+    a.extra = extra;
     const SortArgs * tab = nullptr;
-    const int2 * map = nullptr;
     hipStream_t st = (hipStream_t) stream;
     hipLaunchKernelGGL( k_sort_newnp, dim3( ( nt + 255 ) / 256 ), dim3( 256 ), 0, st, a, tab );
     ZPIC_LAUNCH_CHECK( "k_sort_newnp" );
     hipLaunchKernelGGL( k_sort_count, dim3( 1 ), dim3( SCAN_BLOCK ), 0, st, a, tab );
     ZPIC_LAUNCH_CHECK( "k_sort_count" );
-    hipLaunchKernelGGL( k_sort_out, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
+    return ZPIC_OK;
+}
+
+int zpic_tile_sort_copy( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int ntiles_x, int ntiles_y, int nx,
+                         int ny, int periodic_x, int periodic_y, int mode, void * stream ) {
+    if ( !p || !tmp || !s || ntiles_x < 1 || ntiles_y < 1 || nx < 1 || ny < 1 )
+        return set_error( ZPIC_EINVAL, "invalid tile_sort arguments" );
+    if ( mode != 0 && mode != 1 ) return set_error( ZPIC_EINVAL, "tile_sort copy mode must be 0 or 1" );
+    const int nt = ntiles_x * ntiles_y;
+    const SortArgs a = sort_args( p, tmp, s, ntiles_x, ntiles_y, nx, ny, periodic_x, periodic_y );
+    const SortArgs * tab = nullptr;
+    const int2 * map = nullptr;
+    hipStream_t st = (hipStream_t) stream;
+    if ( mode == 1 ) {
+        hipLaunchKernelGGL( k_sort_out<true>, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
+        ZPIC_LAUNCH_CHECK( "k_sort_out (copy_sorted)" );
+        return ZPIC_OK;
+    }
+    hipLaunchKernelGGL( k_sort_out<false>, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
     ZPIC_LAUNCH_CHECK( "k_sort_out" );
     hipLaunchKernelGGL( k_sort_in, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
     ZPIC_LAUNCH_CHECK( "k_sort_in" );
     return ZPIC_OK;
+}
+
+int zpic_tile_sort( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int ntiles_x, int ntiles_y, int nx,
+                    int ny, int periodic_x, int periodic_y, void * stream ) {
+    if ( int rc = zpic_tile_sort_count( p, tmp, s, ntiles_x, ntiles_y, nx, ny, periodic_x, periodic_y, nullptr,
+                                        stream ) )
+        return rc;
+    return zpic_tile_sort_copy( p, tmp, s, ntiles_x, ntiles_y, nx, ny, periodic_x, periodic_y, 0, stream );
 }
 
 int zpic_set_threads_per_tile( int threads ) {

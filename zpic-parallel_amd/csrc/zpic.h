@@ -321,6 +321,13 @@ class Particles {
 
     zpic_particles view() const;
     uint64_t np_total() const;
+    // Particles::grow_buffer (particles.h:174-187): reallocates the particle
+    // arrays (contents discarded) to new_max rounded up to 64 Ki particles,
+    // if new_max exceeds the current size
+    void grow_buffer( uint32_t new_max );
+    // swap_buffers (particles.h:195-206): exchanges the particle arrays, the
+    // counts and offsets, and the sizes
+    friend void swap_buffers( Particles & a, Particles & b );
     std::vector<int> get_np() const;
     std::vector<int> get_offset() const;
 };
@@ -331,6 +338,9 @@ class ParticleSort {
     const int ntiles;
     ParticleSort( uint2 const ntiles, uint32_t const max_part );
     ~ParticleSort();
+    // leaver list sized for max_part particles (grown with the particle buffer)
+    void grow( uint32_t max_part );
+    uint32_t idx_size;
     ParticleSort( const ParticleSort & ) = delete;
     ParticleSort & operator=( const ParticleSort & ) = delete;
 };
@@ -377,34 +387,65 @@ class ThermalCorr : public Type {
 }  // namespace UDistribution
 
 // --------------------------------------------------------------- Density
+// Density profiles (density.h:11-132): the density n0 sets the charge per
+// particle; the profile selects which points of the regular injection
+// lattice are kept.  desc() gives the selection as plain data with lengths
+// relative to the window origin `ref`, as the reference's inject /
+// np_inject compute them (density.cpp:415-442, 674-718, 950-967): a step or
+// a slab along either axis is placed with ref.x and dx.x.
 namespace Density {
 class Profile {
   public:
     const float n0;
     Profile( float const n0_ ) : n0( n0_ < 0 ? -n0_ : n0_ ) {}
     virtual Profile * clone() const = 0;
-    // particles injected per tile for the whole box
-    virtual void np_inject( uint2 ntiles, uint2 nx, uint2 ppc, int * np ) const = 0;
-    // positions in mk2 order (density.cpp:91-186) at [offset[t], offset[t]+np[t])
-    virtual void inject( uint2 ntiles, uint2 nx, uint2 ppc, const int * offset, int2 * ix, float2 * x ) const = 0;
+    virtual zpic_inject_profile desc( float2 const dx, float2 const ref ) const = 0;
     virtual ~Profile() = default;
 };
 class None : public Profile {
   public:
     None() : Profile( 0.f ) {}
     None * clone() const override { return new None(); }
-    void np_inject( uint2 ntiles, uint2, uint2, int * np ) const override {
-        for ( unsigned t = 0; t < ntiles.x * ntiles.y; t++ ) np[t] = 0;
-    }
-    void inject( uint2, uint2, uint2, const int *, int2 *, float2 * ) const override {}
+    zpic_inject_profile desc( float2 const dx, float2 const ref ) const override;
 };
 class Uniform : public Profile {
   public:
     Uniform( float const n0_ ) : Profile( n0_ ) {}
     Uniform * clone() const override { return new Uniform( n0 ); }
-    void np_inject( uint2 ntiles, uint2 nx, uint2 ppc, int * np ) const override;
-    void inject( uint2 ntiles, uint2 nx, uint2 ppc, const int * offset, int2 * ix, float2 * x ) const override;
+    zpic_inject_profile desc( float2 const dx, float2 const ref ) const override;
 };
+class Step : public Profile {
+  public:
+    const float pos;
+    const coord::cart dir;
+    Step( coord::cart dir_, float const n0_, float const pos_ ) : Profile( n0_ ), pos( pos_ ), dir( dir_ ) {}
+    Step * clone() const override { return new Step( dir, n0, pos ); }
+    zpic_inject_profile desc( float2 const dx, float2 const ref ) const override;
+};
+class Slab : public Profile {
+  public:
+    const float begin, end;
+    const coord::cart dir;
+    Slab( coord::cart dir_, float const n0_, float begin_, float end_ )
+        : Profile( n0_ ), begin( begin_ ), end( end_ ), dir( dir_ ) {}
+    Slab * clone() const override { return new Slab( dir, n0, begin, end ); }
+    zpic_inject_profile desc( float2 const dx, float2 const ref ) const override;
+};
+class Sphere : public Profile {
+  public:
+    const float2 center;
+    const float radius;
+    Sphere( float const n0_, float2 center_, float radius_ ) : Profile( n0_ ), center( center_ ), radius( radius_ ) {}
+    Sphere * clone() const override { return new Sphere( n0, center, radius ); }
+    zpic_inject_profile desc( float2 const dx, float2 const ref ) const override;
+};
+
+// Host injection of a profile over the global cell range `range`
+// ({x lower, x upper, y lower, y upper}, inclusive): particles per tile, and
+// their cells and positions at [offset[t], offset[t] + np[t]).
+void np_inject_host( zpic_inject_profile const & d, uint2 ntiles, uint2 nx, uint2 ppc, const int range[4], int * np );
+void inject_host( zpic_inject_profile const & d, uint2 ntiles, uint2 nx, uint2 ppc, const int range[4],
+                  const int * offset, int2 * ix, float2 * x );
 }  // namespace Density
 
 namespace species {
@@ -455,6 +496,8 @@ class Current {
     }
     void advance();
     void zero();
+    // Current::set_moving_window (current.h:123-136): 0, or -1 after the first step
+    int set_moving_window();
     // Current::save (current.cpp:263-313): CURRENT/J<c>-<iter>.zdf
     void save( fcomp::cart const jc );
     void note_step() { iter++; }
@@ -486,6 +529,11 @@ class EMF {
     void get_energy( double3 & ene_E, double3 & ene_B );
     // EMF::process_bc (emf.cpp:431-471)
     void process_bc();
+    // EMF::set_moving_window (emf.h:152-165): 0, or -1 after the first step
+    int set_moving_window();
+    // EMF::move_window (emf.cpp:120-129): E and B shifted one cell left
+    // when the window is due to move
+    void move_window();
     // EMF::save (emf.cpp:589-655): EMF/<E|B><c>-<iter>.zdf
     void save( emf::field const field, fcomp::cart const fc );
     void note_step() { iter++; }
@@ -513,7 +561,11 @@ class Species {
     double * d_tile_energy;  // [ntiles] per-tile push energy sums
     double * d_energy_dev;   // [1]
     uint64_t d_nmove;
-    uint64_t np_count;       // particles held (constant for periodic boundaries)
+    // Upper bound of the particles held: set by every load, raised by the
+    // largest possible count of every injection, lowered to the exact total
+    // whenever that is read back (particles leave only through open walls)
+    uint64_t np_bound;
+    int * d_np_inj;          // [ntiles] particles to inject (moving window)
     bool deterministic;
     unsigned seed_offset;
 
@@ -540,9 +592,25 @@ class Species {
     void push( vec3grid<float3> * const E, vec3grid<float3> * const B );
     void move( vec3grid<float3> * const J );
     void tile_sort();
+    // Particles::tile_sort(tmp, sort, extra) (particles.cpp:855-904): room
+    // for extra[t] (device array) more particles after each tile; max_extra
+    // bounds their sum.  Takes the copy_sorted path, growing the buffers,
+    // when the total exceeds the particle buffer, as the reference does.
+    void tile_sort( const int * extra, uint64_t max_extra );
     virtual void advance( EMF const & emf, Current & current );
-    // advance() recording `after_push` (may be null) between the particle pass and the sort
-    void advance_timed( EMF const & emf, Current & current, hipEvent_t after_push );
+    // advance() (window: advance_mov_window()) recording `after_push` (may be
+    // null) between the particle pass and the sort
+    void advance_timed( EMF const & emf, Current & current, hipEvent_t after_push, bool window = false );
+    // Species::advance_mov_window (species.cpp:2083-2128): on the steps where
+    // the window moves, the particles shift one cell left and the density
+    // profile is injected into the rightmost column of cells
+    virtual void advance_mov_window( EMF const & emf, Current & current );
+    // Species::set_moving_window (species.h:254-265): 0, or -1 after the first step
+    virtual int set_moving_window();
+    // Species::np_inject / inject over a global cell range
+    // ({x lower, x upper, y lower, y upper}) on the device (species.cpp:1819-1842)
+    void np_inject( const int range[4], int * d_np );
+    void inject( const int range[4] );
 
     // Deterministic (reference single-thread) deposit / energy order
     void set_deterministic( bool on ) { deterministic = on; }
@@ -589,6 +657,9 @@ class Species {
     zpic_batch_species batch_item( EMF const & emf, Current & current ) const;
     // Host-side bookkeeping of one step advanced by a batch (iteration, nmove)
     void note_step() { iter++; }
+
+  private:
+    void step( EMF const & emf, Current & current, hipEvent_t after_push, bool window );
 };
 
 // --------------------------------------------------------------- Simulation
@@ -620,6 +691,9 @@ class Simulation {
         return nullptr;
     }
     void advance();
+    // Simulation::set_moving_window / advance_mov_window (simulation.h:56-62, 113-127)
+    void set_moving_window();
+    void advance_mov_window();
     unsigned int get_iter() const { return iter; }
     double get_t() const { return iter * double( dt ); }
     void energy_info();
@@ -636,6 +710,9 @@ class Simulation {
     void note_step();
     // Arguments of this simulation's fields for a batched advance
     zpic_batch_fields_item batch_fields() const;
+
+  private:
+    void step( bool window );
 };
 
 // ------------------------------------------------------------------- Batch
@@ -657,7 +734,7 @@ class Batch {
     void step_kernels();
 
   public:
-    explicit Batch( int threads_per_tile = 512 );
+    explicit Batch( int threads_per_tile = 256 );
     ~Batch();
     Batch( const Batch & ) = delete;
     Batch & operator=( const Batch & ) = delete;
@@ -671,3 +748,52 @@ class Batch {
     void set_graph( bool on );
     size_t size() const { return sims.size(); }
 };
+
+// -------------------------------------------------------------------- Laser
+// Laser pulses (laser.h, laser.cpp:172-437).  launch() fills E and B with
+// the pulse on the host, in the reference's float / double arithmetic and
+// with the host math library (as the reference, which builds it on the CPU),
+// then the guard copy, the compensated x filter (Pulse::filter passes, if
+// > 0), and for the Gaussian beam the divergence correction (div_corr_x) run
+// on the device; add() launches into temporary grids and adds them to the
+// simulation fields.
+namespace Laser {
+class Pulse {
+  public:
+    float start;   // front edge of the pulse
+    float fwhm;    // if > 0: rise = fall = fwhm, flat = 0
+    float rise, flat, fall;
+    float a0;      // normalised peak vector potential
+    float omega0;  // frequency
+    float polarization;
+    float cos_pol, sin_pol;  // both 0: taken from `polarization`
+    unsigned int filter;
+
+    Pulse()
+        : start( 0 ), fwhm( 0 ), rise( 0 ), flat( 0 ), fall( 0 ), a0( 0 ), omega0( 0 ), polarization( 0 ), cos_pol( 0 ),
+          sin_pol( 0 ), filter( 1 ) {}
+    virtual ~Pulse() = default;
+    // longitudinal envelope (laser.h:38-63)
+    float lon_env( const float z ) const;
+    // 0, or -1 for invalid parameters (laser.cpp:178-210)
+    virtual int validate();
+    virtual int launch( vec3grid<float3> & E, vec3grid<float3> & B, float2 box ) = 0;
+    // Pulse::add (laser.h:67-81): 0, or -1 for invalid parameters
+    virtual int add( EMF & emf );
+};
+
+class PlaneWave : public Pulse {
+  public:
+    int launch( vec3grid<float3> & E, vec3grid<float3> & B, float2 box ) override;
+};
+
+class Gaussian : public Pulse {
+  public:
+    float W0;     // waist
+    float focus;  // focal plane position (x)
+    float axis;   // beam axis position (y)
+    Gaussian() : Pulse(), W0( 0 ), focus( 0 ), axis( 0 ) {}
+    int validate() override;
+    int launch( vec3grid<float3> & E, vec3grid<float3> & B, float2 box ) override;
+};
+}  // namespace Laser

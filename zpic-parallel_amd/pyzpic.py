@@ -114,6 +114,11 @@ def lib():
         "zpic_simulation_copy_fields": (i, [vp, vp]),
         "zpic_species_subsample": (i, [vp, vp, i, i]),
         "zpic_autotune": (i, [u, u, u, u, i, ctypes.c_longlong, ctypes.c_char_p, vp]),
+        "zpic_species_set_density": (i, [vp, vp]),
+        "zpic_simulation_set_moving_window": (i, [vp]),
+        "zpic_simulation_advance_mov_window": (i, [vp, i]),
+        "zpic_simulation_window_moves": (i, [vp, vp]),
+        "zpic_laser_add": (i, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -178,10 +183,46 @@ def zdf_save_grid(data, name, path, label=None, units=None, axes=None, iteration
     _chk(lib().zpic_zdf_save_grid(_p(data), dtype, ctypes.byref(info), ctypes.byref(it), path.encode()))
 
 
-class Species:
-    """Species(name, m_q, ppc) with the reference's udist / density setters."""
+class _Density(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("dir", ctypes.c_int), ("n0", ctypes.c_float), ("pos", ctypes.c_float),
+                ("begin", ctypes.c_float), ("end", ctypes.c_float), ("center", ctypes.c_float * 2),
+                ("radius", ctypes.c_float)]
 
-    def __init__(self, name, m_q, ppc, udist="none", uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2, n0=None, seed=0):
+
+_AXIS = {"x": 0, "y": 1, 0: 0, 1: 1}
+
+
+class Density:
+    """Density profiles (density.h:30-132; the reference's Cython classes
+    em2d.density.None / Uniform / Step / Slab, plus Sphere)."""
+
+    @staticmethod
+    def none():
+        return _Density(kind=0)
+
+    @staticmethod
+    def uniform(n0=1.0):
+        return _Density(kind=1, n0=n0)
+
+    @staticmethod
+    def step(dir, n0, pos):
+        return _Density(kind=2, dir=_AXIS[dir], n0=n0, pos=pos)
+
+    @staticmethod
+    def slab(dir, n0, begin, end):
+        return _Density(kind=3, dir=_AXIS[dir], n0=n0, begin=begin, end=end)
+
+    @staticmethod
+    def sphere(n0, center, radius):
+        return _Density(kind=4, n0=n0, center=(ctypes.c_float * 2)(*center), radius=radius)
+
+
+class Species:
+    """Species(name, m_q, ppc) with the reference's udist / density setters.
+    density: a Density profile (default uniform n0 = 1, or n0 if given)."""
+
+    def __init__(self, name, m_q, ppc, udist="none", uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2, n0=None, seed=0,
+                 density=None):
         L = lib()
         h = L.zpic_species_new(name.encode(), m_q, ppc[0], ppc[1])
         if not h:
@@ -191,9 +232,15 @@ class Species:
         self.sim = None
         if n0 is not None:
             _chk(L.zpic_species_set_density_uniform(h, n0))
+        if density is not None:
+            self.set_density(density)
         if seed:
             _chk(L.zpic_species_set_seed(h, seed))
         self.set_udist(udist, uth, ufl, npmin)
+
+    def set_density(self, d):
+        """Species::set_density (before add_species)."""
+        _chk(lib().zpic_species_set_density(self.h, ctypes.byref(d)))
 
     def set_udist(self, kind, uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2):
         uth = np.asarray(uth, np.float32)
@@ -271,9 +318,13 @@ class Species:
 
 
 class Simulation:
-    """Simulation(ntiles, nx, box, dt) (simulation.h:39-43) on the GPU."""
+    """Simulation(ntiles, nx, box, dt) (simulation.h:39-43) on the GPU.
 
-    def __init__(self, ntiles, nx, box, dt, filter=None, deterministic=False):
+    As the reference's Cython Simulation (em2d.pyx:1449-1550), species may
+    be given here, and moving_window=True turns the moving window on (after
+    the species are added) and makes advance() call advance_mov_window()."""
+
+    def __init__(self, ntiles, nx, box, dt, filter=None, deterministic=False, species=None, moving_window=False):
         L = lib()
         h = L.zpic_simulation_new(ntiles[0], ntiles[1], nx[0], nx[1], float(box[0]), float(box[1]), float(dt))
         if not h:
@@ -284,10 +335,33 @@ class Simulation:
         _chk(L.zpic_field_geom(h, ctypes.byref(g)))
         self.geom = g
         self.ntiles = g.ntiles_x * g.ntiles_y
+        self.mov_window = False
         if filter:
             self.set_filter(*filter)
         if deterministic:
             self.set_deterministic(True)
+        for sp in ( [species] if isinstance(species, Species) else (species or []) ):
+            self.add_species(sp)
+        if moving_window:
+            self.set_moving_window()
+
+    def set_moving_window(self):
+        """Simulation::set_moving_window (before the first step)."""
+        _chk(lib().zpic_simulation_set_moving_window(self.h))
+        self.mov_window = True
+
+    def advance_mov_window(self, n=1):
+        _chk(lib().zpic_simulation_advance_mov_window(self.h, n))
+
+    def window_moves(self):
+        n = ctypes.c_uint(0)
+        _chk(lib().zpic_simulation_window_moves(self.h, ctypes.byref(n)))
+        return int(n.value)
+
+    def add_laser(self, laser):
+        """Simulation.add_laser (em2d.pyx:1521-1536): adds a PlaneWave or
+        Gaussian pulse to E and B."""
+        laser.add(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -313,7 +387,10 @@ class Simulation:
         _chk(lib().zpic_simulation_set_deterministic(self.h, 1 if on else 0))
 
     def advance(self, n=1):
-        _chk(lib().zpic_simulation_advance(self.h, n))
+        if self.mov_window:
+            _chk(lib().zpic_simulation_advance_mov_window(self.h, n))
+        else:
+            _chk(lib().zpic_simulation_advance(self.h, n))
 
     def stage(self, what, k=0):
         _chk(lib().zpic_simulation_stage(self.h, STAGES[what], k))
@@ -396,6 +473,40 @@ class Simulation:
         out = np.zeros(5)
         _chk(lib().zpic_simulation_get_timing(self.h, _p(out)))
         return dict(zip(("push_move", "sort", "current", "yee", "step"), out.tolist()))
+
+
+class _Laser(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("start", ctypes.c_float), ("fwhm", ctypes.c_float),
+                ("rise", ctypes.c_float), ("flat", ctypes.c_float), ("fall", ctypes.c_float),
+                ("a0", ctypes.c_float), ("omega0", ctypes.c_float), ("polarization", ctypes.c_float),
+                ("cos_pol", ctypes.c_float), ("sin_pol", ctypes.c_float), ("filter", ctypes.c_uint),
+                ("W0", ctypes.c_float), ("focus", ctypes.c_float), ("axis", ctypes.c_float)]
+
+
+class PlaneWave:
+    """Laser::PlaneWave (the reference's Cython em2d.laser.PlaneWave)."""
+    kind = 0
+
+    def __init__(self, *, start=0.0, fwhm=0.0, rise=0.0, flat=0.0, fall=0.0, a0=0.0, omega0=0.0,
+                 polarization=0.0, cos_pol=0.0, sin_pol=0.0, filter=1):
+        self.p = dict(start=start, fwhm=fwhm, rise=rise, flat=flat, fall=fall, a0=a0, omega0=omega0,
+                      polarization=polarization, cos_pol=cos_pol, sin_pol=sin_pol, filter=filter)
+
+    def _struct(self):
+        return _Laser(kind=self.kind, **self.p)
+
+    def add(self, sim):
+        """Pulse::add: raises ZpicError for invalid pulse parameters."""
+        _chk(lib().zpic_laser_add(sim.h, ctypes.byref(self._struct())))
+
+
+class Gaussian(PlaneWave):
+    """Laser::Gaussian (the reference's Cython em2d.laser.Gaussian)."""
+    kind = 1
+
+    def __init__(self, *, W0=0.0, focus=0.0, axis=0.0, **kw):
+        super().__init__(**kw)
+        self.p.update(W0=W0, focus=focus, axis=axis)
 
 
 class Batch:
