@@ -136,6 +136,29 @@ int orc_add_species( void * p, float m_q, int ppcx, int ppcy, int udist, const f
     } );
 }
 
+// orc_add_species with a density profile: kind 0 none, 1 uniform, 2 step,
+// 3 slab, 4 sphere; prm = {pos or begin, end, centre x, centre y, radius}.
+int orc_add_species_profile( void * p, float m_q, int ppcx, int ppcy, int udist, const float * uth, const float * ufl,
+                             int npmin, float n0, int kind, int dir, const float * prm ) {
+    Handle * h = (Handle *) p;
+    return guard( [&] {
+        auto fill = [&]( auto & s ) {
+            s.m_q = m_q; s.ppcx = ppcx; s.ppcy = ppcy; s.n0 = std::fabs( n0 );
+            s.ud.kind = udist; s.ud.npmin = npmin;
+            for ( int i = 0; i < 3; i++ ) { s.ud.uth[i] = uth ? uth[i] : 0; s.ud.ufl[i] = ufl ? ufl[i] : 0; }
+            s.prof.kind = kind; s.prof.dir = dir;
+            s.prof.pos = s.prof.begin = prm[0]; s.prof.end = prm[1];
+            s.prof.cx = prm[2]; s.prof.cy = prm[3]; s.prof.radius = prm[4];
+        };
+        if ( m_q == 0 ) throw std::runtime_error( "Invalid m_q value, must be not 0" );
+        if ( ppcx < 1 || ppcy < 1 ) throw std::runtime_error( "Invalid ppc value, must be >= 1 in all directions" );
+        if ( kind < 0 || kind > 4 || !prm ) throw std::runtime_error( "invalid density profile" );
+        if ( h->f ) { Species<float> s; fill( s ); h->f->add_species( s ); }
+        else { Species<double> s; fill( s ); h->d->add_species( s ); }
+        return 0;
+    } );
+}
+
 int orc_advance( void * p, int nsteps ) {
     Handle * h = (Handle *) p;
     return guard( [&] {

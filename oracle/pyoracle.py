@@ -40,6 +40,7 @@ def lib():
         L.orc_destroy.argtypes = [vp]
         L.orc_set_filter.argtypes = [vp, i, i, i]
         L.orc_add_species.argtypes = [vp, f, i, i, i, vp, vp, i, f]
+        L.orc_add_species_profile.argtypes = [vp, f, i, i, i, vp, vp, i, f, i, i, vp]
         L.orc_advance.argtypes = [vp, i]
         L.orc_stage.argtypes = [vp, i, i]
         L.orc_geom.argtypes = [vp, vp]
@@ -101,10 +102,26 @@ class OracleSim:
         self._chk(lib().orc_set_filter(self.h, FILTER[kind] if isinstance(kind, str) else kind,
                                        direction, order))
 
-    def add_species(self, m_q, ppc, udist="thermal_corr", uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2, n0=1.0):
+    def add_species(self, m_q, ppc, udist="thermal_corr", uth=(0, 0, 0), ufl=(0, 0, 0), npmin=2, n0=1.0,
+                    density=None):
+        """density: None (uniform), ("none",), ("step", axis, pos), ("slab", axis, begin, end) or
+        ("sphere", (cx, cy), radius); axis "x" or "y"."""
         uth = np.asarray(uth, np.float32)
         ufl = np.asarray(ufl, np.float32)
-        self._chk(lib().orc_add_species(self.h, m_q, ppc[0], ppc[1], UDIST[udist], _p(uth), _p(ufl), npmin, n0))
+        if density is None:
+            self._chk(lib().orc_add_species(self.h, m_q, ppc[0], ppc[1], UDIST[udist], _p(uth), _p(ufl), npmin, n0))
+        else:
+            kind = {"none": 0, "uniform": 1, "step": 2, "slab": 3, "sphere": 4}[density[0]]
+            prm = np.zeros(5, np.float32)
+            d = 0
+            if kind in (2, 3):
+                d = {"x": 0, "y": 1}[density[1]]
+                prm[:len(density) - 2] = density[2:]
+            elif kind == 4:
+                prm[2:4] = density[1]
+                prm[4] = density[2]
+            self._chk(lib().orc_add_species_profile(self.h, m_q, ppc[0], ppc[1], UDIST[udist], _p(uth), _p(ufl),
+                                                    npmin, n0, kind, d, _p(prm)))
         self.nspecies += 1
 
     def advance(self, n=1):

@@ -32,7 +32,9 @@
 //   kernel3_x / kernel3_y .. grid.h:495-593, Filter:: filter.h:9-78
 //   yee_b / yeeJ_e / advance emf.cpp:67-81, 486-580
 //   get_energy ............. emf.cpp:662-713, species.h:365-368
-//   initialize / inject .... species.cpp:1730-1788, density.cpp:103-298
+//   initialize / inject .... species.cpp:1730-1788, density.cpp:103-1067
+//                            (Uniform, Step, Slab, Sphere; whole box, window
+//                            origin at 0)
 //   ThermalCorr / Thermal .. udist.cpp:42-136, zrandom random.h:23-155
 //   dep_charge_kernel ...... species.cpp:2348-2410
 #pragma once
@@ -261,6 +263,15 @@ struct UDist {
     int npmin = 2;
 };
 
+// Density profile (density.h:30-132): 0 none, 1 uniform, 2 step, 3 slab,
+// 4 sphere; dir 0 = x, 1 = y; lengths in simulation units.
+struct Profile {
+    int kind = 1;
+    int dir = 0;
+    float pos = 0, begin = 0, end = 0;
+    float cx = 0, cy = 0, radius = 0;
+};
+
 // ---------------------------------------------------------------------------
 template <class R>
 struct Particles {
@@ -288,6 +299,7 @@ struct Species {
     int id = 0;
     float n0 = 1.0f;
     UDist ud;
+    Profile prof;
     int per_x = 1, per_y = 1;
 
     // derived at initialize()
@@ -490,33 +502,79 @@ struct Sim {
         s.nleav.assign( nt, 0 );
         s.npt.assign( 9 * nt, 0 );
         s.new_np.assign( nt, 0 );
-        inject_uniform( s );
+        inject_profile( s );
         set_udist( s );
         sp.push_back( s );
     }
 
-    // density.cpp:103-186 (mk2 order) + np_inject + exclusive scan
-    void inject_uniform( Species<R> & s ) {
+    // Density::*::np_inject + exclusive scan + Density::*::inject over the
+    // whole box (density.cpp:103-1067).  Per tile, lattice point (j, i) outer,
+    // cell inner (the "mk2" order); a point is kept when the profile says so:
+    //   step  (:316-573):  t = float(cell + (pos + 0.5)) > (pos0 - 0) / dx.x
+    //   slab  (:575-854):  (begin / dx.x) <= t < (end / dx.x)
+    //   sphere (:856-1067): |float((cell + pos + 0.5) * dx) - centre|^2 < r^2
+    // the step and the slab are placed with dx.x along either axis, as the
+    // reference does; t and the sphere point are formed in double and
+    // rounded to float.
+    static bool keeps( const Profile & pr, float dxx, float dxy, int gx, int gy, float px, float py ) {
+        switch ( pr.kind ) {
+        case 1: return true;
+        case 2:
+        case 3: {
+            const float t = ( pr.dir == 0 ) ? (float) ( (double) gx + ( (double) px + 0.5 ) )
+                                            : (float) ( (double) gy + ( (double) py + 0.5 ) );
+            if ( pr.kind == 2 ) return t > ( pr.pos - 0.0f ) / dxx;
+            return t >= ( pr.begin - 0.0f ) / dxx && t < ( pr.end - 0.0f ) / dxx;
+        }
+        case 4: {
+            const float X = (float) ( ( (double) gx + ( (double) px + 0.5 ) ) * (double) dxx );
+            const float Y = (float) ( ( (double) gy + ( (double) py + 0.5 ) ) * (double) dxy );
+            const float r2 = pr.radius * pr.radius;
+            const float ax = X - pr.cx, ay = Y - pr.cy;
+            return ax * ax + ay * ay < r2;
+        }
+        default: return false;
+        }
+    }
+
+    void inject_profile( Species<R> & s ) {
         const int nt = g.ntiles();
         const int vol = g.nx * g.ny;
-        int acc = 0;
-        for ( int t = 0; t < nt; t++ ) { s.p.off[t] = acc; s.p.np[t] = 0; acc += vol * s.ppcx * s.ppcy; }
         const double fx = 1.0 / s.ppcx, fy = 1.0 / s.ppcy;
+        const float dxx = (float) s.dx.x, dxy = (float) s.dx.y;
+        // lattice positions as float (the reference's make_float2), as R for storage
+        std::vector<float> lx( s.ppcx ), ly( s.ppcy );
+        for ( int i = 0; i < s.ppcx; i++ ) lx[i] = (float) ( fx * ( i + 0.5 ) - 0.5 );
+        for ( int j = 0; j < s.ppcy; j++ ) ly[j] = (float) ( fy * ( j + 0.5 ) - 0.5 );
+        int acc = 0;
         for ( int t = 0; t < nt; t++ ) {
-            const int base = s.p.off[t];
+            const int tx = t % g.ntx, ty = t / g.ntx;
+            int n = 0;
+            for ( int j = 0; j < s.ppcy; j++ )
+                for ( int i = 0; i < s.ppcx; i++ )
+                    for ( int c = 0; c < vol; c++ )
+                        n += keeps( s.prof, dxx, dxy, tx * g.nx + c % g.nx, ty * g.ny + c / g.nx, lx[i], ly[j] );
+            s.p.off[t] = acc;
+            s.p.np[t] = n;
+            acc += n;
+        }
+        if ( (size_t) acc > s.max_part ) throw std::runtime_error( "particle buffer too small for the injection" );
+        for ( int t = 0; t < nt; t++ ) {
+            const int tx = t % g.ntx, ty = t / g.ntx;
+            int n = s.p.off[t];
             for ( int j = 0; j < s.ppcy; j++ )
                 for ( int i = 0; i < s.ppcx; i++ ) {
-                    V2<R> pos{ R( (float) ( fx * ( i + 0.5 ) - 0.5 ) ), R( (float) ( fy * ( j + 0.5 ) - 0.5 ) ) };
+                    V2<R> pos{ R( lx[i] ), R( ly[j] ) };
                     if ( sizeof( R ) == 8 ) pos = { R( fx * ( i + 0.5 ) - 0.5 ), R( fy * ( j + 0.5 ) - 0.5 ) };
-                    const int k = j * s.ppcx + i;
                     for ( int c = 0; c < vol; c++ ) {
-                        const int n = base + vol * k + c;
+                        if ( !keeps( s.prof, dxx, dxy, tx * g.nx + c % g.nx, ty * g.ny + c / g.nx, lx[i], ly[j] ) )
+                            continue;
                         s.p.ix[n] = { c % g.nx, c / g.nx };
                         s.p.x[n] = pos;
                         s.p.u[n] = { 0, 0, 0 };
+                        n++;
                     }
                 }
-            s.p.np[t] = vol * s.ppcx * s.ppcy;
         }
     }
 

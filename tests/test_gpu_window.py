@@ -118,7 +118,7 @@ def test_fast_window_close_to_reference(zp, case):
 
 
 def test_window_api_errors(zp):
-    sim = build_sim(zp, ["ntiles=2,2", "nx=8,8", "ppc=1,1", "box=0.8,0.8", "dt=0.04", "plasma=electrons",
+    sim = build_sim(zp, ["ntiles=2,2", "nx=8,8", "ppc=1,1", "box=1.6,1.6", "dt=0.04", "plasma=electrons",
                          "uth=0,0,0", "ufl=0,0,0"])
     sim.advance(1)
     with pytest.raises(zp.ZpicError):

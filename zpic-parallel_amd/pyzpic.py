@@ -401,6 +401,9 @@ class Simulation:
     def get_iter(self):
         return int(lib().zpic_simulation_get_iter(self.h))
 
+    def get_t(self):
+        return float(lib().zpic_simulation_get_t(self.h))
+
     def get_nmove(self):
         return int(lib().zpic_simulation_get_nmove(self.h))
 
