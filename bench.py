@@ -1,6 +1,6 @@
 """Benchmark of the MI355X em2d PIC timestep (push + deposit + sort + FDTD).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload weibel2048|weibel128]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload weibel2048|weibel128|batch64|lwfa]
 
 Workload (default): the 2D Weibel instability at single-GPU production size,
 2048x2048 cells (128x128 tiles of 16x16, dx = dy = 0.1, dt = 0.07, periodic),
@@ -39,6 +39,14 @@ WORKLOADS = {
     # 512}^2, ppc from {2, 4, 8}^2, drift uz from U[0.2, 0.9], uth from
     # U[0.02, 0.2], dt 0.07, seed 2024
     "batch64": dict(instances=64, seed=2024),
+    # laser wakefield with a moving window: 4096x512 cells (dx 0.05, dy 0.2,
+    # dt 0.04), cold electrons (n0 0.01, ppc 4x2) behind a linear density ramp
+    # of length 20 starting at the right edge, Gaussian pulse a0 2, omega0 10,
+    # fwhm 4, waist 8, polarised in y, compensated x filter of order 4,
+    # absorbing fields and open particle walls along y
+    "lwfa": dict(ntiles=(256, 32), nx=(16, 16), box=(204.8, 102.4), dt=0.04, ppc=(4, 2), n0=0.01,
+                 ramp=(204.8, 224.8), laser=dict(start=200.0, fwhm=4.0, a0=2.0, omega0=10.0, W0=8.0,
+                                                focus=214.8, axis=51.2, cos_pol=1.0, sin_pol=0.0)),
 }
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_push_move_deposit_weibel2048.json")
@@ -116,6 +124,8 @@ def main():
         pyzpic.set_threads_per_tile(args.threads)
     if args.workload == "batch64":
         return run_batch(args, pyzpic)
+    if args.workload == "lwfa":
+        return run_lwfa(args, pyzpic)
 
     wl = WORKLOADS[args.workload]
     print(f"[bench] building {args.workload}: ntiles {wl['ntiles']} tile {wl['nx']} ppc {wl['ppc']}",
@@ -194,6 +204,116 @@ def main():
             "bytes_per_launch": bytes_per_launch,
         },
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
+        "host": platform.node(),
+    }
+    sim.close()
+    print(json.dumps(result), flush=True)
+
+
+def lwfa_sim(pyzpic, wl):
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    sim = pyzpic.Simulation(wl["ntiles"], wl["nx"], tuple(f32(b) for b in wl["box"]), f32(wl["dt"]),
+                            filter=("compensated", 0, 4))
+    dens = pyzpic.Density.ramp("x", wl["n0"], *wl["ramp"])
+    sim.add_species(pyzpic.Species("electrons", -1.0, wl["ppc"], "cold", density=dens))
+    sim.species[0].set_bc(("periodic", "periodic"), ("open", "open"))
+    sim.set_emf_bc(("periodic", "periodic"), ("absorbing", "absorbing"))
+    sim.set_current_bc(("periodic", "periodic"), ("none", "none"))
+    sim.add_laser(pyzpic.Gaussian(**wl["laser"]))
+    sim.set_moving_window()
+    return sim
+
+
+def lwfa_cpu_baseline(wl, steps):
+    """The reference's LWFA path (advance_mov_window) on the same grid with
+    the profiles it has: a density step at the ramp's start (it has no ramp)
+    and field boundaries "none" along y (it has no absorbing boundary)."""
+    threads = max(1, min(16, os.cpu_count() or 1))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="close")
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(ref):
+        return None
+    L = wl["laser"]
+    las = "gauss:" + ",".join(str(L[k]) for k in ("start", "fwhm", "a0", "omega0", "W0", "focus", "axis",
+                                                  "cos_pol", "sin_pol"))
+    cmd = [ref, "bench", "ntiles=%d,%d" % wl["ntiles"], "nx=%d,%d" % wl["nx"], "ppc=%d,%d" % wl["ppc"],
+           "box=%g,%g" % wl["box"], "dt=%g" % wl["dt"], "plasma=electrons", "uth=0,0,0", "ufl=0,0,0",
+           "dens=step:x,%g,%g" % (wl["n0"], wl["ramp"][0]), "laser=" + las, "window=1", "filter=cx4",
+           "sbc=1,1,0,0", "ebc=1,1,0,0", "cbc=1,1,0,0", "steps=%d" % steps, "warmup=0"]
+    sample = (f"reference LWFA 4096x512 advance_mov_window, first {steps} steps, density step at x=204.8 "
+              "(the reference has no ramp), field bc none along y (no absorbing bc)")
+    try:
+        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+        r = json.loads([l for l in out.stdout.splitlines() if l.startswith("{")][-1])
+        return {"value": r["particles_per_ns"], "unit": "particles/ns", "cores": r["threads"], "kind": "reference",
+                "sample": sample + f"; {r['ms_per_step']:.2f} ms/step on {r['threads']} threads",
+                "ms_per_step": r["ms_per_step"]}
+    except Exception as e:
+        return {"value": None, "unit": "particles/ns", "cores": threads, "kind": "reference",
+                "sample": f"failed: {e}"}
+
+
+def run_lwfa(args, pyzpic):
+    """Whole LWFA run from t = 0 (the plasma enters as the window moves):
+    `steps` advance_mov_window steps timed after `warmup` untimed ones."""
+    wl = WORKLOADS["lwfa"]
+    print("[bench] building lwfa 4096x512", file=sys.stderr, flush=True)
+    sim = lwfa_sim(pyzpic, wl)
+    sim.sync()
+    ncells = wl["ntiles"][0] * wl["nx"][0] * wl["ntiles"][1] * wl["nx"][1]
+    sim.advance(args.warmup)
+    sim.sync()
+    sim.set_timing(True)
+    n0 = sim.get_nmove()
+    t0 = time.perf_counter()
+    # the first 200 steps timed separately for the comparison with the CPU sample
+    k0 = min(200, args.steps)
+    sim.advance(k0)
+    sim.sync()
+    t_first = time.perf_counter() - t0
+    if args.steps > k0:
+        sim.advance(args.steps - k0)
+    sim.sync()
+    sec = time.perf_counter() - t0
+    moved = sim.get_nmove() - n0
+    tm = sim.get_timing()
+    npart = sim.species[0].np_total()
+    ms_step = 1e3 * sec / args.steps
+    print(f"[bench] lwfa {args.steps} steps, {ms_step:.3f} ms/step, {npart} particles at the end, "
+          f"{sim.window_moves()} window moves, split {tm}", file=sys.stderr, flush=True)
+    # whole-step algorithmic bytes at the average particle count: 56 B per
+    # particle, (96 + 24 x 5 filter passes) B per cell, 2 x 24 B per cell for
+    # the window shift of E and B on the steps that move
+    avg_np = moved / args.steps
+    moves = sim.window_moves()
+    step_bytes = 56.0 * avg_np + (96.0 + 24.0 * 5) * ncells + 48.0 * ncells * moves / max(1, args.warmup + args.steps)
+    achieved = step_bytes / (ms_step * 1e-3) / 1e9
+    cpu = None if args.no_cpu_baseline else lwfa_cpu_baseline(wl, k0)
+    result = {
+        "metric": METRIC,
+        "value": moved / (sec * 1e9),
+        "unit": "particles/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "lwfa", "grid": [4096, 512], "tiles": list(wl["ntiles"]), "tile": list(wl["nx"]),
+                   "ppc": list(wl["ppc"]), "dt": wl["dt"], "dx": [0.05, 0.2], "n0": wl["n0"],
+                   "ramp": list(wl["ramp"]), "laser": wl["laser"], "filter": "compensated x order 4",
+                   "bc_y": "fields absorbing, particles open", "moving_window": True,
+                   "particles_end": npart, "window_moves": moves,
+                   "threads_per_tile": pyzpic.get_threads_per_tile(), "parallelism": "single GPU"},
+        "ms_per_step_first_%d" % k0: 1e3 * t_first / k0,
+        "stage_ms": {k: round(v, 4) for k, v in tm.items()},
+        "roofline": {"kernel": "whole step", "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "bytes_per_step": step_bytes},
+        "cpu_baseline": cpu,
         "host": platform.node(),
     }
     sim.close()

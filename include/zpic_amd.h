@@ -155,12 +155,15 @@ int zpic_tile_sort_copy(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, in
  * Sphere, density.h:30-132) as plain data, lengths relative to the window
  * origin: a step keeps the lattice points past `start` (in cells) along
  * `dir`, a slab those in [start, end), a sphere those within `radius` of the
- * centre (simulation units, cell size dx, dy). */
+ * centre (simulation units, cell size dx, dy).  A ramp (no reference
+ * counterpart) rises linearly from 0 at `start` to full density at `end`
+ * (cells along dir) by keeping a fraction of each cell's lattice points. */
 #define ZPIC_DENSITY_NONE 0
 #define ZPIC_DENSITY_UNIFORM 1
 #define ZPIC_DENSITY_STEP 2
 #define ZPIC_DENSITY_SLAB 3
 #define ZPIC_DENSITY_SPHERE 4
+#define ZPIC_DENSITY_RAMP 5   /* no reference counterpart: linear ramp      */
 typedef struct {
     int kind;                          /* ZPIC_DENSITY_*                      */
     int dir;                           /* step / slab axis: 0 = x, 1 = y      */
@@ -239,6 +242,16 @@ int zpic_phasespace(const zpic_particles *p, int ntiles_x, int ntiles_y, int nx,
  * 1 periodic: nothing to do). */
 int zpic_emf_bc(float *E, float *B, const zpic_geom *g, const int bc[4], void *stream);
 int zpic_current_bc(float *J, const zpic_geom *g, const int bc[4], void *stream);
+
+/* Absorbing field boundary (no reference counterpart; the EMF boundary
+ * value ZPIC_EMF_ABSORBING next to the reference's none 0, periodic 1, pec
+ * 2, pmc 3): within `width` cells of each absorbing wall E and B are scaled
+ * by exp(-(alpha (width - i))^2) every step, i the distance in cells from the
+ * wall (the mask of Cerjan et al., Geophysics 50 (1985) 705); guard cells
+ * included.  bc[4] = {x lower, x upper, y lower, y upper}. */
+#define ZPIC_EMF_ABSORBING 4
+int zpic_emf_absorb(float *E, float *B, const zpic_geom *g, const int bc[4], int width, float alpha,
+                    void *stream);
 
 /* vec3grid::gather (vec3grid.h:76-158): component fc (0,1,2) of the interior
  * cells into a contiguous [ny*ntiles_y][nx*ntiles_x] device array. */
@@ -366,7 +379,9 @@ int zpic_species_set_density_uniform(zpic_species *sp, float n0);
  * density.h:30-132 (the Cython classes openmp/python/em2d/density/defs.pxd
  * bind None, Uniform, Step and Slab): kind ZPIC_DENSITY_*, density n0; a
  * step starts at `pos` along dir (0 x, 1 y), a slab spans [begin, end), a
- * sphere has centre and radius.  Before add_species. */
+ * sphere has centre and radius; a ramp (ZPIC_DENSITY_RAMP, no reference
+ * counterpart) rises from 0 at `begin` to n0 at `end` along dir.  Before
+ * add_species. */
 typedef struct {
     int kind;
     int dir;
@@ -384,6 +399,13 @@ int zpic_simulation_add_species(zpic_simulation *sim, zpic_species *sp);
 
 /* Simulation::advance (simulation.h:90-107), n times. */
 int zpic_simulation_advance(zpic_simulation *sim, int n);
+
+/* Replaces the species' particles by a device injection of its density
+ * profile over the whole box (Species::np_inject + inject,
+ * species.cpp:1808-1842, through zpic_np_inject / zpic_inject), momenta 0:
+ * the particles a cold initial load places, made by the kernels of the
+ * moving window.  For checking those kernels against the initial load. */
+int zpic_species_reinject(zpic_species *sp);
 
 /* Simulation::set_moving_window (simulation.h:56-62; Cython:
  * Simulation(..., moving_window=True), em2d.pyx:1478-1480): fields and
@@ -423,7 +445,8 @@ int zpic_laser_add(zpic_simulation *sim, const zpic_laser *laser);
 /* Stages of one Simulation::advance, for stage-level parity checks:
  * 0 push(k), 1 move(k), 2 tile_sort(k), 3 Species::advance(k),
  * 4 Current::zero, 5 Current::advance, 6 EMF::advance(current),
- * 7 J add_from_gc, 8 J copy_to_gc, 9 J filter. */
+ * 7 J add_from_gc, 8 J copy_to_gc, 9 J filter, 10 EMF::process_bc,
+ * 11 EMF::advance() (no current). */
 int zpic_simulation_stage(zpic_simulation *sim, int stage, int species);
 
 /* Current::set_filter (current.h:137) with Filter::None/Binomial/Compensated. */
@@ -549,10 +572,14 @@ int zpic_simulation_add_pair(zpic_denoiser *d, zpic_simulation *in, zpic_simulat
 /* Boundary conditions (Species::set_bc species.h:210-237, EMF::set_bc
  * emf.h:112-143, Current::set_bc current.h:84-114): bc[4] = {x lower,
  * x upper, y lower, y upper}; species 0 open, 1 periodic, 2 reflecting; EMF
- * 0 none, 1 periodic, 2 pec, 3 pmc; current 0 none, 1 periodic, 2
+ * 0 none, 1 periodic, 2 pec, 3 pmc (4 absorbing: no reference
+ * counterpart, see zpic_emf_absorb); current 0 none, 1 periodic, 2
  * reflecting.  Periodic must be set on both sides of an axis. */
 int zpic_species_set_bc(zpic_species *sp, const int bc[4]);
 int zpic_emf_set_bc(zpic_simulation *sim, const int bc[4]);
+/* Width (cells, default 20) and strength (default 0.015) of the absorbing
+ * field layers (no reference counterpart). */
+int zpic_emf_set_absorbing(zpic_simulation *sim, int width, float alpha);
 int zpic_current_set_bc(zpic_simulation *sim, const int bc[4]);
 
 /* Learned-filter hooks: the simulation's current filter becomes the

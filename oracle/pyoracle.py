@@ -282,3 +282,47 @@ def phasespace(np_tile, ix, x, u, ntiles, nx, dx, q, axes):
         if ia1 and ib1:
             out[(b + 1) * s0 + a + 1] += wa * wb * norm
     return out.reshape(s1, s0)
+
+
+def ramp_stride(npc):
+    """Rank stride of the density ramp's lattice thinning: the largest
+    s <= 0.618 npc (at least 1) coprime with npc.  Design-only oracle: the
+    ramp has no reference counterpart."""
+    import math
+    s = max(1, int(np.float32(0.618034) * np.float32(npc)))
+    while s > 1 and math.gcd(s, npc) != 1:
+        s -= 1
+    return s
+
+
+def ramp_load(ntiles, nx, ppc, start, end, axis=0):
+    """Cells and in-cell positions of a whole-box load of the linear density
+    ramp (0 at `start`, full at `end`, in cells along `axis`), per tile in
+    the injection order (lattice point outer, cell inner): a lattice point k
+    of its cell is kept when (k * stride mod npc) + 1/2 < f npc, f the ramp
+    fraction at the point.  Returns np[ntiles], ix [n, 2], x [n, 2] (float32).
+    Design-only oracle (no reference counterpart)."""
+    npc = ppc[0] * ppc[1]
+    stride = ramp_stride(npc)
+    lat = [np.float32(1.0 / ppc[a] * (np.arange(ppc[a]) + 0.5) - 0.5) for a in (0, 1)]
+    counts, ixs, xs = [], [], []
+    for ty in range(ntiles[1]):
+        for tx in range(ntiles[0]):
+            n = 0
+            for i1 in range(ppc[1]):
+                for i0 in range(ppc[0]):
+                    k = i1 * ppc[0] + i0
+                    rank = (k * stride) % npc
+                    cy, cx = np.divmod(np.arange(nx[0] * nx[1]), nx[0])
+                    g = (tx * nx[0] + cx) if axis == 0 else (ty * nx[1] + cy)
+                    p = lat[axis][i0 if axis == 0 else i1]
+                    t = (g.astype(np.float64) + (np.float64(p) + 0.5)).astype(np.float32)
+                    f = (t - np.float32(start)) / (np.float32(end) - np.float32(start))
+                    keep = np.float32(rank + 0.5) < f * np.float32(npc)
+                    m = int(keep.sum())
+                    ixs.append(np.stack([cx[keep], cy[keep]], 1))
+                    xs.append(np.tile(np.array([lat[0][i0], lat[1][i1]], np.float32), (m, 1)))
+                    n += m
+            counts.append(n)
+    return (np.array(counts, np.int32), np.concatenate(ixs).astype(np.int32),
+            np.concatenate(xs).astype(np.float32))

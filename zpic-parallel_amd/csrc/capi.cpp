@@ -99,7 +99,8 @@ int zpic_species_set_density( zpic_species * h, const zpic_density * d ) {
     return guarded( [&] {
         if ( !h || !d ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
         if ( h->added ) return zpic::set_error( ZPIC_ESTATE, "density must be set before add_species" );
-        if ( ( d->kind == ZPIC_DENSITY_STEP || d->kind == ZPIC_DENSITY_SLAB ) && d->dir != 0 && d->dir != 1 )
+        if ( ( d->kind == ZPIC_DENSITY_STEP || d->kind == ZPIC_DENSITY_SLAB || d->kind == ZPIC_DENSITY_RAMP ) &&
+             d->dir != 0 && d->dir != 1 )
             return zpic::set_error( ZPIC_EINVAL, "profile direction must be 0 (x) or 1 (y)" );
         const coord::cart dir = d->dir == 1 ? coord::y : coord::x;
         switch ( d->kind ) {
@@ -110,6 +111,7 @@ int zpic_species_set_density( zpic_species * h, const zpic_density * d ) {
         case ZPIC_DENSITY_SPHERE:
             h->sp->set_density( Density::Sphere( d->n0, make_float2( d->center[0], d->center[1] ), d->radius ) );
             break;
+        case ZPIC_DENSITY_RAMP: h->sp->set_density( Density::Ramp( dir, d->n0, d->begin, d->end ) ); break;
         default: return zpic::set_error( ZPIC_EINVAL, "unknown density profile" );
         }
         return ZPIC_OK;
@@ -131,6 +133,15 @@ int zpic_simulation_advance( zpic_simulation * s, int n ) {
     return guarded( [&] {
         if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
         for ( int i = 0; i < n; i++ ) s->sim->advance();
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_species_reinject( zpic_species * h ) {
+    return guarded( [&] {
+        if ( !h ) return zpic::set_error( ZPIC_EINVAL, "null species" );
+        if ( !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not added to a simulation" );
+        h->sp->reinject();
         return ZPIC_OK;
     } );
 }
@@ -207,6 +218,8 @@ int zpic_simulation_stage( zpic_simulation * s, int stage, int k ) {
         case 7: sim.current.J->add_from_gc(); break;
         case 8: sim.current.J->copy_to_gc(); break;
         case 9: sim.current.filter->apply( *sim.current.J ); break;
+        case 10: sim.emf.process_bc(); break;
+        case 11: sim.emf.advance(); break;
         default: return zpic::set_error( ZPIC_EINVAL, "bad stage" );
         }
         return ZPIC_OK;
@@ -585,11 +598,20 @@ int zpic_species_set_bc( zpic_species * h, const int bc[4] ) {
     } );
 }
 
+int zpic_emf_set_absorbing( zpic_simulation * s, int width, float alpha ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        s->sim->emf.set_absorbing( width, alpha );
+        return ZPIC_OK;
+    } );
+}
+
 int zpic_emf_set_bc( zpic_simulation * s, const int bc[4] ) {
     return guarded( [&] {
         if ( !s || !bc ) return zpic::set_error( ZPIC_EINVAL, "null argument" );
         for ( int k = 0; k < 4; k++ )
-            if ( bc[k] < 0 || bc[k] > 3 ) return zpic::set_error( ZPIC_EINVAL, "emf bc must be 0 none, 1 periodic, 2 pec, 3 pmc" );
+            if ( bc[k] < 0 || bc[k] > ZPIC_EMF_ABSORBING )
+                return zpic::set_error( ZPIC_EINVAL, "emf bc must be 0 none, 1 periodic, 2 pec, 3 pmc, 4 absorbing" );
         emf::bc_type b;
         b.x.lower = (emf::bc::type) bc[0];
         b.x.upper = (emf::bc::type) bc[1];

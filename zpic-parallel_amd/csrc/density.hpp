@@ -44,16 +44,41 @@ __host__ __device__ inline float lattice_pos( unsigned i, unsigned n ) {
     return (float) ( d * ( i + 0.5 ) - 0.5 );
 }
 
+// Rank of lattice point k among the npc points of a cell, for the ramp's
+// thinning: k * stride mod npc with stride coprime to npc near 0.618 npc, so
+// that points of consecutive rank lie apart in the cell.
+__host__ __device__ inline int ramp_stride( int npc ) {
+    int s = (int) ( 0.618034f * npc );
+    if ( s < 1 ) s = 1;
+    for ( ; s > 1; s-- ) {
+        int a = s, b = npc;
+        while ( b ) {
+            const int r = a % b;
+            a = b;
+            b = r;
+        }
+        if ( a == 1 ) break;
+    }
+    return s;
+}
+
 // Whether the profile keeps the lattice point at in-cell position (px, py)
-// of global cell (gx, gy) (cell indices counted from the window origin).
+// of global cell (gx, gy) (cell indices counted from the window origin); k
+// is the point's index i1 * ppc.x + i0 among the npc points of its cell.
 //  step:   t = cell + (pos + 0.5) along dir, kept if t > start
 //          (inject_step_kernel, density.cpp:315-413)
 //  slab:   kept if start <= t < end (inject_slab_kernel, :575-672)
 //  sphere: kept if |((cell + pos + 0.5) * dx) - center|^2 < radius^2,
 //          in float (inject_sphere_kernel, :856-948)
+//  ramp:   no reference counterpart: density rising linearly from 0 at
+//          `start` to n0 at `end` (cells along dir), n0 beyond.  Particles
+//          keep the equal charge of the other profiles; a point is kept when
+//          its rank (ramp_stride) satisfies rank + 1/2 < f npc, f the ramp
+//          fraction at the point, so a cell holds about f npc particles.
 // t and the sphere coordinates are formed in double from the integer cell
 // and the float position and rounded to float, as in the reference.
-__host__ __device__ inline bool profile_keeps( const zpic_inject_profile & d, int gx, int gy, float px, float py ) {
+__host__ __device__ inline bool profile_keeps( const zpic_inject_profile & d, int gx, int gy, float px, float py,
+                                               int k, int npc, int stride ) {
     switch ( d.kind ) {
     case ZPIC_DENSITY_UNIFORM:
         return true;
@@ -62,6 +87,12 @@ __host__ __device__ inline bool profile_keeps( const zpic_inject_profile & d, in
         const float t = ( d.dir == 0 ) ? (float) ( gx + ( px + 0.5 ) ) : (float) ( gy + ( py + 0.5 ) );
         if ( d.kind == ZPIC_DENSITY_STEP ) return t > d.start;
         return ( t >= d.start ) && ( t < d.end );
+    }
+    case ZPIC_DENSITY_RAMP: {
+        const float t = ( d.dir == 0 ) ? (float) ( gx + ( px + 0.5 ) ) : (float) ( gy + ( py + 0.5 ) );
+        const float f = ( t - d.start ) / ( d.end - d.start );
+        const int rank = ( k * stride ) % npc;
+        return ( rank + 0.5f ) < f * npc;
     }
     case ZPIC_DENSITY_SPHERE: {
         const float x = (float) ( ( gx + ( px + 0.5 ) ) * d.dx );

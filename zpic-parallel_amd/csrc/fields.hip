@@ -361,6 +361,54 @@ __global__ void __launch_bounds__( TILE_BLOCK ) k_x_shift_left( float * buf, G g
     }
 }
 
+// Absorbing layers (no reference counterpart): E and B scaled by
+// exp(-(alpha (L - i))^2) at i cells from an absorbing wall, i < L (the mask
+// of Cerjan, Kosloff, Kosloff & Reshef, Geophysics 50 (1985) 705), every
+// element of the tile including guard cells, with factors of both axes
+// multiplied.  abs[4] = {x lower, x upper, y lower, y upper}: 1 if absorbing.
+struct AbsArgs {
+    float * E;
+    float * B;
+    G g;
+    int abs[4];
+    int width;
+    float alpha;
+};
+
+__device__ inline float layer_factor( int i, int width, float alpha ) {
+    if ( i >= width ) return 1.0f;
+    const float a = alpha * (float) ( width - i );
+    return expf( -a * a );
+}
+
+__global__ void k_emf_absorb( AbsArgs a ) {
+    const G g = a.g;
+    const int t = blockIdx.x;
+    const int tx = t % g.ntx, ty = t / g.ntx;
+    const int X = g.ntx * g.nx, Y = g.nty * g.ny;
+    // skip tiles (with their guards) entirely outside every layer
+    const int x0 = tx * g.nx - g.gl, x1 = tx * g.nx + g.nx + g.gu - 1;
+    const int y0 = ty * g.ny - g.gl, y1 = ty * g.ny + g.ny + g.gu - 1;
+    const bool hit = ( a.abs[0] && x0 < a.width ) || ( a.abs[1] && X - 1 - x1 < a.width ) ||
+                     ( a.abs[2] && y0 < a.width ) || ( a.abs[3] && Y - 1 - y1 < a.width );
+    if ( !hit ) return;
+    const gptr<float> Et = gbl( a.E ) + (size_t) t * g.vol * 3;
+    const gptr<float> Bt = gbl( a.B ) + (size_t) t * g.vol * 3;
+    for ( int e = threadIdx.x; e < g.ext_x * g.ext_y; e += blockDim.x ) {
+        const int gx = tx * g.nx + e % g.ext_x - g.gl, gy = ty * g.ny + e / g.ext_x - g.gl;
+        float f = 1.0f;
+        if ( a.abs[0] ) f *= layer_factor( gx, a.width, a.alpha );
+        if ( a.abs[1] ) f *= layer_factor( X - 1 - gx, a.width, a.alpha );
+        if ( a.abs[2] ) f *= layer_factor( gy, a.width, a.alpha );
+        if ( a.abs[3] ) f *= layer_factor( Y - 1 - gy, a.width, a.alpha );
+        if ( f == 1.0f ) continue;
+        for ( int c = 0; c < 3; c++ ) {
+            Et[3 * e + c] *= f;
+            Bt[3 * e + c] *= f;
+        }
+    }
+}
+
 // div_corr_x (laser.cpp:20-70): one thread per row of cells, tiles right to
 // left and cells right to left, the divergence accumulated in double.
 __global__ void k_div_corr_x( float * E, float * B, G g, double dx_dy ) {
@@ -982,6 +1030,23 @@ int zpic_x_shift_left( float * buf, const zpic_geom * gg, int ncomp, int shift, 
         hipLaunchKernelGGL( k_x_shift_left<1>, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, g, shift );
     ZPIC_LAUNCH_CHECK( "k_x_shift_left" );
     return gc( 2, buf, gg, ncomp, px, py, stream );
+}
+
+int zpic_emf_absorb( float * E, float * B, const zpic_geom * gg, const int bc[4], int width, float alpha,
+                     void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !E || !B || !bc ) return set_error( ZPIC_EINVAL, "null argument" );
+    if ( width < 1 || !( alpha >= 0 ) ) return set_error( ZPIC_EINVAL, "absorbing layer needs width >= 1, alpha >= 0" );
+    AbsArgs a{ E, B, make( gg ), { 0, 0, 0, 0 }, width, alpha };
+    bool any = false;
+    for ( int i = 0; i < 4; i++ ) {
+        a.abs[i] = ( bc[i] == ZPIC_EMF_ABSORBING );
+        any = any || a.abs[i];
+    }
+    if ( !any ) return ZPIC_OK;
+    hipLaunchKernelGGL( k_emf_absorb, dim3( a.g.ntx * a.g.nty ), dim3( TILE_BLOCK ), 0, (hipStream_t) stream, a );
+    ZPIC_LAUNCH_CHECK( "k_emf_absorb" );
+    return ZPIC_OK;
 }
 
 int zpic_div_corr_x( float * E, float * B, const zpic_geom * gg, float dx, float dy, void * stream ) {

@@ -244,6 +244,23 @@ zpic_inject_profile Density::Sphere::desc( float2 const dx, float2 const ref ) c
     return d;
 }
 
+// Density::Ramp (no reference counterpart): along x placed like a step
+// (window origin subtracted), along y in y cells
+zpic_inject_profile Density::Ramp::desc( float2 const dx, float2 const ref ) const {
+    zpic_inject_profile d{};
+    d.kind = ZPIC_DENSITY_RAMP;
+    d.dir = ( dir == coord::x ) ? 0 : 1;
+    if ( dir == coord::x ) {
+        d.start = ( begin - ref.x ) / dx.x;
+        d.end = ( end - ref.x ) / dx.x;
+    } else {
+        d.start = ( begin - ref.y ) / dx.y;
+        d.end = ( end - ref.y ) / dx.y;
+    }
+    d.dx = dx.x; d.dy = dx.y;
+    return d;
+}
+
 // The lattice points of the range kept by the profile, per tile, in the
 // reference's order (lattice point i1, i0, then cell in the clipped range).
 void Density::np_inject_host( zpic_inject_profile const & d, uint2 ntiles, uint2 nx, uint2 ppc, const int range[4],
@@ -252,12 +269,15 @@ void Density::np_inject_host( zpic_inject_profile const & d, uint2 ntiles, uint2
         const int tx = t % ntiles.x, ty = t / ntiles.x;
         TileRange r;
         int n = 0;
+        const int npc = ppc.x * ppc.y, stride = ramp_stride( npc );
         if ( d.kind != ZPIC_DENSITY_NONE && tile_range( range, tx, ty, nx.x, nx.y, r ) ) {
             for ( unsigned i1 = 0; i1 < ppc.y; i1++ )
                 for ( unsigned i0 = 0; i0 < ppc.x; i0++ ) {
                     const float px = lattice_pos( i0, ppc.x ), py = lattice_pos( i1, ppc.y );
+                    const int k = i1 * ppc.x + i0;
                     for ( int c = 0; c < r.vol; c++ )
-                        n += profile_keeps( d, tx * nx.x + c % r.row + r.i0, ty * nx.y + c / r.row + r.j0, px, py );
+                        n += profile_keeps( d, tx * nx.x + c % r.row + r.i0, ty * nx.y + c / r.row + r.j0, px, py, k,
+                                            npc, stride );
                 }
         }
         np[t] = n;
@@ -271,12 +291,14 @@ void Density::inject_host( zpic_inject_profile const & d, uint2 ntiles, uint2 nx
         TileRange r;
         if ( d.kind == ZPIC_DENSITY_NONE || !tile_range( range, tx, ty, nx.x, nx.y, r ) ) return;
         int n = offset[t];
+        const int npc = ppc.x * ppc.y, stride = ramp_stride( npc );
         for ( unsigned i1 = 0; i1 < ppc.y; i1++ )
             for ( unsigned i0 = 0; i0 < ppc.x; i0++ ) {
                 const float2 pos = make_float2( lattice_pos( i0, ppc.x ), lattice_pos( i1, ppc.y ) );
+                const int k = i1 * ppc.x + i0;
                 for ( int c = 0; c < r.vol; c++ ) {
                     const int2 cell = make_int2( c % r.row + r.i0, c / r.row + r.j0 );
-                    if ( profile_keeps( d, tx * nx.x + cell.x, ty * nx.y + cell.y, pos.x, pos.y ) ) {
+                    if ( profile_keeps( d, tx * nx.x + cell.x, ty * nx.y + cell.y, pos.x, pos.y, k, npc, stride ) ) {
                         ix[n] = cell;
                         x[n] = pos;
                         n++;
@@ -455,7 +477,17 @@ void EMF::process_bc() {
         const int b[4] = { bc.x.lower, bc.x.upper, bc.y.lower, bc.y.upper };
         zpic_geom g = E->geom();
         check( zpic_emf_bc( E->fbuf(), B->fbuf(), &g, b, stream() ) );
+        if ( b[0] == emf::bc::absorbing || b[1] == emf::bc::absorbing || b[2] == emf::bc::absorbing ||
+             b[3] == emf::bc::absorbing )
+            check( zpic_emf_absorb( E->fbuf(), B->fbuf(), &g, b, abs_width, abs_alpha, stream() ) );
     }
+}
+
+void EMF::set_absorbing( int width, float alpha ) {
+    if ( width < 1 || !( alpha >= 0 ) )
+        throw std::runtime_error( "(*error*) absorbing layer needs width >= 1 and alpha >= 0" );
+    abs_width = width;
+    abs_alpha = alpha;
 }
 
 // EMF::advance(Current&) (emf.cpp:513-580): dt/dx is evaluated in double and
@@ -687,6 +719,25 @@ void Species::inject( const int range[4] ) {
     const uint2 nt = particles->ntiles, nx = particles->nx;
     zpic_particles p = particles->view();
     check( zpic_inject( &prof, &p, nt.x, nt.y, nx.x, nx.y, ppc.x, ppc.y, range, stream() ) );
+}
+
+void Species::reinject() {
+    const uint2 nt = particles->ntiles, gd = particles->dims;
+    const int n = nt.x * nt.y;
+    const int range[4] = { 0, (int) gd.x - 1, 0, (int) gd.y - 1 };
+    np_inject( range, d_np_inj );
+    std::vector<int> np( n ), off( n );
+    to_host( np.data(), d_np_inj, n );
+    sync();
+    size_t total = 0;
+    for ( int t = 0; t < n; t++ ) { off[t] = (int) total; total += np[t]; }
+    if ( total > particles->max_part ) throw std::runtime_error( "(*error*) particle buffer too small for the injection" );
+    std::vector<int> zero( n, 0 );
+    to_device( particles->offset, off.data(), n );
+    to_device( particles->np, zero.data(), n );
+    inject( range );
+    sync();
+    np_bound = total;
 }
 
 int Species::set_moving_window() {

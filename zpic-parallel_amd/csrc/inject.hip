@@ -17,6 +17,7 @@ struct InjArgs {
     zpic_inject_profile d;
     int range[4];
     int ntx, nx, ny, ppcx, ppcy;
+    int stride;  // ramp_stride(ppcx * ppcy)
 };
 
 __device__ inline unsigned long long lanes_below() {
@@ -32,7 +33,7 @@ __device__ inline bool candidate( const InjArgs & a, const TileRange & r, int tx
     const int i1 = p / a.ppcx, i0 = p - i1 * a.ppcx;
     cell = make_int2( idx % r.row + r.i0, idx / r.row + r.j0 );
     pos = make_float2( lattice_pos( i0, a.ppcx ), lattice_pos( i1, a.ppcy ) );
-    return profile_keeps( a.d, tx * a.nx + cell.x, ty * a.ny + cell.y, pos.x, pos.y );
+    return profile_keeps( a.d, tx * a.nx + cell.x, ty * a.ny + cell.y, pos.x, pos.y, p, a.ppcx * a.ppcy, a.stride );
 }
 
 // Kept candidates of this workgroup's chunk: returns the number kept by the
@@ -110,8 +111,10 @@ int check_inject( const zpic_inject_profile * d, int ntx, int nty, int nx, int n
     if ( !d || !range ) return set_error( ZPIC_EINVAL, "null argument" );
     if ( ntx < 1 || nty < 1 || nx < 1 || ny < 1 ) return set_error( ZPIC_EINVAL, "invalid tiling" );
     if ( ppcx < 1 || ppcy < 1 ) return set_error( ZPIC_EINVAL, "Invalid ppc value, must be >= 1 in all directions" );
-    if ( d->kind < ZPIC_DENSITY_NONE || d->kind > ZPIC_DENSITY_SPHERE )
+    if ( d->kind < ZPIC_DENSITY_NONE || d->kind > ZPIC_DENSITY_RAMP )
         return set_error( ZPIC_EINVAL, "unknown density profile" );
+    if ( d->kind == ZPIC_DENSITY_RAMP && !( d->end > d->start ) )
+        return set_error( ZPIC_EINVAL, "ramp end must lie beyond its start" );
     if ( d->dir != 0 && d->dir != 1 ) return set_error( ZPIC_EINVAL, "profile direction must be 0 (x) or 1 (y)" );
     if ( range[0] > range[1] || range[2] > range[3] ) return set_error( ZPIC_EINVAL, "empty injection range" );
     if ( (long long) ppcx * ppcy * nx * ny > ( 1ll << 30 ) ) return set_error( ZPIC_ERANGE, "too many lattice points per tile" );
@@ -123,6 +126,7 @@ InjArgs inj_args( const zpic_inject_profile * d, int ntx, int nx, int ny, int pp
     a.d = *d;
     for ( int i = 0; i < 4; i++ ) a.range[i] = range[i];
     a.ntx = ntx; a.nx = nx; a.ny = ny; a.ppcx = ppcx; a.ppcy = ppcy;
+    a.stride = ramp_stride( ppcx * ppcy );
     return a;
 }
 

@@ -431,6 +431,19 @@ class Slab : public Profile {
     Slab * clone() const override { return new Slab( dir, n0, begin, end ); }
     zpic_inject_profile desc( float2 const dx, float2 const ref ) const override;
 };
+// No reference counterpart: density rising linearly from 0 at `begin` to n0
+// at `end` along dir, n0 beyond (equal-charge particles, thinned lattice)
+class Ramp : public Profile {
+  public:
+    const float begin, end;
+    const coord::cart dir;
+    Ramp( coord::cart dir_, float const n0_, float begin_, float end_ )
+        : Profile( n0_ ), begin( begin_ ), end( end_ ), dir( dir_ ) {
+        if ( !( end > begin ) ) throw std::runtime_error( "(*error*) Invalid ramp, end must be > begin" );
+    }
+    Ramp * clone() const override { return new Ramp( dir, n0, begin, end ); }
+    zpic_inject_profile desc( float2 const dx, float2 const ref ) const override;
+};
 class Sphere : public Profile {
   public:
     const float2 center;
@@ -459,7 +472,8 @@ typedef bnd<bc::type> bc_type;
 namespace emf {
 enum field { e = 0, b };
 namespace bc {
-enum type { none = 0, periodic, pec, pmc };
+// absorbing: no reference counterpart (damping layer, zpic_emf_absorb)
+enum type { none = 0, periodic, pec, pmc, absorbing };
 }
 typedef bnd<bc::type> bc_type;
 }  // namespace emf
@@ -527,8 +541,12 @@ class EMF {
     void advance();
     void advance( Current & current );
     void get_energy( double3 & ene_E, double3 & ene_B );
-    // EMF::process_bc (emf.cpp:431-471)
+    // EMF::process_bc (emf.cpp:431-471), then the absorbing layers
     void process_bc();
+    // Width (cells) and strength of absorbing layers (no reference counterpart)
+    void set_absorbing( int width, float alpha );
+    int abs_width = 20;
+    float abs_alpha = 0.015f;
     // EMF::set_moving_window (emf.h:152-165): 0, or -1 after the first step
     int set_moving_window();
     // EMF::move_window (emf.cpp:120-129): E and B shifted one cell left
@@ -611,6 +629,10 @@ class Species {
     // ({x lower, x upper, y lower, y upper}) on the device (species.cpp:1819-1842)
     void np_inject( const int range[4], int * d_np );
     void inject( const int range[4] );
+    // The particles replaced by a device injection of the density profile
+    // over the whole box (np_inject, exclusive scan, inject; momenta 0): the
+    // same particles a cold initial load places, by the window's kernels
+    void reinject();
 
     // Deterministic (reference single-thread) deposit / energy order
     void set_deterministic( bool on ) { deterministic = on; }

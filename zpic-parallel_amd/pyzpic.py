@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libzpic_amd.so")
 UDIST = {"none": 0, "cold": 1, "thermal": 2, "thermal_corr": 3}
 FILTER = {"none": 0, "binomial": 1, "compensated": 2}
 STAGES = {"push": 0, "move": 1, "sort": 2, "species": 3, "zero_j": 4, "current": 5, "yee": 6,
-          "add_gc": 7, "copy_gc": 8, "filter": 9}
+          "add_gc": 7, "copy_gc": 8, "filter": 9, "emf_bc": 10, "yee_vacuum": 11}
 
 
 class ZpicError(RuntimeError):
@@ -119,6 +119,8 @@ def lib():
         "zpic_simulation_advance_mov_window": (i, [vp, i]),
         "zpic_simulation_window_moves": (i, [vp, vp]),
         "zpic_laser_add": (i, [vp, vp]),
+        "zpic_emf_set_absorbing": (i, [vp, i, f]),
+        "zpic_species_reinject": (i, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -216,6 +218,11 @@ class Density:
     def sphere(n0, center, radius):
         return _Density(kind=4, n0=n0, center=(ctypes.c_float * 2)(*center), radius=radius)
 
+    @staticmethod
+    def ramp(dir, n0, begin, end):
+        """No reference counterpart: 0 at begin rising linearly to n0 at end."""
+        return _Density(kind=5, dir=_AXIS[dir], n0=n0, begin=begin, end=end)
+
 
 class Species:
     """Species(name, m_q, ppc) with the reference's udist / density setters.
@@ -300,6 +307,11 @@ class Species:
         names = {"open": 0, "periodic": 1, "reflecting": 2}
         b = np.array([names[x[0]], names[x[1]], names[y[0]], names[y[1]]], np.int32)
         _chk(lib().zpic_species_set_bc(self.h, _p(b)))
+
+    def reinject(self):
+        """Particles replaced by a device injection of the density profile
+        over the whole box (the moving window's kernels), momenta 0."""
+        _chk(lib().zpic_species_reinject(self.h))
 
     def subsample_from(self, src, stride, phase=0):
         """Every stride-th particle of each of src's tiles (same tiling)."""
@@ -440,10 +452,15 @@ class Simulation:
         _chk(lib().zpic_field_save_file(self.h, "EBJ".index(which), "xyz".index(fc), filename.encode()))
 
     def set_emf_bc(self, x=("periodic", "periodic"), y=("periodic", "periodic")):
-        """EMF::set_bc: each side "none", "periodic", "pec" or "pmc"."""
-        names = {"none": 0, "periodic": 1, "pec": 2, "pmc": 3}
+        """EMF::set_bc: each side "none", "periodic", "pec" or "pmc", or
+        "absorbing" (damping layer, no reference counterpart)."""
+        names = {"none": 0, "periodic": 1, "pec": 2, "pmc": 3, "absorbing": 4}
         b = np.array([names[x[0]], names[x[1]], names[y[0]], names[y[1]]], np.int32)
         _chk(lib().zpic_emf_set_bc(self.h, _p(b)))
+
+    def set_absorbing(self, width=20, alpha=0.015):
+        """Width (cells) and strength of the absorbing field layers."""
+        _chk(lib().zpic_emf_set_absorbing(self.h, int(width), float(alpha)))
 
     def set_current_bc(self, x=("periodic", "periodic"), y=("periodic", "periodic")):
         """Current::set_bc: each side "none", "periodic" or "reflecting"."""
