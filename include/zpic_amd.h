@@ -423,6 +423,31 @@ int zpic_simulation_advance_mov_window(zpic_simulation *sim, int n);
 /* Cells the window has moved (EMF's MovingWindow::n_move, moving_window.h). */
 int zpic_simulation_window_moves(const zpic_simulation *sim, unsigned *n_move);
 
+/* Device buffer, geometry and periodic flags (x, y) of a field of the
+ * simulation (which: 0 E, 1 B, 2 J), for the stage entry points (guard
+ * exchange, filters, shifts) on the simulation's own grids; the buffer stays
+ * owned by the simulation.  The Cython Vec3Grid view (em2d.pyx:280-500)
+ * works through it. */
+int zpic_field_device(zpic_simulation *sim, int which, float **buf, zpic_geom *g, int periodic[2]);
+
+/* Simulation geometry: tiles, cells per tile, box, time step
+ * (Simulation::ntiles / nx / box / dt, simulation.h:22-25). */
+int zpic_simulation_info(const zpic_simulation *sim, unsigned ntiles[2], unsigned nx[2], float box[2],
+                         double *dt);
+
+/* EMF::get_iter / Current::get_iter (emf.h:91, current.h:63). */
+int zpic_emf_get_iter(const zpic_simulation *sim);
+int zpic_current_get_iter(const zpic_simulation *sim);
+
+/* Simulation::energy_info (simulation.h:150-169): the energy report on stdout. */
+int zpic_simulation_energy_info(zpic_simulation *sim);
+
+/* Species::dep_phasespace (species.cpp:2579-2620, 2786-2850) into a host
+ * array: [size0] (size1 <= 0) or [size1][size0]; q 0 x, 1 y, 2 ux, 3 uy,
+ * 4 uz (Cython Species.get_phasespace, em2d.pyx:1217-1288). */
+int zpic_species_get_phasespace(zpic_species *sp, int q0, float min0, float max0, int size0, int q1, float min1,
+                                float max1, int size1, float *out);
+
 /* Laser pulses (Laser::PlaneWave / Laser::Gaussian, laser.h; Cython
  * openmp/python/em2d/laser/defs.pxd): kind 0 plane wave, 1 Gaussian beam;
  * fwhm > 0 overrides rise / flat / fall; cos_pol = sin_pol = 0 takes the

@@ -308,6 +308,69 @@ static vec3grid<float3> * field_of( zpic_simulation * s, int which ) {
     return nullptr;
 }
 
+int zpic_field_device( zpic_simulation * s, int which, float ** buf, zpic_geom * g, int periodic[2] ) {
+    return guarded( [&] {
+        vec3grid<float3> * f = s ? field_of( s, which ) : nullptr;
+        if ( !f || !buf || !g || !periodic ) return zpic::set_error( ZPIC_EINVAL, "bad field request" );
+        *buf = f->fbuf();
+        *g = f->geom();
+        const int2 p = f->get_periodic();
+        periodic[0] = p.x;
+        periodic[1] = p.y;
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_simulation_info( const zpic_simulation * s, unsigned ntiles[2], unsigned nx[2], float box[2], double * dt ) {
+    if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+    const Simulation & sim = *s->sim;
+    if ( ntiles ) { ntiles[0] = sim.ntiles.x; ntiles[1] = sim.ntiles.y; }
+    if ( nx ) { nx[0] = sim.nx.x; nx[1] = sim.nx.y; }
+    if ( box ) { box[0] = sim.box.x; box[1] = sim.box.y; }
+    if ( dt ) *dt = sim.dt;
+    return ZPIC_OK;
+}
+
+int zpic_emf_get_iter( const zpic_simulation * s ) { return s ? s->sim->emf.get_iter() : -1; }
+int zpic_current_get_iter( const zpic_simulation * s ) { return s ? s->sim->current.get_iter() : -1; }
+
+int zpic_simulation_energy_info( zpic_simulation * s ) {
+    return guarded( [&] {
+        if ( !s ) return zpic::set_error( ZPIC_EINVAL, "null simulation" );
+        s->sim->energy_info();
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_species_get_phasespace( zpic_species * h, int q0, float min0, float max0, int size0, int q1, float min1,
+                                 float max1, int size1, float * out ) {
+    return guarded( [&] {
+        if ( !h || !h->added ) return zpic::set_error( ZPIC_ESTATE, "species not initialised" );
+        if ( !out ) return zpic::set_error( ZPIC_EINVAL, "null output" );
+        if ( q0 < 0 || q0 > 4 || size0 < 1 || !( max0 > min0 ) ) return zpic::set_error( ZPIC_EINVAL, "bad axis 0" );
+        const size_t n = (size_t) size0 * ( size1 > 0 ? size1 : 1 );
+        float * d = zpic::device_alloc<float>( n );
+        try {
+            if ( size1 <= 0 ) {
+                h->sp->dep_phasespace( d, static_cast<phasespace::quant>( q0 ), make_float2( min0, max0 ), size0 );
+            } else {
+                if ( q1 < 0 || q1 > 4 || q1 == q0 || !( max1 > min1 ) )
+                    throw std::runtime_error( "Cannot generate a 2d phasespace with these axes" );
+                h->sp->dep_phasespace( d, static_cast<phasespace::quant>( q0 ), make_float2( min0, max0 ), size0,
+                                       static_cast<phasespace::quant>( q1 ), make_float2( min1, max1 ), size1 );
+            }
+            zpic::hip_check( hipMemcpyAsync( out, d, n * sizeof( float ), hipMemcpyDeviceToHost, zpic::stream() ),
+                             "phasespace copy" );
+            zpic::hip_check( hipStreamSynchronize( zpic::stream() ), "sync" );
+        } catch ( ... ) {
+            zpic::device_free( d );
+            throw;
+        }
+        zpic::device_free( d );
+        return ZPIC_OK;
+    } );
+}
+
 int zpic_field_get( zpic_simulation * s, int which, float * out ) {
     return guarded( [&] {
         vec3grid<float3> * f = s ? field_of( s, which ) : nullptr;

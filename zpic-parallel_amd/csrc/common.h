@@ -29,6 +29,13 @@ template <class T> using gptr = T *;
 #endif
 template <class T> __device__ inline gptr<T> gbl( T * p ) { return (gptr<T>) p; }
 
+// The library's stream (default priority, non-blocking), or null if no
+// device is present (launches then fail and report the error).
+hipStream_t default_stream_or_null();
+
+// The stream of a stage entry point: the caller's, or the library's for NULL.
+inline hipStream_t as_stream( void * s ) { return s ? (hipStream_t) s : default_stream_or_null(); }
+
 }  // namespace zpic
 
 #define ZPIC_HIP_TRY( call )                                               \

@@ -588,7 +588,7 @@ int zpic_denoiser_add_pair( zpic_denoiser * d, const float * J_in, const float *
     if ( d->npairs >= d->max_pairs ) return set_error( ZPIC_ERANGE, "pair store full" );
     const size_t P = dn_P( d );
     // the grids were written on the caller's stream
-    if ( stream ) ZPIC_HIP_TRY( hipStreamSynchronize( (hipStream_t) stream ) );
+    ZPIC_HIP_TRY( hipStreamSynchronize( as_stream( stream ) ) );
     float * pi = d->pairs_in + (size_t) d->npairs * P * 3;
     float * pt = d->pairs_tgt + (size_t) d->npairs * P * 3;
     hipLaunchKernelGGL( k_dn_gather, dim3( 1024 ), dim3( 256 ), 0, d->st, J_in, *g, 1.0f, (bf16 *) nullptr, pi );
@@ -697,7 +697,7 @@ int zpic_denoiser_get_pair( zpic_denoiser * d, int k, float * in, float * tgt ) 
 int zpic_denoiser_apply( zpic_denoiser * d, float * J, const zpic_geom * g, void * stream ) {
     if ( !d || !J ) return set_error( ZPIC_EINVAL, "null argument" );
     if ( int rc = dn_check_geom( d, g ) ) return rc;
-    hipStream_t caller = (hipStream_t) stream;
+    hipStream_t caller = as_stream( stream );
     hipStream_t own = d->st;
     d->st = caller;  // run on the caller's stream: ordered with the step
     hipLaunchKernelGGL( k_dn_gather, dim3( 1024 ), dim3( 256 ), 0, d->st, J, *g, 1.0f / d->scale, d->x0,
@@ -716,7 +716,7 @@ int zpic_denoiser_apply( zpic_denoiser * d, float * J, const zpic_geom * g, void
 int zpic_denoiser_reldiff( zpic_denoiser * d, const float * a, const float * b, size_t n, void * stream,
                            double * out ) {
     if ( !d || !a || !b || !out ) return set_error( ZPIC_EINVAL, "null argument" );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     hipLaunchKernelGGL( k_dn_diff, dim3( RED_BLOCKS ), dim3( 256 ), 0, st, a, b, n, d->red );
     ZPIC_LAUNCH_CHECK( "k_dn_diff" );
     std::vector<double> h( 2 * RED_BLOCKS );

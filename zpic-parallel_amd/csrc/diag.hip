@@ -159,7 +159,7 @@ extern "C" int zpic_phasespace( const zpic_particles * p, int ntiles_x, int ntil
     a.norm = norm;
     const long long bins = (long long) size0 * ( two ? size1 : 1 );
     if ( bins > ( 1ll << 30 ) ) return set_error( ZPIC_ERANGE, "phase-space grid too large" );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     ZPIC_HIP_TRY( hipMemsetAsync( out, 0, sizeof( float ) * bins, st ) );
     const int nt = ntiles_x * ntiles_y;
     if ( deterministic ) {
@@ -203,7 +203,7 @@ extern "C" int zpic_subsample( const zpic_particles * src, zpic_particles * dst,
                                void * stream ) {
     if ( !src || !dst || ntiles < 1 || stride < 1 || phase < 0 || phase >= stride )
         return set_error( ZPIC_EINVAL, "invalid subsample request" );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     std::vector<int> np( ntiles ), off( ntiles ), dnp( ntiles ), doff( ntiles );
     ZPIC_HIP_TRY( hipMemcpyAsync( np.data(), src->np, sizeof( int ) * ntiles, hipMemcpyDeviceToHost, st ) );
     ZPIC_HIP_TRY( hipMemcpyAsync( off.data(), src->offset, sizeof( int ) * ntiles, hipMemcpyDeviceToHost, st ) );

@@ -670,7 +670,7 @@ int gc_pass( int which, float * buf, const zpic_geom * gg, int px, int py, hipSt
 int gc( int which, float * buf, const zpic_geom * g, int ncomp, int px, int py, void * stream ) {
     if ( int rc = check( g ) ) return rc;
     if ( !buf ) return set_error( ZPIC_EINVAL, "null buffer" );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     if ( ncomp == 3 ) return gc_pass<3>( which, buf, g, px ? 1 : 0, py ? 1 : 0, st );
     if ( ncomp == 1 ) return gc_pass<1>( which, buf, g, px ? 1 : 0, py ? 1 : 0, st );
     return set_error( ZPIC_EINVAL, "ncomp must be 1 or 3" );
@@ -877,7 +877,7 @@ int zpic_kernel3( float * buf, const zpic_geom * gg, int dir, float a, float b, 
     const G g = make( gg );
     const size_t lds = sizeof( float ) * 3 * g.vol;
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     const K3Args args{ buf, g, dir, a, b, c };
     hipLaunchKernelGGL( k_kernel3, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, args, (const K3Args *) nullptr,
                         (const int2 *) nullptr );
@@ -911,7 +911,7 @@ int zpic_yee( float * E, float * B, const float * J, const zpic_geom * gg, float
     const G g = make( gg );
     const size_t lds = sizeof( float ) * 6 * g.vol;
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     const YeeArgs args{ E, B, J, g, dtdx, dtdy, dt };
     hipLaunchKernelGGL( k_yee, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, args, (const YeeArgs *) nullptr,
                         (const int2 *) nullptr );
@@ -931,7 +931,7 @@ static int physical_bc( float * F, float * Bf, const zpic_geom * gg, const int b
     if ( emf && ( bc[3] == 2 || bc[3] == 3 ) && gg->nx > gg->ny + 2 )
         return set_error( ZPIC_EINVAL, "upper y pec/pmc needs tile nx <= ny + 2 (reference row indexing)" );
     const G g = make( gg );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     // bc values: 0 none, 1 periodic, >1 physical
     if ( bc[0] > 1 || bc[1] > 1 ) {
         const BcArgs a{ F, Bf, g, bc[0], bc[1], g.ntx > 1 ? 1 : 2 };
@@ -967,7 +967,7 @@ int zpic_field_energy( const float * E, const float * B, const zpic_geom * gg, d
         return set_error( ZPIC_ERANGE, "field energy loop would leave the tile (ny too large for nx)" );
     const G g = make( gg );
     const int nt = g.ntx * g.nty;
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     hipLaunchKernelGGL( k_energy_tiles, dim3( ( 6 * nt + 255 ) / 256 ), dim3( 256 ), 0, st, E, B, g, scratch );
     ZPIC_LAUNCH_CHECK( "k_energy_tiles" );
     hipLaunchKernelGGL( k_sum_tiles, dim3( 1 ), dim3( SUM_BLOCK ), 0, st, scratch, nt, 6, out6 );
@@ -978,9 +978,9 @@ int zpic_field_energy( const float * E, const float * B, const zpic_geom * gg, d
 int zpic_sum_tiles( const double * v, int n, double * out, int deterministic, void * stream ) {
     if ( !v || !out || n < 0 ) return set_error( ZPIC_EINVAL, "invalid arguments" );
     if ( deterministic )
-        hipLaunchKernelGGL( k_sum_tiles, dim3( 1 ), dim3( SUM_BLOCK ), 0, (hipStream_t) stream, v, n, 1, out );
+        hipLaunchKernelGGL( k_sum_tiles, dim3( 1 ), dim3( SUM_BLOCK ), 0, as_stream( stream ), v, n, 1, out );
     else
-        hipLaunchKernelGGL( k_sum_tiles_fast, dim3( 1 ), dim3( SUM_BLOCK ), 0, (hipStream_t) stream, v, n, out );
+        hipLaunchKernelGGL( k_sum_tiles_fast, dim3( 1 ), dim3( SUM_BLOCK ), 0, as_stream( stream ), v, n, out );
     ZPIC_LAUNCH_CHECK( "k_sum_tiles" );
     return ZPIC_OK;
 }
@@ -989,7 +989,7 @@ int zpic_gather_component( const float * buf, const zpic_geom * gg, int ncomp, i
     if ( int rc = check( gg ) ) return rc;
     if ( !buf || !out || fc < 0 || fc >= ncomp ) return set_error( ZPIC_EINVAL, "invalid gather arguments" );
     const G g = make( gg );
-    hipLaunchKernelGGL( k_gather, dim3( 1024 ), dim3( 256 ), 0, (hipStream_t) stream, buf, g, ncomp, fc, out );
+    hipLaunchKernelGGL( k_gather, dim3( 1024 ), dim3( 256 ), 0, as_stream( stream ), buf, g, ncomp, fc, out );
     ZPIC_LAUNCH_CHECK( "k_gather" );
     return ZPIC_OK;
 }
@@ -999,7 +999,7 @@ int zpic_fill( float * buf, size_t n, float v, void * stream ) {
     if ( n == 0 ) return ZPIC_OK;
     size_t blocks = ( n + 255 ) / 256;
     if ( blocks > 8192 ) blocks = 8192;
-    hipLaunchKernelGGL( k_fill, dim3( blocks ), dim3( 256 ), 0, (hipStream_t) stream, buf, n, v );
+    hipLaunchKernelGGL( k_fill, dim3( blocks ), dim3( 256 ), 0, as_stream( stream ), buf, n, v );
     ZPIC_LAUNCH_CHECK( "k_fill" );
     return ZPIC_OK;
 }
@@ -1009,7 +1009,7 @@ int zpic_grid_add( float * dst, const float * src, size_t n, void * stream ) {
     if ( n == 0 ) return ZPIC_OK;
     size_t blocks = ( n + 255 ) / 256;
     if ( blocks > 8192 ) blocks = 8192;
-    hipLaunchKernelGGL( k_grid_add, dim3( blocks ), dim3( 256 ), 0, (hipStream_t) stream, dst, src, n );
+    hipLaunchKernelGGL( k_grid_add, dim3( blocks ), dim3( 256 ), 0, as_stream( stream ), dst, src, n );
     ZPIC_LAUNCH_CHECK( "k_grid_add" );
     return ZPIC_OK;
 }
@@ -1023,7 +1023,7 @@ int zpic_x_shift_left( float * buf, const zpic_geom * gg, int ncomp, int shift, 
     const G g = make( gg );
     const size_t lds = sizeof( float ) * ncomp * g.ext_x * g.ext_y;
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     if ( ncomp == 3 )
         hipLaunchKernelGGL( k_x_shift_left<3>, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, g, shift );
     else
@@ -1044,7 +1044,7 @@ int zpic_emf_absorb( float * E, float * B, const zpic_geom * gg, const int bc[4]
         any = any || a.abs[i];
     }
     if ( !any ) return ZPIC_OK;
-    hipLaunchKernelGGL( k_emf_absorb, dim3( a.g.ntx * a.g.nty ), dim3( TILE_BLOCK ), 0, (hipStream_t) stream, a );
+    hipLaunchKernelGGL( k_emf_absorb, dim3( a.g.ntx * a.g.nty ), dim3( TILE_BLOCK ), 0, as_stream( stream ), a );
     ZPIC_LAUNCH_CHECK( "k_emf_absorb" );
     return ZPIC_OK;
 }
@@ -1057,7 +1057,7 @@ int zpic_div_corr_x( float * E, float * B, const zpic_geom * gg, float dx, float
     const G g = make( gg );
     const int rows = g.nty * g.ny;
     const double dx_dy = ( (double) dx ) / ( (double) dy );
-    hipLaunchKernelGGL( k_div_corr_x, dim3( ( rows + 63 ) / 64 ), dim3( 64 ), 0, (hipStream_t) stream, E, B, g,
+    hipLaunchKernelGGL( k_div_corr_x, dim3( ( rows + 63 ) / 64 ), dim3( 64 ), 0, as_stream( stream ), E, B, g,
                         dx_dy );
     ZPIC_LAUNCH_CHECK( "k_div_corr_x" );
     return ZPIC_OK;
@@ -1079,7 +1079,7 @@ int zpic_batch_zero_current( zpic_batch_fields * b, void * stream ) {
     if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch" );
     size_t blocks = ( b->plan->max_fill + 255 ) / 256;
     if ( blocks > 1024 ) blocks = 1024;
-    hipLaunchKernelGGL( k_fill_batch, dim3( blocks, b->plan->n ), dim3( 256 ), 0, (hipStream_t) stream,
+    hipLaunchKernelGGL( k_fill_batch, dim3( blocks, b->plan->n ), dim3( 256 ), 0, as_stream( stream ),
                         (const FillArgs *) b->plan->d_fill );
     ZPIC_LAUNCH_CHECK( "k_fill_batch" );
     return ZPIC_OK;
@@ -1088,14 +1088,14 @@ int zpic_batch_zero_current( zpic_batch_fields * b, void * stream ) {
 int zpic_batch_current_advance( zpic_batch_fields * b, void * stream ) {
     if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch" );
     for ( const Group & g : b->plan->current )
-        if ( int rc = run_group( g, (hipStream_t) stream ) ) return rc;
+        if ( int rc = run_group( g, as_stream( stream ) ) ) return rc;
     return ZPIC_OK;
 }
 
 int zpic_batch_yee( zpic_batch_fields * b, void * stream ) {
     if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch" );
     for ( const Group & g : b->plan->yee )
-        if ( int rc = run_group( g, (hipStream_t) stream ) ) return rc;
+        if ( int rc = run_group( g, as_stream( stream ) ) ) return rc;
     return ZPIC_OK;
 }
 

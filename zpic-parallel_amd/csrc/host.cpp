@@ -30,6 +30,14 @@ hipStream_t stream() {
     return s;
 }
 
+hipStream_t default_stream_or_null() {
+    try {
+        return stream();
+    } catch ( ... ) {
+        return nullptr;
+    }
+}
+
 void check( int rc ) {
     if ( rc != ZPIC_OK ) throw std::runtime_error( zpic_last_error() );
 }

@@ -142,7 +142,7 @@ int zpic_np_inject( const zpic_inject_profile * d, int ntx, int nty, int nx, int
     if ( int rc = check_inject( d, ntx, nty, nx, ny, ppcx, ppcy, range ) ) return rc;
     if ( !np ) return set_error( ZPIC_EINVAL, "null np buffer" );
     const InjArgs a = inj_args( d, ntx, nx, ny, ppcx, ppcy, range );
-    hipLaunchKernelGGL( k_np_inject, dim3( ntx * nty ), dim3( INJ_BLOCK ), 0, (hipStream_t) stream, a, np );
+    hipLaunchKernelGGL( k_np_inject, dim3( ntx * nty ), dim3( INJ_BLOCK ), 0, as_stream( stream ), a, np );
     ZPIC_LAUNCH_CHECK( "k_np_inject" );
     return ZPIC_OK;
 }
@@ -153,7 +153,7 @@ int zpic_inject( const zpic_inject_profile * d, const zpic_particles * p, int nt
     if ( !p || !p->np || !p->offset || !p->ix || !p->x || !p->u ) return set_error( ZPIC_EINVAL, "null particle buffer" );
     if ( d->kind == ZPIC_DENSITY_NONE ) return ZPIC_OK;
     const InjArgs a = inj_args( d, ntx, nx, ny, ppcx, ppcy, range );
-    hipLaunchKernelGGL( k_inject, dim3( ntx * nty ), dim3( INJ_BLOCK ), 0, (hipStream_t) stream, a, p->np, p->offset,
+    hipLaunchKernelGGL( k_inject, dim3( ntx * nty ), dim3( INJ_BLOCK ), 0, as_stream( stream ), a, p->np, p->offset,
                         (int2 *) p->ix, (float2 *) p->x, (float3 *) p->u, (unsigned) p->max_part );
     ZPIC_LAUNCH_CHECK( "k_inject" );
     return ZPIC_OK;

@@ -1139,7 +1139,7 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
     a.P = view( p ); a.g = make_geo( fld );
     a.E = (const float3 *) E; a.B = (const float3 *) B; a.J = (float3 *) J;
     a.alpha = alpha; a.m = m; a.tile_energy = tile_energy; a.S = view( s );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     if ( det )
         hipLaunchKernelGGL( ( k_particles<PUSH, MOVE, true> ), dim3( nt ), dim3( threads ), lds, st, a,
                             (const PartArgs *) nullptr, (const int2 *) nullptr );
@@ -1263,7 +1263,7 @@ __global__ void __launch_bounds__( 256 ) k_batch_energy( const EnergyArgs * tab 
 }
 
 int batch_particles_step( BatchParticlePlan * plan, void * stream ) {
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     const size_t lds = particles_lds_bytes( plan->max_vol, plan->threads / WAVE, true, true );
     PartArgs dummy{};
     hipLaunchKernelGGL( ( k_particles<true, true, false> ), dim3( plan->nblocks ), dim3( plan->threads ), lds, st,
@@ -1339,7 +1339,7 @@ int zpic_tile_sort_count( zpic_particles * p, zpic_particles * tmp, zpic_sort * 
     SortArgs a = sort_args( p, tmp, s, ntiles_x, ntiles_y, nx, ny, periodic_x, periodic_y );
     a.extra = extra;
     const SortArgs * tab = nullptr;
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     hipLaunchKernelGGL( k_sort_newnp, dim3( ( nt + 255 ) / 256 ), dim3( 256 ), 0, st, a, tab );
     ZPIC_LAUNCH_CHECK( "k_sort_newnp" );
     hipLaunchKernelGGL( k_sort_count, dim3( 1 ), dim3( SCAN_BLOCK ), 0, st, a, tab );
@@ -1356,7 +1356,7 @@ int zpic_tile_sort_copy( zpic_particles * p, zpic_particles * tmp, zpic_sort * s
     const SortArgs a = sort_args( p, tmp, s, ntiles_x, ntiles_y, nx, ny, periodic_x, periodic_y );
     const SortArgs * tab = nullptr;
     const int2 * map = nullptr;
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     if ( mode == 1 ) {
         hipLaunchKernelGGL( k_sort_out<true>, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
         ZPIC_LAUNCH_CHECK( "k_sort_out (copy_sorted)" );
@@ -1420,7 +1420,7 @@ int zpic_deposit_charge( const zpic_particles * p, const zpic_geom * g, float * 
     const int nt = g->ntiles_x * g->ntiles_y;
     const size_t lds = sizeof( float ) * g->tile_vol;
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
-    hipStream_t st = (hipStream_t) stream;
+    hipStream_t st = as_stream( stream );
     if ( deterministic )
         hipLaunchKernelGGL( k_charge<true>, dim3( nt ), dim3( WAVE ), lds, st, view( p ), make_geo( g ), rho, q );
     else
