@@ -187,15 +187,35 @@ int zpic_inject(const zpic_inject_profile *d, const zpic_particles *p, int ntile
                 int ny, int ppc_x, int ppc_y, const int range[4], void *stream);
 
 /* grid<T>::add_from_gc / copy_to_gc (grid.h:253-445) on a tile-major grid
- * of `ncomp` floats per element (3 for vec3grid<float3>, 1 for grid<float>). */
+ * of `ncomp` floats per element (3 for vec3grid<float3>, 1 for grid<float>).
+ * copy_to_gc (the x pass then the y pass) runs as one launch. */
 int zpic_add_from_gc(float *buf, const zpic_geom *g, int ncomp, int periodic_x, int periodic_y, void *stream);
 int zpic_copy_to_gc(float *buf, const zpic_geom *g, int ncomp, int periodic_x, int periodic_y, void *stream);
+
+/* copy_to_gc of two grids of the same geometry (E and B after the Yee
+ * update, emf.cpp:569-570) in one launch. */
+int zpic_copy_to_gc2(float *buf0, float *buf1, const zpic_geom *g, int ncomp, int periodic_x, int periodic_y,
+                     void *stream);
+
+/* add_from_gc followed by copy_to_gc (Current::advance, current.cpp:230-231):
+ * the two add passes, then the copy in one launch. */
+int zpic_add_copy_gc(float *buf, const zpic_geom *g, int ncomp, int periodic_x, int periodic_y, void *stream);
 
 /* grid<T>::kernel3_x / kernel3_y (grid.h:495-593) on vec3 grids: one pass of
  * the 3-point convolution [a,b,c] along dir (0 = x, 1 = y), then the guard
  * copy along that direction. */
 int zpic_kernel3(float *buf, const zpic_geom *g, int dir, float a, float b, float c,
                  int periodic_x, int periodic_y, void *stream);
+
+/* `passes` kernel3_x passes (coef[3p..3p+2] = a, b, c of pass p), each
+ * followed by the x guard copy: Filter::Binomial / Compensated along x
+ * (filter.h:22-78).  Two to 16 passes on tiles at least gc + passes wide run
+ * as one tile-parallel launch into a library-held scratch grid (one per
+ * device and stream, allocated on first use) and a device copy back, with
+ * results identical to the pass-by-pass sequence (every element computed as
+ * A[i-1]*a + A[i]*b + A[i+1]*c); otherwise pass by pass. */
+int zpic_filter_x(float *buf, const zpic_geom *g, int passes, const float *coef, int periodic_x, int periodic_y,
+                  void *stream);
 
 /* Current::advance (current.cpp:227-246) for periodic boundaries: guard add,
  * guard copy and the digital filter (filter.h:9-78).  filter_kind 0 none,
@@ -269,6 +289,11 @@ int zpic_grid_add(float *dst, const float *src, size_t n, void *stream);
  * right, then the x guard copy.  Requires 0 < shift < gc_upper. */
 int zpic_x_shift_left(float *buf, const zpic_geom *g, int ncomp, int shift, int periodic_x, int periodic_y,
                       void *stream);
+
+/* zpic_x_shift_left of one or two float3 grids of the same geometry (buf1
+ * may be NULL) in one launch, with the same results: EMF::move_window
+ * (emf.cpp:120-129) shifts E and B together. */
+int zpic_x_shift_left2(float *buf0, float *buf1, const zpic_geom *g, int shift, int periodic_x, void *stream);
 
 /* div_corr_x (laser.cpp:20-70): Ex and Bx of the interior cells set so that
  * the discrete divergence of E and B vanishes, accumulated in double from

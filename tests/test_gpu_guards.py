@@ -1,0 +1,190 @@
+"""Guard-cell exchange on the GPU against the reference's two passes.
+
+grid<T>::add_from_gc and copy_to_gc (grid.h:253-445) each run an x pass over
+every tile, then a y pass.  The library runs copy_to_gc as one launch
+(zpic_copy_to_gc, and zpic_copy_to_gc2 for two grids), and add_from_gc
+followed by copy_to_gc as two (zpic_add_copy_gc, used by Current::advance);
+zpic_add_from_gc keeps the two passes.  Each is compared here, bit for bit
+and over the whole buffer (guards and tile padding included), with the two
+passes restated in numpy, on random data, over tile counts of 1 (a tile its
+own neighbour when periodic) to several, tiles as narrow as the guard bands,
+both guard widths and every periodicity.
+"""
+import ctypes
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-guards] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def nbr(tx, ty, dx, dy, ntx, nty, px, py):
+    x, y = tx + dx, ty + dy
+    if px:
+        x %= ntx
+    if py:
+        y %= nty
+    if x < 0 or x >= ntx or y < 0 or y >= nty:
+        return -1
+    return y * ntx + x
+
+
+def add_from_gc(A, g):
+    """grid.h:367-444: x pass (lower neighbour's upper guards into the first
+    gu interior columns, then the upper neighbour's lower guards into the
+    last gl), then the y pass on full rows; every pass reads the state the
+    previous pass left."""
+    ntx, nty, nx, ny, gl, gu, px, py = g
+    for axis in (0, 1):
+        S = A.copy()
+        for t in range(ntx * nty):
+            tx, ty = t % ntx, t // ntx
+            if axis == 0:
+                lo, hi = nbr(tx, ty, -1, 0, ntx, nty, px, py), nbr(tx, ty, 1, 0, ntx, nty, px, py)
+                if lo >= 0:
+                    A[t, :, gl:gl + gu] += S[lo, :, gl + nx:gl + nx + gu]
+                if hi >= 0:
+                    A[t, :, nx:nx + gl] += S[hi, :, 0:gl]
+            else:
+                lo, hi = nbr(tx, ty, 0, -1, ntx, nty, px, py), nbr(tx, ty, 0, 1, ntx, nty, px, py)
+                if lo >= 0:
+                    A[t, gl:gl + gu] += S[lo, gl + ny:gl + ny + gu]
+                if hi >= 0:
+                    A[t, ny:ny + gl] += S[hi, 0:gl]
+    return A
+
+
+def copy_to_gc(A, g):
+    """grid.h:253-340: x guard columns from the x neighbours' edge columns
+    (all rows), then y guard rows (full width) from the y neighbours."""
+    ntx, nty, nx, ny, gl, gu, px, py = g
+    for axis in (0, 1):
+        S = A.copy()
+        for t in range(ntx * nty):
+            tx, ty = t % ntx, t // ntx
+            if axis == 0:
+                lo, hi = nbr(tx, ty, -1, 0, ntx, nty, px, py), nbr(tx, ty, 1, 0, ntx, nty, px, py)
+                if lo >= 0:
+                    A[t, :, 0:gl] = S[lo, :, nx:nx + gl]
+                if hi >= 0:
+                    A[t, :, gl + nx:gl + nx + gu] = S[hi, :, gl:gl + gu]
+            else:
+                lo, hi = nbr(tx, ty, 0, -1, ntx, nty, px, py), nbr(tx, ty, 0, 1, ntx, nty, px, py)
+                if lo >= 0:
+                    A[t, 0:gl] = S[lo, ny:ny + gl]
+                if hi >= 0:
+                    A[t, gl + ny:gl + ny + gu] = S[hi, gl:gl + gu]
+    return A
+
+
+GEOMS = [  # ntx, nty, nx, ny, gl, gu
+    (1, 1, 16, 16, 1, 2), (3, 1, 16, 16, 1, 2), (1, 3, 16, 16, 1, 2), (5, 4, 16, 16, 1, 2),
+    (4, 3, 4, 6, 1, 2), (3, 4, 2, 3, 1, 2), (2, 2, 3, 2, 2, 2), (3, 2, 5, 4, 1, 1), (2, 3, 4, 4, 0, 1),
+]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+    import pyzpic
+    L = pyzpic.lib()
+    for name in ("zpic_copy_to_gc", "zpic_add_from_gc", "zpic_add_copy_gc"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.zpic_copy_to_gc2.restype = ctypes.c_int
+    L.zpic_copy_to_gc2.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_void_p]
+    return L, pyzpic, torch
+
+
+def _setup(lib, geom, ncomp, seed):
+    L, zp, torch = lib
+    ntx, nty, nx, ny, gl, gu = geom
+    g = zp.Geom()
+    assert L.zpic_geom_init(ctypes.byref(g), ntx, nty, nx, ny, gl, gu) == 0
+    rng = np.random.default_rng(seed)
+    flat = rng.standard_normal(ntx * nty * g.tile_vol * ncomp).astype(np.float32)
+    dev = torch.from_numpy(flat.copy()).to("cuda")
+    torch.cuda.synchronize()
+    return g, flat, dev
+
+
+def _grid_view(flat, g, ncomp):
+    """(tile, row, column, component) view of the interior-and-guard
+    elements of a tile-major buffer (the tile padding stays outside)."""
+    nt = g.ntiles_x * g.ntiles_y
+    E, Ey = g.ext_x, g.ext_y
+    return flat.reshape(nt, g.tile_vol, ncomp)[:, :E * Ey].reshape(nt, Ey, E, ncomp)
+
+
+def _run(lib, fn, *bufs_and_args):
+    L, zp, torch = lib
+    st = torch.cuda.current_stream().cuda_stream
+    rc = fn(*bufs_and_args, ctypes.c_void_p(st))
+    torch.cuda.synchronize()
+    assert rc == 0, L.zpic_last_error()
+
+
+def _ndiff(got, want):
+    return int(np.sum(got.view(np.uint32) != want.view(np.uint32)))
+
+
+CASES = [(geom, per, nc) for geom in GEOMS for per in itertools.product((0, 1), (0, 1)) for nc in (3, 1)]
+
+
+def _id(c):
+    (ntx, nty, nx, ny, gl, gu), (px, py), nc = c
+    return f"{ntx}x{nty}t-{nx}x{ny}-gc{gl}{gu}-p{px}{py}-c{nc}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=_id)
+def test_guard_exchange_matches_two_passes(lib, case):
+    geom, (px, py), nc = case
+    L = lib[0]
+    say(f"case {_id(case)}")
+    gfull = (*geom, px, py)
+    # copy_to_gc in one launch
+    g, flat, dev = _setup(lib, geom, nc, 1)
+    want = flat.copy()
+    copy_to_gc(_grid_view(want, g, nc), gfull)
+    _run(lib, L.zpic_copy_to_gc, ctypes.c_void_p(dev.data_ptr()), ctypes.byref(g), nc, px, py)
+    assert _ndiff(dev.cpu().numpy(), want) == 0
+    # add_from_gc, two passes, guard sums kept
+    g, flat, dev = _setup(lib, geom, nc, 2)
+    want = flat.copy()
+    add_from_gc(_grid_view(want, g, nc), gfull)
+    _run(lib, L.zpic_add_from_gc, ctypes.c_void_p(dev.data_ptr()), ctypes.byref(g), nc, px, py)
+    assert _ndiff(dev.cpu().numpy(), want) == 0
+    # add_from_gc + copy_to_gc in two launches
+    g, flat, dev = _setup(lib, geom, nc, 3)
+    want = flat.copy()
+    v = _grid_view(want, g, nc)
+    copy_to_gc(add_from_gc(v, gfull), gfull)
+    _run(lib, L.zpic_add_copy_gc, ctypes.c_void_p(dev.data_ptr()), ctypes.byref(g), nc, px, py)
+    assert _ndiff(dev.cpu().numpy(), want) == 0
+
+
+@pytest.mark.parametrize("per", [(1, 1), (0, 1), (1, 0), (0, 0)])
+def test_copy_to_gc2_matches_two_copies(lib, per):
+    """Two grids in one launch equal each grid's own copy_to_gc."""
+    L = lib[0]
+    px, py = per
+    geom = (4, 3, 16, 16, 1, 2)
+    g, f0, d0 = _setup(lib, geom, 3, 4)
+    _, f1, d1 = _setup(lib, geom, 3, 5)
+    w0, w1 = f0.copy(), f1.copy()
+    copy_to_gc(_grid_view(w0, g, 3), (*geom, px, py))
+    copy_to_gc(_grid_view(w1, g, 3), (*geom, px, py))
+    _run(lib, L.zpic_copy_to_gc2, ctypes.c_void_p(d0.data_ptr()), ctypes.c_void_p(d1.data_ptr()), ctypes.byref(g),
+         3, px, py)
+    assert _ndiff(d0.cpu().numpy(), w0) == 0
+    assert _ndiff(d1.cpu().numpy(), w1) == 0
+    say(f"copy_to_gc2 periodic {per}: identical")

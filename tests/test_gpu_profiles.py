@@ -210,3 +210,97 @@ def test_lwfa_with_ramp_and_absorbing_layers(zp):
     # window moved m_d cells into the ramp; counts within 1 % between modes
     assert abs(n_f - n_d) <= 0.01 * n_d
     assert abs(e_f.sum() - e_d.sum()) <= 1e-3 * abs(e_d.sum())
+
+
+def _pass_coefs(filt):
+    """(a, b, c) of every kernel3_x pass of Filter::Binomial / Compensated
+    (filter.h:22-78), with the compensator's float arithmetic."""
+    kind, _, order = filt
+    c = [(np.float32(0.25), np.float32(0.5), np.float32(0.25))] * order
+    if kind == "compensated":
+        b = np.float32((4.0 + 2.0 * order) / order)
+        norm = np.float32(np.float32(-2.0) + b)
+        c.append((np.float32(-1.0) / norm, b / norm, np.float32(-1.0) / norm))
+    return c
+
+
+@pytest.mark.parametrize("per_x,filt", [(True, ("compensated", 0, 3)), (False, ("compensated", 0, 4)),
+                                        (False, ("binomial", 0, 2)), (True, ("binomial", 0, 1))])
+def test_fused_row_filter_matches_pass_by_pass(zp, per_x, filt):
+    """The x filter in one launch (zpic_filter_x: all passes and the final
+    guard copy) against the pass-by-pass path (zpic_kernel3: one kernel3_x
+    and x guard copy per pass), on random J with periodic or open x
+    boundaries: bit-identical, guard cells included."""
+    import ctypes
+    L = zp.lib()
+    L.zpic_field_device.restype = ctypes.c_int
+    L.zpic_field_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.zpic_kernel3.restype = ctypes.c_int
+    L.zpic_kernel3.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    rng = np.random.default_rng(11)
+    fused = zp.Simulation((5, 3), (8, 12), (4.0, 3.6), 0.05, filter=filt)
+    ref = zp.Simulation((5, 3), (8, 12), (4.0, 3.6), 0.05)
+    if not per_x:
+        for s in (fused, ref):
+            s.set_current_bc(("none", "none"), ("periodic", "periodic"))
+    J = rng.standard_normal(fused.field("J").size).astype(np.float32)
+    fused.set_field("J", J)
+    ref.set_field("J", J)
+    fused.stage("filter")
+    buf = ctypes.c_void_p()
+    geom = zp.Geom()
+    per = (ctypes.c_int * 2)()
+    assert L.zpic_field_device(ref.h, 2, ctypes.byref(buf), ctypes.byref(geom), per) == 0
+    assert per[0] == (1 if per_x else 0)
+    for ca, cb, cc in _pass_coefs(filt):
+        assert L.zpic_kernel3(buf, ctypes.byref(geom), 0, ca, cb, cc, per[0], per[1], None) == 0
+    got, want = fused.field("J"), ref.field("J")
+    ndiff = int(np.sum(got.view(np.uint32) != want.view(np.uint32)))
+    say(f"fused x filter {filt} periodic={per_x}: {ndiff} differing words")
+    assert ndiff == 0
+    fused.close()
+    ref.close()
+
+
+@pytest.mark.parametrize("per_x", [True, False])
+def test_fused_window_shift_matches_per_tile_shift(zp, per_x):
+    """E and B shifted together in one launch (zpic_x_shift_left2) against
+    the per-tile shift and x guard copy of each (zpic_x_shift_left), on
+    random fields with periodic or open x: bit-identical, guards included."""
+    import ctypes
+    L = zp.lib()
+    L.zpic_field_device.restype = ctypes.c_int
+    L.zpic_field_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.zpic_x_shift_left.restype = ctypes.c_int
+    L.zpic_x_shift_left.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_void_p]
+    L.zpic_x_shift_left2.restype = ctypes.c_int
+    L.zpic_x_shift_left2.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p]
+    rng = np.random.default_rng(5)
+    sims = [zp.Simulation((6, 2), (8, 12), (4.8, 2.4), 0.05) for _ in range(2)]
+    fields = {f: rng.standard_normal(sims[0].field(f).size).astype(np.float32) for f in "EB"}
+    dev = []
+    for s in sims:
+        if not per_x:
+            s.set_emf_bc(("none", "none"), ("periodic", "periodic"))
+        for f in "EB":
+            s.set_field(f, fields[f])
+        bufs = []
+        for which in (0, 1):
+            buf = ctypes.c_void_p()
+            geom = zp.Geom()
+            per = (ctypes.c_int * 2)()
+            assert L.zpic_field_device(s.h, which, ctypes.byref(buf), ctypes.byref(geom), per) == 0
+            bufs.append((buf, geom, per))
+        dev.append(bufs)
+    (e0, g0, p0), (b0, _, _) = dev[0]
+    assert L.zpic_x_shift_left2(e0, b0, ctypes.byref(g0), 1, p0[0], None) == 0
+    for buf, geom, per in dev[1]:
+        assert L.zpic_x_shift_left(buf, ctypes.byref(geom), 3, 1, per[0], per[1], None) == 0
+    for f in "EB":
+        a, b = sims[0].field(f), sims[1].field(f)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f
+    for s in sims:
+        s.close()

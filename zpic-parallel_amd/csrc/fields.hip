@@ -7,6 +7,8 @@
 #include "common.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <memory>
 #include <vector>
 
@@ -187,6 +189,68 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_y( GcArgs a, const GcArg
     }
 }
 
+// Where element (i, j) of tile t takes its value from in copy_to_gc_x
+// followed by copy_to_gc_y (grid.h:253-340): the y pass copies the y
+// neighbour's row (as its own x pass left it), the x pass the x neighbour's
+// column, so a guard element resolves first along y (tile row, row index),
+// then along x within that tile row.  Returns false when the element keeps
+// its own value (interior, or a guard at a non-periodic edge).  Every source
+// it names is an interior element or a guard element that keeps its value,
+// so one launch may copy all guards of all tiles at once.
+__device__ inline bool copy_source( const G & g, int px, int py, int t, int i, int j, int & st, int & si, int & sj ) {
+    int tx = t % g.ntx, ty = t / g.ntx;
+    si = i;
+    sj = j;
+    if ( j < g.gl ) {
+        const int n = nbr( tx, ty, 0, -1, g, px, py );
+        if ( n >= 0 ) { ty = n / g.ntx; sj = g.ny + j; }
+    } else if ( j >= g.gl + g.ny ) {
+        const int n = nbr( tx, ty, 0, +1, g, px, py );
+        if ( n >= 0 ) { ty = n / g.ntx; sj = j - g.ny; }
+    }
+    if ( i < g.gl ) {
+        const int n = nbr( tx, ty, -1, 0, g, px, py );
+        if ( n >= 0 ) { tx = n % g.ntx; si = g.nx + i; }
+    } else if ( i >= g.gl + g.nx ) {
+        const int n = nbr( tx, ty, +1, 0, g, px, py );
+        if ( n >= 0 ) { tx = n % g.ntx; si = i - g.nx; }
+    }
+    st = ty * g.ntx + tx;
+    return st != t || si != i || sj != j;
+}
+
+// copy_to_gc (x then y passes, grid.h:253-340) of up to two grids of the same
+// element count in one launch (blockIdx.y selects the grid): every guard
+// element of the tile copied from where copy_source resolves it.
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_xy( GcArgs a0, GcArgs a1 ) {
+    const GcArgs a = blockIdx.y ? a1 : a0;
+    const G g = a.g;
+    const int t = blockIdx.x;
+    const gptr<float> buf = gbl( a.buf );
+    const int E = g.ext_x, gw = g.gl + g.gu;
+    const int nrow = gw * E, ncell = nrow + g.ny * gw;
+    for ( int k = threadIdx.x; k < ncell * NC; k += blockDim.x ) {
+        const int c = k % NC, e = k / NC;
+        int i, j;
+        if ( e < g.gl * E ) {
+            j = e / E;
+            i = e % E;
+        } else if ( e < nrow ) {
+            const int f = e - g.gl * E;
+            j = g.gl + g.ny + f / E;
+            i = f % E;
+        } else {
+            const int f = e - nrow, q = f % gw;
+            j = g.gl + f / gw;
+            i = q < g.gl ? q : g.nx + q;
+        }
+        int st, si, sj;
+        if ( copy_source( g, a.px, a.py, t, i, j, st, si, sj ) )
+            buf[( (size_t) t * g.vol + i + j * E ) * NC + c] = buf[( (size_t) st * g.vol + si + sj * E ) * NC + c];
+    }
+}
+
 constexpr int TILE_BLOCK = 256;
 
 // kernel3_x / kernel3_y (grid.h:495-593): B = A[-1]*a + A[0]*b + A[+1]*c per
@@ -358,6 +422,160 @@ __global__ void __launch_bounds__( TILE_BLOCK ) k_x_shift_left( float * buf, G g
     for ( int k = threadIdx.x; k < n; k += blockDim.x ) {
         const int r = k % ( g.ext_x * NC );
         tb[k] = ( r < keep ) ? A[k + shift * NC] : 0.0f;
+    }
+}
+
+// Several kernel3_x passes, each followed by copy_to_gc_x (grid.h:495-539
+// and 253-295), in one launch, tile-parallel and out of place: workgroup t
+// stages its tile's rows (all components, guard rows included) widened by
+// `passes` columns of the x neighbours' interiors on each side, runs the
+// passes in LDS and writes the whole tile (interior and x guards) to `dst`.
+// Every loaded column is one of three tiles' data (lower neighbour L, the
+// tile T, upper neighbour R), and the passes reproduce the pass-by-pass
+// results exactly: the first pass reads each tile's own x guards as they
+// are (the element next to a tile edge uses that tile's guard, as kernel3_x
+// does); later passes read the neighbour's new edge value, which is what
+// the guard copy between passes puts there; at a non-periodic edge the
+// guards keep their values through all passes.  After the last pass the x
+// guards hold the neighbours' final edge values, as the last copy_to_gc_x
+// leaves them.  Needs gc_lower + passes <= nx and gc_upper + passes <= nx
+// (the widened rows reach no further than the neighbours).
+constexpr int MAX_PASSES = 16;
+constexpr int TF_BLOCK = 288;  // threads per workgroup at most (3 x window width x rows at once)
+struct TileFilterArgs {
+    const float * src;
+    float * dst;
+    G g;
+    int px;
+    int passes;
+    float a[MAX_PASSES], b[MAX_PASSES], c[MAX_PASSES];
+};
+
+__global__ void __launch_bounds__( TF_BLOCK ) k_filter_tiles_x( TileFilterArgs a ) {
+    // thread (q, r): float q = 3 w + comp of every blockDim.y-th row
+    extern __shared__ float lds[];
+    const G g = a.g;
+    const int t = blockIdx.x;
+    const int tx = t % g.ntx, ty = t / g.ntx;
+    const int n = a.passes;
+    const int WL = g.gl + n;                   // window columns left of the tile's first cell
+    const int W = WL + g.nx + g.gu + n;        // window width
+    const int Q = 3 * W;                       // floats per window row
+    const int NE = g.ext_y * Q;                // floats per window copy
+    float * A = lds;
+    float * B = lds + NE;
+    float * bg = lds + 2 * NE;                 // per row and component: L's upper, T's lower, T's upper, R's lower guard
+    const int lo = nbr( tx, ty, -1, 0, g, a.px, 0 ), hi = nbr( tx, ty, +1, 0, g, a.px, 0 );
+    const gptr<const float> src = gbl( a.src );
+    const int q = threadIdx.x, r0 = threadIdx.y, dr = blockDim.y;
+    const int w = q / 3, comp = q - 3 * w;
+    const int ix = w - WL;
+    // the column's source: (tile, column) or none
+    int stile = -1, scol = 0;
+    if ( ix >= 0 && ix < g.nx ) { stile = t; scol = g.gl + ix; }
+    else if ( ix < 0 ) {
+        if ( lo >= 0 ) { stile = lo; scol = g.gl + g.nx + ix; }
+        else if ( ix >= -g.gl ) { stile = t; scol = g.gl + ix; }
+    } else {
+        if ( hi >= 0 ) { stile = hi; scol = g.gl + ix - g.nx; }
+        else if ( ix < g.nx + g.gu ) { stile = t; scol = g.gl + ix; }
+    }
+    const size_t sbase = ( (size_t) ( stile < 0 ? 0 : stile ) * g.vol + scol ) * 3 + comp;
+    for ( int j = r0; j < g.ext_y; j += dr ) A[j * Q + q] = stile >= 0 ? src[sbase + (size_t) j * g.ext_x * 3] : 0.0f;
+    if ( q < 12 ) {
+        const int side = q / 3;  // 0 L upper, 1 T lower, 2 T upper, 3 R lower
+        const int gt = side == 0 ? lo : side == 3 ? hi : t;
+        const int gcol = ( side == 0 || side == 2 ) ? g.gl + g.nx : g.gl - 1;
+        for ( int j = r0; j < g.ext_y; j += dr )
+            bg[j * 12 + q] = gt >= 0 ? src[( (size_t) gt * g.vol + gcol + j * g.ext_x ) * 3 + comp] : 0.0f;
+    }
+    __syncthreads();
+    // held columns: the window's ends and the guards at a non-periodic edge
+    const bool held = ( lo < 0 && ix < 0 ) || ( hi < 0 && ix >= g.nx ) || w == 0 || w == W - 1;
+    const int side = ix == -1 ? 0 : ix == 0 ? 1 : ix == g.nx - 1 ? 2 : ix == g.nx ? 3 : -1;
+    for ( int p = 0; p < n; p++ ) {
+        const float ca = a.a[p], cb = a.b[p], cc = a.c[p];
+        for ( int j = r0; j < g.ext_y; j += dr ) {
+            const int k = j * Q + q;
+            if ( held ) {
+                B[k] = A[k];
+                continue;
+            }
+            float l = A[k - 3], r = A[k + 3];
+            if ( p == 0 && side >= 0 ) {
+                // each tile's own guard next to its edge
+                const float gv = bg[j * 12 + side * 3 + comp];
+                if ( side == 0 || side == 2 ) r = gv;
+                else l = gv;
+            }
+            B[k] = l * ca + A[k] * cb + r * cc;
+        }
+        __syncthreads();
+        float * tmp = A;
+        A = B;
+        B = tmp;
+    }
+    // the tile: interior and x guards, every row
+    const gptr<float> dst = gbl( a.dst ) + (size_t) t * g.vol * 3;
+    const int E = g.ext_x;
+    if ( q < 3 * E ) {
+        const int col = q / 3, wo = col - g.gl + WL;
+        for ( int j = r0; j < g.ext_y; j += dr ) dst[( col + j * E ) * 3 + comp] = A[j * Q + wo * 3 + comp];
+    }
+    // tile padding as it was
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x, nth = blockDim.x * blockDim.y;
+    for ( int k = g.ext_y * E * 3 + tid; k < g.vol * 3; k += nth ) dst[k] = src[(size_t) t * g.vol * 3 + k];
+}
+
+constexpr int ROW_BLOCK = 512;
+constexpr int ROW_REGS = 32;  // floats per thread of the shift: rows of up to ROW_BLOCK * ROW_REGS floats
+
+// x_shift_left + copy_to_gc_x (grid.h:455-485) of up to two float3 grids of
+// the same geometry in one launch: workgroup (row, grid) stages a row of
+// tiles' elements (every tile's full row, guards included, all components)
+// in LDS in memory order, shifted as the reference does (element e <- e +
+// shift, zeros entering at the right), then writes it back with the x guards
+// taken from the neighbours' new interior values (wrapped when periodic,
+// left as shifted at a non-periodic edge).
+__global__ void __launch_bounds__( ROW_BLOCK ) k_shift_rows_x( float * buf0, float * buf1, G g, int shift, int px ) {
+    extern __shared__ float row[];  // 3 ntx ext_x shifted values
+    const int ty = blockIdx.x / g.ext_y, iy = blockIdx.x % g.ext_y;
+    const gptr<float> buf = gbl( blockIdx.y == 0 ? buf0 : buf1 );
+    const int E = g.ext_x, RT = 3 * E, M3 = g.ntx * RT;
+    auto base = [&]( int tx ) { return ( (size_t) ( ty * g.ntx + tx ) * g.vol + iy * E ) * 3; };
+    float v[ROW_REGS];
+#pragma unroll
+    for ( int k = 0; k < ROW_REGS; k++ ) {
+        const int L = threadIdx.x + k * ROW_BLOCK;
+        v[k] = 0.0f;
+        if ( L < M3 ) {
+            const int tx = L / RT, r = L - tx * RT, e = r / 3;
+            if ( e < E - shift ) v[k] = buf[base( tx ) + r + 3 * shift];
+        }
+    }
+#pragma unroll
+    for ( int k = 0; k < ROW_REGS; k++ ) {
+        const int L = threadIdx.x + k * ROW_BLOCK;
+        if ( L < M3 ) row[L] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for ( int k = 0; k < ROW_REGS; k++ ) {
+        const int L = threadIdx.x + k * ROW_BLOCK;
+        if ( L < M3 ) {
+            const int tx = L / RT, r = L - tx * RT, e = r / 3, comp = r - 3 * e;
+            float v = row[L];
+            if ( e < g.gl ) {
+                int src = tx - 1;
+                if ( src < 0 ) src = px ? g.ntx - 1 : -1;
+                if ( src >= 0 ) v = row[src * RT + ( g.nx + e ) * 3 + comp];
+            } else if ( e >= g.gl + g.nx ) {
+                int src = tx + 1;
+                if ( src >= g.ntx ) src = px ? 0 : -1;
+                if ( src >= 0 ) v = row[src * RT + ( e - g.nx ) * 3 + comp];
+            }
+            buf[base( tx ) + r] = v;
+        }
     }
 }
 
@@ -676,6 +894,22 @@ int gc( int which, float * buf, const zpic_geom * g, int ncomp, int px, int py, 
     return set_error( ZPIC_EINVAL, "ncomp must be 1 or 3" );
 }
 
+// copy_to_gc of one grid (buf1 null) or of two grids of one geometry, one launch.
+int copy_xy( float * buf0, float * buf1, const zpic_geom * gg, int ncomp, int px, int py, void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !buf0 ) return set_error( ZPIC_EINVAL, "null buffer" );
+    const G g = make( gg );
+    const GcArgs a0{ buf0, g, px ? 1 : 0, py ? 1 : 0 };
+    const GcArgs a1{ buf1 ? buf1 : buf0, g, px ? 1 : 0, py ? 1 : 0 };
+    const dim3 grid( g.ntx * g.nty, buf1 ? 2 : 1 );
+    hipStream_t st = as_stream( stream );
+    if ( ncomp == 3 ) hipLaunchKernelGGL( k_copy_gc_xy<3>, grid, dim3( GC_BLOCK ), 0, st, a0, a1 );
+    else if ( ncomp == 1 ) hipLaunchKernelGGL( k_copy_gc_xy<1>, grid, dim3( GC_BLOCK ), 0, st, a0, a1 );
+    else return set_error( ZPIC_EINVAL, "ncomp must be 1 or 3" );
+    ZPIC_LAUNCH_CHECK( "k_copy_gc_xy" );
+    return ZPIC_OK;
+}
+
 // ------------------------------------------------------------------ batches
 // Fill of whole buffers, one row of blocks (blockIdx.y) per entry.
 struct FillArgs {
@@ -865,8 +1099,17 @@ int zpic_add_from_gc( float * buf, const zpic_geom * g, int ncomp, int px, int p
 }
 
 int zpic_copy_to_gc( float * buf, const zpic_geom * g, int ncomp, int px, int py, void * stream ) {
-    if ( int rc = gc( 2, buf, g, ncomp, px, py, stream ) ) return rc;
-    return gc( 3, buf, g, ncomp, px, py, stream );
+    return copy_xy( buf, nullptr, g, ncomp, px, py, stream );
+}
+
+int zpic_copy_to_gc2( float * buf0, float * buf1, const zpic_geom * g, int ncomp, int px, int py, void * stream ) {
+    if ( !buf1 ) return set_error( ZPIC_EINVAL, "null buffer" );
+    return copy_xy( buf0, buf1, g, ncomp, px, py, stream );
+}
+
+int zpic_add_copy_gc( float * buf, const zpic_geom * gg, int ncomp, int px, int py, void * stream ) {
+    if ( int rc = zpic_add_from_gc( buf, gg, ncomp, px, py, stream ) ) return rc;
+    return copy_xy( buf, nullptr, gg, ncomp, px, py, stream );
 }
 
 int zpic_kernel3( float * buf, const zpic_geom * gg, int dir, float a, float b, float c, int px, int py,
@@ -885,21 +1128,96 @@ int zpic_kernel3( float * buf, const zpic_geom * gg, int dir, float a, float b, 
     return gc( dir == 0 ? 2 : 3, buf, gg, 3, px, py, stream );
 }
 
+}  // extern "C"
+
+namespace zpic {
+namespace {
+// Device scratch of at least `bytes`, one per (device, stream) so that work
+// on different streams never shares it, grown on demand and kept for the
+// process (staging of the tile-parallel x filter).
+float * filter_scratch( size_t bytes, hipStream_t st ) {
+    static std::mutex m;
+    static std::map<std::pair<int, hipStream_t>, std::pair<void *, size_t>> cache;
+    int dev = 0;
+    if ( hipGetDevice( &dev ) != hipSuccess ) return nullptr;
+    std::lock_guard<std::mutex> lk( m );
+    auto & e = cache[std::make_pair( dev, st )];
+    if ( e.second < bytes ) {
+        if ( e.first ) (void) hipFree( e.first );
+        e.first = nullptr;
+        e.second = 0;
+        if ( hipMalloc( &e.first, bytes ) != hipSuccess ) return nullptr;
+        e.second = bytes;
+    }
+    return static_cast<float *>( e.first );
+}
+}  // namespace
+}  // namespace zpic
+
+extern "C" {
+
+int zpic_filter_x( float * buf, const zpic_geom * gg, int passes, const float * coef, int px, int py,
+                   void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !buf || !coef ) return set_error( ZPIC_EINVAL, "null argument" );
+    if ( passes < 1 ) return set_error( ZPIC_EINVAL, "filter passes must be >= 1" );
+    if ( gg->gc_lower < 1 || gg->gc_upper < 1 )
+        return set_error( ZPIC_EINVAL, "kernel3 requires at least 1 guard cell at both boundaries" );
+    const G g = make( gg );
+    // one pass, or tiles too narrow for the widened rows: pass by pass
+    const int W = g.gl + g.nx + g.gu + 2 * passes;
+    const size_t lds = sizeof( float ) * ( 2 * (size_t) g.ext_y * W * 3 + (size_t) g.ext_y * 12 );
+    const size_t bytes = sizeof( float ) * 3 * (size_t) g.vol * g.ntx * g.nty;
+    hipStream_t st = as_stream( stream );
+    float * tmp = nullptr;
+    if ( passes > 1 && passes <= MAX_PASSES && g.gl + passes <= g.nx && g.gu + passes <= g.nx && lds <= 65536 &&
+         3 * W <= TF_BLOCK )
+        tmp = filter_scratch( bytes, st );
+    if ( !tmp ) {
+        for ( int p = 0; p < passes; p++ )
+            if ( int rc = zpic_kernel3( buf, gg, 0, coef[3 * p], coef[3 * p + 1], coef[3 * p + 2], px, py, stream ) )
+                return rc;
+        return ZPIC_OK;
+    }
+    TileFilterArgs a;
+    a.src = buf;
+    a.dst = tmp;
+    a.g = g;
+    a.px = px ? 1 : 0;
+    a.passes = passes;
+    for ( int p = 0; p < passes; p++ ) {
+        a.a[p] = coef[3 * p];
+        a.b[p] = coef[3 * p + 1];
+        a.c[p] = coef[3 * p + 2];
+    }
+    const int Q = 3 * W;
+    const dim3 block( Q, std::max( 1, std::min( g.ext_y, TF_BLOCK / Q ) ) );
+    hipLaunchKernelGGL( k_filter_tiles_x, dim3( g.ntx * g.nty ), block, lds, st, a );
+    ZPIC_LAUNCH_CHECK( "k_filter_tiles_x" );
+    ZPIC_HIP_TRY( hipMemcpyAsync( buf, tmp, bytes, hipMemcpyDeviceToDevice, st ) );
+    (void) py;
+    return ZPIC_OK;
+}
+
 int zpic_current_advance( float * J, const zpic_geom * g, int kind, int dir, int order, int px, int py,
                           void * stream ) {
-    if ( int rc = zpic_add_from_gc( J, g, 3, px, py, stream ) ) return rc;
-    if ( int rc = zpic_copy_to_gc( J, g, 3, px, py, stream ) ) return rc;
+    if ( int rc = zpic_add_copy_gc( J, g, 3, px, py, stream ) ) return rc;
     if ( kind == 0 ) return ZPIC_OK;
     const unsigned n = order > 0 ? order : 1;
-    for ( unsigned i = 0; i < n; i++ )
-        if ( int rc = zpic_kernel3( J, g, dir, 0.25f, 0.5f, 0.25f, px, py, stream ) ) return rc;
+    std::vector<float> coef;
+    for ( unsigned i = 0; i < n; i++ ) coef.insert( coef.end(), { 0.25f, 0.5f, 0.25f } );
     if ( kind == 2 ) {
         // Filter::Compensated (filter.h:58-76)
         const float a = -1.0f;
         const float b = ( 4.0 + 2.0 * n ) / n;
         const float norm = 2 * a + b;
-        if ( int rc = zpic_kernel3( J, g, dir, a / norm, b / norm, a / norm, px, py, stream ) ) return rc;
+        coef.insert( coef.end(), { a / norm, b / norm, a / norm } );
     }
+    const int passes = (int) coef.size() / 3;
+    if ( dir == 0 ) return zpic_filter_x( J, g, passes, coef.data(), px, py, stream );
+    for ( int p = 0; p < passes; p++ )
+        if ( int rc = zpic_kernel3( J, g, dir, coef[3 * p], coef[3 * p + 1], coef[3 * p + 2], px, py, stream ) )
+            return rc;
     return ZPIC_OK;
 }
 
@@ -916,8 +1234,7 @@ int zpic_yee( float * E, float * B, const float * J, const zpic_geom * gg, float
     hipLaunchKernelGGL( k_yee, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, args, (const YeeArgs *) nullptr,
                         (const int2 *) nullptr );
     ZPIC_LAUNCH_CHECK( "k_yee" );
-    if ( int rc = zpic_copy_to_gc( E, gg, 3, px, py, stream ) ) return rc;
-    return zpic_copy_to_gc( B, gg, 3, px, py, stream );
+    return zpic_copy_to_gc2( E, B, gg, 3, px, py, stream );
 }
 
 // EMF::process_bc / Current::process_bc (emf.cpp:431-471, current.cpp:178-218).
@@ -1011,6 +1328,24 @@ int zpic_grid_add( float * dst, const float * src, size_t n, void * stream ) {
     if ( blocks > 8192 ) blocks = 8192;
     hipLaunchKernelGGL( k_grid_add, dim3( blocks ), dim3( 256 ), 0, as_stream( stream ), dst, src, n );
     ZPIC_LAUNCH_CHECK( "k_grid_add" );
+    return ZPIC_OK;
+}
+
+int zpic_x_shift_left2( float * buf0, float * buf1, const zpic_geom * gg, int shift, int px, void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !buf0 ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( !( shift > 0 && shift < gg->gc_upper ) )
+        return set_error( ZPIC_EINVAL, "x_shift_left(), invalid shift value, must be 0 < shift < gc.x.upper" );
+    const G g = make( gg );
+    const size_t lds = sizeof( float ) * 3 * (size_t) g.ntx * g.ext_x;
+    const bool fits = 3 * (size_t) g.ntx * g.ext_x <= (size_t) ROW_BLOCK * ROW_REGS && lds <= 65536;
+    if ( !fits ) {
+        if ( int rc = zpic_x_shift_left( buf0, gg, 3, shift, px, 0, stream ) ) return rc;
+        return buf1 ? zpic_x_shift_left( buf1, gg, 3, shift, px, 0, stream ) : ZPIC_OK;
+    }
+    hipLaunchKernelGGL( k_shift_rows_x, dim3( g.nty * g.ext_y, buf1 ? 2 : 1 ), dim3( ROW_BLOCK ), lds,
+                        as_stream( stream ), buf0, buf1, g, shift, px ? 1 : 0 );
+    ZPIC_LAUNCH_CHECK( "k_shift_rows_x" );
     return ZPIC_OK;
 }
 

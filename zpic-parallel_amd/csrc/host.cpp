@@ -409,8 +409,7 @@ void Current::set_bc( current::bc_type new_bc ) {
 
 // Current::advance (current.cpp:227-246)
 void Current::advance() {
-    J->add_from_gc();
-    J->copy_to_gc();
+    J->add_copy_gc();
     if ( bc.x.lower > current::bc::periodic || bc.x.upper > current::bc::periodic ||
          bc.y.lower > current::bc::periodic || bc.y.upper > current::bc::periodic ) {
         const int b[4] = { bc.x.lower, bc.x.upper, bc.y.lower, bc.y.upper };
@@ -538,8 +537,7 @@ void EMF::move_window() {
     if ( moving_window.needs_move( iter * dt ) ) {
         zpic_geom g = E->geom();
         const int2 p = E->get_periodic();
-        check( zpic_x_shift_left( E->fbuf(), &g, 3, 1, p.x, p.y, stream() ) );
-        check( zpic_x_shift_left( B->fbuf(), &g, 3, 1, p.x, p.y, stream() ) );
+        check( zpic_x_shift_left2( E->fbuf(), B->fbuf(), &g, 1, p.x, stream() ) );
         moving_window.advance();
     }
 }
@@ -1485,8 +1483,7 @@ void finish_launch( vec3grid<float3> & E, vec3grid<float3> & B, std::vector<floa
                     unsigned filter ) {
     E.upload( hE.data() );
     B.upload( hB.data() );
-    E.copy_to_gc();
-    B.copy_to_gc();
+    E.copy_to_gc( B );
     if ( filter > 0 ) {
         Filter::Compensated fcomp( coord::x, filter );
         fcomp.apply( E );

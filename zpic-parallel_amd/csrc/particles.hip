@@ -51,9 +51,9 @@ struct Geo {
     int nx, ny, ext_x, ext_y, vol, off, ntx, nty;
 };
 
-__device__ inline unsigned long long lanes_below() {
-    const unsigned lane = __lane_id();
-    return lane == 0 ? 0ull : ( ~0ull >> ( 64 - lane ) );
+// Number of set bits of m below this lane (v_mbcnt_lo / v_mbcnt_hi).
+__device__ inline int count_below( unsigned long long m ) {
+    return (int) __builtin_amdgcn_mbcnt_hi( (unsigned) ( m >> 32 ), __builtin_amdgcn_mbcnt_lo( (unsigned) m, 0u ) );
 }
 
 // ---- interpolate_fld (species.cpp:43-92) on LDS float planes
@@ -69,10 +69,12 @@ __device__ inline void interpolate( const float * __restrict__ Ex, const float *
     const float s0xh = ( 1 - hx ) - x.x, s1xh = hx + x.x;
     const float s0yh = ( 1 - hy ) - x.y, s1yh = hy + x.y;
 
-    const int k00 = ih + j * ys;  // (ih, j)
-    const int kjh = i + jh * ys;  // (i, jh)
-    const int kij = i + j * ys;   // (i, j)
-    const int khh = ih + jh * ys; // (ih, jh)
+    // row offsets by 24-bit multiplies (cell rows and tile widths are small)
+    const int jr = __mul24( j, ys ), jhr = __mul24( jh, ys );
+    const int k00 = ih + jr;   // (ih, j)
+    const int kjh = i + jhr;   // (i, jh)
+    const int kij = i + jr;    // (i, j)
+    const int khh = ih + jhr;  // (ih, jh)
 
     e.x = ( Ex[k00] * s0xh + Ex[k00 + 1] * s1xh ) * s0y + ( Ex[k00 + ys] * s0xh + Ex[k00 + ys + 1] * s1xh ) * s1y;
     e.y = ( Ey[kjh] * s0x + Ey[kjh + 1] * s1x ) * s0yh + ( Ey[kjh + ys] * s0x + Ey[kjh + ys + 1] * s1x ) * s1yh;
@@ -109,14 +111,20 @@ __device__ inline float div_ge1( float a, float b ) {
     return fmaf( fmaf( -b, q, a ), r, q );
 }
 
-// ---- dudt_boris (species.cpp:117-178); returns u and the energy term
+// ---- dudt_boris (species.cpp:117-178); returns u and the energy term.
+// EXACT_ENE: the energy term |u|^2 / (gamma + 1) by the general division, as
+// the reference (the deterministic mode); otherwise by div_ge1, correctly
+// rounded except for a subnormal quotient (|u| below 1e-19, where the general
+// division's rescaling matters), in 6 instructions instead of 13.
+template <bool EXACT_ENE>
 __device__ inline float3 boris( float alpha, float3 e, float3 b, float3 u, float & ene ) {
     e.x *= alpha; e.y *= alpha; e.z *= alpha;
     float3 ut = make_float3( u.x + e.x, u.y + e.y, u.z + e.z );
     {
         const float usq = fmaf( ut.z, ut.z, fmaf( ut.y, ut.y, ut.x * ut.x ) );
         const float g = sqrt_ge1( 1.0f + usq );
-        ene = usq / ( g + 1.0f );  // usq may be 0 or tiny: general division
+        if constexpr ( EXACT_ENE ) ene = usq / ( g + 1.0f );  // usq may be 0 or tiny: general division
+        else ene = div_ge1( usq, g + 1.0f );
         const float ag = div_ge1( alpha, g );
         b.x *= ag; b.y *= ag; b.z *= ag;
     }
@@ -230,6 +238,30 @@ __device__ inline void segment_current( const Seg & s, float qnx, float qny, flo
     v[7] = q * ( A1x * A1y + B1x * B1y + ( A1x * B1y - B1x * A1y ) / 2.0f );
 }
 
+// The same eight contributions with fused multiply-adds, for the fast mode
+// (whose J is compared to a tolerance: the deposit order differs anyway):
+// wy0 = (A0y + B0y) / 2 = 1/2 - (a.y + b.y) / 2, wy1 = 1/2 + (a.y + b.y) / 2,
+// and Jz = q (Ax (Ay + By / 2) + Bx (By - Ay / 2)), the reference's
+// Ax Ay + Bx By + (Ax By - Bx Ay) / 2 regrouped: 40 instead of 58 operations.
+__device__ inline void segment_current_fma( const Seg & s, float qnx, float qny, float v[8] ) {
+    const float A0x = 0.5f - s.a.x, A1x = 0.5f + s.a.x, B0x = 0.5f - s.b.x, B1x = 0.5f + s.b.x;
+    const float A0y = 0.5f - s.a.y, A1y = 0.5f + s.a.y, B0y = 0.5f - s.b.y, B1y = 0.5f + s.b.y;
+    const float jx = qnx * ( s.b.x - s.a.x );
+    const float jy = qny * ( s.b.y - s.a.y );
+    const float mx = 0.5f * ( s.a.x + s.b.x ), my = 0.5f * ( s.a.y + s.b.y );
+    v[0] = jx * ( 0.5f - my );
+    v[5] = jx * ( 0.5f + my );
+    v[1] = jy * ( 0.5f - mx );
+    v[3] = jy * ( 0.5f + mx );
+    const float q = s.qvz;
+    const float p0 = fmaf( 0.5f, B0y, A0y ), r0 = fmaf( -0.5f, A0y, B0y );
+    const float p1 = fmaf( 0.5f, B1y, A1y ), r1 = fmaf( -0.5f, A1y, B1y );
+    v[2] = q * fmaf( A0x, p0, B0x * r0 );
+    v[4] = q * fmaf( A1x, p0, B1x * r0 );
+    v[6] = q * fmaf( A0x, p1, B0x * r1 );
+    v[7] = q * fmaf( A1x, p1, B1x * r1 );
+}
+
 // LDS (address space 3) element types: volatile accesses through generic
 // pointers would be lowered to flat instructions.
 typedef __attribute__( ( address_space( 3 ) ) ) float lds_f32;
@@ -288,12 +320,11 @@ __device__ inline void add_current_batched( volatile lds_f32 * Jx, volatile lds_
 __device__ inline void deposit_elected( bool pend, const Seg & s, float qnx, float qny, volatile lds_f32 * Jx,
                                         volatile lds_f32 * Jy, volatile lds_f32 * Jz, volatile lds_u8 * tag, int ys,
                                         unsigned lane ) {
+    // computed on every lane (SIMD lanes cost the same masked or not); used
+    // only by pending ones
     float v[8];
-    int k = 0;
-    if ( pend ) {
-        segment_current( s, qnx, qny, v );
-        k = s.c.x + ys * s.c.y;
-    }
+    segment_current_fma( s, qnx, qny, v );
+    const int k = s.c.x + __mul24( ys, s.c.y );
     while ( __ballot( pend ) ) {
         if ( pend ) tag[k] = (unsigned char) lane;
         if ( pend && tag[k] == (unsigned char) lane ) {
@@ -350,6 +381,7 @@ __device__ inline bool reflect_walls( int refl, int tx, int ty, int ntx, int nty
 // second and third segments do not occupy the whole wave every iteration.
 constexpr int QCAP = WAVE;  // crossing-queue entries per wave
 constexpr int QWORDS = 6;   // cell, x0.x, x0.y, d.x, d.y, qvz
+constexpr int PREFETCH = 2;  // iterations of 64 particles loaded ahead
 
 __host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move ) {
     return sizeof( float ) * (size_t) vol * ( ( push ? 6 : 0 ) + ( move ? 3 * W : 0 ) ) +
@@ -398,8 +430,11 @@ __device__ inline SortG gview( const SortView & v ) {
 template <class T>
 struct TileArray {
     gptr<char> base;
-    __device__ T get( unsigned i ) const { return *(gptr<T>) ( base + i * (unsigned) sizeof( T ) ); }
-    __device__ void set( unsigned i, const T & v ) const { *(gptr<T>) ( base + i * (unsigned) sizeof( T ) ) = v; }
+    __device__ T get( unsigned i ) const { return load( i * (unsigned) sizeof( T ) ); }
+    __device__ void set( unsigned i, const T & v ) const { store( i * (unsigned) sizeof( T ), v ); }
+    // by byte offset
+    __device__ T load( unsigned b ) const { return *(gptr<T>) ( base + b ); }
+    __device__ void store( unsigned b, const T & v ) const { *(gptr<T>) ( base + b ) = v; }
 };
 
 // Single-instance launches pass map == nullptr and use `a` with tile =
@@ -495,7 +530,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
 
     // split + deposit of the first `count` queued paths, one per lane
     auto flush_queue = [&]( int count ) {
-        Seg sg[3];
+        Seg sg[3] = {};
         int ns = 0;
         if ( lane < count ) {
             const int cp = __float_as_int( q[lane] );
@@ -512,34 +547,54 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
         if ( __ballot( ns > 2 ) ) deposit_elected( ns > 2, sg[2], m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
     };
 
-    // the next 64 particles are loaded while the current ones are processed
-    int2 cn = make_int2( 0, 0 );
-    float2 xn = make_float2( 0.f, 0.f );
-    float3 un = make_float3( 0.f, 0.f, 0.f );
-    if ( beg + lane < end ) {
-        cn = pix.get( beg + lane );
-        xn = px.get( beg + lane );
-        un = pu.get( beg + lane );
+    // The particles of the next PREFETCH iterations are loaded while the
+    // current ones are processed (more bytes in flight per wave: the pass is
+    // bound by memory latency at the occupancy the LDS allows).  Byte offsets
+    // of the lane's particle in the 8-byte (ix, x) and 12-byte (u) arrays are
+    // carried across iterations; prefetch offsets are clamped to the chunk's
+    // last particle, so that every lane loads (lanes past the end load data
+    // that they never store or deposit).
+    int2 cq[PREFETCH] = {};
+    float2 xq[PREFETCH] = {};
+    float3 uq[PREFETCH] = {};
+    unsigned o8 = (unsigned) ( beg + lane ) * 8u, o12 = (unsigned) ( beg + lane ) * 12u;
+    const unsigned last8 = (unsigned) ( end - 1 ) * 8u, last12 = (unsigned) ( end - 1 ) * 12u;
+    if ( beg < end ) {
+#pragma unroll
+        for ( int d = 0; d < PREFETCH; d++ ) {
+            const unsigned n8 = min( o8 + 8u * WAVE * d, last8 ), n12 = min( o12 + 12u * WAVE * d, last12 );
+            cq[d] = pix.load( n8 );
+            xq[d] = px.load( n8 );
+            uq[d] = pu.load( n12 );
+        }
     }
 
-    for ( int i0 = beg; i0 < end; i0 += WAVE ) {
+    for ( int i0 = beg; i0 < end; i0 += WAVE, o8 += 8u * WAVE, o12 += 12u * WAVE ) {
         const int i = i0 + lane;
         const bool act = i < end;
-        int2 c = cn;
-        float2 x = xn;
-        float3 u = un;
-        if ( i + WAVE < end ) {
-            cn = pix.get( i + WAVE );
-            xn = px.get( i + WAVE );
-            un = pu.get( i + WAVE );
+        int2 c = cq[0];
+        float2 x = xq[0];
+        float3 u = uq[0];
+#pragma unroll
+        for ( int d = 0; d + 1 < PREFETCH; d++ ) {
+            cq[d] = cq[d + 1];
+            xq[d] = xq[d + 1];
+            uq[d] = uq[d + 1];
+        }
+        {
+            const unsigned n8 = min( o8 + 8u * WAVE * PREFETCH, last8 );
+            const unsigned n12 = min( o12 + 12u * WAVE * PREFETCH, last12 );
+            cq[PREFETCH - 1] = pix.load( n8 );
+            xq[PREFETCH - 1] = px.load( n8 );
+            uq[PREFETCH - 1] = pu.load( n12 );
         }
         if constexpr ( PUSH ) {
             float ene = 0.0f;
             if ( act ) {
                 float3 e, b;
                 interpolate( e0x, e0y, e0z, b0x, b0y, b0z, ys, c, x, e, b );
-                u = boris( alpha, e, b, u, ene );
-                pu.set( i, u );
+                u = boris<DET>( alpha, e, b, u, ene );
+                pu.store( o12, u );
             }
             if constexpr ( DET ) {
                 // energy += term, in particle order
@@ -552,37 +607,36 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
             }
         }
         if constexpr ( MOVE && !DET ) {
-            int dir = 4;
-            bool cross = false;
+            // computed on every lane (lanes past the end hold some particle's
+            // data); stored and deposited by active lanes only
+            const float rg = inv_gamma( u );
+            const float2 d = make_float2( m.dtdx * rg * u.x, m.dtdy * rg * u.y );
             Seg s0;
-            float2 d = make_float2( 0.f, 0.f );
+            s0.c = c;
+            s0.a = x;
+            s0.b = make_float2( x.x + d.x, x.y + d.y );
+            s0.qvz = m.q * u.z * rg * 0.5f;
+            // deltai (species.cpp:342-343)
+            const int2 di = make_int2( ( s0.b.x >= 0.5f ) - ( s0.b.x < -0.5f ), ( s0.b.y >= 0.5f ) - ( s0.b.y < -0.5f ) );
+            const bool cross = act && ( di.x != 0 || di.y != 0 );
+            int2 c1 = make_int2( c.x + di.x + m.shx, c.y + di.y + m.shy );
+            float2 x1 = make_float2( s0.b.x - di.x, s0.b.y - di.y );
             if ( act ) {
-                const float rg = inv_gamma( u );
-                d = make_float2( m.dtdx * rg * u.x, m.dtdy * rg * u.y );
-                s0.c = c;
-                s0.a = x;
-                s0.b = make_float2( x.x + d.x, x.y + d.y );
-                s0.qvz = m.q * u.z * rg * 0.5f;
-                // deltai (species.cpp:342-343)
-                const int2 di = make_int2( ( s0.b.x >= 0.5f ) - ( s0.b.x < -0.5f ), ( s0.b.y >= 0.5f ) - ( s0.b.y < -0.5f ) );
-                cross = di.x != 0 || di.y != 0;
-                int2 c1 = make_int2( c.x + di.x + m.shx, c.y + di.y + m.shy );
-                float2 x1 = make_float2( s0.b.x - di.x, s0.b.y - di.y );
                 if ( m.refl && reflect_walls( m.refl, t % g.ntx, t / g.ntx, g.ntx, g.nty, g.nx, g.ny, c1, x1, u ) )
-                    pu.set( i, u );
-                px.set( i, x1 );
-                pix.set( i, c1 );
-                const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
-                const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
-                dir = ( yc + 1 ) * 3 + ( xc + 1 );
+                    pu.store( o12, u );
+                px.store( o8, x1 );
+                pix.store( o8, c1 );
             }
+            const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
+            const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
+            const int dir = act ? ( yc + 1 ) * 3 + ( xc + 1 ) : 4;
             // paths inside one cell: their single segment now
             deposit_elected( act && !cross, s0, m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
             // paths that cross: queued, split and deposited 64 at a time
             const unsigned long long cm = __ballot( cross );
             if ( cm ) {
                 const int nc = __popcll( cm );
-                const int pos = qn + __popcll( cm & lanes_below() );
+                const int pos = qn + count_below( cm );
                 auto put = [&]( int k ) {
                     q[k] = __int_as_float( ( s0.c.y << 16 ) | s0.c.x );
                     q[QCAP + k] = s0.a.x;
@@ -591,19 +645,21 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                     q[4 * QCAP + k] = d.y;
                     q[5 * QCAP + k] = s0.qvz;
                 };
-                if ( cross && pos < QCAP ) put( pos );
-                if ( qn + nc >= QCAP ) {
+                // positions [base, base + QCAP) of the unbounded queue are
+                // its slots; a full queue is flushed and the window moves on
+                int base = 0;
+                for ( ;; ) {
+                    if ( cross && pos >= base && pos < base + QCAP ) put( pos - base );
+                    if ( qn + nc - base < QCAP ) break;
                     flush_queue( QCAP );
-                    if ( cross && pos >= QCAP ) put( pos - QCAP );
-                    qn += nc - QCAP;
-                } else {
-                    qn += nc;
+                    base += QCAP;
                 }
+                qn += nc - base;
             }
             const bool leave = act && dir != 4;
             const unsigned long long mask = __ballot( leave );
             if ( leave ) {
-                const int pos = nl + __popcll( mask & lanes_below() );
+                const int pos = nl + count_below( mask );
                 sidx[beg + pos] = ( i << 4 ) | dir;
                 atomicAdd( &s_cnt[dir], 1 );
             }
@@ -624,9 +680,9 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 x1.y -= di.y;
                 int2 c1 = make_int2( c.x + di.x + m.shx, c.y + di.y + m.shy );
                 if ( m.refl && reflect_walls( m.refl, t % g.ntx, t / g.ntx, g.ntx, g.nty, g.nx, g.ny, c1, x1, u ) )
-                    pu.set( i, u );
-                px.set( i, x1 );
-                pix.set( i, c1 );
+                    pu.store( o12, u );
+                px.store( o8, x1 );
+                pix.store( o8, c1 );
                 const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
                 const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
                 dir = ( yc + 1 ) * 3 + ( xc + 1 );
@@ -652,7 +708,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
             const bool leave = act && dir != 4;
             const unsigned long long mask = __ballot( leave );
             if ( leave ) {
-                const int pos = nl + __popcll( mask & lanes_below() );
+                const int pos = nl + count_below( mask );
                 sidx[beg + pos] = ( i << 4 ) | dir;
                 atomicAdd( &s_cnt[dir], 1 );
             }
@@ -758,46 +814,71 @@ __global__ void k_sort_newnp( SortArgs a, const SortArgs * tab ) {
 }
 
 // Exclusive scan of the new counts into the new tile offsets
-// (update_tile_info, particles.cpp:314-357).  One workgroup per instance;
-// each thread owns a contiguous run of tiles.
+// (update_tile_info, particles.cpp:314-357).  One workgroup per instance
+// walks the tiles in chunks of SCAN_BLOCK, one tile per thread (coalesced
+// loads, the next chunk's loads issued before the current chunk is
+// scanned): a wave scan by shuffles, the wave totals scanned by the first
+// wave, and a running carry between chunks.
 constexpr int SCAN_BLOCK = 1024;
 __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const SortArgs * tab ) {
     if ( tab ) a = gbl( tab )[blockIdx.x];
     const SortG S = gview( a.S );
     const gptr<int> new_off = gbl( a.T.off );
-    const int ntiles = a.ntiles;
-    __shared__ int s_sum[SCAN_BLOCK];
-    const int tid = threadIdx.x;
-    const int per = ( ntiles + SCAN_BLOCK - 1 ) / SCAN_BLOCK;
-    const int t0 = min( ntiles, tid * per ), t1 = min( ntiles, t0 + per );
     const gptr<const int> extra = gbl( a.extra );
-    int local = 0;
-    for ( int t = t0; t < t1; t++ ) local += S.new_np[t] + ( a.extra ? extra[t] : 0 );
-    s_sum[tid] = local;
-    __syncthreads();
-    // Hillis-Steele inclusive scan of the per-thread sums
-    for ( int o = 1; o < SCAN_BLOCK; o <<= 1 ) {
-        const int v = tid >= o ? s_sum[tid - o] : 0;
+    const gptr<const int> np_old = gbl( (const int *) a.P.np );
+    const bool count_moved = a.S.nmove != nullptr;
+    const int ntiles = a.ntiles;
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+    __shared__ int s_wave[SCAN_BLOCK / WAVE];
+    __shared__ unsigned long long s_moved;
+    if ( tid == 0 ) s_moved = 0;
+    auto load = [&]( int t, int & v, int & m ) {
+        v = 0;
+        m = 0;
+        if ( t < ntiles ) {
+            v = S.new_np[t] + ( a.extra ? extra[t] : 0 );
+            if ( count_moved ) m = np_old[t];
+        }
+    };
+    int v, m;
+    load( tid, v, m );
+    int carry = 0;
+    unsigned long long moved = 0;
+    for ( int t0 = 0; t0 < ntiles; t0 += SCAN_BLOCK ) {
+        int vn, mn;
+        load( t0 + SCAN_BLOCK + tid, vn, mn );
+        // inclusive scan within the wave
+        int x = v;
+        for ( int o = 1; o < WAVE; o <<= 1 ) {
+            const int y = __shfl_up( x, o );
+            if ( lane >= o ) x += y;
+        }
+        if ( lane == WAVE - 1 ) s_wave[w] = x;
         __syncthreads();
-        s_sum[tid] += v;
+        if ( w == 0 ) {
+            int z = lane < SCAN_BLOCK / WAVE ? s_wave[lane] : 0;
+            for ( int o = 1; o < SCAN_BLOCK / WAVE; o <<= 1 ) {
+                const int y = __shfl_up( z, o );
+                if ( lane >= o ) z += y;
+            }
+            if ( lane < SCAN_BLOCK / WAVE ) s_wave[lane] = z;  // inclusive wave totals
+        }
         __syncthreads();
+        const int before = carry + ( w > 0 ? s_wave[w - 1] : 0 ) + x - v;
+        if ( t0 + tid < ntiles ) new_off[t0 + tid] = before;
+        carry += s_wave[SCAN_BLOCK / WAVE - 1];
+        moved += (unsigned) m;
+        __syncthreads();  // s_wave is rewritten by the next chunk
+        v = vn;
+        m = mn;
     }
-    int base = s_sum[tid] - local;
-    for ( int t = t0; t < t1; t++ ) {
-        new_off[t] = base;
-        base += S.new_np[t] + ( a.extra ? extra[t] : 0 );
-    }
-    if ( tid == SCAN_BLOCK - 1 ) S.total[0] = s_sum[tid];
+    if ( tid == 0 ) S.total[0] = carry;
     // Species::move's d_nmove += np (species.cpp:2162): the counts the
     // particle pass just moved, i.e. before this sort
-    if ( a.S.nmove ) {
-        __shared__ unsigned long long s_moved;
-        if ( tid == 0 ) s_moved = 0;
-        __syncthreads();
-        const gptr<const int> np_old = gbl( (const int *) a.P.np );
-        unsigned long long mine = 0;
-        for ( int t = t0; t < t1; t++ ) mine += (unsigned) np_old[t];
-        if ( mine ) atomicAdd( &s_moved, mine );
+    if ( count_moved ) {
+        __syncthreads();  // s_moved cleared
+        for ( int o = WAVE / 2; o > 0; o >>= 1 ) moved += __shfl_xor( moved, o );
+        if ( lane == 0 && moved ) atomicAdd( &s_moved, moved );
         __syncthreads();
         if ( tid == 0 ) S.nmove[0] += s_moved;
     }
@@ -925,7 +1006,7 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
             for ( int dd = 0; dd < 9; dd++ ) {
                 if ( dd == 4 ) continue;
                 const unsigned long long mk = __ballot( valid && d == dd );
-                if ( d == dd ) rank = run[dd] + __popcll( mk & lanes_below() );
+                if ( d == dd ) rank = run[dd] + count_below( mk );
                 run[dd] += __popcll( mk );
             }
             if ( valid && s_start[d] >= 0 ) {
@@ -956,7 +1037,7 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
             }
             const unsigned long long mk = __ballot( stay );
             if ( stay ) {
-                const int r = r0 + __popcll( mk & lanes_below() );
+                const int r = r0 + count_below( mk );
                 const int hole = o + ( leaver_at( r, s_pre, chunk, S.idx, o ) >> 4 );
                 P.ix[hole] = P.ix[o + n0 + j];
                 P.x[hole] = P.x[o + n0 + j];

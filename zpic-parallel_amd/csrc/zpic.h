@@ -157,6 +157,17 @@ class grid {
         zpic_geom g = geom();
         zpic::check( zpic_add_from_gc( fbuf(), &g, ncomp, periodic.x, periodic.y, zpic::stream() ) );
     }
+    // copy_to_gc of this grid and of `other` (same geometry), one launch
+    void copy_to_gc( grid & other ) {
+        zpic_geom g = geom();
+        zpic::check( zpic_copy_to_gc2( fbuf(), other.fbuf(), &g, ncomp, periodic.x, periodic.y, zpic::stream() ) );
+    }
+    // add_from_gc followed by copy_to_gc; the guard sums the copy replaces
+    // are not kept
+    void add_copy_gc() {
+        zpic_geom g = geom();
+        zpic::check( zpic_add_copy_gc( fbuf(), &g, ncomp, periodic.x, periodic.y, zpic::stream() ) );
+    }
     void kernel3_x( float a, float b, float c ) {
         zpic_geom g = geom();
         zpic::check( zpic_kernel3( fbuf(), &g, 0, a, b, c, periodic.x, periodic.y, zpic::stream() ) );
@@ -164,6 +175,12 @@ class grid {
     void kernel3_y( float a, float b, float c ) {
         zpic_geom g = geom();
         zpic::check( zpic_kernel3( fbuf(), &g, 1, a, b, c, periodic.x, periodic.y, zpic::stream() ) );
+    }
+    // kernel3_x passes with coefficients coef[3p .. 3p+2], one launch
+    void filter_x( const std::vector<float> & coef ) {
+        zpic_geom g = geom();
+        zpic::check( zpic_filter_x( fbuf(), &g, (int) coef.size() / 3, coef.data(), periodic.x, periodic.y,
+                                    zpic::stream() ) );
     }
     // Whole tile-major buffer to / from the host (synchronous).
     void download( T * host ) const {
@@ -244,13 +261,21 @@ class Binomial : public Digital {
   public:
     Binomial( coord::cart dir_, unsigned int order_ = 0 ) : order( order_ > 0 ? order_ : 1 ), dir( dir_ ) {}
     Binomial * clone() const override { return new Binomial( dir, order ); }
+    // along x all passes run as one fused launch (zpic_filter_x)
     void apply( vec3grid<float3> & fld ) override {
-        for ( unsigned i = 0; i < order; i++ ) {
-            if ( dir == coord::x ) fld.kernel3_x( 0.25f, 0.5f, 0.25f );
-            else fld.kernel3_y( 0.25f, 0.5f, 0.25f );
+        if ( dir == coord::x ) {
+            fld.filter_x( passes() );
+            return;
         }
+        for ( unsigned i = 0; i < order; i++ ) fld.kernel3_y( 0.25f, 0.5f, 0.25f );
     }
     int kind() const override { return 1; }
+    // coefficients (a, b, c) of the passes of this filter
+    virtual std::vector<float> passes() const {
+        std::vector<float> c;
+        for ( unsigned i = 0; i < order; i++ ) c.insert( c.end(), { 0.25f, 0.5f, 0.25f } );
+        return c;
+    }
     unsigned get_order() const { return order; }
     coord::cart get_dir() const { return dir; }
 };
@@ -259,14 +284,23 @@ class Compensated : public Binomial {
     Compensated( coord::cart dir_, unsigned int order_ = 0 ) : Binomial( dir_, order_ ) {}
     Compensated * clone() const override { return new Compensated( dir, order ); }
     void apply( vec3grid<float3> & fld ) override {
+        if ( dir == coord::x ) {
+            fld.filter_x( passes() );
+            return;
+        }
         Binomial::apply( fld );
+        const std::vector<float> c = passes();
+        fld.kernel3_y( c[c.size() - 3], c[c.size() - 2], c[c.size() - 1] );
+    }
+    int kind() const override { return 2; }
+    std::vector<float> passes() const override {
+        std::vector<float> c = Binomial::passes();
         const float a = -1.0f;
         const float b = ( 4.0 + 2.0 * order ) / order;
         const float norm = 2 * a + b;
-        if ( dir == coord::x ) fld.kernel3_x( a / norm, b / norm, a / norm );
-        else fld.kernel3_y( a / norm, b / norm, a / norm );
+        c.insert( c.end(), { a / norm, b / norm, a / norm } );
+        return c;
     }
-    int kind() const override { return 2; }
 };
 // Learned denoiser in place of the binomial filter (no reference
 // counterpart): applies a zpic_denoiser (not owned; it must outlive the
