@@ -89,6 +89,47 @@ def cpu_baseline():
                 "sample": f"failed: {e}"}
 
 
+def bandwidth_ceilings():
+    """GB/s (bytes read + written per second) of a 4 GiB device copy and of
+    the particle records of one species of the 2048^2 Weibel run (268 M x 28
+    B read and written, no arithmetic), 5 timed repetitions each on the
+    current stream."""
+    import ctypes
+    import torch
+    import pyzpic
+    L = pyzpic.lib()
+    L.zpic_particle_stream_probe.restype = ctypes.c_int
+    L.zpic_particle_stream_probe.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_size_t, ctypes.c_void_p]
+    st = torch.cuda.Stream()  # a stream of its own: a null stream handle would mean the library's stream
+    res = {}
+    n = 1 << 28
+    ix = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    x = torch.zeros(2 * n, dtype=torch.float32, device="cuda")
+    u = torch.zeros(3 * n, dtype=torch.float32, device="cuda")
+
+    def probe():
+        rc = L.zpic_particle_stream_probe(ctypes.c_void_p(ix.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                          ctypes.c_void_p(u.data_ptr()), n, ctypes.c_void_p(st.cuda_stream))
+        assert rc == 0, L.zpic_last_error()
+
+    src = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for name, fn, nbytes in (("particle_records", probe, 56 * n), ("copy_4GiB", lambda: dst.copy_(src), 8 << 30)):
+        with torch.cuda.stream(st):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(5):
+                fn()
+            e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        res[name] = {"ms": round(ms, 4), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    return {"bandwidth_ceilings": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,9 +139,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=0,
                     help="threads per tile of the particle pass (multiple of 64 in [64, 512]; 0 = library default)")
+    ap.add_argument("--bandwidth", action="store_true",
+                    help="time only the HBM ceilings: a 4 GiB device copy and the particle pass's traffic "
+                         "(zpic_particle_stream_probe over one species of the 2048^2 run)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 2 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.bandwidth:
+        print(json.dumps(bandwidth_ceilings()), flush=True)
+        return
 
     if args.calibrate_copy:
         import torch

@@ -284,6 +284,12 @@ int zpic_fill(float *buf, size_t n, float value, void *stream);
  * guard cells included). */
 int zpic_grid_add(float *dst, const float *src, size_t n, void *stream);
 
+/* Measurement only, no reference counterpart: reads and writes back every
+ * record of n particles (int2 ix, float2 x, float3 u: 28 B each, modified),
+ * the particle pass's HBM traffic without its arithmetic, to time the
+ * ceiling that traffic allows on the device. */
+int zpic_particle_stream_probe(int *ix, float *x, float *u, size_t n, void *stream);
+
 /* grid<T>::x_shift_left (grid.h:455-485): every row of every tile, guard
  * cells included, shifted left by `shift` cells with zeros entering at the
  * right, then the x guard copy.  Requires 0 < shift < gc_upper. */

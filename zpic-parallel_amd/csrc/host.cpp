@@ -767,18 +767,18 @@ void Species::advance_timed( EMF const & emf, Current & current, hipEvent_t afte
 
 void Species::advance_mov_window( EMF const & emf, Current & current ) { step( emf, current, nullptr, true ); }
 
-void Species::step( EMF const & emf, Current & current, hipEvent_t after_push, bool window ) {
+void Species::particle_pass( EMF const & emf, Current & current, bool shift ) {
     const float alpha = 0.5 * dt / m_q;
     const float2 dt_dx = make_float2( dt / dx.x, dt / dx.y );
     const float2 qnx = make_float2( q * dx.x / dt, q * dx.y / dt );
-    // species.cpp:2088: the window time is (iter + 1) * dt in float
-    const bool shift = window && moving_window.needs_move( ( iter + 1 ) * dt );
     zpic_geom g = emf.E->geom();
     zpic_particles p = particles->view();
     check( zpic_push_move_deposit_shift( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha, dt_dx.x,
                                          dt_dx.y, q, qnx.x, qnx.y, d_tile_energy, &sort->buf, reflect_mask(),
                                          shift ? -1 : 0, 0, deterministic, stream() ) );
-    if ( after_push ) hip_check( hipEventRecord( after_push, stream() ), "hipEventRecord" );
+}
+
+void Species::finish_step( bool shift ) {
     if ( shift ) {
         // inject into the rightmost column of cells (species.cpp:2094-2112)
         const uint2 gd = particles->dims;
@@ -791,7 +791,16 @@ void Species::step( EMF const & emf, Current & current, hipEvent_t after_push, b
         tile_sort();
     }
     iter++;
-    check( zpic_sum_tiles( d_tile_energy, g.ntiles_x * g.ntiles_y, d_energy_dev, deterministic, stream() ) );
+    const uint2 nt = particles->ntiles;
+    check( zpic_sum_tiles( d_tile_energy, (int) ( nt.x * nt.y ), d_energy_dev, deterministic, stream() ) );
+}
+
+void Species::step( EMF const & emf, Current & current, hipEvent_t after_push, bool window ) {
+    // species.cpp:2088: the window time is (iter + 1) * dt in float
+    const bool shift = window && moving_window.needs_move( ( iter + 1 ) * dt );
+    particle_pass( emf, current, shift );
+    if ( after_push ) hip_check( hipEventRecord( after_push, stream() ), "hipEventRecord" );
+    finish_step( shift );
 }
 
 double Species::get_energy() const {
@@ -872,19 +881,21 @@ void Species::gather( part::quant quant, float * out ) const {
 
 // --------------------------------------------------------------- Simulation
 Simulation::Simulation( uint2 const ntiles_, uint2 const nx_, float2 const box_, double dt_ )
-    : iter( 0 ), timing( false ), t_steps( 0 ), ntiles( ntiles_ ), nx( nx_ ), box( box_ ), dt( dt_ ),
+    : iter( 0 ), timing( false ), t_ring( 0 ), t_steps( 0 ), ntiles( ntiles_ ), nx( nx_ ), box( box_ ), dt( dt_ ),
       emf( ntiles_, nx_, box_, dt_ ), current( ntiles_, nx_, box_, dt_ ) {
     for ( double & v : t_acc ) v = 0;
 }
 
 Simulation::~Simulation() {
-    for ( auto e : ev ) (void) hipEventDestroy( e );
+    for ( auto & v : ev )
+        for ( auto e : v ) (void) hipEventDestroy( e );
 }
 
 void Simulation::set_timing( bool on ) {
     timing = on;
     for ( double & v : t_acc ) v = 0;
     t_steps = 0;
+    for ( bool & p : ev_pending ) p = false;
 }
 
 // Simulation::advance (simulation.h:90-107) and advance_mov_window
@@ -898,50 +909,75 @@ void Simulation::set_moving_window() {
     for ( auto * s : species ) s->set_moving_window();
 }
 
+// One step.  Stage timing records events in one of TSLOTS slots and reads a
+// slot's elapsed times only when the slot comes round again (or in
+// get_timing()), so that timing adds no host wait per step.  (Running a
+// species' sort on a second stream beside the next species' particle pass
+// was measured and gave no gain: the particle pass, bound by HBM traffic,
+// slowed by as much as the sort hid.)
 void Simulation::step( bool window ) {
     const size_t nsp = species.size();
-    const size_t nev = 2 * nsp + 3;
-    if ( timing && ev.size() < nev ) {
-        for ( size_t i = ev.size(); i < nev; i++ ) {
+    const size_t nev = 2 * nsp + 4;
+    const int slot = (int) ( t_ring % TSLOTS );
+    if ( timing ) {
+        if ( ev_pending[slot] ) accumulate( slot );
+        while ( ev[slot].size() < nev ) {
             hipEvent_t e;
             hip_check( hipEventCreate( &e ), "hipEventCreate" );
-            ev.push_back( e );
+            ev[slot].push_back( e );
         }
     }
+    std::vector<hipEvent_t> & E = ev[slot];
+    auto rec = [&]( size_t i ) {
+        if ( timing ) hip_check( hipEventRecord( E[i], stream() ), "hipEventRecord" );
+    };
     current.zero();
-    if ( timing ) hip_check( hipEventRecord( ev[0], stream() ), "hipEventRecord" );
+    rec( 0 );
     for ( size_t k = 0; k < nsp; k++ ) {
-        species[k]->advance_timed( emf, current, timing ? ev[1 + 2 * k] : nullptr, window );
-        if ( timing ) hip_check( hipEventRecord( ev[2 + 2 * k], stream() ), "hipEventRecord" );
+        species[k]->advance_timed( emf, current, timing ? E[1 + 2 * k] : nullptr, window );
+        rec( 2 + 2 * k );
     }
     current.advance();
-    if ( timing ) hip_check( hipEventRecord( ev[2 * nsp + 1], stream() ), "hipEventRecord" );
+    rec( 2 * nsp + 1 );
     emf.advance( current );
+    rec( 2 * nsp + 2 );
+    rec( 2 * nsp + 3 );
     if ( timing ) {
-        hip_check( hipEventRecord( ev[2 * nsp + 2], stream() ), "hipEventRecord" );
-        hip_check( hipEventSynchronize( ev[2 * nsp + 2] ), "hipEventSynchronize" );
-        float ms;
-        double pm = 0, so = 0;
-        for ( size_t k = 0; k < nsp; k++ ) {
-            hip_check( hipEventElapsedTime( &ms, k ? ev[2 * k] : ev[0], ev[1 + 2 * k] ), "elapsed" );
-            pm += ms;
-            hip_check( hipEventElapsedTime( &ms, ev[1 + 2 * k], ev[2 + 2 * k] ), "elapsed" );
-            so += ms;
-        }
-        t_acc[0] += pm;
-        t_acc[1] += so;
-        hip_check( hipEventElapsedTime( &ms, ev[2 * nsp], ev[2 * nsp + 1] ), "elapsed" );
-        t_acc[2] += ms;
-        hip_check( hipEventElapsedTime( &ms, ev[2 * nsp + 1], ev[2 * nsp + 2] ), "elapsed" );
-        t_acc[3] += ms;
-        hip_check( hipEventElapsedTime( &ms, ev[0], ev[2 * nsp + 2] ), "elapsed" );
-        t_acc[4] += ms;
-        t_steps++;
+        ev_pending[slot] = true;
+        t_ring++;
     }
     iter++;
 }
 
+// Stage times of the step recorded in `slot` (waits for its last event):
+// particle passes, sorts, current update, field update, whole step.
+void Simulation::accumulate( int slot ) {
+    const size_t nsp = species.size();
+    const std::vector<hipEvent_t> & E = ev[slot];
+    hip_check( hipEventSynchronize( E[2 * nsp + 3] ), "hipEventSynchronize" );
+    auto el = [&]( size_t a, size_t b ) {
+        float ms = 0;
+        hip_check( hipEventElapsedTime( &ms, E[a], E[b] ), "elapsed" );
+        return (double) ms;
+    };
+    double pm = 0, so = 0;
+    for ( size_t k = 0; k < nsp; k++ ) {
+        const size_t start = k == 0 ? 0 : 2 * k;
+        pm += el( start, 1 + 2 * k );
+        so += el( 1 + 2 * k, 2 + 2 * k );
+    }
+    t_acc[0] += pm;
+    t_acc[1] += so;
+    t_acc[2] += el( 2 * nsp, 2 * nsp + 1 );
+    t_acc[3] += el( 2 * nsp + 1, 2 * nsp + 2 );
+    t_acc[4] += el( 0, 2 * nsp + 3 );
+    t_steps++;
+    ev_pending[slot] = false;
+}
+
 void Simulation::get_timing( double out[5] ) {
+    for ( int s = 0; s < TSLOTS; s++ )
+        if ( ev_pending[s] ) accumulate( s );
     for ( int i = 0; i < 5; i++ ) out[i] = t_steps ? t_acc[i] / t_steps : 0.0;
 }
 

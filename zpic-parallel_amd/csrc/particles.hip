@@ -894,14 +894,20 @@ __device__ inline int shifted_stayers( int t, gptr<int> old_off, gptr<int> new_o
     return ns > n0 ? n0 : ns;
 }
 
-// Position j of the tile's ordered leaver list.
-__device__ inline int leaver_at( int j, const int * seg_pre, int chunk, gptr<int> idx, int off ) {
+// Position j of the tile's ordered leaver list; pre[] holds the segments'
+// exclusive prefix counts (non-decreasing), in registers.
+__device__ inline int leaver_at( int j, const int ( &pre )[MAX_SEG + 1], int chunk, gptr<int> idx, int off ) {
     int w = 0;
-    while ( w + 1 < MAX_SEG && seg_pre[w + 1] <= j ) w++;
-    return idx[off + w * chunk + ( j - seg_pre[w] )];
+#pragma unroll
+    for ( int v = 1; v < MAX_SEG; v++ ) w += pre[v] <= j;
+    int base = 0;
+#pragma unroll
+    for ( int v = 0; v < MAX_SEG; v++ ) base = ( v == w ) ? pre[v] : base;
+    return idx[off + w * chunk + ( j - base )];
 }
 
 constexpr int SORT_BLOCK = 256;
+constexpr int SORT_CACHE = 4;  // iterations per wave whose list entries and cells stay in registers
 
 // copy_out (particles.cpp:379-550): leavers to their staging slots, holes
 // below n0 refilled from the stayers above n0, shifted stayers staged.  The
@@ -922,6 +928,39 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
     const SortGeo q = a.q;
     const int tid = threadIdx.x;
 
+    // Set-up with every global load issued at once: thread 9 d + e loads the
+    // count source e sends to this tile's target in direction d, thread
+    // 81 + d that target's new offset, threads 96.. the wave segment counts
+    // and the tile header; then nine threads form the staging starts.
+    __shared__ int s_cnt9[9][9];
+    __shared__ int s_src9[9][9];
+    __shared__ int s_toff[9];
+    __shared__ int s_seg[MAX_SEG];
+    if ( tid < 81 ) {
+        const int d = tid / 9, e = tid - 9 * ( tid / 9 );
+        const int tg = ( d == 4 ) ? s : target_of( s, d, q );
+        int se = -1, cnt = 0;
+        if ( tg >= 0 ) {
+            se = ( e == 4 ) ? tg : source_of( tg, e, q );
+            if ( se >= 0 ) {
+                if ( e != 4 ) cnt = S.npt[9 * se + e];
+                else if ( SORTED ) cnt = S.npt[9 * tg + 4];
+                else cnt = shifted_stayers( tg, P.off, T.off, S );
+            }
+        }
+        s_cnt9[d][e] = cnt;
+        s_src9[d][e] = se;
+    } else if ( tid < 90 ) {
+        const int d = tid - 81;
+        const int tg = ( d == 4 ) ? s : target_of( s, d, q );
+        s_toff[d] = tg >= 0 ? T.off[tg] : 0;
+    } else if ( tid >= 96 && tid < 96 + MAX_SEG ) {
+        s_seg[tid - 96] = S.seg[16 * s + ( tid - 96 )];
+    } else if ( tid >= 112 && tid < 116 ) {
+        const int h = tid - 112;
+        s_hdr[h] = h == 0 ? P.off[s] : h == 1 ? P.np[s] : h == 2 ? S.npt[9 * s + 4] : S.seg[16 * s + MAX_SEG];
+    }
+    __syncthreads();
     if ( tid < 9 ) {
         const int d = tid;
         const int tg = ( d == 4 ) ? s : target_of( s, d, q );
@@ -930,31 +969,27 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
             const int mine = ( d == 4 && !SORTED ) ? -1 : d;
             int before = 0, total = 0;
             for ( int e = 0; e < 9; e++ ) {
-                const int se = ( e == 4 ) ? tg : source_of( tg, e, q );
+                const int se = s_src9[d][e];
                 if ( se < 0 ) continue;
-                int cnt;
-                if ( e != 4 ) cnt = S.npt[9 * se + e];
-                else if ( SORTED ) cnt = S.npt[9 * tg + 4];
-                else cnt = shifted_stayers( tg, P.off, T.off, S );
+                const int cnt = s_cnt9[d][e];
                 const int rank = ( e == 4 && !SORTED ) ? -1 : e;
                 if ( se < s || ( se == s && rank < mine ) ) before += cnt;
                 total += cnt;
             }
-            start = T.off[tg] + before;
+            start = s_toff[d] + before;
             if ( d == 4 && !SORTED ) S.tmp_np[s] = total;
         }
         s_start[d] = start;
     }
     if ( tid == 0 ) {
         int acc = 0;
-        for ( int w = 0; w < MAX_SEG; w++ ) { s_pre[w] = acc; acc += S.seg[16 * s + w]; }
+        for ( int w = 0; w < MAX_SEG; w++ ) { s_pre[w] = acc; acc += s_seg[w]; }
         s_pre[MAX_SEG] = acc;
-        s_hdr[0] = P.off[s];
-        s_hdr[1] = P.np[s];
-        s_hdr[2] = S.npt[9 * s + 4];
-        s_hdr[3] = S.seg[16 * s + MAX_SEG];
     }
     __syncthreads();
+    int pre[MAX_SEG + 1];
+#pragma unroll
+    for ( int v = 0; v <= MAX_SEG; v++ ) pre[v] = s_pre[v];
 
     const int o = s_hdr[0], np = s_hdr[1], n0 = s_hdr[2], chunk = s_hdr[3];
     const int nl = np - n0;
@@ -968,20 +1003,35 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
     const int j0 = min( nl, w * qlen ), j1 = min( nl, j0 + qlen );
     __shared__ int s_dir[SORT_BLOCK / WAVE][9];
     __shared__ int s_stay[SORT_BLOCK / WAVE];
+    // The first SORT_CACHE iterations of a wave keep their leaver entries and
+    // tail cells in registers from pass 1 for passes 2 and 3 (a wave's
+    // quarter is rarely longer), so that those passes start from the records'
+    // loads instead of reloading the list first.
+    int ent_c[SORT_CACHE];
+    int2 tix_c[SORT_CACHE];
+    auto in_tile = [&]( int2 c ) { return c.x >= 0 && c.x < q.nx && c.y >= 0 && c.y < q.ny; };
     {
         int cnt[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
         int st = 0;
-        for ( int jb = j0; jb < j1; jb += WAVE ) {
+        auto pass1 = [&]( int jb, int & ent, int2 & c ) {
             const int j = jb + lane;
             const bool valid = j < j1;
-            const int d = valid ? ( leaver_at( j, s_pre, chunk, S.idx, o ) & 15 ) : 4;
+            ent = valid ? leaver_at( j, pre, chunk, S.idx, o ) : 4;
+            c = valid ? P.ix[o + n0 + j] : make_int2( -1, -1 );
+            const int d = ent & 15;
             for ( int dd = 0; dd < 9; dd++ ) cnt[dd] += __popcll( __ballot( d == dd && valid ) );
-            bool stay = false;
-            if ( valid ) {
-                const int2 c = P.ix[o + n0 + j];
-                stay = c.x >= 0 && c.x < q.nx && c.y >= 0 && c.y < q.ny;
-            }
-            st += __popcll( __ballot( stay ) );
+            st += __popcll( __ballot( valid && in_tile( c ) ) );
+        };
+#pragma unroll
+        for ( int it = 0; it < SORT_CACHE; it++ ) {
+            ent_c[it] = 4;
+            tix_c[it] = make_int2( -1, -1 );
+            if ( j0 + it * WAVE < j1 ) pass1( j0 + it * WAVE, ent_c[it], tix_c[it] );
+        }
+        for ( int jb = j0 + SORT_CACHE * WAVE; jb < j1; jb += WAVE ) {
+            int ent;
+            int2 c;
+            pass1( jb, ent, c );
         }
         if ( lane == 0 ) {
             for ( int dd = 0; dd < 9; dd++ ) s_dir[w][dd] = cnt[dd];
@@ -997,10 +1047,9 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
             run[dd] = 0;
             for ( int v = 0; v < w; v++ ) run[dd] += s_dir[v][dd];
         }
-        for ( int jb = j0; jb < j1; jb += WAVE ) {
+        auto pass2 = [&]( int jb, int ent ) {
             const int j = jb + lane;
             const bool valid = j < j1;
-            const int ent = valid ? leaver_at( j, s_pre, chunk, S.idx, o ) : 0;
             const int d = ent & 15;
             int rank = 0;
             for ( int dd = 0; dd < 9; dd++ ) {
@@ -1012,13 +1061,22 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
             if ( valid && s_start[d] >= 0 ) {
                 const int k = o + ( ent >> 4 );
                 const int2 c = P.ix[k];
+                const float2 x = P.x[k];
+                const float3 u = P.u[k];
                 const int xc = ( c.x >= q.nx ) - ( c.x < 0 );
                 const int yc = ( c.y >= q.ny ) - ( c.y < 0 );
                 const int dst = s_start[d] + rank;
                 T.ix[dst] = make_int2( c.x - xc * q.nx, c.y - yc * q.ny );
-                T.x[dst] = P.x[k];
-                T.u[dst] = P.u[k];
+                T.x[dst] = x;
+                T.u[dst] = u;
             }
+        };
+#pragma unroll
+        for ( int it = 0; it < SORT_CACHE; it++ )
+            if ( j0 + it * WAVE < j1 ) pass2( j0 + it * WAVE, ent_c[it] );
+        for ( int jb = j0 + SORT_CACHE * WAVE; jb < j1; jb += WAVE ) {
+            const int j = jb + lane;
+            pass2( jb, j < j1 ? leaver_at( j, pre, chunk, S.idx, o ) : 4 );
         }
     }
     // every leaver is read before any hole (a leaver slot below n0) is overwritten
@@ -1028,22 +1086,27 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
     {
         int r0 = 0;
         for ( int v = 0; v < w; v++ ) r0 += s_stay[v];
-        for ( int jb = j0; jb < j1; jb += WAVE ) {
+        auto pass3 = [&]( int jb, int2 c ) {
             const int j = jb + lane;
-            bool stay = false;
-            if ( j < j1 ) {
-                const int2 c = P.ix[o + n0 + j];
-                stay = c.x >= 0 && c.x < q.nx && c.y >= 0 && c.y < q.ny;
-            }
+            const bool stay = j < j1 && in_tile( c );
             const unsigned long long mk = __ballot( stay );
             if ( stay ) {
+                const float2 x = P.x[o + n0 + j];
+                const float3 u = P.u[o + n0 + j];
                 const int r = r0 + count_below( mk );
-                const int hole = o + ( leaver_at( r, s_pre, chunk, S.idx, o ) >> 4 );
-                P.ix[hole] = P.ix[o + n0 + j];
-                P.x[hole] = P.x[o + n0 + j];
-                P.u[hole] = P.u[o + n0 + j];
+                const int hole = o + ( leaver_at( r, pre, chunk, S.idx, o ) >> 4 );
+                P.ix[hole] = c;
+                P.x[hole] = x;
+                P.u[hole] = u;
             }
             r0 += __popcll( mk );
+        };
+#pragma unroll
+        for ( int it = 0; it < SORT_CACHE; it++ )
+            if ( j0 + it * WAVE < j1 ) pass3( j0 + it * WAVE, tix_c[it] );
+        for ( int jb = j0 + SORT_CACHE * WAVE; jb < j1; jb += WAVE ) {
+            const int j = jb + lane;
+            pass3( jb, j < j1 ? P.ix[o + n0 + j] : make_int2( -1, -1 ) );
         }
     }
     __syncthreads();
@@ -1368,6 +1431,28 @@ int batch_particles_step( BatchParticlePlan * plan, void * stream ) {
     return ZPIC_OK;
 }
 
+// The particle pass's HBM traffic without its arithmetic: every particle
+// record (int2 cell, float2 position, float3 momentum: 28 B) read and written
+// back changed, grid-stride.  A measurement of the ceiling that traffic
+// allows on the device, not part of the timestep.
+__global__ void __launch_bounds__( 256 ) k_stream_probe( int2 * __restrict__ ix, float2 * __restrict__ x,
+                                                         float3 * __restrict__ u, size_t n ) {
+    const gptr<int2> gix = gbl( ix );
+    const gptr<float2> gx = gbl( x );
+    const gptr<float3> gu = gbl( u );
+    for ( size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x ) {
+        int2 c = gix[i];
+        float2 p = gx[i];
+        float3 v = gu[i];
+        c.x ^= 1;
+        p.x = p.x + 1.0f;
+        v.z = v.z + 1.0f;
+        gix[i] = c;
+        gx[i] = p;
+        gu[i] = v;
+    }
+}
+
 }  // namespace zpic
 
 using namespace zpic;
@@ -1507,6 +1592,17 @@ int zpic_deposit_charge( const zpic_particles * p, const zpic_geom * g, float * 
     else
         hipLaunchKernelGGL( k_charge<false>, dim3( nt ), dim3( FAST_BLOCK ), lds, st, view( p ), make_geo( g ), rho, q );
     ZPIC_LAUNCH_CHECK( "k_charge" );
+    return ZPIC_OK;
+}
+
+int zpic_particle_stream_probe( int * ix, float * x, float * u, size_t n, void * stream ) {
+    if ( ( !ix || !x || !u ) && n ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( n == 0 ) return ZPIC_OK;
+    size_t blocks = ( n + 255 ) / 256;
+    if ( blocks > 65536 ) blocks = 65536;
+    hipLaunchKernelGGL( k_stream_probe, dim3( blocks ), dim3( 256 ), 0, as_stream( stream ), (int2 *) ix, (float2 *) x,
+                        (float3 *) u, n );
+    ZPIC_LAUNCH_CHECK( "k_stream_probe" );
     return ZPIC_OK;
 }
 

@@ -81,6 +81,7 @@ namespace zpic {
 
 // Library stream shared by all objects (default priority).
 hipStream_t stream();
+
 // Throws std::runtime_error for a non-zero zpic_* return code.
 void check( int rc );
 void hip_check( hipError_t e, const char * what );
@@ -668,6 +669,13 @@ class Species {
     // same particles a cold initial load places, by the window's kernels
     void reinject();
 
+    // The two halves of advance(): the particle pass (push, move, deposit,
+    // leaver classification; with the window's cell shift when `shift`),
+    // and the sort, the injection when `shift`, the iteration count and the
+    // energy sum.
+    void particle_pass( EMF const & emf, Current & current, bool shift );
+    void finish_step( bool shift );
+
     // Deterministic (reference single-thread) deposit / energy order
     void set_deterministic( bool on ) { deterministic = on; }
 
@@ -722,9 +730,15 @@ class Species {
 class Simulation {
     unsigned int iter;
     bool timing;
-    std::vector<hipEvent_t> ev;
+    // stage timing without a host wait per step: the events of the last few
+    // steps, read when their slot comes round again or in get_timing()
+    static constexpr int TSLOTS = 3;
+    std::vector<hipEvent_t> ev[TSLOTS];
+    bool ev_pending[TSLOTS] = { false, false, false };
+    long long t_ring;
     double t_acc[5];
     long long t_steps;
+    void accumulate( int slot );
 
   public:
     const uint2 ntiles;
