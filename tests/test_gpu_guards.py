@@ -2,9 +2,9 @@
 
 grid<T>::add_from_gc and copy_to_gc (grid.h:253-445) each run an x pass over
 every tile, then a y pass.  The library runs copy_to_gc as one launch
-(zpic_copy_to_gc, and zpic_copy_to_gc2 for two grids), and add_from_gc
-followed by copy_to_gc as two (zpic_add_copy_gc, used by Current::advance);
-zpic_add_from_gc keeps the two passes.  Each is compared here, bit for bit
+(zpic_copy_to_gc, and zpic_copy_to_gc2 for two grids); add_from_gc keeps
+its two passes (zpic_add_from_gc), and zpic_add_copy_gc (Current::advance)
+runs them followed by the one-launch copy.  Each is compared here, bit for bit
 and over the whole buffer (guards and tile padding included), with the two
 passes restated in numpy, on random data, over tile counts of 1 (a tile its
 own neighbour when periodic) to several, tiles as narrow as the guard bands,
@@ -90,11 +90,42 @@ GEOMS = [  # ntx, nty, nx, ny, gl, gu
 ]
 
 
+class DeviceBuffer:
+    """Device memory through the HIP runtime the library itself uses
+    (hipMalloc / hipMemcpy by ctypes), so that the test needs no other GPU
+    framework."""
+
+    def __init__(self, hip, host):
+        self.hip, self.nbytes = hip, host.nbytes
+        self.ptr = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(self.nbytes)) == 0
+        assert hip.hipMemcpy(self.ptr, host.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(self.nbytes), 1) == 0
+
+    def numpy(self, dtype=np.float32):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        assert self.hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(self.nbytes), 2) == 0
+        return out
+
+    def free(self):
+        self.hip.hipFree(self.ptr)
+
+
+def _hip_runtime():
+    """The HIP runtime the library loaded (the one its kernels run on)."""
+    for line in open("/proc/self/maps"):
+        if "libamdhip64" in line:
+            return ctypes.CDLL(line.split()[-1])
+    return ctypes.CDLL("libamdhip64.so")
+
+
 @pytest.fixture(scope="module")
 def lib():
-    import torch
     import pyzpic
     L = pyzpic.lib()
+    hip = _hip_runtime()
+    hip.hipMalloc.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
     for name in ("zpic_copy_to_gc", "zpic_add_from_gc", "zpic_add_copy_gc"):
         f = getattr(L, name)
         f.restype = ctypes.c_int
@@ -102,18 +133,17 @@ def lib():
     L.zpic_copy_to_gc2.restype = ctypes.c_int
     L.zpic_copy_to_gc2.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_void_p]
-    return L, pyzpic, torch
+    return L, pyzpic, hip
 
 
 def _setup(lib, geom, ncomp, seed):
-    L, zp, torch = lib
+    L, zp, hip = lib
     ntx, nty, nx, ny, gl, gu = geom
     g = zp.Geom()
     assert L.zpic_geom_init(ctypes.byref(g), ntx, nty, nx, ny, gl, gu) == 0
     rng = np.random.default_rng(seed)
     flat = rng.standard_normal(ntx * nty * g.tile_vol * ncomp).astype(np.float32)
-    dev = torch.from_numpy(flat.copy()).to("cuda")
-    torch.cuda.synchronize()
+    dev = DeviceBuffer(hip, flat)
     return g, flat, dev
 
 
@@ -126,10 +156,9 @@ def _grid_view(flat, g, ncomp):
 
 
 def _run(lib, fn, *bufs_and_args):
-    L, zp, torch = lib
-    st = torch.cuda.current_stream().cuda_stream
-    rc = fn(*bufs_and_args, ctypes.c_void_p(st))
-    torch.cuda.synchronize()
+    L, zp, hip = lib
+    rc = fn(*bufs_and_args, None)  # the library's stream
+    assert hip.hipDeviceSynchronize() == 0
     assert rc == 0, L.zpic_last_error()
 
 
@@ -155,21 +184,24 @@ def test_guard_exchange_matches_two_passes(lib, case):
     g, flat, dev = _setup(lib, geom, nc, 1)
     want = flat.copy()
     copy_to_gc(_grid_view(want, g, nc), gfull)
-    _run(lib, L.zpic_copy_to_gc, ctypes.c_void_p(dev.data_ptr()), ctypes.byref(g), nc, px, py)
-    assert _ndiff(dev.cpu().numpy(), want) == 0
+    _run(lib, L.zpic_copy_to_gc, dev.ptr, ctypes.byref(g), nc, px, py)
+    assert _ndiff(dev.numpy(), want) == 0
+    dev.free()
     # add_from_gc, two passes, guard sums kept
     g, flat, dev = _setup(lib, geom, nc, 2)
     want = flat.copy()
     add_from_gc(_grid_view(want, g, nc), gfull)
-    _run(lib, L.zpic_add_from_gc, ctypes.c_void_p(dev.data_ptr()), ctypes.byref(g), nc, px, py)
-    assert _ndiff(dev.cpu().numpy(), want) == 0
+    _run(lib, L.zpic_add_from_gc, dev.ptr, ctypes.byref(g), nc, px, py)
+    assert _ndiff(dev.numpy(), want) == 0
+    dev.free()
     # add_from_gc + copy_to_gc in two launches
     g, flat, dev = _setup(lib, geom, nc, 3)
     want = flat.copy()
     v = _grid_view(want, g, nc)
     copy_to_gc(add_from_gc(v, gfull), gfull)
-    _run(lib, L.zpic_add_copy_gc, ctypes.c_void_p(dev.data_ptr()), ctypes.byref(g), nc, px, py)
-    assert _ndiff(dev.cpu().numpy(), want) == 0
+    _run(lib, L.zpic_add_copy_gc, dev.ptr, ctypes.byref(g), nc, px, py)
+    assert _ndiff(dev.numpy(), want) == 0
+    dev.free()
 
 
 @pytest.mark.parametrize("per", [(1, 1), (0, 1), (1, 0), (0, 0)])
@@ -183,8 +215,9 @@ def test_copy_to_gc2_matches_two_copies(lib, per):
     w0, w1 = f0.copy(), f1.copy()
     copy_to_gc(_grid_view(w0, g, 3), (*geom, px, py))
     copy_to_gc(_grid_view(w1, g, 3), (*geom, px, py))
-    _run(lib, L.zpic_copy_to_gc2, ctypes.c_void_p(d0.data_ptr()), ctypes.c_void_p(d1.data_ptr()), ctypes.byref(g),
-         3, px, py)
-    assert _ndiff(d0.cpu().numpy(), w0) == 0
-    assert _ndiff(d1.cpu().numpy(), w1) == 0
+    _run(lib, L.zpic_copy_to_gc2, d0.ptr, d1.ptr, ctypes.byref(g), 3, px, py)
+    assert _ndiff(d0.numpy(), w0) == 0
+    assert _ndiff(d1.numpy(), w1) == 0
+    d0.free()
+    d1.free()
     say(f"copy_to_gc2 periodic {per}: identical")

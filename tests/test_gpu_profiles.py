@@ -224,13 +224,18 @@ def _pass_coefs(filt):
     return c
 
 
-@pytest.mark.parametrize("per_x,filt", [(True, ("compensated", 0, 3)), (False, ("compensated", 0, 4)),
-                                        (False, ("binomial", 0, 2)), (True, ("binomial", 0, 1))])
-def test_fused_row_filter_matches_pass_by_pass(zp, per_x, filt):
-    """The x filter in one launch (zpic_filter_x: all passes and the final
-    guard copy) against the pass-by-pass path (zpic_kernel3: one kernel3_x
-    and x guard copy per pass), on random J with periodic or open x
-    boundaries: bit-identical, guard cells included."""
+@pytest.mark.parametrize("ntx,per_x,filt", [(5, True, ("compensated", 0, 3)), (5, False, ("compensated", 0, 4)),
+                                            (5, False, ("binomial", 0, 2)), (5, True, ("binomial", 0, 1)),
+                                            (1, True, ("compensated", 0, 4)), (2, True, ("compensated", 0, 3)),
+                                            (1, False, ("binomial", 0, 3))])
+def test_fused_x_filter_matches_pass_by_pass(zp, ntx, per_x, filt):
+    """The x filter as zpic_filter_x runs it (two or more passes: the
+    tile-parallel launch into scratch and the copy back) against the
+    pass-by-pass path (zpic_kernel3: one kernel3_x and x guard copy per
+    pass), on random J (guards inconsistent with their neighbours, so that
+    the first pass's use of each tile's own guards is checked), periodic or
+    open x, one tile (its own neighbour when periodic), two or five tiles
+    along x: bit-identical, guard cells included."""
     import ctypes
     L = zp.lib()
     L.zpic_field_device.restype = ctypes.c_int
@@ -239,8 +244,8 @@ def test_fused_row_filter_matches_pass_by_pass(zp, per_x, filt):
     L.zpic_kernel3.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
     rng = np.random.default_rng(11)
-    fused = zp.Simulation((5, 3), (8, 12), (4.0, 3.6), 0.05, filter=filt)
-    ref = zp.Simulation((5, 3), (8, 12), (4.0, 3.6), 0.05)
+    fused = zp.Simulation((ntx, 3), (8, 12), (0.8 * ntx, 3.6), 0.05, filter=filt)
+    ref = zp.Simulation((ntx, 3), (8, 12), (0.8 * ntx, 3.6), 0.05)
     if not per_x:
         for s in (fused, ref):
             s.set_current_bc(("none", "none"), ("periodic", "periodic"))
@@ -257,7 +262,7 @@ def test_fused_row_filter_matches_pass_by_pass(zp, per_x, filt):
         assert L.zpic_kernel3(buf, ctypes.byref(geom), 0, ca, cb, cc, per[0], per[1], None) == 0
     got, want = fused.field("J"), ref.field("J")
     ndiff = int(np.sum(got.view(np.uint32) != want.view(np.uint32)))
-    say(f"fused x filter {filt} periodic={per_x}: {ndiff} differing words")
+    say(f"fused x filter {filt} {ntx} tiles periodic={per_x}: {ndiff} differing words")
     assert ndiff == 0
     fused.close()
     ref.close()
