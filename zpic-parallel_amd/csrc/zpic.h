@@ -81,6 +81,19 @@ namespace zpic {
 
 // Library stream shared by all objects (default priority).
 hipStream_t stream();
+// Second library stream (default priority) for work that may run beside the
+// first's: the last species' sort while the current and fields advance.
+hipStream_t side_stream();
+// While alive, stream() returns `s` on the constructing thread.
+class StreamScope {
+    hipStream_t prev;
+
+  public:
+    explicit StreamScope( hipStream_t s );
+    ~StreamScope();
+    StreamScope( const StreamScope & ) = delete;
+    StreamScope & operator=( const StreamScope & ) = delete;
+};
 
 // Throws std::runtime_error for a non-zero zpic_* return code.
 void check( int rc );
@@ -675,6 +688,9 @@ class Species {
     // energy sum.
     void particle_pass( EMF const & emf, Current & current, bool shift );
     void finish_step( bool shift );
+    // Whether this step shifts the particles (the window moves), for a step
+    // of advance_mov_window (window) or advance (not window)
+    bool shifts( bool window ) const;
 
     // Deterministic (reference single-thread) deposit / energy order
     void set_deterministic( bool on ) { deterministic = on; }
@@ -735,6 +751,9 @@ class Simulation {
     static constexpr int TSLOTS = 3;
     std::vector<hipEvent_t> ev[TSLOTS];
     bool ev_pending[TSLOTS] = { false, false, false };
+    bool ev_overlap[TSLOTS] = { false, false, false };
+    // ordering events between the two streams (no timing)
+    hipEvent_t dep[2] = { nullptr, nullptr };
     long long t_ring;
     double t_acc[5];
     long long t_steps;
