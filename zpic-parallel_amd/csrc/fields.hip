@@ -441,7 +441,7 @@ __global__ void __launch_bounds__( TILE_BLOCK ) k_x_shift_left( float * buf, G g
 // leaves them.  Needs gc_lower + passes <= nx and gc_upper + passes <= nx
 // (the widened rows reach no further than the neighbours).
 constexpr int MAX_PASSES = 16;
-constexpr int TF_BLOCK = 288;  // threads per workgroup at most (3 x window width x rows at once)
+constexpr int TF_BLOCK = 180;  // threads per workgroup at most (3 x window width x rows at once; 96-1024 measured)
 struct TileFilterArgs {
     const float * src;
     float * dst;
