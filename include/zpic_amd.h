@@ -297,8 +297,9 @@ int zpic_x_shift_left(float *buf, const zpic_geom *g, int ncomp, int shift, int 
                       void *stream);
 
 /* zpic_x_shift_left of one or two float3 grids of the same geometry (buf1
- * may be NULL) in one launch, with the same results: EMF::move_window
- * (emf.cpp:120-129) shifts E and B together. */
+ * may be NULL), both grids per launch (the shift, then the x guard copy),
+ * with the same results: EMF::move_window (emf.cpp:120-129) shifts E and B
+ * together. */
 int zpic_x_shift_left2(float *buf0, float *buf1, const zpic_geom *g, int shift, int periodic_x, void *stream);
 
 /* div_corr_x (laser.cpp:20-70): Ex and Bx of the interior cells set so that

@@ -269,10 +269,11 @@ def test_fused_x_filter_matches_pass_by_pass(zp, ntx, per_x, filt):
 
 
 @pytest.mark.parametrize("per_x", [True, False])
-def test_fused_window_shift_matches_per_tile_shift(zp, per_x):
-    """E and B shifted together in one launch (zpic_x_shift_left2) against
-    the per-tile shift and x guard copy of each (zpic_x_shift_left), on
-    random fields with periodic or open x: bit-identical, guards included."""
+def test_two_grid_window_shift_matches_one_grid_shift(zp, per_x):
+    """E and B shifted together (zpic_x_shift_left2: both grids per launch)
+    against the shift and x guard copy of each grid alone
+    (zpic_x_shift_left), on random fields with periodic or open x:
+    bit-identical, guards included."""
     import ctypes
     L = zp.lib()
     L.zpic_field_device.restype = ctypes.c_int

@@ -135,10 +135,10 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( GcArgs a, const GcArgs
     }
 }
 
-// copy_to_gc_x (grid.h:253-297): x guard columns from the neighbours' edges.
+// copy_to_gc_x (grid.h:253-297) of tile t: x guard columns from the
+// neighbours' edges.
 template <int NC>
-__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( GcArgs a, const GcArgs * tab, const int2 * map ) {
-    const int t = select_instance( a, tab, map );
+__device__ inline void copy_gc_x_tile( const GcArgs & a, int t ) {
     const gptr<float> buf = gbl( a.buf );
     const G g = a.g;
     const int px = a.px, py = a.py;
@@ -160,6 +160,18 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( GcArgs a, const GcArg
             loc[( g.gl + g.nx + i + j * E ) * NC + c] = src[( g.gl + i + j * E ) * NC + c];
         }
     }
+}
+
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x( GcArgs a, const GcArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    copy_gc_x_tile<NC>( a, t );
+}
+
+// copy_to_gc_x of two grids of one geometry (blockIdx.y selects the grid).
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x2( GcArgs a0, GcArgs a1 ) {
+    copy_gc_x_tile<NC>( blockIdx.y ? a1 : a0, blockIdx.x );
 }
 
 // copy_to_gc_y (grid.h:303-340): y guard rows (full width) from the neighbours.
@@ -411,10 +423,11 @@ __global__ void k_grid_add( float * __restrict__ dst, const float * __restrict__
 
 // x_shift_left (grid.h:455-485) of one tile: rows of ext_x elements,
 // guard rows and columns included, staged in LDS and written back shifted.
+// blockIdx.y selects the grid (buf0 or buf1).
 template <int NC>
-__global__ void __launch_bounds__( TILE_BLOCK ) k_x_shift_left( float * buf, G g, int shift ) {
+__global__ void __launch_bounds__( TILE_BLOCK ) k_x_shift_left( float * buf0, float * buf1, G g, int shift ) {
     extern __shared__ float A[];
-    const gptr<float> tb = gbl( buf ) + (size_t) blockIdx.x * g.vol * NC;
+    const gptr<float> tb = gbl( blockIdx.y ? buf1 : buf0 ) + (size_t) blockIdx.x * g.vol * NC;
     const int n = g.ext_x * g.ext_y * NC;
     for ( int k = threadIdx.x; k < n; k += blockDim.x ) A[k] = tb[k];
     __syncthreads();
@@ -525,58 +538,6 @@ __global__ void __launch_bounds__( TF_BLOCK ) k_filter_tiles_x( TileFilterArgs a
     // tile padding as it was
     const int tid = threadIdx.y * blockDim.x + threadIdx.x, nth = blockDim.x * blockDim.y;
     for ( int k = g.ext_y * E * 3 + tid; k < g.vol * 3; k += nth ) dst[k] = src[(size_t) t * g.vol * 3 + k];
-}
-
-constexpr int ROW_BLOCK = 512;
-constexpr int ROW_REGS = 32;  // floats per thread of the shift: rows of up to ROW_BLOCK * ROW_REGS floats
-
-// x_shift_left + copy_to_gc_x (grid.h:455-485) of up to two float3 grids of
-// the same geometry in one launch: workgroup (row, grid) stages a row of
-// tiles' elements (every tile's full row, guards included, all components)
-// in LDS in memory order, shifted as the reference does (element e <- e +
-// shift, zeros entering at the right), then writes it back with the x guards
-// taken from the neighbours' new interior values (wrapped when periodic,
-// left as shifted at a non-periodic edge).
-__global__ void __launch_bounds__( ROW_BLOCK ) k_shift_rows_x( float * buf0, float * buf1, G g, int shift, int px ) {
-    extern __shared__ float row[];  // 3 ntx ext_x shifted values
-    const int ty = blockIdx.x / g.ext_y, iy = blockIdx.x % g.ext_y;
-    const gptr<float> buf = gbl( blockIdx.y == 0 ? buf0 : buf1 );
-    const int E = g.ext_x, RT = 3 * E, M3 = g.ntx * RT;
-    auto base = [&]( int tx ) { return ( (size_t) ( ty * g.ntx + tx ) * g.vol + iy * E ) * 3; };
-    float v[ROW_REGS];
-#pragma unroll
-    for ( int k = 0; k < ROW_REGS; k++ ) {
-        const int L = threadIdx.x + k * ROW_BLOCK;
-        v[k] = 0.0f;
-        if ( L < M3 ) {
-            const int tx = L / RT, r = L - tx * RT, e = r / 3;
-            if ( e < E - shift ) v[k] = buf[base( tx ) + r + 3 * shift];
-        }
-    }
-#pragma unroll
-    for ( int k = 0; k < ROW_REGS; k++ ) {
-        const int L = threadIdx.x + k * ROW_BLOCK;
-        if ( L < M3 ) row[L] = v[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for ( int k = 0; k < ROW_REGS; k++ ) {
-        const int L = threadIdx.x + k * ROW_BLOCK;
-        if ( L < M3 ) {
-            const int tx = L / RT, r = L - tx * RT, e = r / 3, comp = r - 3 * e;
-            float v = row[L];
-            if ( e < g.gl ) {
-                int src = tx - 1;
-                if ( src < 0 ) src = px ? g.ntx - 1 : -1;
-                if ( src >= 0 ) v = row[src * RT + ( g.nx + e ) * 3 + comp];
-            } else if ( e >= g.gl + g.nx ) {
-                int src = tx + 1;
-                if ( src >= g.ntx ) src = px ? 0 : -1;
-                if ( src >= 0 ) v = row[src * RT + ( e - g.nx ) * 3 + comp];
-            }
-            buf[base( tx ) + r] = v;
-        }
-    }
 }
 
 // Absorbing layers (no reference counterpart): E and B scaled by
@@ -1337,15 +1298,16 @@ int zpic_x_shift_left2( float * buf0, float * buf1, const zpic_geom * gg, int sh
     if ( !( shift > 0 && shift < gg->gc_upper ) )
         return set_error( ZPIC_EINVAL, "x_shift_left(), invalid shift value, must be 0 < shift < gc.x.upper" );
     const G g = make( gg );
-    const size_t lds = sizeof( float ) * 3 * (size_t) g.ntx * g.ext_x;
-    const bool fits = 3 * (size_t) g.ntx * g.ext_x <= (size_t) ROW_BLOCK * ROW_REGS && lds <= 65536;
-    if ( !fits ) {
-        if ( int rc = zpic_x_shift_left( buf0, gg, 3, shift, px, 0, stream ) ) return rc;
-        return buf1 ? zpic_x_shift_left( buf1, gg, 3, shift, px, 0, stream ) : ZPIC_OK;
-    }
-    hipLaunchKernelGGL( k_shift_rows_x, dim3( g.nty * g.ext_y, buf1 ? 2 : 1 ), dim3( ROW_BLOCK ), lds,
-                        as_stream( stream ), buf0, buf1, g, shift, px ? 1 : 0 );
-    ZPIC_LAUNCH_CHECK( "k_shift_rows_x" );
+    hipStream_t st = as_stream( stream );
+    // both grids shifted in one launch (tile by tile), then both x guard copies in one
+    const size_t lds = sizeof( float ) * 3 * g.ext_x * g.ext_y;
+    if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    const dim3 grid( g.ntx * g.nty, buf1 ? 2 : 1 );
+    hipLaunchKernelGGL( k_x_shift_left<3>, grid, dim3( TILE_BLOCK ), lds, st, buf0, buf1 ? buf1 : buf0, g, shift );
+    ZPIC_LAUNCH_CHECK( "k_x_shift_left" );
+    const GcArgs a0{ buf0, g, px ? 1 : 0, 0 }, a1{ buf1 ? buf1 : buf0, g, px ? 1 : 0, 0 };
+    hipLaunchKernelGGL( k_copy_gc_x2<3>, grid, dim3( GC_BLOCK ), 0, st, a0, a1 );
+    ZPIC_LAUNCH_CHECK( "k_copy_gc_x2" );
     return ZPIC_OK;
 }
 
@@ -1360,9 +1322,9 @@ int zpic_x_shift_left( float * buf, const zpic_geom * gg, int ncomp, int shift, 
     if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     hipStream_t st = as_stream( stream );
     if ( ncomp == 3 )
-        hipLaunchKernelGGL( k_x_shift_left<3>, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, g, shift );
+        hipLaunchKernelGGL( k_x_shift_left<3>, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, buf, g, shift );
     else
-        hipLaunchKernelGGL( k_x_shift_left<1>, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, g, shift );
+        hipLaunchKernelGGL( k_x_shift_left<1>, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, buf, buf, g, shift );
     ZPIC_LAUNCH_CHECK( "k_x_shift_left" );
     return gc( 2, buf, gg, ncomp, px, py, stream );
 }
