@@ -996,6 +996,7 @@ void Simulation::step( bool window ) {
     if ( timing ) {
         ev_pending[slot] = true;
         ev_overlap[slot] = overlap;
+        ev_nsp[slot] = nsp;
         t_ring++;
     }
     iter++;
@@ -1004,7 +1005,7 @@ void Simulation::step( bool window ) {
 // Stage times of the step recorded in `slot` (waits for its last event):
 // particle passes, sorts, current update, field update, whole step.
 void Simulation::accumulate( int slot ) {
-    const size_t nsp = species.size();
+    const size_t nsp = ev_nsp[slot];
     const std::vector<hipEvent_t> & E = ev[slot];
     hip_check( hipEventSynchronize( E[2 * nsp + 3] ), "hipEventSynchronize" );
     auto el = [&]( size_t a, size_t b ) {

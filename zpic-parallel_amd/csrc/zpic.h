@@ -752,6 +752,7 @@ class Simulation {
     std::vector<hipEvent_t> ev[TSLOTS];
     bool ev_pending[TSLOTS] = { false, false, false };
     bool ev_overlap[TSLOTS] = { false, false, false };
+    size_t ev_nsp[TSLOTS] = { 0, 0, 0 };  // species count of the recorded step
     // ordering events between the two streams (no timing)
     hipEvent_t dep[2] = { nullptr, nullptr };
     long long t_ring;
