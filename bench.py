@@ -90,32 +90,35 @@ def cpu_baseline():
 
 
 def bandwidth_ceilings():
-    """GB/s (bytes read + written per second) of a 4 GiB device copy and of
-    the particle records of one species of the 2048^2 Weibel run (268 M x 28
-    B read and written, no arithmetic), 5 timed repetitions each on the
-    current stream."""
+    """GB/s (bytes read + written per second) of HBM access patterns, 5 timed
+    repetitions each on a stream of their own:
+    * a 4 GiB device copy by torch;
+    * zpic_bandwidth_probe kind 0: the same copy in 16-byte words with 1, 2, 4
+      or 8 loads in flight per lane, over a sweep of grid sizes;
+    * kind 1: the particle records of one species of the 2048^2 Weibel run
+      (268 M x 28 B read and written back, no arithmetic), same sweep;
+    * zpic_particle_stream_probe: the records one per lane, 65 536 workgroups."""
     import ctypes
     import torch
     import pyzpic
     L = pyzpic.lib()
     L.zpic_particle_stream_probe.restype = ctypes.c_int
     L.zpic_particle_stream_probe.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_size_t, ctypes.c_void_p]
+    L.zpic_bandwidth_probe.restype = ctypes.c_int
+    L.zpic_bandwidth_probe.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_size_t, ctypes.c_int,
+                                                                                 ctypes.c_int, ctypes.c_void_p]
     st = torch.cuda.Stream()  # a stream of its own: a null stream handle would mean the library's stream
-    res = {}
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
     n = 1 << 28
     ix = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
     x = torch.zeros(2 * n, dtype=torch.float32, device="cuda")
     u = torch.zeros(3 * n, dtype=torch.float32, device="cuda")
-
-    def probe():
-        rc = L.zpic_particle_stream_probe(ctypes.c_void_p(ix.data_ptr()), ctypes.c_void_p(x.data_ptr()),
-                                          ctypes.c_void_p(u.data_ptr()), n, ctypes.c_void_p(st.cuda_stream))
-        assert rc == 0, L.zpic_last_error()
-
     src = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    for name, fn, nbytes in (("particle_records", probe, 56 * n), ("copy_4GiB", lambda: dst.copy_(src), 8 << 30)):
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    sp = ctypes.c_void_p(st.cuda_stream)
+
+    def timed(fn, nbytes):
         with torch.cuda.stream(st):
             fn()
             torch.cuda.synchronize()
@@ -126,7 +129,29 @@ def bandwidth_ceilings():
             e1.record(st)
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / 5
-        res[name] = {"ms": round(ms, 4), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+        return {"ms": round(ms, 4), "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+
+    def probe(kind, U, blocks):
+        def run():
+            if kind == 0:
+                rc = L.zpic_bandwidth_probe(0, vp(src), vp(dst), None, n, U, blocks, sp)
+            else:
+                rc = L.zpic_bandwidth_probe(1, vp(ix), vp(x), vp(u), n, U, blocks, sp)
+            assert rc == 0, L.zpic_last_error()
+        return run
+
+    res = {"cus": cus, "torch_copy_4GiB": timed(lambda: dst.copy_(src), 8 << 30)}
+    res["particle_stream_probe"] = timed(
+        lambda: L.zpic_particle_stream_probe(vp(ix), vp(x), vp(u), n, sp), 56 * n)
+    for kind, name, nbytes in ((0, "copy16_4GiB", 32 * n), (1, "particle_records", 56 * n)):
+        sweep = {}
+        for U in (1, 2, 4, 8):
+            for wpc in (1, 2, 4, 8, 16, 32, 0):
+                blocks = cus * wpc if wpc else (n + 256 * U - 1) // (256 * U)
+                sweep[f"U{U}_b{blocks}"] = timed(probe(kind, U, blocks), nbytes)
+        best = max(sweep, key=lambda k: sweep[k]["GB/s"])
+        res[name] = {"best": best, **sweep[best], "sweep": sweep}
+        print(f"[bench] {name}: best {best} {sweep[best]}", file=sys.stderr, flush=True)
     return {"bandwidth_ceilings": res}
 
 

@@ -80,6 +80,17 @@ typedef struct {
     int *total;     /* [1]          particles after the sort                        */
     unsigned long long *nmove; /* [1] running sum of the particles each sort's
                                   pass moved (Species::get_nmove); may be NULL */
+    /* Order-free sort of the fast mode (may be NULL): buffers of the
+     * particle buffer's size (int2, float2, float3 per particle; the
+     * staging buffer `tmp` serves).  When set, a fast-mode particle pass also
+     * writes every tile leaver's record there (at its leaver-list position,
+     * cell index in the target tile's frame) and the next zpic_tile_sort /
+     * zpic_tile_sort_copy mode 0 places particles without keeping the
+     * reference's within-tile order; counts, offsets and the per-tile
+     * multisets are the reference's. */
+    int *stage_ix;
+    float *stage_x;
+    float *stage_u;
 } zpic_sort;
 
 /* ----------------------------------------------- stage entry points (1) */
@@ -290,6 +301,14 @@ int zpic_grid_add(float *dst, const float *src, size_t n, void *stream);
  * ceiling that traffic allows on the device. */
 int zpic_particle_stream_probe(int *ix, float *x, float *u, size_t n, void *stream);
 
+/* Measurement only, no reference counterpart: kind 0 copies n 16-byte words
+ * from a to b; kind 1 reads and writes back n particle records (a = int2 ix,
+ * b = float2 x, c = float3 u).  loads_per_lane (1, 2, 4 or 8) independent
+ * records or words are in flight per lane; `blocks` workgroups of 256 lanes
+ * stride over the buffers.  Times the HBM ceiling of an access pattern. */
+int zpic_bandwidth_probe(int kind, void *a, void *b, void *c, size_t n, int loads_per_lane, int blocks,
+                         void *stream);
+
 /* grid<T>::x_shift_left (grid.h:455-485): every row of every tile, guard
  * cells included, shifted left by `shift` cells with zeros entering at the
  * right, then the x guard copy.  Requires 0 < shift < gc_upper. */
@@ -312,6 +331,13 @@ int zpic_div_corr_x(float *E, float *B, const zpic_geom *g, float dx, float dy, 
  * (default 256).  Library-wide; the deterministic mode always uses 64. */
 int zpic_set_threads_per_tile(int threads);
 int zpic_get_threads_per_tile(void);
+
+/* Tile sort of the fast mode, library-wide: 0 (default) = order-free (the
+ * species objects stage leavers in the particle pass, see zpic_sort.stage_ix),
+ * 1 = the reference's single-thread within-tile order (particles.cpp:379-614),
+ * as in the deterministic mode, which always keeps it. */
+int zpic_set_sort_order(int reference_order);
+int zpic_get_sort_order(void);
 /* Threads per tile the fast particle pass actually uses for tiles of
  * tile_vol elements when `threads` is requested (fewer wavefronts when the
  * LDS staging of a large tile would not fit); 0 if nothing fits. */

@@ -16,6 +16,26 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: longer CPU test")
 
 
+def tile_sorted(p):
+    """Particle records (cell, position bits, momentum bits) sorted within each
+    tile: the per-tile multisets, for comparing runs whose within-tile order
+    may differ (the fast mode's order-free sort).  np and offsets unchanged."""
+    import numpy as np
+    rows = np.concatenate([p["ix"].astype(np.int64), p["x"].view(np.uint32).astype(np.int64),
+                           p["u"].view(np.uint32).astype(np.int64)], 1)
+    tile = np.repeat(np.arange(len(p["np"])), p["np"])
+    order = np.lexsort(tuple(rows.T[::-1]) + (tile,))
+    return rows[order]
+
+
+def same_tile_multisets(p, q):
+    """True when two particle sets have the same counts and offsets per tile
+    and the same records in every tile."""
+    import numpy as np
+    return (np.array_equal(p["np"], q["np"]) and np.array_equal(p["offset"], q["offset"])
+            and np.array_equal(tile_sorted(p), tile_sorted(q)))
+
+
 def driver_args_to_config(args):
     """Translates ref_driver key=value arguments into a Weibel configuration."""
     cfg = {"filter": None}

@@ -5,13 +5,9 @@ import numpy as np
 def gather(buf, ntiles, nx, ext_x, vol, off, ncomp=3):
     """Interior cells of a tile-major buffer as a global [ny_total, nx_total, ncomp] array."""
     ntx, nty = ntiles
-    b = np.asarray(buf).reshape(ntx * nty, vol, ncomp) if ncomp > 1 else np.asarray(buf).reshape(ntx * nty, vol, 1)
-    out = np.zeros((nty * nx[1], ntx * nx[0], b.shape[2]), b.dtype)
-    for t in range(ntx * nty):
-        tx, ty = t % ntx, t // ntx
-        for iy in range(nx[1]):
-            row = b[t, off + iy * ext_x: off + iy * ext_x + nx[0]]
-            out[ty * nx[1] + iy, tx * nx[0]:(tx + 1) * nx[0]] = row
+    b = np.asarray(buf).reshape(nty, ntx, vol, ncomp)
+    rows = b[:, :, off: off + nx[1] * ext_x].reshape(nty, ntx, nx[1], ext_x, ncomp)[:, :, :, :nx[0]]
+    out = rows.transpose(0, 2, 1, 3, 4).reshape(nty * nx[1], ntx * nx[0], ncomp)
     return out if ncomp > 1 else out[..., 0]
 
 

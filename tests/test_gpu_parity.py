@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from conftest import apply_bc, driver_args_to_config
+from conftest import apply_bc, driver_args_to_config, same_tile_multisets
 from gridutil import gather, global_positions
 
 pytestmark = pytest.mark.gpu
@@ -189,8 +189,8 @@ def test_fast_first_step(zp):
     osim.advance(1)
     for k in range(2):
         p, q = sim.species[k].particles(), osim.particles(k)
-        for name in ("np", "offset", "ix", "x", "u"):
-            assert same_bits(p[name], q[name]), f"species {k} {name}"
+        # the fast mode's sort does not keep the reference's within-tile order
+        assert same_tile_multisets(p, q), f"species {k}"
     for f in "JEB":
         err = relnorm(sim.field(f), osim.field(f))
         say(f"  {f}: relative norm error {err:.2e}")
@@ -215,12 +215,81 @@ def test_fast_step_from_evolved_state(zp):
     osim.advance(1)
     for k in range(2):
         p, q = sim.species[k].particles(), osim.particles(k)
-        for name in ("np", "offset", "ix", "x", "u"):
-            assert same_bits(p[name], q[name]), f"species {k} {name}"
+        assert same_tile_multisets(p, q), f"species {k}"
     for f in "JEB":
         err = relnorm(sim.field(f), osim.field(f))
         assert err <= 1e-5, f"{f}: {err}"
     np.testing.assert_allclose(sim.energies()[:2], osim.energies()[:2], rtol=1e-9)
+
+
+def test_fast_reference_order_sort(zp):
+    """The fast mode with the reference's within-tile order kept
+    (zpic_set_sort_order(1)): particles bit-exact in buffer order after one
+    step, as the reference's single-thread tile_sort leaves them
+    (particles.cpp:379-614)."""
+    cfg = cfg_of((4, 4), (16, 16), (4, 4), uth=(0.3, 0.3, 0.3))
+    say("fast mode, reference sort order: one step vs oracle")
+    zp.set_sort_order(True)
+    try:
+        sim = gpu_weibel(zp, cfg)
+        osim = cpu_weibel(cfg)
+        sim.advance(1)
+        osim.advance(1)
+        for k in range(2):
+            p, q = sim.species[k].particles(), osim.particles(k)
+            for name in ("np", "offset", "ix", "x", "u"):
+                assert same_bits(p[name], q[name]), f"species {k} {name}"
+        sim.close()
+    finally:
+        zp.set_sort_order(False)
+
+
+def test_fast_sort_more_leavers_than_slot_rounds(zp):
+    """Order-free sort with more than 2048 leavers per tile (4x4-cell tiles,
+    ppc 32x32, relativistic thermal spread): the placement runs in several
+    rounds of hole slots.  Per-tile particle sets equal the oracle's after one
+    step, and after a second step from the GPU's own state."""
+    cfg = cfg_of((3, 3), (4, 4), (32, 32), uth=(2.0, 2.0, 2.0), ufl=(0.3, -0.2, 0.0))
+    say("order-free sort, > 2048 leavers per tile")
+    sim = gpu_weibel(zp, cfg)
+    osim = cpu_weibel(cfg)
+    sim.advance(1)
+    osim.advance(1)
+    for k in range(2):
+        p, q = sim.species[k].particles(), osim.particles(k)
+        assert same_tile_multisets(p, q), f"species {k}"
+    # leavers per tile of that step: cells outside the tile after the oracle's move alone
+    chk = cpu_weibel(cfg)
+    chk.stage("zero_j")
+    chk.stage("push", 0)
+    chk.stage("move", 0)
+    c = chk.particles(0)
+    out = (c["ix"][:, 0] < 0) | (c["ix"][:, 0] >= 4) | (c["ix"][:, 1] < 0) | (c["ix"][:, 1] >= 4)
+    leavers = np.add.reduceat(out.astype(np.int64), c["offset"])
+    say(f"  leavers per tile: max {leavers.max()}, min {leavers.min()}")
+    assert leavers.max() > 2048
+    copy_state_to_oracle(sim, osim)
+    sim.advance(1)
+    osim.advance(1)
+    for k in range(2):
+        p, q = sim.species[k].particles(), osim.particles(k)
+        assert same_tile_multisets(p, q), f"species {k} second step"
+    sim.close()
+
+
+def test_fast_sort_is_reproducible(zp):
+    """Two fast-mode runs of the same set-up leave identical buffers (the
+    order-free placement is a fixed enumeration, not a race)."""
+    cfg = cfg_of((4, 3), (16, 16), (3, 3), uth=(0.4, 0.4, 0.4), ufl=(0.2, 0.1, 0.5))
+    runs = []
+    for _ in range(2):
+        sim = gpu_weibel(zp, cfg)
+        sim.advance(5)
+        runs.append([sim.species[k].particles() for k in range(2)])
+        sim.close()
+    for k in range(2):
+        for name in ("np", "offset", "ix", "x", "u"):
+            assert same_bits(runs[0][k][name], runs[1][k][name]), f"species {k} {name}"
 
 
 @pytest.mark.parametrize("case", ["hot_small_tiles", "wrap_1x3_by2", "odd_tiles_bx1", "comp_cx1"])
@@ -233,18 +302,21 @@ def test_fast_sort_edge_cases(zp, case):
     osim.advance(1)
     for k in range(2):
         p, q = sim.species[k].particles(), osim.particles(k)
-        for name in ("np", "offset", "ix", "x", "u"):
-            assert same_bits(p[name], q[name]), f"{case} species {k} {name}"
+        assert same_tile_multisets(p, q), f"{case} species {k}"
     for _ in range(2):
         copy_state_to_oracle(sim, osim)
         sim.advance(1)
         osim.advance(1)
         for k in range(2):
             p, q = sim.species[k].particles(), osim.particles(k)
-            for name in ("np", "offset", "ix", "x", "u"):
-                assert same_bits(p[name], q[name]), f"{case} species {k} {name}"
+            assert same_tile_multisets(p, q), f"{case} species {k}"
         for f in "JEB":
             assert relnorm(sim.field(f), osim.field(f)) <= 1e-5, f
+
+
+# Norm-wise bounds of the E, B, J differences from the oracle after 500 fast
+# steps of the 128x128 Weibel problem.
+FIELD_DRIFT_500 = {"E": 1.0, "B": 1.0, "J": 2.0}
 
 
 def test_fast_weibel_500_steps(zp):
@@ -264,6 +336,14 @@ def test_fast_weibel_500_steps(zp):
         worst = max(worst, kin_err.max(), fld_err, bz_err)
         say(f"  step {50 * (blk + 1)}: kinetic rel err {kin_err.max():.2e}, field {fld_err:.2e}, B {bz_err:.2e}")
     assert worst <= 1e-3, worst
+    # Norm-wise E, B, J differences at step 500.  J's summation order differs
+    # from the oracle's from step 1 on, and the instability amplifies the
+    # difference; the bound is the measured drift with a margin (see the
+    # measurement notes), not a rounding-level bar.
+    errs = {f: relnorm(sim.field(f), osim.field(f)) for f in "EBJ"}
+    say("  step 500 norm-wise: " + ", ".join(f"{f} {v:.2e}" for f, v in errs.items()))
+    for f, v in errs.items():
+        assert v <= FIELD_DRIFT_500[f], (f, v)
 
 
 def test_push_stage_bit_exact(zp):
@@ -304,33 +384,83 @@ def test_move_deposit_stage(zp, det):
         assert relnorm(sim.field("J"), osim.field("J")) <= 1e-6
 
 
+def continuity_residual(r0, r1, J, dtdx):
+    """(r1 - r0) + dt/dx (Jx_i - Jx_{i-1}) + dt/dy (Jy_j - Jy_{j-1}), the
+    discrete continuity residual, evaluated in float64."""
+    r0, r1, J = (np.asarray(a, np.float64) for a in (r0, r1, J))
+    jx, jy = J[..., 0], J[..., 1]
+    return (r1 - r0) + dtdx * (jx - np.roll(jx, 1, 1)) + dtdx * (jy - np.roll(jy, 1, 0))
+
+
 def test_charge_continuity_gpu(zp):
-    """Discrete continuity of the GPU deposit: FP32 rounding level, against the
-    FP64 oracle's residual on the same state."""
+    """Discrete charge continuity of one GPU fast-mode step, checked against the
+    FP64 oracle on the same state (species.cpp:2348-2410 for the charge,
+    :1400-1493 for the deposit).
+
+    After 3 GPU steps the particles, E and B are loaded into the FP64 and the
+    FP32 builds of the oracle.  All three then run one species step (push,
+    move + deposit, sort) and the current's guard add from that state.  The
+    scale is the largest per-cell sum of the species' charge magnitudes
+    (electron and positron charge cancel in the total, their rounding does
+    not):
+    * FP64 residual <= 1e-13 of the scale: the continuity identity holds to
+      double rounding on the GPU's state;
+    * GPU FP32 residual <= 64 FP32 ulps (64 * 2^-24) of the scale, and at most
+      4x the residual of the reference's own FP32 arithmetic (the FP32 oracle,
+      serial deposit) on the same state;
+    * GPU J and charge within 2e-6 (norm-wise) of the FP64 values.
+    """
     cfg = cfg_of((2, 2), (16, 16), (3, 2), uth=(0.4, 0.4, 0.4), ufl=(0.5, -0.3, 0.6))
-    say("charge continuity on the GPU fast path")
+    say("charge continuity on the GPU fast path against the FP64 oracle on the same state")
     sim = gpu_weibel(zp, cfg)
     sim.advance(3)
     geo = ((2, 2), (16, 16))
+    cvol = (17 * 17 + 3) & ~3
+    o64 = cpu_weibel(cfg, prec=8)
+    o32 = cpu_weibel(cfg, prec=4)
+    copy_state_to_oracle(sim, o64)
+    copy_state_to_oracle(sim, o32)
 
-    def rho(s):
-        tot = 0
-        for sp in s.species:
-            tot = tot + gather(sp.charge(), geo[0], geo[1], 17, (17 * 17 + 3) & ~3, 0, ncomp=1)
-        return tot
+    def rho_gpu():
+        return [gather(sp.charge(), geo[0], geo[1], 17, cvol, 0, ncomp=1) for sp in sim.species]
 
-    r0 = rho(sim)
-    sim.stage("zero_j")
-    for k in range(2):
-        sim.stage("species", k)
-    sim.stage("add_gc")
-    r1 = rho(sim)
-    J = gather(sim.field("J"), geo[0], geo[1], 19, 364, 20)
-    dtdx = np.float32(0.07) / (np.float32(3.2) / np.float32(32))
-    res = (r1 - r0) + dtdx * (J[..., 0] - np.roll(J[..., 0], 1, 1)) + dtdx * (J[..., 1] - np.roll(J[..., 1], 1, 0))
-    scale = np.abs(r1).max()
-    say(f"  max residual / max rho = {np.abs(res).max() / scale:.2e}")
-    assert np.abs(res).max() <= 3e-6 * scale
+    def rho_orc(o):
+        return [gather(o.charge(k)[0], geo[0], geo[1], 17, cvol, 0, ncomp=1) for k in range(2)]
+
+    def jfield(s):
+        return gather(s.field("J"), geo[0], geo[1], 19, 364, 20)
+
+    r0, r0_64, r0_32 = sum(rho_gpu()), sum(rho_orc(o64)), sum(rho_orc(o32))
+    assert relnorm(r0, r0_64) <= 1e-6, "charge of the loaded state"
+    for s in (sim, o64, o32):
+        s.stage("zero_j")
+        for k in range(2):
+            s.stage("species", k)
+        s.stage("add_gc")
+    rs64 = rho_orc(o64)
+    scale = float(np.max(np.abs(rs64[0]) + np.abs(rs64[1])))
+    r1, r1_64, r1_32 = sum(rho_gpu()), sum(rs64), sum(rho_orc(o32))
+    J, J64, J32 = jfield(sim), jfield(o64), jfield(o32)
+
+    # the reference's dt_dx: float dt / float dx (species.cpp:2140-2148)
+    dtdx32 = float(np.float32(0.07) / (np.float32(3.2) / np.float32(32)))
+    dtdx64 = 0.07 / (np.float64(np.float32(3.2)) / 32)
+    res_gpu = np.abs(continuity_residual(r0, r1, J, dtdx32)).max()
+    res_32 = np.abs(continuity_residual(r0_32, r1_32, J32, dtdx32)).max()
+    res_64 = np.abs(continuity_residual(r0_64, r1_64, J64, dtdx64)).max()
+    ulp32 = 2.0 ** -24
+    say(f"  scale {scale:.3f}; FP64 residual {res_64 / scale:.2e}; GPU FP32 residual {res_gpu / scale / ulp32:.1f} "
+        f"ulps (oracle FP32 {res_32 / scale / ulp32:.1f}); J vs FP64 {relnorm(J[..., :2], J64[..., :2]):.2e}, "
+        f"rho {relnorm(r1, r1_64):.2e}")
+    assert res_64 <= 1e-13 * scale
+    assert res_gpu <= 64 * ulp32 * scale
+    assert res_gpu <= 4 * res_32
+    assert relnorm(J[..., :2], J64[..., :2]) <= 2e-6
+    assert relnorm(r1, r1_64) <= 2e-6
+    # the check is sensitive: the FP64 residual with Jy's sign flipped is O(1)
+    bad = J64.copy()
+    bad[..., 1] *= -1
+    assert np.abs(continuity_residual(r0_64, r1_64, bad, dtdx64)).max() > 1e-3 * scale
 
 
 @pytest.mark.parametrize("filt", [("binomial", 0, 1), ("binomial", 1, 2), ("compensated", 0, 1),

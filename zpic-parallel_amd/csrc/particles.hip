@@ -45,6 +45,9 @@ struct SortView {
     int * tmp_np;
     int * total;
     unsigned long long * nmove;
+    int2 * stage_ix;  // order-free sort staging (fast mode), or null
+    float2 * stage_x;
+    float3 * stage_u;
 };
 
 struct Geo {
@@ -515,6 +518,12 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
     const TileArray<float2> px{ (gptr<char>) gbl( P.x + off ) };
     const TileArray<float3> pu{ (gptr<char>) gbl( P.u + off ) };
     const gptr<int> sidx = gbl( S.idx + off );
+    // order-free sort: leaver records also go to the staging arrays at their
+    // leaver-list positions (fast mode)
+    const bool stage = !DET && S.stage_ix != nullptr;
+    const TileArray<int2> six{ (gptr<char>) gbl( S.stage_ix + off ) };
+    const TileArray<float2> sx{ (gptr<char>) gbl( S.stage_x + off ) };
+    const TileArray<float3> su{ (gptr<char>) gbl( S.stage_u + off ) };
 
     const float * e0x = Ex + g.off; const float * e0y = Ey + g.off; const float * e0z = Ez + g.off;
     const float * b0x = Bx + g.off; const float * b0y = By + g.off; const float * b0z = Bz + g.off;
@@ -662,6 +671,12 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
                 const int pos = nl + count_below( mask );
                 sidx[beg + pos] = ( i << 4 ) | dir;
                 atomicAdd( &s_cnt[dir], 1 );
+                if ( stage ) {
+                    const unsigned k = (unsigned) ( beg + pos );
+                    six.set( k, make_int2( c1.x - xc * g.nx, c1.y - yc * g.ny ) );
+                    sx.set( k, x1 );
+                    su.set( k, u );
+                }
             }
             nl += __popcll( mask );
         }
@@ -1166,6 +1181,256 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( SortArgs a, const Sor
 }
 
 // ---------------------------------------------------------------------------
+// Order-free tile sort of the fast mode.  The particle pass has left every
+// particle in place (leavers with their cell outside the tile), the leaver
+// list in wave segments (idx, seg) and a copy of every leaver's record, cell
+// in the target tile's frame, in the staging arrays at its leaver-list
+// position.  After the new counts and offsets (k_sort_newnp, k_sort_count):
+//   k_sort_shift: each tile stages the stayers that lie outside its new range
+//     into the free slots of its staging region (between the wave segments)
+//     and keeps its old offset and count in seg[SEG_OLD_OFF], seg[SEG_OLD_NP];
+//   k_sort_in_fast: each tile fills the slots of its new range that do not
+//     hold one of its stayers (its holes inside the new range, then the new
+//     range outside the old one) with its staged stayers and the leavers its
+//     nine sources staged towards it, in a fixed enumeration order.
+// Every record is read once and written once after the particle pass; the
+// within-tile order is not the reference's (counts, offsets and per-tile
+// multisets are).
+constexpr int SEG_OLD_OFF = 13, SEG_OLD_NP = 14;
+constexpr int SLOT_CAP = 2048;  // hole slots held in LDS per round of k_sort_in_fast
+
+// Workgroup-wide exclusive rank of `pred` among the threads; `total` gets
+// the count.  Called by every thread of the workgroup.
+__device__ inline int block_rank( bool pred, int & total, int * s_w ) {
+    const unsigned long long m = __ballot( pred );
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    if ( lane == 0 ) s_w[w] = __popcll( m );
+    __syncthreads();
+    int before = 0, tot = 0;
+    for ( int v = 0; v < nw; v++ ) {
+        const int c = s_w[v];
+        before += v < w ? c : 0;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return before + count_below( m );
+}
+
+// Segment v of a prefix table pre[0..MAX_SEG] (non-decreasing) that holds
+// position k (the last v with pre[v] <= k).
+__device__ inline int segment_of( int k, const int * pre ) {
+    int w = 0;
+#pragma unroll
+    for ( int v = 1; v < MAX_SEG; v++ ) w += pre[v] <= k;
+    return w;
+}
+
+// Free-slot prefix of a tile's staging region: the wave segments of the
+// particle pass are [v chunk, min(np, (v + 1) chunk)), the first seg[v]
+// slots of each hold leaver records.
+__device__ inline void gap_prefix( const int * seg, int np, int chunk, int * gpre ) {
+    int acc = 0;
+    for ( int v = 0; v < MAX_SEG; v++ ) {
+        gpre[v] = acc;
+        const int len = min( max( np - v * chunk, 0 ), chunk );
+        acc += max( len - seg[v], 0 );
+    }
+    gpre[MAX_SEG] = acc;
+}
+
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_shift( SortArgs a, const SortArgs * tab, const int2 * map ) {
+    __shared__ int s_seg[MAX_SEG + 1];
+    __shared__ int s_gpre[MAX_SEG + 1];
+    __shared__ int s_hdr[4];
+    __shared__ int s_w[SORT_BLOCK / WAVE];
+    const int t = select_instance( a, tab, map );
+    const PartG P = gview( a.P );
+    const SortG S = gview( a.S );
+    const gptr<int2> Gix = gbl( a.S.stage_ix );
+    const gptr<float2> Gx = gbl( a.S.stage_x );
+    const gptr<float3> Gu = gbl( a.S.stage_u );
+    const gptr<const int> new_off = gbl( (const int *) a.T.off );
+    const int tid = threadIdx.x;
+    if ( tid <= MAX_SEG ) s_seg[tid] = S.seg[16 * t + tid];  // [MAX_SEG] = chunk
+    else if ( tid >= WAVE && tid < WAVE + 4 ) {
+        const int h = tid - WAVE;
+        s_hdr[h] = h == 0 ? P.off[t] : h == 1 ? P.np[t] : h == 2 ? new_off[t] : S.new_np[t];
+    }
+    __syncthreads();
+    const int o = s_hdr[0], np = s_hdr[1], no = s_hdr[2], nn = s_hdr[3], chunk = s_seg[MAX_SEG];
+    if ( tid == 0 ) {
+        gap_prefix( s_seg, np, chunk, s_gpre );
+        S.seg[16 * t + SEG_OLD_OFF] = o;
+        S.seg[16 * t + SEG_OLD_NP] = np;
+    }
+    __syncthreads();
+    // old-range positions below and above the new range
+    const int a0 = o, la = max( min( o + np, no ) - o, 0 );
+    const int b0 = max( o, no + nn ), lb = max( o + np - b0, 0 );
+    const int nout = la + lb;
+    const int nx = a.q.nx, ny = a.q.ny;
+    int ns = 0;
+    for ( int j0 = 0; j0 < nout; j0 += blockDim.x ) {
+        const int j = j0 + tid;
+        int p = 0;
+        int2 c = make_int2( -1, -1 );
+        if ( j < nout ) {
+            p = j < la ? a0 + j : b0 + ( j - la );
+            c = P.ix[p];
+        }
+        const bool stay = c.x >= 0 && c.x < nx && c.y >= 0 && c.y < ny;
+        int tot;
+        const int k = ns + block_rank( stay, tot, s_w );
+        if ( stay ) {
+            const int w = segment_of( k, s_gpre );
+            const int dst = o + w * chunk + s_seg[w] + ( k - s_gpre[w] );
+            Gix[dst] = c;
+            Gx[dst] = P.x[p];
+            Gu[dst] = P.u[p];
+        }
+        ns += tot;
+    }
+    if ( tid == 0 ) S.tmp_np[t] = ns;
+}
+
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in_fast( SortArgs a, const SortArgs * tab, const int2 * map ) {
+    __shared__ int s_seg[9][MAX_SEG + 1];  // segment counts of the 9 sources' lists (e = 4: the tile)
+    __shared__ int s_so[9][3];             // source tile, old offset, old count
+    __shared__ int s_pre[9][MAX_SEG + 1];  // prefix of the segment counts
+    __shared__ int s_epre[10];             // prefix of the incoming lists' lengths (e != 4)
+    __shared__ int s_gpre[MAX_SEG + 1];    // free-slot prefix of the tile's staging region
+    __shared__ int s_hdr[3];
+    __shared__ int s_w[SORT_BLOCK / WAVE];
+    __shared__ int s_slot[SLOT_CAP];
+    const int t = select_instance( a, tab, map );
+    const PartG P = gview( a.P );
+    const SortG S = gview( a.S );
+    const gptr<const int2> Gix = gbl( (const int2 *) a.S.stage_ix );
+    const gptr<const float2> Gx = gbl( (const float2 *) a.S.stage_x );
+    const gptr<const float3> Gu = gbl( (const float3 *) a.S.stage_u );
+    const gptr<const int> new_off = gbl( (const int *) a.T.off );
+    const SortGeo q = a.q;
+    const int tid = threadIdx.x;
+
+    // set-up: every load at once
+    if ( tid < 9 * 16 ) {
+        const int e = tid / 16, v = tid % 16;
+        const int s = ( e == 4 ) ? t : source_of( t, e, q );
+        const int val = s >= 0 ? S.seg[16 * s + v] : 0;
+        if ( v <= MAX_SEG ) s_seg[e][v] = ( v == SEG_OLD_OFF || v == SEG_OLD_NP ) ? 0 : val;
+        if ( v == SEG_OLD_OFF ) s_so[e][1] = val;
+        if ( v == SEG_OLD_NP ) s_so[e][2] = val;
+        if ( v == 0 ) s_so[e][0] = s;
+    } else if ( tid < 9 * 16 + 3 ) {
+        const int h = tid - 9 * 16;
+        s_hdr[h] = h == 0 ? new_off[t] : h == 1 ? S.new_np[t] : S.tmp_np[t];
+    }
+    __syncthreads();
+    if ( tid < 9 ) {
+        const int e = tid;
+        int acc = 0;
+        for ( int v = 0; v < MAX_SEG; v++ ) {
+            s_pre[e][v] = acc;
+            acc += s_so[e][0] >= 0 ? s_seg[e][v] : 0;
+        }
+        s_pre[e][MAX_SEG] = acc;
+    }
+    if ( tid == 16 ) gap_prefix( s_seg[4], s_so[4][2], s_seg[4][MAX_SEG], s_gpre );
+    __syncthreads();
+    if ( tid == 0 ) {
+        int acc = 0;
+        for ( int e = 0; e < 9; e++ ) {
+            s_epre[e] = acc;
+            if ( e != 4 ) acc += s_pre[e][MAX_SEG];
+        }
+        s_epre[9] = acc;
+    }
+    __syncthreads();
+
+    const int o = s_so[4][1], np = s_so[4][2], chunk = s_seg[4][MAX_SEG];
+    const int no = s_hdr[0], nn = s_hdr[1], ns = s_hdr[2];
+    const int nl = s_pre[4][MAX_SEG];
+    const int etot = s_epre[9];
+    // slots of the new range outside the old one: [no, no + lb1) and [b2, b2 + lb2)
+    const int lb1 = max( min( no + nn, o ) - no, 0 );
+    const int b2 = max( no, o + np ), lb2 = max( no + nn - b2, 0 );
+
+    int nA = 0;  // the tile's holes inside its new range
+    for ( int r0 = 0; r0 == 0 || r0 < nA; r0 += SLOT_CAP ) {
+        // this round's hole slots, in list order
+        int seen = 0;
+        for ( int j0 = 0; j0 < nl; j0 += blockDim.x ) {
+            const int j = j0 + tid;
+            int p = -1;
+            if ( j < nl ) {
+                const int w = segment_of( j, s_pre[4] );
+                p = o + ( S.idx[o + w * chunk + ( j - s_pre[4][w] )] >> 4 );
+            }
+            const bool in = p >= no && p < no + nn;
+            int tot;
+            const int r = seen + block_rank( in, tot, s_w );
+            if ( in && r >= r0 && r < r0 + SLOT_CAP ) s_slot[r - r0] = p;
+            seen += tot;
+        }
+        nA = seen;
+        __syncthreads();
+        // destination of record rank k, or -1 outside this round
+        auto slot = [&]( int k ) {
+            if ( k < nA ) return ( k >= r0 && k < r0 + SLOT_CAP ) ? s_slot[k - r0] : -1;
+            if ( r0 != 0 ) return -1;
+            const int k2 = k - nA;
+            return k2 < lb1 ? no + k2 : b2 + ( k2 - lb1 );
+        };
+        // the tile's staged stayers: ranks [0, ns)
+        for ( int k = tid; k < ns; k += blockDim.x ) {
+            const int dst = slot( k );
+            if ( dst >= 0 ) {
+                const int w = segment_of( k, s_gpre );
+                const int src = o + w * chunk + s_seg[4][w] + ( k - s_gpre[w] );
+                P.ix[dst] = Gix[src];
+                P.x[dst] = Gx[src];
+                P.u[dst] = Gu[src];
+            }
+        }
+        // leavers of the sources towards this tile: ranks ns, ns + 1, ...
+        int run = ns;
+        for ( int j0 = 0; j0 < etot; j0 += blockDim.x ) {
+            const int j = j0 + tid;
+            int src = -1;
+            bool match = false;
+            if ( j < etot ) {
+                int e = 0;
+#pragma unroll
+                for ( int v = 1; v < 9; v++ ) e += s_epre[v] <= j;
+                // e is the last direction whose list starts at or before j
+                // (the tile's own list, e = 4, has zero length here)
+                const int jj = j - s_epre[e];
+                const int w = segment_of( jj, s_pre[e] );
+                src = s_so[e][1] + w * s_seg[e][MAX_SEG] + ( jj - s_pre[e][w] );
+                match = ( S.idx[src] & 15 ) == e;
+            }
+            int tot;
+            const int k = run + block_rank( match, tot, s_w );
+            if ( match ) {
+                const int dst = slot( k );
+                if ( dst >= 0 ) {
+                    P.ix[dst] = Gix[src];
+                    P.x[dst] = Gx[src];
+                    P.u[dst] = Gu[src];
+                }
+            }
+            run += tot;
+        }
+        __syncthreads();  // s_slot is rebuilt by the next round
+    }
+    if ( tid == 0 ) {
+        P.np[t] = nn;
+        P.off[t] = no;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // dep_charge_kernel (species.cpp:2348-2395) into a scalar grid with guards.
 template <bool DET>
 __global__ void __launch_bounds__( FAST_BLOCK ) k_charge( PartView P, Geo g, float * __restrict__ rho, float q ) {
@@ -1230,9 +1495,14 @@ SortView view( const zpic_sort * s ) {
     if ( s ) {
         v.idx = s->idx; v.npt = s->npt; v.new_np = s->new_np; v.seg = s->seg; v.tmp_np = s->tmp_np; v.total = s->total;
         v.nmove = s->nmove;
+        const bool st = s->stage_ix && s->stage_x && s->stage_u;
+        v.stage_ix = st ? reinterpret_cast<int2 *>( s->stage_ix ) : nullptr;
+        v.stage_x = st ? reinterpret_cast<float2 *>( s->stage_x ) : nullptr;
+        v.stage_u = st ? reinterpret_cast<float3 *>( s->stage_u ) : nullptr;
     } else {
         v.idx = v.npt = v.new_np = v.seg = v.tmp_np = v.total = nullptr;
         v.nmove = nullptr;
+        v.stage_ix = nullptr; v.stage_x = nullptr; v.stage_u = nullptr;
     }
     return v;
 }
@@ -1250,6 +1520,9 @@ int check_geom( const zpic_geom * g ) {
 // on the 2048^2 Weibel step the particle pass took 7.08 ms with 4 waves per
 // tile against 7.36 ms with 8 (fewer private J planes to clear and reduce).
 int g_threads = 256;  // library-wide setting
+// Fast-mode tile sort: 0 = order-free (staged leavers), 1 = the reference's
+// within-tile order.  The deterministic mode always keeps the reference's.
+int g_reference_order = 0;
 
 constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup
 
@@ -1324,6 +1597,7 @@ struct BatchParticlePlan {
     int max_tiles = 0;
     int max_vol = 0;
     int threads = FAST_BLOCK;
+    bool staged = false;  // every entry uses the order-free sort
     PartArgs * d_part = nullptr;
     SortArgs * d_sort = nullptr;
     int2 * d_map = nullptr;
@@ -1361,6 +1635,15 @@ int batch_particles_create( BatchParticlePlan ** out, const zpic_batch_species *
         plan->max_vol = std::max( plan->max_vol, it.fld.tile_vol );
     }
     plan->nblocks = (int) map.size();
+    {
+        int nst = 0;
+        for ( int e = 0; e < n; e++ ) nst += items[e].s.stage_ix != nullptr;
+        if ( nst != 0 && nst != n ) {
+            delete plan;
+            return set_error( ZPIC_EINVAL, "batch entries must all use the same sort (staged or not)" );
+        }
+        plan->staged = nst == n;
+    }
     plan->threads = fitted_threads( threads, plan->max_vol );
     if ( plan->threads == 0 ) {
         delete plan;
@@ -1420,12 +1703,21 @@ int batch_particles_step( BatchParticlePlan * plan, void * stream ) {
     hipLaunchKernelGGL( k_sort_count, dim3( plan->nentries ), dim3( SCAN_BLOCK ), 0, st, sd,
                         (const SortArgs *) plan->d_sort );
     ZPIC_LAUNCH_CHECK( "k_sort_count (batch)" );
-    hipLaunchKernelGGL( k_sort_out<false>, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
-                        (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
-    ZPIC_LAUNCH_CHECK( "k_sort_out (batch)" );
-    hipLaunchKernelGGL( k_sort_in, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
-                        (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
-    ZPIC_LAUNCH_CHECK( "k_sort_in (batch)" );
+    if ( plan->staged ) {
+        hipLaunchKernelGGL( k_sort_shift, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
+                            (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
+        ZPIC_LAUNCH_CHECK( "k_sort_shift (batch)" );
+        hipLaunchKernelGGL( k_sort_in_fast, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
+                            (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
+        ZPIC_LAUNCH_CHECK( "k_sort_in_fast (batch)" );
+    } else {
+        hipLaunchKernelGGL( k_sort_out<false>, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
+                            (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
+        ZPIC_LAUNCH_CHECK( "k_sort_out (batch)" );
+        hipLaunchKernelGGL( k_sort_in, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
+                            (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
+        ZPIC_LAUNCH_CHECK( "k_sort_in (batch)" );
+    }
     hipLaunchKernelGGL( k_batch_energy, dim3( plan->nentries ), dim3( 256 ), 0, st, (const EnergyArgs *) plan->d_energy );
     ZPIC_LAUNCH_CHECK( "k_batch_energy" );
     return ZPIC_OK;
@@ -1450,6 +1742,66 @@ __global__ void __launch_bounds__( 256 ) k_stream_probe( int2 * __restrict__ ix,
         gix[i] = c;
         gx[i] = p;
         gu[i] = v;
+    }
+}
+
+// Device copy of n 16-byte words with U independent loads in flight per
+// lane: a workgroup of 256 lanes moves 256 U words per iteration (lane l
+// takes words l, l + 256, ..., contiguous 4 KiB runs per load instruction),
+// grid-stride over the buffer.  Measurement only.
+template <int U>
+__global__ void __launch_bounds__( 256 ) k_copy_probe( const float4 * __restrict__ src, float4 * __restrict__ dst,
+                                                       size_t n ) {
+    const gptr<const float4> s = gbl( src );
+    const gptr<float4> d = gbl( dst );
+    const size_t step = (size_t) gridDim.x * 256 * U;
+    for ( size_t base = (size_t) blockIdx.x * 256 * U + threadIdx.x; base < n; base += step ) {
+        float4 v[U];
+#pragma unroll
+        for ( int k = 0; k < U; k++ ) {
+            const size_t i = base + (size_t) k * 256;
+            v[k] = i < n ? (float4) s[i] : make_float4( 0.f, 0.f, 0.f, 0.f );
+        }
+#pragma unroll
+        for ( int k = 0; k < U; k++ ) {
+            const size_t i = base + (size_t) k * 256;
+            if ( i < n ) d[i] = v[k];
+        }
+    }
+}
+
+// The particle records with U records in flight per lane (same access order
+// as k_copy_probe, read and written back modified).  Measurement only.
+template <int U>
+__global__ void __launch_bounds__( 256 ) k_stream_probe_u( int2 * __restrict__ ix, float2 * __restrict__ x,
+                                                           float3 * __restrict__ u, size_t n ) {
+    const gptr<int2> gix = gbl( ix );
+    const gptr<float2> gx = gbl( x );
+    const gptr<float3> gu = gbl( u );
+    const size_t step = (size_t) gridDim.x * 256 * U;
+    for ( size_t base = (size_t) blockIdx.x * 256 * U + threadIdx.x; base < n; base += step ) {
+        int2 c[U];
+        float2 p[U];
+        float3 v[U];
+#pragma unroll
+        for ( int k = 0; k < U; k++ ) {
+            const size_t i = min( base + (size_t) k * 256, n - 1 );
+            c[k] = gix[i];
+            p[k] = gx[i];
+            v[k] = gu[i];
+        }
+#pragma unroll
+        for ( int k = 0; k < U; k++ ) {
+            const size_t i = base + (size_t) k * 256;
+            if ( i < n ) {
+                c[k].x ^= 1;
+                p[k].x = p[k].x + 1.0f;
+                v[k].z = v[k].z + 1.0f;
+                gix[i] = c[k];
+                gx[i] = p[k];
+                gu[i] = v[k];
+            }
+        }
     }
 }
 
@@ -1528,6 +1880,14 @@ int zpic_tile_sort_copy( zpic_particles * p, zpic_particles * tmp, zpic_sort * s
         ZPIC_LAUNCH_CHECK( "k_sort_out (copy_sorted)" );
         return ZPIC_OK;
     }
+    if ( a.S.stage_ix ) {
+        // order-free placement from the particle pass's staging
+        hipLaunchKernelGGL( k_sort_shift, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
+        ZPIC_LAUNCH_CHECK( "k_sort_shift" );
+        hipLaunchKernelGGL( k_sort_in_fast, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
+        ZPIC_LAUNCH_CHECK( "k_sort_in_fast" );
+        return ZPIC_OK;
+    }
     hipLaunchKernelGGL( k_sort_out<false>, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
     ZPIC_LAUNCH_CHECK( "k_sort_out" );
     hipLaunchKernelGGL( k_sort_in, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
@@ -1550,6 +1910,14 @@ int zpic_set_threads_per_tile( int threads ) {
 }
 
 int zpic_get_threads_per_tile( void ) { return g_threads; }
+
+int zpic_set_sort_order( int reference ) {
+    if ( reference != 0 && reference != 1 ) return set_error( ZPIC_EINVAL, "sort order must be 0 or 1" );
+    g_reference_order = reference;
+    return ZPIC_OK;
+}
+
+int zpic_get_sort_order( void ) { return g_reference_order; }
 
 int zpic_fitted_threads_per_tile( int threads, int tile_vol ) {
     if ( threads < WAVE || threads > FAST_BLOCK || threads % WAVE || tile_vol < 1 ) return 0;
@@ -1603,6 +1971,34 @@ int zpic_particle_stream_probe( int * ix, float * x, float * u, size_t n, void *
     hipLaunchKernelGGL( k_stream_probe, dim3( blocks ), dim3( 256 ), 0, as_stream( stream ), (int2 *) ix, (float2 *) x,
                         (float3 *) u, n );
     ZPIC_LAUNCH_CHECK( "k_stream_probe" );
+    return ZPIC_OK;
+}
+
+int zpic_bandwidth_probe( int kind, void * a, void * b, void * c, size_t n, int loads_per_lane, int blocks,
+                          void * stream ) {
+    if ( kind < 0 || kind > 1 ) return set_error( ZPIC_EINVAL, "probe kind must be 0 (copy) or 1 (particle records)" );
+    if ( ( !a || !b || ( kind == 1 && !c ) ) && n ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( blocks < 1 ) return set_error( ZPIC_EINVAL, "blocks must be positive" );
+    if ( n == 0 ) return ZPIC_OK;
+    hipStream_t st = as_stream( stream );
+#define ZPIC_PROBE( U )                                                                                                \
+    case U:                                                                                                            \
+        if ( kind == 0 )                                                                                               \
+            hipLaunchKernelGGL( k_copy_probe<U>, dim3( blocks ), dim3( 256 ), 0, st, (const float4 *) a, (float4 *) b, \
+                                n );                                                                                   \
+        else                                                                                                           \
+            hipLaunchKernelGGL( k_stream_probe_u<U>, dim3( blocks ), dim3( 256 ), 0, st, (int2 *) a, (float2 *) b,     \
+                                (float3 *) c, n );                                                                     \
+        break;
+    switch ( loads_per_lane ) {
+        ZPIC_PROBE( 1 )
+        ZPIC_PROBE( 2 )
+        ZPIC_PROBE( 4 )
+        ZPIC_PROBE( 8 )
+    default: return set_error( ZPIC_EINVAL, "loads_per_lane must be 1, 2, 4 or 8" );
+    }
+#undef ZPIC_PROBE
+    ZPIC_LAUNCH_CHECK( "bandwidth probe" );
     return ZPIC_OK;
 }
 

@@ -84,6 +84,8 @@ def lib():
         "zpic_batch_advance": (i, [vp, i]),
         "zpic_set_threads_per_tile": (i, [i]),
         "zpic_get_threads_per_tile": (i, []),
+        "zpic_set_sort_order": (i, [i]),
+        "zpic_get_sort_order": (i, []),
         "zpic_emf_save": (i, [vp, i, i]),
         "zpic_current_save": (i, [vp, i]),
         "zpic_field_save_file": (i, [vp, i, i, ctypes.c_char_p]),
@@ -689,6 +691,16 @@ def set_threads_per_tile(n):
 
 def get_threads_per_tile():
     return int(lib().zpic_get_threads_per_tile())
+
+
+def set_sort_order(reference):
+    """Fast-mode tile sort: False (default) order-free, True the reference's
+    within-tile order (the deterministic mode always keeps it)."""
+    _chk(lib().zpic_set_sort_order(1 if reference else 0))
+
+
+def get_sort_order():
+    return bool(lib().zpic_get_sort_order())
 
 
 def weibel(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), box=None, dt=0.07, uth=(0.1, 0.1, 0.1),
