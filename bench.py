@@ -34,10 +34,10 @@ PEAK_HBM_GBS = 8000.0
 WORKLOADS = {
     "weibel2048": dict(ntiles=(128, 128), nx=(16, 16), ppc=(8, 8), filter=("binomial", 0, 1)),
     "weibel128": dict(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), filter=None),
-    # 64 independent Weibel instances advanced as one batch (one launch per
-    # stage, one hipGraph launch per step): grids drawn from {64, 128, 256,
+    # 64 independent Weibel instances advanced as one batch (one persistent
+    # launch per hipGraph-captured step): grids drawn from {64, 128, 256,
     # 512}^2, ppc from {2, 4, 8}^2, drift uz from U[0.2, 0.9], uth from
-    # U[0.02, 0.2], dt 0.07, seed 2024
+    # U[0.02, 0.2], dt 0.07, seed 2024, autotuned tiles
     "batch64": dict(instances=64, seed=2024),
     # laser wakefield with a moving window: 4096x512 cells (dx 0.05, dy 0.2,
     # dt 0.04), cold electrons (n0 0.01, ppc 4x2) behind a linear density ramp
@@ -162,6 +162,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="weibel2048", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-mode", choices=("persistent", "stage"), default="persistent",
+                    help="batch64: one persistent launch per step (default) or one launch per stage")
+    ap.add_argument("--untuned", action="store_true", help="batch64: 16x16 tiles instead of the autotuner's")
+    ap.add_argument("--sort-order", choices=("free", "reference"), default="free",
+                    help="fast-mode tile sort: order-free placement (default) or the reference's within-tile order")
     ap.add_argument("--threads", type=int, default=0,
                     help="threads per tile of the particle pass (multiple of 64 in [64, 512]; 0 = library default)")
     ap.add_argument("--bandwidth", action="store_true",
@@ -195,6 +200,7 @@ def main():
 
     if args.threads:
         pyzpic.set_threads_per_tile(args.threads)
+    pyzpic.set_sort_order(args.sort_order == "reference")
     if args.workload == "batch64":
         return run_batch(args, pyzpic)
     if args.workload == "lwfa":
@@ -393,45 +399,47 @@ def run_lwfa(args, pyzpic):
     print(json.dumps(result), flush=True)
 
 
-def batch_instances(n, seed):
-    rng = np.random.RandomState(seed)
-    out = []
-    for _ in range(n):
-        g = int(rng.choice([64, 128, 256, 512]))
-        p = int(rng.choice([2, 4, 8]))
-        uz = float(rng.uniform(0.2, 0.9))
-        uth = float(rng.uniform(0.02, 0.2))
-        out.append(dict(ntiles=(g // 16, g // 16), nx=(16, 16), ppc=(p, p), uth=(uth, uth, uth), ufl=(0.0, 0.0, uz)))
-    return out
-
-
 def run_batch(args, pyzpic):
+    """The batched parameter scan: 64 Weibel instances (pyzpic.batch64_instances,
+    seed 2024), tile shape and threads per tile of each (grid, ppc) from the
+    autotuner (timed on a 128x128 sample, 2 steps), advanced by one persistent
+    launch per hipGraph-captured step (--batch-mode stage: one launch per
+    stage instead)."""
     wl = WORKLOADS["batch64"]
-    inst = batch_instances(wl["instances"], wl["seed"])
-    print(f"[bench] building batch64: {len(inst)} instances", file=sys.stderr, flush=True)
+    inst = pyzpic.batch64_instances(wl["instances"], wl["seed"])
+    t_tune = time.perf_counter()
+    if not args.untuned:
+        pyzpic.tune_instances(inst)
+    t_tune = time.perf_counter() - t_tune
+    print(f"[bench] building batch64: {len(inst)} instances (autotune {t_tune:.1f} s)", file=sys.stderr, flush=True)
     t_init = time.perf_counter()
-    sims = [pyzpic.weibel(ntiles=c["ntiles"], nx=c["nx"], ppc=c["ppc"], uth=c["uth"], ufl=c["ufl"]) for c in inst]
-    batch = pyzpic.Batch(sims, threads_per_tile=pyzpic.get_threads_per_tile(), graph=True)
+    sims = [pyzpic.scan_simulation(c) for c in inst]
+    batch = pyzpic.Batch(threads_per_tile=pyzpic.get_threads_per_tile(), graph=True,
+                         persistent=args.batch_mode == "persistent")
+    for sim, c in zip(sims, inst):
+        batch.add(sim, c.get("threads", 0))
     for s in sims:
         s.sync()
     t_init = time.perf_counter() - t_init
     npart = sum(sp.np_total() for s in sims for sp in s.species)
-    ncells = sum(c["ntiles"][0] * c["ntiles"][1] * 256 for c in inst)
+    ncells = sum(c["grid"][0] * c["grid"][1] for c in inst)
     print(f"[bench] {npart} particles, {ncells} cells, init {t_init:.1f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
     batch.advance(args.warmup)
-    sims[0].sync()
+    batch.status()
     n0 = sum(s.get_nmove() for s in sims)
     t0 = time.perf_counter()
     batch.advance(args.steps)
     sims[0].sync()
     t1 = time.perf_counter()
+    batch.status()
     moved = sum(s.get_nmove() for s in sims) - n0
     sec = t1 - t0
     ms_step = 1e3 * sec / args.steps
     print(f"[bench] {args.steps} steps, {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
     step_bytes = 56.0 * npart + 96.0 * ncells
     achieved = step_bytes / (ms_step * 1e-3) / 1e9
+    shapes = sorted({(c["grid"][0], c["ppc"][0], tuple(c.get("nx", (16, 16))), c.get("threads", 0)) for c in inst})
     result = {
         "metric": METRIC,
         "value": moved / (sec * 1e9),
@@ -446,9 +454,13 @@ def run_batch(args, pyzpic):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "batch64", "instances": len(inst), "seed": wl["seed"], "particles": npart,
-                   "cells": ncells, "grids": "U{64,128,256,512}^2", "ppc": "U{2,4,8}^2", "tile": [16, 16],
-                   "graph": True, "threads_per_tile": pyzpic.get_threads_per_tile(), "parallelism": "single GPU"},
-        "roofline": {"kernel": "whole step (batched launches)", "bound": "hbm", "achieved": achieved,
+                   "cells": ncells, "grids": "U{64,128,256,512}^2", "ppc": "U{2,4,8}^2",
+                   "tiles": "autotuned" if not args.untuned else "16x16",
+                   "tuned (grid, ppc, tile, threads)": [list(map(str, x)) for x in shapes],
+                   "graph": True, "mode": args.batch_mode, "parallelism": "single GPU"},
+        "roofline": {"kernel": "whole step (%s)" % ("one persistent launch" if args.batch_mode == "persistent"
+                                                      else "one launch per stage"),
+                     "bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "bytes_per_step": step_bytes},
         "cpu_baseline": None,

@@ -245,6 +245,7 @@ int zpic_yee(float *E, float *B, const float *J, const zpic_geom *g, float dt_dx
 int zpic_field_energy(const float *E, const float *B, const zpic_geom *g, double *tile_scratch,
                       double *out6, void *stream);
 
+
 /* Sum of per-tile doubles (the OpenMP reduction of Species::push,
  * species.cpp:2311): in tile order when deterministic != 0 (the reference's
  * single-thread result), as a parallel tree otherwise.  out is a device
@@ -377,6 +378,8 @@ typedef struct {
     double *energy_out;     /* [1] device: sum of the push energies      */
     float alpha, dt_dx, dt_dy, q, qnx, qny;
     int periodic_x, periodic_y;
+    int threads;            /* threads per tile of this item's particle pass
+                               (fitted to its tiles); 0: the plan's          */
 } zpic_batch_species;
 
 typedef struct zpic_batch_particles zpic_batch_particles;
@@ -410,6 +413,29 @@ int zpic_batch_current_advance(zpic_batch_fields *plan, void *stream);
 /* EMF::advance(current) of every instance (Yee + guard copy of E, B). */
 int zpic_batch_yee(zpic_batch_fields *plan, void *stream);
 void zpic_batch_fields_free(zpic_batch_fields *plan);
+
+/* The whole step of every instance in one persistent launch (no reference
+ * counterpart; Simulation::advance, simulation.h:90-107, of each instance).
+ * species[k * n + i] is species slot k of instance i (nslots slots),
+ * fields[i] the fields of instance i.  Every instance runs the particle
+ * pass with the threads per tile that fit its tiles (as
+ * zpic_fitted_threads_per_tile(threads_per_tile, tile_vol)), so each computes
+ * what its own Simulation::advance computes with that setting.  Fast mode
+ * with the order-free sort (every species item's stage buffers set) and
+ * periodic boundaries only.  Workgroups walk a list of (stage, instance,
+ * tiles) work items in dependency order and wait on per-(stage, instance)
+ * completion counters, so instances overlap; advance enqueues a counter
+ * reset and the persistent kernel (two launches, graph-capturable).
+ * status synchronises the device and reports ZPIC_EHIP if a dependency wait
+ * ever exceeded its spin limit (the results of that step are then invalid);
+ * info reports the launch shape (workgroups, threads) and the work items. */
+typedef struct zpic_batch_step zpic_batch_step;
+int zpic_batch_step_create(zpic_batch_step **plan, const zpic_batch_species *species, int nslots,
+                           const zpic_batch_fields_item *fields, int n, int threads_per_tile);
+int zpic_batch_step_advance(zpic_batch_step *plan, void *stream);
+int zpic_batch_step_status(zpic_batch_step *plan);
+int zpic_batch_step_info(const zpic_batch_step *plan, int *workgroups, int *threads, int *items);
+void zpic_batch_step_free(zpic_batch_step *plan);
 
 /* --------------------------------------------- object entry points (2) */
 typedef struct zpic_simulation zpic_simulation;
@@ -677,9 +703,13 @@ int zpic_current_get_fallback(zpic_simulation *sim);
 int zpic_simulation_copy_fields(zpic_simulation *dst, const zpic_simulation *src);
 
 /* Every stride-th particle (from index phase) of each tile of src, in tile
- * order, into dst (same tiling); dst's counts and offsets are rewritten.
- * Species form: the species' particle buffers. */
-int zpic_subsample(const zpic_particles *src, zpic_particles *dst, int ntiles, int stride, int phase, void *stream);
+ * order, into dst; dst's counts and offsets are rewritten.  src_geom and
+ * dst_geom describe the two species' tilings (tile counts and cells per
+ * tile), which must be equal (ZPIC_EINVAL otherwise): the tile tables of both
+ * hold ntiles_x * ntiles_y entries.  Species form: the species' particle
+ * buffers. */
+int zpic_subsample(const zpic_particles *src, const zpic_geom *src_geom, zpic_particles *dst,
+                   const zpic_geom *dst_geom, int stride, int phase, void *stream);
 int zpic_species_subsample(zpic_species *dst, zpic_species *src, int stride, int phase);
 
 /* Device synchronisation of the simulation's stream. */
@@ -693,7 +723,17 @@ typedef struct zpic_batch zpic_batch;
 zpic_batch *zpic_batch_new(int threads_per_tile);
 void zpic_batch_free(zpic_batch *b);
 int zpic_batch_add(zpic_batch *b, zpic_simulation *sim);
+/* As zpic_batch_add, with this instance's threads per tile (0 = the
+ * batch's): its particle pass runs as its own Simulation::advance does
+ * after zpic_set_threads_per_tile(threads_per_tile). */
+int zpic_batch_add_threads(zpic_batch *b, zpic_simulation *sim, int threads_per_tile);
 int zpic_batch_set_graph(zpic_batch *b, int on);
+/* on = 1: every step is one persistent launch over all instances
+ * (zpic_batch_step; fast mode with the order-free sort, periodic
+ * boundaries); on = 0 (default): one launch per stage. */
+int zpic_batch_set_persistent(zpic_batch *b, int on);
+/* Synchronises; ZPIC_EHIP if a persistent step reported an error. */
+int zpic_batch_status(zpic_batch *b);
 int zpic_batch_advance(zpic_batch *b, int n);
 
 #ifdef __cplusplus

@@ -78,3 +78,20 @@ def test_large_tiles_fit_lds():
     assert 64 <= t < 512 and t % 64 == 0
     assert pyzpic.fitted_threads_per_tile(512, 364) == 512
     assert pyzpic.fitted_threads_per_tile(100, 364) == 0
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_subsample_rejects_mismatched_tilings():
+    # the tile tables of both species are sized by their own geometry: a
+    # destination with fewer tiles is refused before any device access
+    import pyzpic
+    L = pyzpic.lib()
+    gs, gd = pyzpic.Geom(), pyzpic.Geom()
+    assert L.zpic_geom_init(ctypes.byref(gs), 4, 4, 16, 16, 0, 0) == 0
+    assert L.zpic_geom_init(ctypes.byref(gd), 2, 4, 16, 16, 0, 0) == 0
+    buf = (ctypes.c_byte * 64)()
+    L.zpic_subsample.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    rc = L.zpic_subsample(ctypes.addressof(buf), ctypes.addressof(gs), ctypes.addressof(buf),
+                          ctypes.addressof(gd), 4, 0, None)
+    assert rc == -1  # ZPIC_EINVAL
+    assert b"same tiling" in L.zpic_last_error()

@@ -18,6 +18,10 @@ SHAPES = [
     dict(ntiles=(6, 3), nx=(16, 16), ppc=(4, 4), filter=None),
     dict(ntiles=(2, 5), nx=(16, 16), ppc=(3, 2), filter=("compensated", 1, 1)),
     dict(ntiles=(1, 1), nx=(16, 16), ppc=(2, 3), filter=("binomial", 1, 2)),
+    # tiles of other sizes in the same batch: 32x32 tiles fit fewer waves in
+    # LDS than the batch's workgroup holds, 8x8 tiles all of them
+    dict(ntiles=(2, 2), nx=(32, 32), ppc=(2, 2), filter=("binomial", 0, 1)),
+    dict(ntiles=(4, 6), nx=(8, 8), ppc=(4, 4), filter=None),
 ]
 
 
@@ -77,20 +81,22 @@ def singles(zp, threads, steps):
         zp.set_threads_per_tile(old)
 
 
+@pytest.mark.parametrize("persistent", [False, True])
 @pytest.mark.parametrize("threads", [256, 512])
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_matches_single_instances(zp, threads, graph):
+def test_batch_matches_single_instances(zp, threads, graph, persistent):
     steps = 12
     ref = singles(zp, threads, steps)
     sims = [make(zp, sh) for sh in SHAPES]
-    batch = zp.Batch(sims, threads_per_tile=threads, graph=graph)
-    say(f"threads={threads} graph={graph}: {len(sims)} instances, {steps} steps")
+    batch = zp.Batch(sims, threads_per_tile=threads, graph=graph, persistent=persistent)
+    say(f"threads={threads} graph={graph} persistent={persistent}: {len(sims)} instances, {steps} steps")
     # Replays are interleaved with eager work and host reads.
     for step in range(steps):
         batch.advance(1)
         if step % 4 == 3:
             for s in sims:
                 s.energies()
+    batch.status()
     for k, (s, (st, en)) in enumerate(zip(sims, ref)):
         assert_same(state(s), st, f"instance {k}")
         np.testing.assert_allclose(s.energies(), en, rtol=1e-12, atol=1e-30, err_msg=f"instance {k} energies")
@@ -133,3 +139,71 @@ def test_batch_rejects_bad_members(zp):
     b.close()
     for s in (a, det, one):
         s.close()
+
+
+def test_batch_mixed_threads_per_instance(zp):
+    """Instances added with threads per tile of their own (as the autotuner
+    chooses them) equal their single runs with those settings, in the
+    per-stage and in the persistent batch."""
+    steps = 6
+    threads = [128, 256, 64, 512, 256, 192]
+    refs = []
+    old = zp.get_threads_per_tile()
+    try:
+        for sh, th in zip(SHAPES, threads):
+            zp.set_threads_per_tile(th)
+            sim = make(zp, sh)
+            sim.advance(steps)
+            refs.append(state(sim))
+            sim.close()
+    finally:
+        zp.set_threads_per_tile(old)
+    for persistent in (False, True):
+        sims = [make(zp, sh) for sh in SHAPES]
+        batch = zp.Batch(threads_per_tile=256, persistent=persistent)
+        for sim, th in zip(sims, threads):
+            batch.add(sim, th)
+        batch.advance(steps)
+        batch.status()
+        for k, (s, st) in enumerate(zip(sims, refs)):
+            assert_same(state(s), st, f"persistent={persistent} instance {k}")
+        batch.close()
+        for s in sims:
+            s.close()
+
+
+def test_persistent_batch_graph_replays(zp):
+    """The captured persistent step replayed many times, with host reads in
+    between, equals the eager persistent step."""
+    steps = 15
+    eager = [make(zp, sh) for sh in SHAPES]
+    graphed = [make(zp, sh) for sh in SHAPES]
+    be = zp.Batch(eager, threads_per_tile=256, graph=False, persistent=True)
+    bg = zp.Batch(graphed, threads_per_tile=256, graph=True, persistent=True)
+    for step in range(steps):
+        be.advance(1)
+        bg.advance(1)
+        for a, b in zip(eager, graphed):
+            np.testing.assert_array_equal(a.energies(), b.energies(), err_msg=f"step {step}")
+    be.status()
+    bg.status()
+    for k, (a, b) in enumerate(zip(eager, graphed)):
+        assert_same(state(a), state(b), f"instance {k}")
+    be.close()
+    bg.close()
+    for s in eager + graphed:
+        s.close()
+
+
+def test_persistent_batch_rejects_reference_order(zp):
+    """The persistent step places particles with the order-free sort only."""
+    a = make(zp, SHAPES[0])
+    zp.set_sort_order(True)
+    try:
+        b = zp.Batch([a], persistent=True)
+        with pytest.raises(zp.ZpicError, match="order-free"):
+            b.advance(1)
+        b.close()
+    finally:
+        zp.set_sort_order(False)
+    a.close()

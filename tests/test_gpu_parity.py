@@ -444,7 +444,8 @@ def test_charge_continuity_gpu(zp):
 
     # the reference's dt_dx: float dt / float dx (species.cpp:2140-2148)
     dtdx32 = float(np.float32(0.07) / (np.float32(3.2) / np.float32(32)))
-    dtdx64 = 0.07 / (np.float64(np.float32(3.2)) / 32)
+    # the FP64 build's dt and dx: the float time step and box widened to double
+    dtdx64 = float(np.float32(0.07)) / (float(np.float32(3.2)) / 32)
     res_gpu = np.abs(continuity_residual(r0, r1, J, dtdx32)).max()
     res_32 = np.abs(continuity_residual(r0_32, r1_32, J32, dtdx32)).max()
     res_64 = np.abs(continuity_residual(r0_64, r1_64, J64, dtdx64)).max()
@@ -505,3 +506,29 @@ def test_error_paths(zp):
         zp.Species("bad", 0.0, (2, 2))
     with pytest.raises(zp.ZpicError, match="ppc"):
         zp.Species("bad", 1.0, (0, 2))
+
+
+def test_set_particles_rejects_bad_input(zp):
+    """Loading a state whose counts are negative or whose cell indices lie
+    outside their tile is refused on the host (the particle pass stages only
+    the tile's own cells and guards in LDS); a valid state still loads."""
+    sim = gpu_weibel(zp, cfg_of((2, 2), (16, 16), (2, 2)))
+    s = sim.species[0]
+    p = s.particles()
+    bad = p["ix"].copy()
+    bad[5, 0] = 16
+    with pytest.raises(zp.ZpicError, match="outside its tile"):
+        s.set_particles(p["np"], bad, p["x"], p["u"])
+    bad = p["ix"].copy()
+    bad[7, 1] = -1
+    with pytest.raises(zp.ZpicError, match="outside its tile"):
+        s.set_particles(p["np"], bad, p["x"], p["u"])
+    npn = p["np"].copy()
+    npn[1] = -npn[1]
+    with pytest.raises(zp.ZpicError, match="negative"):
+        s.set_particles(npn, p["ix"], p["x"], p["u"])
+    s.set_particles(p["np"], p["ix"], p["x"], p["u"])
+    q = s.particles()
+    for name in ("np", "offset", "ix", "x", "u"):
+        assert same_bits(p[name], q[name]), name
+    sim.close()

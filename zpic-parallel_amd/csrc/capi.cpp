@@ -461,11 +461,13 @@ zpic_batch * zpic_batch_new( int threads_per_tile ) {
 
 void zpic_batch_free( zpic_batch * b ) { delete b; }
 
-int zpic_batch_add( zpic_batch * b, zpic_simulation * s ) {
+int zpic_batch_add( zpic_batch * b, zpic_simulation * s ) { return zpic_batch_add_threads( b, s, 0 ); }
+
+int zpic_batch_add_threads( zpic_batch * b, zpic_simulation * s, int threads_per_tile ) {
     return guarded( [&] {
         if ( !b || !s ) return zpic::set_error( ZPIC_EINVAL, "null handle" );
         if ( s->deterministic ) return zpic::set_error( ZPIC_EINVAL, "batches run the fast mode only" );
-        b->b->add( *s->sim );
+        b->b->add( *s->sim, threads_per_tile );
         return ZPIC_OK;
     } );
 }
@@ -474,6 +476,22 @@ int zpic_batch_set_graph( zpic_batch * b, int on ) {
     if ( !b ) return zpic::set_error( ZPIC_EINVAL, "null batch" );
     b->b->set_graph( on != 0 );
     return ZPIC_OK;
+}
+
+int zpic_batch_set_persistent( zpic_batch * b, int on ) {
+    return guarded( [&] {
+        if ( !b ) return zpic::set_error( ZPIC_EINVAL, "null batch" );
+        b->b->set_persistent( on != 0 );
+        return ZPIC_OK;
+    } );
+}
+
+int zpic_batch_status( zpic_batch * b ) {
+    return guarded( [&] {
+        if ( !b ) return zpic::set_error( ZPIC_EINVAL, "null batch" );
+        b->b->check_status();
+        return ZPIC_OK;
+    } );
 }
 
 int zpic_batch_advance( zpic_batch * b, int n ) {

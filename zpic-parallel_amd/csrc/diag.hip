@@ -199,10 +199,17 @@ __global__ void __launch_bounds__( 256 ) k_subsample( const int * __restrict__ s
 }  // namespace
 }  // namespace zpic
 
-extern "C" int zpic_subsample( const zpic_particles * src, zpic_particles * dst, int ntiles, int stride, int phase,
-                               void * stream ) {
-    if ( !src || !dst || ntiles < 1 || stride < 1 || phase < 0 || phase >= stride )
+extern "C" int zpic_subsample( const zpic_particles * src, const zpic_geom * src_geom, zpic_particles * dst,
+                               const zpic_geom * dst_geom, int stride, int phase, void * stream ) {
+    if ( !src || !dst || !src_geom || !dst_geom || stride < 1 || phase < 0 || phase >= stride )
         return set_error( ZPIC_EINVAL, "invalid subsample request" );
+    if ( src_geom->ntiles_x != dst_geom->ntiles_x || src_geom->ntiles_y != dst_geom->ntiles_y ||
+         src_geom->nx != dst_geom->nx || src_geom->ny != dst_geom->ny )
+        return set_error( ZPIC_EINVAL, "subsample needs species with the same tiling" );
+    const long long nt = (long long) src_geom->ntiles_x * src_geom->ntiles_y;
+    if ( src_geom->ntiles_x < 1 || src_geom->ntiles_y < 1 || nt > ( 1 << 30 ) )
+        return set_error( ZPIC_EINVAL, "invalid tile count" );
+    const int ntiles = (int) nt;
     hipStream_t st = as_stream( stream );
     std::vector<int> np( ntiles ), off( ntiles ), dnp( ntiles ), doff( ntiles );
     ZPIC_HIP_TRY( hipMemcpyAsync( np.data(), src->np, sizeof( int ) * ntiles, hipMemcpyDeviceToHost, st ) );

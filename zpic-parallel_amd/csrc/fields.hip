@@ -5,6 +5,7 @@
 // (built with -ffp-contract=off), so every result is bit-identical to the
 // reference's.
 #include "common.h"
+#include "grid_ops.h"
 
 #include <algorithm>
 #include <map>
@@ -25,45 +26,7 @@ int set_error( int code, const std::string & msg ) {
 
 namespace {
 
-struct G {
-    int ntx, nty, nx, ny, gl, gu, ext_x, ext_y, vol, off;
-};
-
-G make( const zpic_geom * g ) {
-    return G{ g->ntiles_x, g->ntiles_y, g->nx, g->ny, g->gc_lower, g->gc_upper,
-              g->ext_x, g->ext_y, g->tile_vol, g->offset };
-}
-
-__device__ inline int nbr( int tx, int ty, int dx, int dy, const G & g, int px, int py ) {
-    int x = tx + dx, y = ty + dy;
-    if ( px ) { if ( x < 0 ) x += g.ntx; if ( x >= g.ntx ) x -= g.ntx; }
-    if ( py ) { if ( y < 0 ) y += g.nty; if ( y >= g.nty ) y -= g.nty; }
-    if ( x < 0 || x >= g.ntx || y < 0 || y >= g.nty ) return -1;
-    return y * g.ntx + x;
-}
-
-// Arguments of one grid operand; batched launches pass a table of them and a
-// map from block to (entry, tile), single launches pass nullptr for both.
-struct GcArgs {
-    float * buf;
-    G g;
-    int px, py;
-};
-
-struct K3Args {
-    float * buf;
-    G g;
-    int dir;
-    float a, b, c;
-};
-
-struct YeeArgs {
-    float * E;
-    float * B;
-    const float * J;
-    G g;
-    float dtdx, dtdy, dt;
-};
+using namespace grid_ops;
 
 template <class A>
 __device__ inline int select_instance( A & a, const A * tab, const int2 * map ) {
@@ -73,93 +36,6 @@ __device__ inline int select_instance( A & a, const A * tab, const int2 * map ) 
         return r.y;
     }
     return blockIdx.x;
-}
-
-constexpr int GC_BLOCK = 128;
-
-// add_from_gc, x pass (grid.h:367-401): lower neighbour's upper guards into
-// the first interior columns, then upper neighbour's lower guards into the
-// last interior columns.
-template <int NC>
-__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( GcArgs a, const GcArgs * tab, const int2 * map ) {
-    const int t = select_instance( a, tab, map );
-    const gptr<float> buf = gbl( a.buf );
-    const G g = a.g;
-    const int px = a.px, py = a.py;
-    const int tx = t % g.ntx, ty = t / g.ntx;
-    const gptr<float> loc = buf + (size_t) t * g.vol * NC;
-    const int lo = nbr( tx, ty, -1, 0, g, px, py ), hi = nbr( tx, ty, +1, 0, g, px, py );
-    const int E = g.ext_x;
-    if ( lo >= 0 ) {
-        const gptr<const float> src = buf + (size_t) lo * g.vol * NC;
-        for ( int k = threadIdx.x; k < g.ext_y * g.gu * NC; k += blockDim.x ) {
-            const int c = k % NC, e = k / NC, i = e % g.gu, j = e / g.gu;
-            loc[( g.gl + i + j * E ) * NC + c] += src[( g.gl + g.nx + i + j * E ) * NC + c];
-        }
-    }
-    __syncthreads();
-    if ( hi >= 0 ) {
-        const gptr<const float> src = buf + (size_t) hi * g.vol * NC;
-        for ( int k = threadIdx.x; k < g.ext_y * g.gl * NC; k += blockDim.x ) {
-            const int c = k % NC, e = k / NC, i = e % g.gl, j = e / g.gl;
-            loc[( g.nx + i + j * E ) * NC + c] += src[( i + j * E ) * NC + c];
-        }
-    }
-}
-
-// add_from_gc, y pass (grid.h:406-444): full rows of the y neighbours' guards.
-template <int NC>
-__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( GcArgs a, const GcArgs * tab, const int2 * map ) {
-    const int t = select_instance( a, tab, map );
-    const gptr<float> buf = gbl( a.buf );
-    const G g = a.g;
-    const int px = a.px, py = a.py;
-    const int tx = t % g.ntx, ty = t / g.ntx;
-    const gptr<float> loc = buf + (size_t) t * g.vol * NC;
-    const int lo = nbr( tx, ty, 0, -1, g, px, py ), hi = nbr( tx, ty, 0, +1, g, px, py );
-    const int E = g.ext_x;
-    if ( lo >= 0 ) {
-        const gptr<const float> src = buf + (size_t) lo * g.vol * NC;
-        for ( int k = threadIdx.x; k < g.gu * E * NC; k += blockDim.x ) {
-            const int c = k % NC, e = k / NC, i = e % E, j = e / E;
-            loc[( i + ( g.gl + j ) * E ) * NC + c] += src[( i + ( g.gl + g.ny + j ) * E ) * NC + c];
-        }
-    }
-    __syncthreads();
-    if ( hi >= 0 ) {
-        const gptr<const float> src = buf + (size_t) hi * g.vol * NC;
-        for ( int k = threadIdx.x; k < g.gl * E * NC; k += blockDim.x ) {
-            const int c = k % NC, e = k / NC, i = e % E, j = e / E;
-            loc[( i + ( g.ny + j ) * E ) * NC + c] += src[( i + j * E ) * NC + c];
-        }
-    }
-}
-
-// copy_to_gc_x (grid.h:253-297) of tile t: x guard columns from the
-// neighbours' edges.
-template <int NC>
-__device__ inline void copy_gc_x_tile( const GcArgs & a, int t ) {
-    const gptr<float> buf = gbl( a.buf );
-    const G g = a.g;
-    const int px = a.px, py = a.py;
-    const int tx = t % g.ntx, ty = t / g.ntx;
-    const gptr<float> loc = buf + (size_t) t * g.vol * NC;
-    const int lo = nbr( tx, ty, -1, 0, g, px, py ), hi = nbr( tx, ty, +1, 0, g, px, py );
-    const int E = g.ext_x;
-    if ( lo >= 0 ) {
-        const gptr<const float> src = buf + (size_t) lo * g.vol * NC;
-        for ( int k = threadIdx.x; k < g.ext_y * g.gl * NC; k += blockDim.x ) {
-            const int c = k % NC, e = k / NC, i = e % g.gl, j = e / g.gl;
-            loc[( i + j * E ) * NC + c] = src[( g.nx + i + j * E ) * NC + c];
-        }
-    }
-    if ( hi >= 0 ) {
-        const gptr<const float> src = buf + (size_t) hi * g.vol * NC;
-        for ( int k = threadIdx.x; k < g.ext_y * g.gu * NC; k += blockDim.x ) {
-            const int c = k % NC, e = k / NC, i = e % g.gu, j = e / g.gu;
-            loc[( g.gl + g.nx + i + j * E ) * NC + c] = src[( g.gl + i + j * E ) * NC + c];
-        }
-    }
 }
 
 template <int NC>
@@ -174,172 +50,39 @@ __global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_x2( GcArgs a0, GcArgs a1
     copy_gc_x_tile<NC>( blockIdx.y ? a1 : a0, blockIdx.x );
 }
 
-// copy_to_gc_y (grid.h:303-340): y guard rows (full width) from the neighbours.
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_xy( GcArgs a0, GcArgs a1 ) {
+    copy_gc_xy_tile<NC>( blockIdx.y ? a1 : a0, blockIdx.x );
+}
+
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_x( GcArgs a, const GcArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    add_gc_x_tile<NC>( a, t );
+}
+
+template <int NC>
+__global__ void __launch_bounds__( GC_BLOCK ) k_add_gc_y( GcArgs a, const GcArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    add_gc_y_tile<NC>( a, t );
+}
+
 template <int NC>
 __global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_y( GcArgs a, const GcArgs * tab, const int2 * map ) {
     const int t = select_instance( a, tab, map );
-    const gptr<float> buf = gbl( a.buf );
-    const G g = a.g;
-    const int px = a.px, py = a.py;
-    const int tx = t % g.ntx, ty = t / g.ntx;
-    const gptr<float> loc = buf + (size_t) t * g.vol * NC;
-    const int lo = nbr( tx, ty, 0, -1, g, px, py ), hi = nbr( tx, ty, 0, +1, g, px, py );
-    const int E = g.ext_x;
-    if ( lo >= 0 ) {
-        const gptr<const float> src = buf + (size_t) lo * g.vol * NC;
-        for ( int k = threadIdx.x; k < g.gl * E * NC; k += blockDim.x ) {
-            const int c = k % NC, e = k / NC, i = e % E, j = e / E;
-            loc[( i + j * E ) * NC + c] = src[( i + ( g.ny + j ) * E ) * NC + c];
-        }
-    }
-    if ( hi >= 0 ) {
-        const gptr<const float> src = buf + (size_t) hi * g.vol * NC;
-        for ( int k = threadIdx.x; k < g.gu * E * NC; k += blockDim.x ) {
-            const int c = k % NC, e = k / NC, i = e % E, j = e / E;
-            loc[( i + ( g.gl + g.ny + j ) * E ) * NC + c] = src[( i + ( g.gl + j ) * E ) * NC + c];
-        }
-    }
+    copy_gc_y_tile<NC>( a, t );
 }
 
-// Where element (i, j) of tile t takes its value from in copy_to_gc_x
-// followed by copy_to_gc_y (grid.h:253-340): the y pass copies the y
-// neighbour's row (as its own x pass left it), the x pass the x neighbour's
-// column, so a guard element resolves first along y (tile row, row index),
-// then along x within that tile row.  Returns false when the element keeps
-// its own value (interior, or a guard at a non-periodic edge).  Every source
-// it names is an interior element or a guard element that keeps its value,
-// so one launch may copy all guards of all tiles at once.
-__device__ inline bool copy_source( const G & g, int px, int py, int t, int i, int j, int & st, int & si, int & sj ) {
-    int tx = t % g.ntx, ty = t / g.ntx;
-    si = i;
-    sj = j;
-    if ( j < g.gl ) {
-        const int n = nbr( tx, ty, 0, -1, g, px, py );
-        if ( n >= 0 ) { ty = n / g.ntx; sj = g.ny + j; }
-    } else if ( j >= g.gl + g.ny ) {
-        const int n = nbr( tx, ty, 0, +1, g, px, py );
-        if ( n >= 0 ) { ty = n / g.ntx; sj = j - g.ny; }
-    }
-    if ( i < g.gl ) {
-        const int n = nbr( tx, ty, -1, 0, g, px, py );
-        if ( n >= 0 ) { tx = n % g.ntx; si = g.nx + i; }
-    } else if ( i >= g.gl + g.nx ) {
-        const int n = nbr( tx, ty, +1, 0, g, px, py );
-        if ( n >= 0 ) { tx = n % g.ntx; si = i - g.nx; }
-    }
-    st = ty * g.ntx + tx;
-    return st != t || si != i || sj != j;
-}
-
-// copy_to_gc (x then y passes, grid.h:253-340) of up to two grids of the same
-// element count in one launch (blockIdx.y selects the grid): every guard
-// element of the tile copied from where copy_source resolves it.
-template <int NC>
-__global__ void __launch_bounds__( GC_BLOCK ) k_copy_gc_xy( GcArgs a0, GcArgs a1 ) {
-    const GcArgs a = blockIdx.y ? a1 : a0;
-    const G g = a.g;
-    const int t = blockIdx.x;
-    const gptr<float> buf = gbl( a.buf );
-    const int E = g.ext_x, gw = g.gl + g.gu;
-    const int nrow = gw * E, ncell = nrow + g.ny * gw;
-    for ( int k = threadIdx.x; k < ncell * NC; k += blockDim.x ) {
-        const int c = k % NC, e = k / NC;
-        int i, j;
-        if ( e < g.gl * E ) {
-            j = e / E;
-            i = e % E;
-        } else if ( e < nrow ) {
-            const int f = e - g.gl * E;
-            j = g.gl + g.ny + f / E;
-            i = f % E;
-        } else {
-            const int f = e - nrow, q = f % gw;
-            j = g.gl + f / gw;
-            i = q < g.gl ? q : g.nx + q;
-        }
-        int st, si, sj;
-        if ( copy_source( g, a.px, a.py, t, i, j, st, si, sj ) )
-            buf[( (size_t) t * g.vol + i + j * E ) * NC + c] = buf[( (size_t) st * g.vol + si + sj * E ) * NC + c];
-    }
-}
-
-constexpr int TILE_BLOCK = 256;
-
-// kernel3_x / kernel3_y (grid.h:495-593): B = A[-1]*a + A[0]*b + A[+1]*c per
-// component over all rows (x) or all columns (y) of the tile, A in LDS.
 __global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( K3Args args, const K3Args * tab, const int2 * map ) {
     extern __shared__ float A[];
     const int t = select_instance( args, tab, map );
-    const gptr<float> buf = gbl( args.buf );
-    const G g = args.g;
-    const int dir = args.dir;
-    const float a = args.a, b = args.b, c = args.c;
-    const gptr<float> tb = buf + (size_t) t * g.vol * 3;
-    for ( int k = threadIdx.x; k < 3 * g.vol; k += blockDim.x ) A[k] = tb[k];
-    __syncthreads();
-    const int E = g.ext_x;
-    const int step = ( dir == 0 ) ? 3 : 3 * E;
-    const int x0 = ( dir == 0 ) ? g.gl : 0, nxr = ( dir == 0 ) ? g.nx : E;
-    const int y0 = ( dir == 0 ) ? 0 : g.gl, nyr = ( dir == 0 ) ? g.ext_y : g.ny;
-    for ( int k = threadIdx.x; k < nxr * nyr * 3; k += blockDim.x ) {
-        const int comp = k % 3, e = k / 3;
-        const int ix = x0 + e % nxr, iy = y0 + e / nxr;
-        const int q = ( iy * E + ix ) * 3 + comp;
-        tb[q] = A[q - step] * a + A[q] * b + A[q + step] * c;
-    }
+    kernel3_tile( args, t, A );
 }
 
-// EMF::advance(Current&) per tile (emf.cpp:513-566, yee_b :67-81,
-// yeeJ_e :486-506): E and B of the tile in LDS planes, J from global.
 __global__ void __launch_bounds__( TILE_BLOCK ) k_yee( YeeArgs a, const YeeArgs * tab, const int2 * map ) {
     extern __shared__ float sm[];
     const int t = select_instance( a, tab, map );
-    const gptr<float> Eg = gbl( a.E );
-    const gptr<float> Bg = gbl( a.B );
-    const gptr<const float> Jg = gbl( a.J );
-    const G g = a.g;
-    const float dtdx = a.dtdx, dtdy = a.dtdy, dt = a.dt;
-    const int vol = g.vol, ys = g.ext_x;
-    float * Ex = sm; float * Ey = Ex + vol; float * Ez = Ey + vol;
-    float * Bx = Ez + vol; float * By = Bx + vol; float * Bz = By + vol;
-    const gptr<float> Et = Eg + (size_t) t * vol * 3;
-    const gptr<float> Bt = Bg + (size_t) t * vol * 3;
-    const gptr<const float> Jt = a.J ? Jg + (size_t) t * vol * 3 : nullptr;
-    for ( int k = threadIdx.x; k < 3 * vol; k += blockDim.x ) {
-        const int el = k / 3, c = k - 3 * el;
-        Ex[c * vol + el] = Et[k];
-        Bx[c * vol + el] = Bt[k];
-    }
-    __syncthreads();
-    const float hx = dtdx / 2, hy = dtdy / 2;
-    const int nb = ( g.nx + 2 ) * ( g.ny + 2 );
-    auto bstep = [&]() {
-        for ( int e = threadIdx.x; e < nb; e += blockDim.x ) {
-            const int ix = e % ( g.nx + 2 ) - 1, iy = e / ( g.nx + 2 ) - 1;
-            const int k = g.off + ix + iy * ys;
-            Bx[k] += ( -hy * ( Ez[k + ys] - Ez[k] ) );
-            By[k] += ( hx * ( Ez[k + 1] - Ez[k] ) );
-            Bz[k] += ( -hx * ( Ey[k + 1] - Ey[k] ) + hy * ( Ex[k + ys] - Ex[k] ) );
-        }
-    };
-    bstep();
-    __syncthreads();
-    for ( int e = threadIdx.x; e < nb; e += blockDim.x ) {
-        const int ix = e % ( g.nx + 2 ), iy = e / ( g.nx + 2 );
-        const int k = g.off + ix + iy * ys;
-        const float jx = Jt ? Jt[3 * k] : 0.0f, jy = Jt ? Jt[3 * k + 1] : 0.0f, jz = Jt ? Jt[3 * k + 2] : 0.0f;
-        Ex[k] += ( dtdy * ( Bz[k] - Bz[k - ys] ) ) - dt * jx;
-        Ey[k] += ( -dtdx * ( Bz[k] - Bz[k - 1] ) ) - dt * jy;
-        Ez[k] += ( dtdx * ( By[k] - By[k - 1] ) - dtdy * ( Bx[k] - Bx[k - ys] ) ) - dt * jz;
-    }
-    __syncthreads();
-    bstep();
-    __syncthreads();
-    for ( int k = threadIdx.x; k < 3 * vol; k += blockDim.x ) {
-        const int el = k / 3, c = k - 3 * el;
-        Et[k] = Ex[c * vol + el];
-        Bt[k] = Bx[c * vol + el];
-    }
+    yee_tile( a, t, sm );
 }
 
 // EMF::get_energy per tile (emf.cpp:680-705): one thread per (tile,

@@ -865,8 +865,21 @@ void Species::set_particles( const int * np, const int * ix, const float * x, co
     const int nt = particles->ntiles.x * particles->ntiles.y;
     std::vector<int> off( nt );
     size_t n = 0;
-    for ( int t = 0; t < nt; t++ ) { off[t] = (int) n; n += np[t]; }
+    for ( int t = 0; t < nt; t++ ) {
+        if ( np[t] < 0 ) throw std::invalid_argument( "(*error*) negative particle count for a tile" );
+        off[t] = (int) n;
+        n += np[t];
+    }
     if ( n > particles->max_part ) throw std::runtime_error( "(*error*) too many particles for the buffer" );
+    // The particle pass stages a tile's fields and current in LDS around its
+    // own cells and the sort moves particles at most one tile per step: every
+    // cell index must lie inside its tile.
+    const int nxt = (int) particles->nx.x, nyt = (int) particles->nx.y;
+    for ( size_t k = 0; k < n; k++ ) {
+        const int cx = ix[2 * k], cy = ix[2 * k + 1];
+        if ( cx < 0 || cx >= nxt || cy < 0 || cy >= nyt )
+            throw std::invalid_argument( "(*error*) particle cell index outside its tile" );
+    }
     to_device( particles->np, np, nt );
     to_device( particles->offset, off.data(), nt );
     to_device( particles->ix, reinterpret_cast<const int2 *>( ix ), n );
@@ -881,7 +894,12 @@ void Species::subsample_from( const Species & src, int stride, int phase ) {
          src.particles->nx.x != particles->nx.x || src.particles->nx.y != particles->nx.y )
         throw std::runtime_error( "(*error*) subsample needs species with the same tiling" );
     zpic_particles s = src.particles->view(), d = particles->view();
-    check( zpic_subsample( &s, &d, particles->ntiles.x * particles->ntiles.y, stride, phase, stream() ) );
+    zpic_geom gs{}, gd{};
+    check( zpic_geom_init( &gs, (int) src.particles->ntiles.x, (int) src.particles->ntiles.y,
+                           (int) src.particles->nx.x, (int) src.particles->nx.y, 0, 0 ) );
+    check( zpic_geom_init( &gd, (int) particles->ntiles.x, (int) particles->ntiles.y, (int) particles->nx.x,
+                           (int) particles->nx.y, 0, 0 ) );
+    check( zpic_subsample( &s, &gs, &d, &gd, stride, phase, stream() ) );
     np_bound = particles->np_total();
 }
 
@@ -1064,7 +1082,7 @@ void Simulation::energy_info() {
 
 // ------------------------------------------------------------------- Batch
 zpic_batch_species Species::batch_item( EMF const & emf, Current & current ) const {
-    zpic_batch_species it;
+    zpic_batch_species it{};
     it.p = particles->view();
     it.tmp = tmp->view();
     it.s = sort->buf;
@@ -1117,7 +1135,8 @@ zpic_batch_fields_item Simulation::batch_fields() const {
 }
 
 Batch::Batch( int threads_per_tile )
-    : fields( nullptr ), threads( threads_per_tile ), use_graph( false ), graph( nullptr ), exec( nullptr ) {}
+    : fields( nullptr ), persistent_step( nullptr ), threads( threads_per_tile ), use_graph( false ),
+      persistent( false ), graph( nullptr ), exec( nullptr ) {}
 
 Batch::~Batch() { release(); }
 
@@ -1130,9 +1149,13 @@ void Batch::release() {
     slots.clear();
     zpic_batch_fields_free( fields );
     fields = nullptr;
+    zpic_batch_step_free( persistent_step );
+    persistent_step = nullptr;
 }
 
-void Batch::add( Simulation & sim ) {
+void Batch::add( Simulation & sim, int threads_per_tile ) {
+    if ( threads_per_tile != 0 && ( threads_per_tile < 64 || threads_per_tile > 512 || threads_per_tile % 64 ) )
+        throw std::invalid_argument( "(*error*) threads per tile must be 0 or a multiple of 64 in [64, 512]" );
     if ( !sims.empty() && sim.species.size() != sims[0]->species.size() )
         throw std::runtime_error( "(*error*) all instances of a batch must hold the same number of species" );
     auto periodic = []( int lo, int hi ) { return lo == 1 && hi == 1; };
@@ -1147,6 +1170,7 @@ void Batch::add( Simulation & sim ) {
     if ( !ok ) throw std::runtime_error( "(*error*) batches run periodic boundaries only" );
     release();
     sims.push_back( &sim );
+    sim_threads.push_back( threads_per_tile );
 }
 
 void Batch::set_graph( bool on ) {
@@ -1157,13 +1181,40 @@ void Batch::set_graph( bool on ) {
     use_graph = on;
 }
 
+void Batch::set_persistent( bool on ) {
+    release();
+    persistent = on;
+}
+
+void Batch::check_status() {
+    if ( persistent_step ) check( zpic_batch_step_status( persistent_step ) );
+}
+
 void Batch::build() {
     if ( sims.empty() ) throw std::runtime_error( "(*error*) empty batch" );
     release();
     const size_t nsp = sims[0]->species.size();
+    if ( persistent ) {
+        std::vector<zpic_batch_species> items;
+        for ( size_t k = 0; k < nsp; k++ )
+            for ( size_t i = 0; i < sims.size(); i++ ) {
+                Simulation * s = sims[i];
+                items.push_back( s->species[k]->batch_item( s->emf, s->current ) );
+                items.back().threads = sim_threads[i];
+            }
+        std::vector<zpic_batch_fields_item> fitems;
+        for ( auto * s : sims ) fitems.push_back( s->batch_fields() );
+        check( zpic_batch_step_create( &persistent_step, items.data(), (int) nsp, fitems.data(), (int) fitems.size(),
+                                       threads ) );
+        return;
+    }
     for ( size_t k = 0; k < nsp; k++ ) {
         std::vector<zpic_batch_species> items;
-        for ( auto * s : sims ) items.push_back( s->species[k]->batch_item( s->emf, s->current ) );
+        for ( size_t i = 0; i < sims.size(); i++ ) {
+            Simulation * s = sims[i];
+            items.push_back( s->species[k]->batch_item( s->emf, s->current ) );
+            items.back().threads = sim_threads[i];
+        }
         zpic_batch_particles * plan = nullptr;
         check( zpic_batch_particles_create( &plan, items.data(), (int) items.size(), threads ) );
         slots.push_back( plan );
@@ -1176,6 +1227,10 @@ void Batch::build() {
 // Simulation::advance (simulation.h:90-107) for every instance: each stage is
 // one launch over all instances.
 void Batch::step_kernels() {
+    if ( persistent_step ) {
+        check( zpic_batch_step_advance( persistent_step, stream() ) );
+        return;
+    }
     check( zpic_batch_zero_current( fields, stream() ) );
     for ( auto * plan : slots ) check( zpic_batch_particles_advance( plan, stream() ) );
     check( zpic_batch_current_advance( fields, stream() ) );
@@ -1183,7 +1238,7 @@ void Batch::step_kernels() {
 }
 
 void Batch::advance() {
-    if ( !fields ) build();
+    if ( !fields && !persistent_step ) build();
     if ( use_graph ) {
         if ( !exec ) {
             hip_check( hipStreamBeginCapture( stream(), hipStreamCaptureModeThreadLocal ), "hipStreamBeginCapture" );

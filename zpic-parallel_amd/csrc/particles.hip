@@ -18,8 +18,10 @@
 // tile and serialises the deposit and the energy sum over lanes in particle
 // order, which reproduces the reference's single-thread results exactly.
 #include "common.h"
+#include "grid_ops.h"
 
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 namespace zpic {
@@ -402,6 +404,7 @@ struct PartArgs {
     MoveArgs m;
     double * tile_energy;
     SortView S;
+    int waves;  // waves of the workgroup that take particles (0: all)
 };
 
 struct PartG {
@@ -453,10 +456,12 @@ __device__ inline int select_instance( A & a, const A * tab, const int2 * map ) 
     return blockIdx.x;
 }
 
+// The particle pass of tile t (one workgroup).  `smem` is the workgroup's
+// LDS of particles_lds_bytes( vol, W ) bytes, W = the waves that take
+// particles: all of the workgroup's, or a.waves if that is smaller (the
+// other waves then only join the workgroup barriers).
 template <bool PUSH, bool MOVE, bool DET>
-__global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const PartArgs * tab, const int2 * map ) {
-    extern __shared__ float smem[];
-    const int t = select_instance( a, tab, map );
+__device__ inline void particles_tile( const PartArgs & a, const int t, float * smem ) {
     const PartView P = a.P;
     const Geo g = a.g;
     const float3 * __restrict__ E = a.E;
@@ -470,7 +475,7 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
     const int nthr = blockDim.x;
     const int vol = g.vol;
     const int ys = g.ext_x;
-    const int W = nthr / WAVE;
+    const int W = ( a.waves > 0 && a.waves < nthr / WAVE ) ? a.waves : nthr / WAVE;
     const int w = tid / WAVE;
     const int lane = tid % WAVE;
 
@@ -778,6 +783,13 @@ __global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const P
     }
 }
 
+template <bool PUSH, bool MOVE, bool DET>
+__global__ void __launch_bounds__( FAST_BLOCK ) k_particles( PartArgs a, const PartArgs * tab, const int2 * map ) {
+    extern __shared__ float smem[];
+    const int t = select_instance( a, tab, map );
+    particles_tile<PUSH, MOVE, DET>( a, t, smem );
+}
+
 // ---------------------------------------------------------------------------
 // Tile sort.  Directions d = 0..8 encode the motion (d%3-1, d/3-1) of a
 // particle from its source tile to its target tile; d = 4 is "stays".
@@ -815,10 +827,7 @@ struct SortArgs {
 // New particle count per tile (bnd_check's new_np, particles.cpp:212-303):
 // the tile gathers the direction counts its 9 source tiles send to it.
 // Batched launches index the instance with blockIdx.y.
-__global__ void k_sort_newnp( SortArgs a, const SortArgs * tab ) {
-    if ( tab ) a = gbl( tab )[blockIdx.y];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if ( t >= a.ntiles ) return;
+__device__ inline void sort_newnp_tile( const SortArgs & a, const int t ) {
     const SortG S = gview( a.S );
     int n = 0;
     for ( int d = 0; d < 9; d++ ) {
@@ -828,15 +837,23 @@ __global__ void k_sort_newnp( SortArgs a, const SortArgs * tab ) {
     S.new_np[t] = n;
 }
 
+__global__ void k_sort_newnp( SortArgs a, const SortArgs * tab ) {
+    if ( tab ) a = gbl( tab )[blockIdx.y];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( t >= a.ntiles ) return;
+    sort_newnp_tile( a, t );
+}
+
 // Exclusive scan of the new counts into the new tile offsets
 // (update_tile_info, particles.cpp:314-357).  One workgroup per instance
 // walks the tiles in chunks of SCAN_BLOCK, one tile per thread (coalesced
 // loads, the next chunk's loads issued before the current chunk is
 // scanned): a wave scan by shuffles, the wave totals scanned by the first
 // wave, and a running carry between chunks.
+// BLOCK: the workgroup size (a multiple of 64, at most 64 waves).
 constexpr int SCAN_BLOCK = 1024;
-__global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const SortArgs * tab ) {
-    if ( tab ) a = gbl( tab )[blockIdx.x];
+template <int BLOCK>
+__device__ inline void sort_count_body( const SortArgs & a ) {
     const SortG S = gview( a.S );
     const gptr<int> new_off = gbl( a.T.off );
     const gptr<const int> extra = gbl( a.extra );
@@ -844,7 +861,7 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const 
     const bool count_moved = a.S.nmove != nullptr;
     const int ntiles = a.ntiles;
     const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
-    __shared__ int s_wave[SCAN_BLOCK / WAVE];
+    __shared__ int s_wave[BLOCK / WAVE];
     __shared__ unsigned long long s_moved;
     if ( tid == 0 ) s_moved = 0;
     auto load = [&]( int t, int & v, int & m ) {
@@ -859,9 +876,9 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const 
     load( tid, v, m );
     int carry = 0;
     unsigned long long moved = 0;
-    for ( int t0 = 0; t0 < ntiles; t0 += SCAN_BLOCK ) {
+    for ( int t0 = 0; t0 < ntiles; t0 += BLOCK ) {
         int vn, mn;
-        load( t0 + SCAN_BLOCK + tid, vn, mn );
+        load( t0 + BLOCK + tid, vn, mn );
         // inclusive scan within the wave
         int x = v;
         for ( int o = 1; o < WAVE; o <<= 1 ) {
@@ -871,17 +888,17 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const 
         if ( lane == WAVE - 1 ) s_wave[w] = x;
         __syncthreads();
         if ( w == 0 ) {
-            int z = lane < SCAN_BLOCK / WAVE ? s_wave[lane] : 0;
-            for ( int o = 1; o < SCAN_BLOCK / WAVE; o <<= 1 ) {
+            int z = lane < BLOCK / WAVE ? s_wave[lane] : 0;
+            for ( int o = 1; o < BLOCK / WAVE; o <<= 1 ) {
                 const int y = __shfl_up( z, o );
                 if ( lane >= o ) z += y;
             }
-            if ( lane < SCAN_BLOCK / WAVE ) s_wave[lane] = z;  // inclusive wave totals
+            if ( lane < BLOCK / WAVE ) s_wave[lane] = z;  // inclusive wave totals
         }
         __syncthreads();
         const int before = carry + ( w > 0 ? s_wave[w - 1] : 0 ) + x - v;
         if ( t0 + tid < ntiles ) new_off[t0 + tid] = before;
-        carry += s_wave[SCAN_BLOCK / WAVE - 1];
+        carry += s_wave[BLOCK / WAVE - 1];
         moved += (unsigned) m;
         __syncthreads();  // s_wave is rewritten by the next chunk
         v = vn;
@@ -897,6 +914,11 @@ __global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const 
         __syncthreads();
         if ( tid == 0 ) S.nmove[0] += s_moved;
     }
+}
+
+__global__ void __launch_bounds__( SCAN_BLOCK ) k_sort_count( SortArgs a, const SortArgs * tab ) {
+    if ( tab ) a = gbl( tab )[blockIdx.x];
+    sort_count_body<SCAN_BLOCK>( a );
 }
 
 // Stayers that move to the staging buffer because the tile's range moved
@@ -1239,12 +1261,11 @@ __device__ inline void gap_prefix( const int * seg, int np, int chunk, int * gpr
     gpre[MAX_SEG] = acc;
 }
 
-__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_shift( SortArgs a, const SortArgs * tab, const int2 * map ) {
+__device__ inline void sort_shift_tile( const SortArgs & a, const int t ) {
     __shared__ int s_seg[MAX_SEG + 1];
     __shared__ int s_gpre[MAX_SEG + 1];
     __shared__ int s_hdr[4];
-    __shared__ int s_w[SORT_BLOCK / WAVE];
-    const int t = select_instance( a, tab, map );
+    __shared__ int s_w[FAST_BLOCK / WAVE];
     const PartG P = gview( a.P );
     const SortG S = gview( a.S );
     const gptr<int2> Gix = gbl( a.S.stage_ix );
@@ -1294,16 +1315,20 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_shift( SortArgs a, const 
     if ( tid == 0 ) S.tmp_np[t] = ns;
 }
 
-__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in_fast( SortArgs a, const SortArgs * tab, const int2 * map ) {
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_shift( SortArgs a, const SortArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    sort_shift_tile( a, t );
+}
+
+// s_slot: LDS room for SLOT_CAP ints.
+__device__ inline void sort_in_fast_tile( const SortArgs & a, const int t, int * s_slot ) {
     __shared__ int s_seg[9][MAX_SEG + 1];  // segment counts of the 9 sources' lists (e = 4: the tile)
     __shared__ int s_so[9][3];             // source tile, old offset, old count
     __shared__ int s_pre[9][MAX_SEG + 1];  // prefix of the segment counts
     __shared__ int s_epre[10];             // prefix of the incoming lists' lengths (e != 4)
     __shared__ int s_gpre[MAX_SEG + 1];    // free-slot prefix of the tile's staging region
     __shared__ int s_hdr[3];
-    __shared__ int s_w[SORT_BLOCK / WAVE];
-    __shared__ int s_slot[SLOT_CAP];
-    const int t = select_instance( a, tab, map );
+    __shared__ int s_w[FAST_BLOCK / WAVE];
     const PartG P = gview( a.P );
     const SortG S = gview( a.S );
     const gptr<const int2> Gix = gbl( (const int2 *) a.S.stage_ix );
@@ -1380,7 +1405,9 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in_fast( SortArgs a, cons
             if ( k < nA ) return ( k >= r0 && k < r0 + SLOT_CAP ) ? s_slot[k - r0] : -1;
             if ( r0 != 0 ) return -1;
             const int k2 = k - nA;
-            return k2 < lb1 ? no + k2 : b2 + ( k2 - lb1 );
+            // counts always balance (nA + lb1 + lb2 = staged stayers + incoming);
+            // a rank past them would not be written
+            return k2 < lb1 ? no + k2 : ( k2 - lb1 < lb2 ? b2 + ( k2 - lb1 ) : -1 );
         };
         // the tile's staged stayers: ranks [0, ns)
         for ( int k = tid; k < ns; k += blockDim.x ) {
@@ -1428,6 +1455,12 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in_fast( SortArgs a, cons
         P.np[t] = nn;
         P.off[t] = no;
     }
+}
+
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in_fast( SortArgs a, const SortArgs * tab, const int2 * map ) {
+    __shared__ int s_slot[SLOT_CAP];
+    const int t = select_instance( a, tab, map );
+    sort_in_fast_tile( a, t, s_slot );
 }
 
 // ---------------------------------------------------------------------------
@@ -1552,7 +1585,7 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
         threads -= WAVE;
     const size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE );
     if ( lds > LDS_LIMIT ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
-    PartArgs a;
+    PartArgs a{};
     a.P = view( p ); a.g = make_geo( fld );
     a.E = (const float3 *) E; a.B = (const float3 *) B; a.J = (float3 *) J;
     a.alpha = alpha; a.m = m; a.tile_energy = tile_energy; a.S = view( s );
@@ -1596,8 +1629,9 @@ struct BatchParticlePlan {
     int nblocks = 0;
     int max_tiles = 0;
     int max_vol = 0;
-    int threads = FAST_BLOCK;
-    bool staged = false;  // every entry uses the order-free sort
+    int threads = FAST_BLOCK;  // workgroup size: the largest of the entries' fitted threads
+    size_t lds = 0;            // the largest of the entries' LDS needs
+    bool staged = false;       // every entry uses the order-free sort
     PartArgs * d_part = nullptr;
     SortArgs * d_sort = nullptr;
     int2 * d_map = nullptr;
@@ -1609,7 +1643,7 @@ int batch_particles_create( BatchParticlePlan ** out, const zpic_batch_species *
     if ( !out || !items || n < 1 ) return set_error( ZPIC_EINVAL, "empty batch" );
     if ( int rc = check_threads( threads ) ) return rc;
     auto * plan = new BatchParticlePlan();
-    plan->threads = threads;
+    plan->threads = WAVE;
     plan->nentries = n;
     std::vector<SortArgs> sorts( n );
     std::vector<EnergyArgs> energies( n );
@@ -1617,13 +1651,21 @@ int batch_particles_create( BatchParticlePlan ** out, const zpic_batch_species *
     for ( int e = 0; e < n; e++ ) {
         const zpic_batch_species & it = items[e];
         if ( int rc = check_geom( &it.fld ) ) { delete plan; return rc; }
-        PartArgs a;
+        PartArgs a{};
         a.P = view( &it.p ); a.g = make_geo( &it.fld );
         a.E = (const float3 *) it.E; a.B = (const float3 *) it.B; a.J = (float3 *) it.J;
         a.alpha = it.alpha; a.m = MoveArgs{ it.dt_dx, it.dt_dy, it.q, it.qnx, it.qny };
         a.tile_energy = it.tile_energy;
         zpic_sort s = it.s;
         a.S = view( &s );
+        // each entry runs with the threads that fit its own tiles, as its own
+        // single-instance pass does
+        if ( it.threads && check_threads( it.threads ) ) { delete plan; return ZPIC_EINVAL; }
+        const int th = fitted_threads( it.threads ? it.threads : threads, it.fld.tile_vol );
+        if ( th == 0 ) { delete plan; return set_error( ZPIC_ERANGE, "tile too large for LDS staging" ); }
+        a.waves = th / WAVE;
+        plan->threads = std::max( plan->threads, th );
+        plan->lds = std::max( plan->lds, particles_lds_bytes( it.fld.tile_vol, a.waves, true, true ) );
         plan->h_part.push_back( a );
         zpic_particles p = it.p, t = it.tmp;
         sorts[e] = sort_args( &p, &t, &s, it.fld.ntiles_x, it.fld.ntiles_y, it.fld.nx, it.fld.ny, it.periodic_x,
@@ -1643,11 +1685,6 @@ int batch_particles_create( BatchParticlePlan ** out, const zpic_batch_species *
             return set_error( ZPIC_EINVAL, "batch entries must all use the same sort (staged or not)" );
         }
         plan->staged = nst == n;
-    }
-    plan->threads = fitted_threads( threads, plan->max_vol );
-    if ( plan->threads == 0 ) {
-        delete plan;
-        return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     }
     hipError_t e1 = hipMalloc( (void **) &plan->d_part, sizeof( PartArgs ) * n );
     hipError_t e2 = hipMalloc( (void **) &plan->d_sort, sizeof( SortArgs ) * n );
@@ -1674,26 +1711,32 @@ void batch_particles_destroy( BatchParticlePlan * plan ) {
     delete plan;
 }
 
-__global__ void __launch_bounds__( 256 ) k_batch_energy( const EnergyArgs * tab ) {
-    __shared__ double buf[256];
-    const EnergyArgs a = gbl( tab )[blockIdx.x];
+// Sum of an entry's per-tile push energies: per-thread runs, a butterfly
+// within each wave, then the wave sums in wave order (any workgroup size).
+__device__ inline void batch_energy_body( const EnergyArgs & a ) {
+    __shared__ double s_wsum[FAST_BLOCK / WAVE];
     const gptr<const double> tile = gbl( a.tile );
     double acc = 0;
     for ( int t = threadIdx.x; t < a.n; t += blockDim.x ) acc += tile[t];
-    buf[threadIdx.x] = acc;
+    for ( int o = WAVE / 2; o > 0; o >>= 1 ) acc += __shfl_xor( acc, o );
+    if ( threadIdx.x % WAVE == 0 ) s_wsum[threadIdx.x / WAVE] = acc;
     __syncthreads();
-    for ( int o = 128; o > 0; o >>= 1 ) {
-        if ( (int) threadIdx.x < o ) buf[threadIdx.x] += buf[threadIdx.x + o];
-        __syncthreads();
+    if ( threadIdx.x == 0 ) {
+        double s = 0;
+        for ( int v = 0; v < (int) blockDim.x / WAVE; v++ ) s += s_wsum[v];
+        gbl( a.out )[0] = s;
     }
-    if ( threadIdx.x == 0 ) gbl( a.out )[0] = buf[0];
+    __syncthreads();  // s_wsum may be reused by the next body in the workgroup
+}
+
+__global__ void __launch_bounds__( 256 ) k_batch_energy( const EnergyArgs * tab ) {
+    batch_energy_body( gbl( tab )[blockIdx.x] );
 }
 
 int batch_particles_step( BatchParticlePlan * plan, void * stream ) {
     hipStream_t st = as_stream( stream );
-    const size_t lds = particles_lds_bytes( plan->max_vol, plan->threads / WAVE, true, true );
     PartArgs dummy{};
-    hipLaunchKernelGGL( ( k_particles<true, true, false> ), dim3( plan->nblocks ), dim3( plan->threads ), lds, st,
+    hipLaunchKernelGGL( ( k_particles<true, true, false> ), dim3( plan->nblocks ), dim3( plan->threads ), plan->lds, st,
                         dummy, (const PartArgs *) plan->d_part, (const int2 *) plan->d_map );
     ZPIC_LAUNCH_CHECK( "k_particles (batch)" );
     SortArgs sd{};
@@ -1722,6 +1765,495 @@ int batch_particles_step( BatchParticlePlan * plan, void * stream ) {
     ZPIC_LAUNCH_CHECK( "k_batch_energy" );
     return ZPIC_OK;
 }
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Persistent batched step: Simulation::advance (simulation.h:90-107) of every
+// instance of a batch in one launch.  The step is a list of work items
+// (stage, instance, tiles) ordered so that every item's predecessors come
+// before it.  Workgroups claim items in that order from STEP_QUEUES queues
+// (their home queue first, then the others), wait until the stages their item
+// depends on are complete for that instance (a counter of finished items per
+// (stage, instance)), run it and count it as finished.  A workgroup only ever
+// waits for items claimed before its own by workgroups that are running, so
+// the step needs no co-resident grid and always drains; a wait longer than
+// the spin limit (which would be a bug) raises an error flag and goes on
+// instead of hanging.  Different instances overlap freely: a small instance
+// runs its sort or its field update while a large one is still in its
+// particle pass.
+enum StepKind : int {
+    SK_ZERO = 0,  // Current::zero of tiles [t0, t1)
+    SK_PART,      // particle pass of tile t0
+    SK_NEWNP,     // new per-tile counts of tiles [t0, t1)
+    SK_COUNT,     // offsets scan, nmove and push energy of the species
+    SK_SHIFT,     // order-free sort: stayers outside the new range staged (tile t0)
+    SK_PLACE,     // order-free sort: placement (tile t0)
+    SK_ADD_X,     // add_from_gc x pass of J (tiles [t0, t1))
+    SK_ADD_Y,     // add_from_gc y pass of J
+    SK_COPY_XY,   // copy_to_gc of one grid (tiles [t0, t1))
+    SK_COPY_X,    // copy_to_gc_x of J after an x filter pass
+    SK_COPY_Y,    // copy_to_gc_y of J after a y filter pass
+    SK_K3,        // one filter pass (tile t0)
+    SK_YEE,       // Yee update (tile t0)
+};
+
+struct StepItem {
+    int kind, arg, t0, t1;
+    int done;                      // counter of this item's (stage, instance)
+    int dep0, need0, dep1, need1;  // counters to wait for (-1: none), final values
+    int pad[3];
+};
+
+constexpr int STEP_QUEUES = 8;
+constexpr int STEP_SPIN_LIMIT = 1 << 22;  // polls of one wait (seconds), then the error flag
+
+struct StepTables {
+    const StepItem * items;
+    int qoff[STEP_QUEUES + 1];  // queue q holds items [qoff[q], qoff[q + 1])
+    int * next;                 // [STEP_QUEUES] claim counters, then the stage counters
+    int * counters;
+    int * error;
+    const PartArgs * part;
+    const SortArgs * sort;
+    const EnergyArgs * energy;
+    const grid_ops::GcArgs * gc;
+    const grid_ops::K3Args * k3;
+    const grid_ops::YeeArgs * yee;
+};
+
+// A table entry read into scalar registers: the entry is the same for the
+// whole workgroup, and values the compiler cannot prove uniform (loads after
+// stores in the kernel) would otherwise live in vector registers.
+template <class X>
+__device__ inline X uniform_load( const X * p ) {
+    static_assert( sizeof( X ) % 4 == 0, "4-byte words" );
+    X out;
+    const gptr<const int> src = gbl( reinterpret_cast<const int *>( p ) );
+    int * dst = reinterpret_cast<int *>( &out );
+#pragma unroll
+    for ( int i = 0; i < (int) ( sizeof( X ) / 4 ); i++ ) dst[i] = __builtin_amdgcn_readfirstlane( src[i] );
+    return out;
+}
+
+__device__ inline void step_wait( const StepTables & T, int c, int need ) {
+    if ( c < 0 ) return;
+    int spins = 0;
+    while ( __hip_atomic_load( T.counters + c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT ) < need ) {
+        if ( ++spins > STEP_SPIN_LIMIT ) {
+            __hip_atomic_fetch_or( T.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+            return;
+        }
+        __builtin_amdgcn_s_sleep( 4 );
+    }
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
+    extern __shared__ float smem[];
+    __shared__ int s_item;
+    const int home = blockIdx.x % STEP_QUEUES;
+    int q = 0;  // queues exhausted so far, counted from the home queue
+    for ( ;; ) {
+        if ( threadIdx.x == 0 ) {
+            int k = -1;
+            while ( q < STEP_QUEUES ) {
+                const int qq = ( home + q ) % STEP_QUEUES;
+                const int n = T.qoff[qq + 1] - T.qoff[qq];
+                if ( n > 0 ) {
+                    const int c = __hip_atomic_fetch_add( T.next + qq, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+                    if ( c < n ) {
+                        k = T.qoff[qq] + c;
+                        break;
+                    }
+                }
+                q++;
+            }
+            s_item = k;
+        }
+        __syncthreads();
+        // uniform across the workgroup: item and argument loads become scalar
+        const int k = __builtin_amdgcn_readfirstlane( s_item );
+        __syncthreads();
+        if ( k < 0 ) break;
+        const StepItem it = uniform_load( T.items + k );
+        if ( threadIdx.x == 0 ) {
+            step_wait( T, it.dep0, it.need0 );
+            step_wait( T, it.dep1, it.need1 );
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
+        switch ( it.kind ) {
+        case SK_ZERO: {
+            const grid_ops::GcArgs g = uniform_load( T.gc + it.arg );
+            for ( int t = it.t0; t < it.t1; t++ ) grid_ops::zero_tile( g.buf, 3 * g.g.vol, t );
+            break;
+        }
+        case SK_PART: particles_tile<true, true, false>( uniform_load( T.part + it.arg ), it.t0, smem ); break;
+        case SK_NEWNP: {
+            const SortArgs a = uniform_load( T.sort + it.arg );
+            for ( int t = it.t0 + (int) threadIdx.x; t < it.t1; t += BLOCK ) sort_newnp_tile( a, t );
+            break;
+        }
+        case SK_COUNT:
+            sort_count_body<BLOCK>( uniform_load( T.sort + it.arg ) );
+            __syncthreads();
+            batch_energy_body( uniform_load( T.energy + it.arg ) );
+            break;
+        case SK_SHIFT: sort_shift_tile( uniform_load( T.sort + it.arg ), it.t0 ); break;
+        case SK_PLACE: sort_in_fast_tile( uniform_load( T.sort + it.arg ), it.t0, reinterpret_cast<int *>( smem ) ); break;
+        case SK_ADD_X:
+        case SK_ADD_Y:
+        case SK_COPY_XY:
+        case SK_COPY_X:
+        case SK_COPY_Y: {
+            const grid_ops::GcArgs g = uniform_load( T.gc + it.arg );
+            for ( int t = it.t0; t < it.t1; t++ ) {
+                if ( it.kind == SK_ADD_X ) grid_ops::add_gc_x_tile<3>( g, t );
+                else if ( it.kind == SK_ADD_Y ) grid_ops::add_gc_y_tile<3>( g, t );
+                else if ( it.kind == SK_COPY_XY ) grid_ops::copy_gc_xy_tile<3>( g, t );
+                else if ( it.kind == SK_COPY_X ) grid_ops::copy_gc_x_tile<3>( g, t );
+                else grid_ops::copy_gc_y_tile<3>( g, t );
+            }
+            break;
+        }
+        case SK_K3: grid_ops::kernel3_tile( uniform_load( T.k3 + it.arg ), it.t0, smem ); break;
+        case SK_YEE: grid_ops::yee_tile( uniform_load( T.yee + it.arg ), it.t0, smem ); break;
+        default: break;
+        }
+        // every wave's writes reach the device scope before the item counts as finished
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "agent" );
+        __syncthreads();
+        if ( threadIdx.x == 0 ) __hip_atomic_fetch_add( T.counters + it.done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT );
+    }
+}
+
+// Clears the claim and stage counters before each step (a kernel, so that the
+// step can be captured in a graph without memset nodes).
+__global__ void k_step_reset( int * v, int n ) {
+    for ( int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x ) gbl( v )[k] = 0;
+}
+
+struct BatchStepPlan {
+    int block = 256;
+    size_t lds = 0;
+    int grid = 0;
+    int nsync = 0;  // claim counters + stage counters
+    StepTables T{};
+    std::vector<void *> allocs;
+    ~BatchStepPlan() {
+        for ( void * p : allocs ) (void) hipFree( p );
+    }
+    template <class X>
+    int upload( X ** dst, const std::vector<X> & v ) {
+        *dst = nullptr;
+        if ( v.empty() ) return ZPIC_OK;
+        void * p = nullptr;
+        ZPIC_HIP_TRY( hipMalloc( &p, sizeof( X ) * v.size() ) );
+        allocs.push_back( p );
+        ZPIC_HIP_TRY( hipMemcpy( p, v.data(), sizeof( X ) * v.size(), hipMemcpyHostToDevice ) );
+        *dst = static_cast<X *>( p );
+        return ZPIC_OK;
+    }
+};
+
+// Filter passes of Current::advance (filter.h:22-78) as (dir, a, b, c).
+struct FilterPass {
+    int dir;
+    float a, b, c;
+};
+std::vector<FilterPass> filter_passes( const zpic_batch_fields_item & it ) {
+    std::vector<FilterPass> out;
+    if ( it.filter_kind == 0 ) return out;
+    const unsigned order = it.filter_order > 0 ? it.filter_order : 1;
+    for ( unsigned k = 0; k < order; k++ ) out.push_back( FilterPass{ it.filter_dir, 0.25f, 0.5f, 0.25f } );
+    if ( it.filter_kind == 2 ) {
+        const float a = -1.0f;
+        const float b = ( 4.0 + 2.0 * order ) / order;
+        const float norm = 2 * a + b;
+        out.push_back( FilterPass{ it.filter_dir, a / norm, b / norm, a / norm } );
+    }
+    return out;
+}
+
+int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int nslots, const zpic_batch_fields_item * fl,
+                       int n, int threads ) {
+    if ( !out || !sp || !fl || n < 1 || nslots < 1 ) return set_error( ZPIC_EINVAL, "empty batch" );
+    if ( int rc = check_threads( threads ) ) return rc;
+    auto plan = std::make_unique<BatchStepPlan>();
+    // instances by decreasing particle count: the big ones claim first
+    std::vector<int> order( n );
+    std::vector<long long> weight( n, 0 );
+    std::vector<int> waves( n );
+    int block = 256;
+    size_t lds = sizeof( int ) * SLOT_CAP;
+    for ( int i = 0; i < n; i++ ) {
+        order[i] = i;
+        const zpic_batch_fields_item & f = fl[i];
+        if ( int rc = check_geom( &f.g ) ) return rc;
+        if ( !f.E || !f.B || !f.J ) return set_error( ZPIC_EINVAL, "null field buffer" );
+        if ( f.filter_kind < 0 || f.filter_kind > 2 )
+            return set_error( ZPIC_EINVAL, "batches run the none, binomial and compensated current filters only" );
+        if ( !f.periodic_x || !f.periodic_y ) return set_error( ZPIC_EINVAL, "the persistent step runs periodic boundaries only" );
+        const int vol = f.g.tile_vol;
+        const int req = sp[i].threads ? sp[i].threads : threads;  // slot 0 of the instance
+        if ( int rc = check_threads( req ) ) return rc;
+        const int th = fitted_threads( req, vol );
+        if ( th == 0 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+        waves[i] = th / WAVE;
+        block = std::max( block, th );
+        lds = std::max( { lds, particles_lds_bytes( vol, waves[i], true, true ), sizeof( float ) * 6 * (size_t) vol } );
+        for ( int k = 0; k < nslots; k++ ) {
+            const zpic_batch_species & s = sp[k * n + i];
+            if ( !s.s.stage_ix || !s.s.stage_x || !s.s.stage_u )
+                return set_error( ZPIC_EINVAL, "the persistent step needs the order-free sort (staging buffers)" );
+            if ( s.fld.ntiles_x != f.g.ntiles_x || s.fld.ntiles_y != f.g.ntiles_y || s.fld.nx != f.g.nx ||
+                 s.fld.ny != f.g.ny || s.E != f.E || s.B != f.B || s.J != f.J )
+                return set_error( ZPIC_EINVAL, "species and fields of an instance do not match" );
+            if ( !s.periodic_x || !s.periodic_y ) return set_error( ZPIC_EINVAL, "the persistent step runs periodic boundaries only" );
+        }
+        weight[i] = (long long) f.g.ntiles_x * f.g.ntiles_y * f.g.tile_vol;
+    }
+    if ( lds > LDS_LIMIT ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    std::stable_sort( order.begin(), order.end(), [&]( int a, int b ) { return weight[a] > weight[b]; } );
+    plan->block = block;
+    plan->lds = lds;
+
+    // argument tables
+    std::vector<PartArgs> part( (size_t) nslots * n );
+    std::vector<SortArgs> sort( (size_t) nslots * n );
+    std::vector<EnergyArgs> ener( (size_t) nslots * n );
+    std::vector<grid_ops::GcArgs> gc( 3 * (size_t) n );
+    std::vector<grid_ops::YeeArgs> yee( n );
+    std::vector<grid_ops::K3Args> k3;
+    std::vector<std::vector<int>> k3_of( n );  // k3 table index of each pass of each instance
+    std::vector<std::vector<FilterPass>> passes( n );
+    size_t max_passes = 0;
+    for ( int i = 0; i < n; i++ ) {
+        const zpic_batch_fields_item & f = fl[i];
+        const grid_ops::G g = grid_ops::make( &f.g );
+        gc[3 * i + 0] = grid_ops::GcArgs{ f.J, g, 1, 1 };
+        gc[3 * i + 1] = grid_ops::GcArgs{ f.E, g, 1, 1 };
+        gc[3 * i + 2] = grid_ops::GcArgs{ f.B, g, 1, 1 };
+        yee[i] = grid_ops::YeeArgs{ f.E, f.B, f.J, g, f.dt_dx, f.dt_dy, f.dt };
+        passes[i] = filter_passes( f );
+        max_passes = std::max( max_passes, passes[i].size() );
+        for ( const FilterPass & p : passes[i] ) {
+            k3_of[i].push_back( (int) k3.size() );
+            k3.push_back( grid_ops::K3Args{ f.J, g, p.dir, p.a, p.b, p.c } );
+        }
+        for ( int k = 0; k < nslots; k++ ) {
+            const zpic_batch_species & it = sp[k * n + i];
+            PartArgs a{};
+            a.P = view( &it.p );
+            a.g = make_geo( &it.fld );
+            a.E = (const float3 *) it.E; a.B = (const float3 *) it.B; a.J = (float3 *) it.J;
+            a.alpha = it.alpha;
+            a.m = MoveArgs{ it.dt_dx, it.dt_dy, it.q, it.qnx, it.qny };
+            a.tile_energy = it.tile_energy;
+            zpic_sort s = it.s;
+            a.S = view( &s );
+            a.waves = waves[i];
+            part[k * n + i] = a;
+            zpic_particles p = it.p, t = it.tmp;
+            sort[k * n + i] = sort_args( &p, &t, &s, it.fld.ntiles_x, it.fld.ntiles_y, it.fld.nx, it.fld.ny, 1, 1 );
+            ener[k * n + i] = EnergyArgs{ it.tile_energy, it.fld.ntiles_x * it.fld.ntiles_y, it.energy_out };
+        }
+    }
+
+    // stages in claim order; counters[stage * n + instance]
+    struct Stage {
+        int kind;
+        int slot;  // species slot, filter pass or grid (1 E+B) as the kind needs
+    };
+    std::vector<Stage> stages;
+    stages.push_back( { SK_ZERO, 0 } );
+    for ( int k = 0; k < nslots; k++ ) stages.push_back( { SK_PART, k } );
+    for ( int k = 0; k < nslots; k++ ) {
+        stages.push_back( { SK_NEWNP, k } );
+        stages.push_back( { SK_COUNT, k } );
+        stages.push_back( { SK_SHIFT, k } );
+        stages.push_back( { SK_PLACE, k } );
+    }
+    stages.push_back( { SK_ADD_X, 0 } );
+    stages.push_back( { SK_ADD_Y, 0 } );
+    stages.push_back( { SK_COPY_XY, 0 } );
+    for ( size_t p = 0; p < max_passes; p++ ) {
+        stages.push_back( { SK_K3, (int) p } );
+        stages.push_back( { SK_COPY_X, (int) p } );  // x or y guard copy by the pass's direction
+    }
+    stages.push_back( { SK_YEE, 0 } );
+    stages.push_back( { SK_COPY_XY, 1 } );
+    const int ns = (int) stages.size();
+
+    // tiles per item of the light stages
+    constexpr int ZERO_TILES = 8, NEWNP_TILES = 1024, GC_TILES = 4;
+    std::vector<int> need( (size_t) ns * n, 0 );  // items per (stage, instance)
+    std::vector<std::vector<StepItem>> per_stage( ns );
+    std::vector<int> last( n, -1 );  // last stage with items, per instance (current chain)
+    std::vector<int> part_stage( n, -1 );
+    for ( int si = 0; si < ns; si++ ) {
+        const Stage st = stages[si];
+        for ( int oi = 0; oi < n; oi++ ) {
+            const int i = order[oi];
+            const zpic_batch_fields_item & f = fl[i];
+            const int nt = f.g.ntiles_x * f.g.ntiles_y;
+            const int ctr = si * n + i;
+            auto push = [&]( int kind, int arg, int t0, int t1, int dep0 ) {
+                StepItem it{};
+                it.kind = kind; it.arg = arg; it.t0 = t0; it.t1 = t1; it.done = ctr;
+                it.dep0 = dep0; it.dep1 = -1; it.need1 = 0;
+                per_stage[si].push_back( it );
+                need[ctr]++;
+            };
+            // predecessor counter of this stage for instance i (-1: none)
+            int dep = -1;
+            switch ( st.kind ) {
+            case SK_ZERO:
+                for ( int t = 0; t < nt; t += ZERO_TILES ) push( SK_ZERO, 3 * i, t, std::min( nt, t + ZERO_TILES ), -1 );
+                last[i] = si;
+                break;
+            case SK_PART:
+                dep = last[i] * n + i;
+                for ( int t = 0; t < nt; t++ ) push( SK_PART, st.slot * n + i, t, t + 1, dep );
+                last[i] = si;
+                part_stage[i] = si;
+                break;
+            case SK_NEWNP:
+                // after the species' own particle pass
+                dep = ( 1 + st.slot ) * n + i;
+                for ( int t = 0; t < nt; t += NEWNP_TILES ) push( SK_NEWNP, st.slot * n + i, t, std::min( nt, t + NEWNP_TILES ), dep );
+                break;
+            case SK_COUNT:
+            case SK_SHIFT:
+            case SK_PLACE:
+                dep = ( si - 1 ) * n + i;
+                if ( st.kind == SK_COUNT ) push( SK_COUNT, st.slot * n + i, 0, nt, dep );
+                else
+                    for ( int t = 0; t < nt; t++ ) push( st.kind, st.slot * n + i, t, t + 1, dep );
+                break;
+            case SK_ADD_X:
+            case SK_ADD_Y:
+            case SK_COPY_XY:
+            case SK_YEE: {
+                dep = last[i] * n + i;
+                const int arg = st.kind == SK_YEE ? i : 3 * i + st.slot;
+                if ( st.kind == SK_COPY_XY && st.slot == 1 ) {
+                    // E and B after the Yee update: two items per tile range
+                    for ( int t = 0; t < nt; t += GC_TILES ) {
+                        push( SK_COPY_XY, 3 * i + 1, t, std::min( nt, t + GC_TILES ), dep );
+                        push( SK_COPY_XY, 3 * i + 2, t, std::min( nt, t + GC_TILES ), dep );
+                    }
+                } else if ( st.kind == SK_YEE ) {
+                    for ( int t = 0; t < nt; t++ ) push( SK_YEE, arg, t, t + 1, dep );
+                } else {
+                    for ( int t = 0; t < nt; t += GC_TILES ) push( st.kind, arg, t, std::min( nt, t + GC_TILES ), dep );
+                }
+                last[i] = si;
+                break;
+            }
+            case SK_K3:
+            case SK_COPY_X: {
+                if ( (size_t) st.slot >= passes[i].size() ) break;  // no such pass: the chain skips it
+                dep = last[i] * n + i;
+                if ( st.kind == SK_K3 ) {
+                    for ( int t = 0; t < nt; t++ ) push( SK_K3, k3_of[i][st.slot], t, t + 1, dep );
+                } else {
+                    const int kind = passes[i][st.slot].dir == 0 ? SK_COPY_X : SK_COPY_Y;
+                    for ( int t = 0; t < nt; t += GC_TILES ) push( kind, 3 * i, t, std::min( nt, t + GC_TILES ), dep );
+                }
+                last[i] = si;
+                break;
+            }
+            }
+        }
+    }
+    // final counts of the predecessors; queues: item j of a stage to queue j % STEP_QUEUES
+    std::vector<std::vector<StepItem>> queues( STEP_QUEUES );
+    for ( int si = 0; si < ns; si++ ) {
+        int j = 0;
+        for ( StepItem it : per_stage[si] ) {
+            it.need0 = it.dep0 >= 0 ? need[it.dep0] : 0;
+            queues[j++ % STEP_QUEUES].push_back( it );
+        }
+    }
+    std::vector<StepItem> items;
+    for ( int qq = 0; qq < STEP_QUEUES; qq++ ) {
+        plan->T.qoff[qq] = (int) items.size();
+        items.insert( items.end(), queues[qq].begin(), queues[qq].end() );
+    }
+    plan->T.qoff[STEP_QUEUES] = (int) items.size();
+    plan->nsync = STEP_QUEUES + ns * n;
+
+    StepItem * d_items = nullptr;
+    if ( int rc = plan->upload( &d_items, items ) ) return rc;
+    plan->T.items = d_items;
+    std::vector<int> sync( plan->nsync + 1, 0 );  // claim counters, stage counters, error flag
+    int * d_sync = nullptr;
+    if ( int rc = plan->upload( &d_sync, sync ) ) return rc;
+    plan->T.next = d_sync;
+    plan->T.counters = d_sync + STEP_QUEUES;
+    plan->T.error = d_sync + plan->nsync;
+    PartArgs * d_part = nullptr;
+    SortArgs * d_sort = nullptr;
+    EnergyArgs * d_ener = nullptr;
+    grid_ops::GcArgs * d_gc = nullptr;
+    grid_ops::K3Args * d_k3 = nullptr;
+    grid_ops::YeeArgs * d_yee = nullptr;
+    if ( int rc = plan->upload( &d_part, part ) ) return rc;
+    if ( int rc = plan->upload( &d_sort, sort ) ) return rc;
+    if ( int rc = plan->upload( &d_ener, ener ) ) return rc;
+    if ( int rc = plan->upload( &d_gc, gc ) ) return rc;
+    if ( int rc = plan->upload( &d_k3, k3 ) ) return rc;
+    if ( int rc = plan->upload( &d_yee, yee ) ) return rc;
+    plan->T.part = d_part; plan->T.sort = d_sort; plan->T.energy = d_ener;
+    plan->T.gc = d_gc; plan->T.k3 = d_k3; plan->T.yee = d_yee;
+
+    // the dynamic LDS beside the kernel's static LDS must fit one workgroup
+    int dev = 0, cus = 0, per_cu = 0, max_lds = 0;
+    ZPIC_HIP_TRY( hipGetDevice( &dev ) );
+    ZPIC_HIP_TRY( hipDeviceGetAttribute( &max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev ) );
+    hipFuncAttributes fa{};
+    if ( block == 256 ) ZPIC_HIP_TRY( hipFuncGetAttributes( &fa, reinterpret_cast<const void *>( &k_batch_step<256> ) ) );
+    else ZPIC_HIP_TRY( hipFuncGetAttributes( &fa, reinterpret_cast<const void *>( &k_batch_step<512> ) ) );
+    if ( lds + fa.sharedSizeBytes > (size_t) max_lds )
+        return set_error( ZPIC_ERANGE, "persistent step: a tile's LDS staging does not fit one workgroup" );
+    // as many workgroups as fit on the device at once
+    ZPIC_HIP_TRY( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
+    if ( block == 256 )
+        ZPIC_HIP_TRY( hipOccupancyMaxActiveBlocksPerMultiprocessor( &per_cu, k_batch_step<256>, 256, lds ) );
+    else
+        ZPIC_HIP_TRY( hipOccupancyMaxActiveBlocksPerMultiprocessor( &per_cu, k_batch_step<512>, 512, lds ) );
+    plan->grid = std::max( 1, std::min( cus * std::max( per_cu, 1 ), (int) items.size() ) );
+    *out = plan.release();
+    return ZPIC_OK;
+}
+
+int batch_step_run( BatchStepPlan * plan, void * stream ) {
+    hipStream_t st = as_stream( stream );
+    hipLaunchKernelGGL( k_step_reset, dim3( ( plan->nsync + 255 ) / 256 ), dim3( 256 ), 0, st, plan->T.next, plan->nsync );
+    ZPIC_LAUNCH_CHECK( "k_step_reset" );
+    if ( plan->block == 256 )
+        hipLaunchKernelGGL( k_batch_step<256>, dim3( plan->grid ), dim3( 256 ), plan->lds, st, plan->T );
+    else
+        hipLaunchKernelGGL( k_batch_step<512>, dim3( plan->grid ), dim3( 512 ), plan->lds, st, plan->T );
+    ZPIC_LAUNCH_CHECK( "k_batch_step" );
+    return ZPIC_OK;
+}
+
+int batch_step_status( BatchStepPlan * plan ) {
+    int err = 0;
+    ZPIC_HIP_TRY( hipDeviceSynchronize() );
+    ZPIC_HIP_TRY( hipMemcpy( &err, plan->T.error, sizeof( int ), hipMemcpyDeviceToHost ) );
+    if ( err ) {
+        const int zero = 0;
+        ZPIC_HIP_TRY( hipMemcpy( plan->T.error, &zero, sizeof( int ), hipMemcpyHostToDevice ) );
+        return set_error( ZPIC_EHIP, "persistent step: a dependency wait exceeded its spin limit" );
+    }
+    return ZPIC_OK;
+}
+
+}  // namespace
 
 // The particle pass's HBM traffic without its arithmetic: every particle
 // record (int2 cell, float2 position, float3 momentum: 28 B) read and written
@@ -1945,6 +2477,43 @@ int zpic_batch_particles_advance( zpic_batch_particles * b, void * stream ) {
 void zpic_batch_particles_free( zpic_batch_particles * b ) {
     if ( !b ) return;
     batch_particles_destroy( b->plan );
+    delete b;
+}
+
+struct zpic_batch_step {
+    BatchStepPlan * plan;
+};
+
+int zpic_batch_step_create( zpic_batch_step ** out, const zpic_batch_species * species, int nslots,
+                            const zpic_batch_fields_item * fields, int n, int threads_per_tile ) {
+    if ( !out ) return set_error( ZPIC_EINVAL, "null output" );
+    BatchStepPlan * plan = nullptr;
+    if ( int rc = batch_step_create( &plan, species, nslots, fields, n, threads_per_tile ) ) return rc;
+    *out = new zpic_batch_step{ plan };
+    return ZPIC_OK;
+}
+
+int zpic_batch_step_advance( zpic_batch_step * b, void * stream ) {
+    if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch step" );
+    return batch_step_run( b->plan, stream );
+}
+
+int zpic_batch_step_status( zpic_batch_step * b ) {
+    if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch step" );
+    return batch_step_status( b->plan );
+}
+
+int zpic_batch_step_info( const zpic_batch_step * b, int * workgroups, int * threads, int * items ) {
+    if ( !b || !b->plan ) return set_error( ZPIC_EINVAL, "null batch step" );
+    if ( workgroups ) *workgroups = b->plan->grid;
+    if ( threads ) *threads = b->plan->block;
+    if ( items ) *items = b->plan->T.qoff[STEP_QUEUES];
+    return ZPIC_OK;
+}
+
+void zpic_batch_step_free( zpic_batch_step * b ) {
+    if ( !b ) return;
+    delete b->plan;
     delete b;
 }
 

@@ -815,10 +815,13 @@ class Simulation {
 // counterpart (the reference advances one simulation per process).
 class Batch {
     std::vector<Simulation *> sims;
+    std::vector<int> sim_threads;               // per instance, 0 = `threads`
     std::vector<zpic_batch_particles *> slots;  // one plan per species slot
     zpic_batch_fields * fields;
+    zpic_batch_step * persistent_step;          // the whole step in one launch
     int threads;
     bool use_graph;
+    bool persistent;
     hipGraph_t graph;
     hipGraphExec_t exec;
     void release();
@@ -831,12 +834,19 @@ class Batch {
     Batch & operator=( const Batch & ) = delete;
 
     // Non-owning; all instances must hold the same number of species.
-    void add( Simulation & sim );
+    // threads_per_tile: this instance's particle pass (0 = the batch's).
+    void add( Simulation & sim, int threads_per_tile = 0 );
     // Builds the launch tables (called by the first advance if needed).
     void build();
     void advance();
     // Capture the step in a hipGraph on the first advance and replay it.
     void set_graph( bool on );
+    // One persistent launch per step (zpic_batch_step) instead of one launch
+    // per stage; needs the fast mode's order-free sort and periodic
+    // boundaries.
+    void set_persistent( bool on );
+    // Synchronises and throws if a persistent step reported an error.
+    void check_status();
     size_t size() const { return sims.size(); }
 };
 

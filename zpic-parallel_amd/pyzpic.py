@@ -82,6 +82,9 @@ def lib():
         "zpic_batch_add": (i, [vp, vp]),
         "zpic_batch_set_graph": (i, [vp, i]),
         "zpic_batch_advance": (i, [vp, i]),
+        "zpic_batch_set_persistent": (i, [vp, i]),
+        "zpic_batch_add_threads": (i, [vp, vp, i]),
+        "zpic_batch_status": (i, [vp]),
         "zpic_set_threads_per_tile": (i, [i]),
         "zpic_get_threads_per_tile": (i, []),
         "zpic_set_sort_order": (i, [i]),
@@ -538,7 +541,7 @@ class Batch:
     The simulations are held by reference and must stay open while the batch
     lives; each must hold the same number of species."""
 
-    def __init__(self, sims=(), threads_per_tile=256, graph=False):
+    def __init__(self, sims=(), threads_per_tile=256, graph=False, persistent=False):
         L = lib()
         h = L.zpic_batch_new(threads_per_tile)
         if not h:
@@ -548,9 +551,21 @@ class Batch:
         for s in sims:
             self.add(s)
         self.set_graph(graph)
+        self.set_persistent(persistent)
 
-    def add(self, sim):
-        _chk(lib().zpic_batch_add(self.h, sim.h))
+    def set_persistent(self, on=True):
+        """One persistent launch per step over every instance (see
+        zpic_batch_step) instead of one launch per stage."""
+        _chk(lib().zpic_batch_set_persistent(self.h, 1 if on else 0))
+
+    def status(self):
+        """Synchronises; raises if a persistent step reported an error."""
+        _chk(lib().zpic_batch_status(self.h))
+
+    def add(self, sim, threads_per_tile=0):
+        """Adds an instance; threads_per_tile: its particle pass's threads
+        (0 = the batch's)."""
+        _chk(lib().zpic_batch_add_threads(self.h, sim.h, threads_per_tile))
         self.sims.append(sim)
 
     def set_graph(self, on=True):
@@ -713,3 +728,43 @@ def weibel(ntiles=(8, 8), nx=(16, 16), ppc=(8, 8), box=None, dt=0.07, uth=(0.1, 
     sim.add_species(Species("electrons", -1.0, ppc, "thermal_corr", uth, ufl, seed=16 * seed))
     sim.add_species(Species("positrons", +1.0, ppc, "thermal_corr", uth, tuple(-v for v in ufl), seed=16 * seed))
     return sim
+
+
+def batch64_instances(n=64, seed=2024):
+    """The batched Weibel parameter scan: n instances with the grid drawn from
+    {64, 128, 256, 512}^2, ppc from {2, 4, 8}^2, drift uz from U[0.2, 0.9] and
+    thermal spread uth from U[0.02, 0.2] (numpy RandomState(seed), drawn in
+    that order per instance); dx = dy = 0.1, dt = 0.07, periodic."""
+    rng = np.random.RandomState(seed)
+    out = []
+    for _ in range(n):
+        g = int(rng.choice([64, 128, 256, 512]))
+        p = int(rng.choice([2, 4, 8]))
+        uz = float(rng.uniform(0.2, 0.9))
+        uth = float(rng.uniform(0.02, 0.2))
+        out.append(dict(grid=(g, g), ppc=(p, p), uth=(uth, uth, uth), ufl=(0.0, 0.0, uz)))
+    return out
+
+
+def tune_instances(instances, steps=2, max_cells=128 * 128, cache=None):
+    """Autotuned tile shape and threads per tile (zpic_autotune) of every
+    distinct (grid, ppc) of a parameter scan; adds "nx", "ntiles" and
+    "threads" to each instance (in place) and returns them."""
+    tuned = {}
+    for c in instances:
+        key = (c["grid"], c["ppc"])
+        if key not in tuned:
+            tuned[key] = autotune(c["grid"], c["ppc"], steps=steps, max_cells=max_cells, cache=cache)
+        r = tuned[key]
+        c["nx"] = tuple(r["nx"])
+        c["ntiles"] = (c["grid"][0] // r["nx"][0], c["grid"][1] // r["nx"][1])
+        c["threads"] = r["threads"]
+    return instances
+
+
+def scan_simulation(c):
+    """One instance of a parameter scan (see batch64_instances) with its tile
+    shape c["nx"] (default 16x16)."""
+    nx = tuple(c.get("nx", (16, 16)))
+    ntiles = tuple(c.get("ntiles", (c["grid"][0] // nx[0], c["grid"][1] // nx[1])))
+    return weibel(ntiles=ntiles, nx=nx, ppc=c["ppc"], uth=c["uth"], ufl=c["ufl"])
