@@ -140,6 +140,10 @@ def bandwidth_ceilings():
             assert rc == 0, L.zpic_last_error()
         return run
 
+    def probe_tiles(kind, U, tiles):
+        rc = L.zpic_bandwidth_probe(kind, vp(ix), vp(x), vp(u), n, U, tiles, sp)
+        assert rc == 0, L.zpic_last_error()
+
     res = {"cus": cus, "torch_copy_4GiB": timed(lambda: dst.copy_(src), 8 << 30)}
     res["particle_stream_probe"] = timed(
         lambda: L.zpic_particle_stream_probe(vp(ix), vp(x), vp(u), n, sp), 56 * n)
@@ -149,6 +153,16 @@ def bandwidth_ceilings():
             for wpc in (1, 2, 4, 8, 16, 32, 0):
                 blocks = cus * wpc if wpc else (n + 256 * U - 1) // (256 * U)
                 sweep[f"U{U}_b{blocks}"] = timed(probe(kind, U, blocks), nbytes)
+        best = max(sweep, key=lambda k: sweep[k]["GB/s"])
+        res[name] = {"best": best, **sweep[best], "sweep": sweep}
+    # the records as tiles, one workgroup per tile (the particle pass's shape):
+    # waves on contiguous quarters (kind 2) or walking the tile together (kind 3)
+    for kind, name in ((2, "tiles_wave_quarters"), (3, "tiles_wave_rounds")):
+        sweep = {}
+        for U in (1, 2, 4):
+            for tiles in (16384, 65536):
+                sweep[f"U{U}_t{tiles}"] = timed(lambda kind=kind, U=U, tiles=tiles: probe_tiles(kind, U, tiles),
+                                                56 * n)
         best = max(sweep, key=lambda k: sweep[k]["GB/s"])
         res[name] = {"best": best, **sweep[best], "sweep": sweep}
         print(f"[bench] {name}: best {best} {sweep[best]}", file=sys.stderr, flush=True)

@@ -306,7 +306,11 @@ int zpic_particle_stream_probe(int *ix, float *x, float *u, size_t n, void *stre
  * from a to b; kind 1 reads and writes back n particle records (a = int2 ix,
  * b = float2 x, c = float3 u).  loads_per_lane (1, 2, 4 or 8) independent
  * records or words are in flight per lane; `blocks` workgroups of 256 lanes
- * stride over the buffers.  Times the HBM ceiling of an access pattern. */
+ * stride over the buffers.  Kinds 2 and 3: the records as `blocks` tiles of
+ * n / blocks records, one workgroup per tile; kind 2 with each of the four
+ * waves streaming a contiguous quarter of the tile (the particle pass's
+ * order), kind 3 with the waves walking the tile together (loads_per_lane
+ * 1, 2 or 4).  Times the HBM ceiling of an access pattern. */
 int zpic_bandwidth_probe(int kind, void *a, void *b, void *c, size_t n, int loads_per_lane, int blocks,
                          void *stream);
 
@@ -333,10 +337,12 @@ int zpic_div_corr_x(float *E, float *B, const zpic_geom *g, float dx, float dy, 
 int zpic_set_threads_per_tile(int threads);
 int zpic_get_threads_per_tile(void);
 
-/* Tile sort of the fast mode, library-wide: 0 (default) = order-free (the
- * species objects stage leavers in the particle pass, see zpic_sort.stage_ix),
- * 1 = the reference's single-thread within-tile order (particles.cpp:379-614),
- * as in the deterministic mode, which always keeps it. */
+/* Tile sort of the fast mode, library-wide: 1 (default) = the reference's
+ * single-thread within-tile order (particles.cpp:379-614), as in the
+ * deterministic mode, which always keeps it; 0 = order-free (the species
+ * objects stage leavers in the particle pass, see zpic_sort.stage_ix, and
+ * one placement pass fills each tile; counts, offsets and per-tile multisets
+ * are the reference's, the order within a tile is not). */
 int zpic_set_sort_order(int reference_order);
 int zpic_get_sort_order(void);
 /* Threads per tile the fast particle pass actually uses for tiles of
@@ -421,8 +427,9 @@ void zpic_batch_fields_free(zpic_batch_fields *plan);
  * pass with the threads per tile that fit its tiles (as
  * zpic_fitted_threads_per_tile(threads_per_tile, tile_vol)), so each computes
  * what its own Simulation::advance computes with that setting.  Fast mode
- * with the order-free sort (every species item's stage buffers set) and
- * periodic boundaries only.  Workgroups walk a list of (stage, instance,
+ * and periodic boundaries only; the sort is the order-free one when every
+ * species item's stage buffers are set, the reference-order one when none
+ * is.  Workgroups walk a list of (stage, instance,
  * tiles) work items in dependency order and wait on per-(stage, instance)
  * completion counters, so instances overlap; advance enqueues a counter
  * reset and the persistent kernel (two launches, graph-capturable).
@@ -729,8 +736,8 @@ int zpic_batch_add(zpic_batch *b, zpic_simulation *sim);
 int zpic_batch_add_threads(zpic_batch *b, zpic_simulation *sim, int threads_per_tile);
 int zpic_batch_set_graph(zpic_batch *b, int on);
 /* on = 1: every step is one persistent launch over all instances
- * (zpic_batch_step; fast mode with the order-free sort, periodic
- * boundaries); on = 0 (default): one launch per stage. */
+ * (zpic_batch_step; periodic boundaries, the library's fast-mode sort
+ * order); on = 0 (default): one launch per stage. */
 int zpic_batch_set_persistent(zpic_batch *b, int on);
 /* Synchronises; ZPIC_EHIP if a persistent step reported an error. */
 int zpic_batch_status(zpic_batch *b);

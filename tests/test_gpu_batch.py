@@ -195,15 +195,24 @@ def test_persistent_batch_graph_replays(zp):
         s.close()
 
 
-def test_persistent_batch_rejects_reference_order(zp):
-    """The persistent step places particles with the order-free sort only."""
-    a = make(zp, SHAPES[0])
-    zp.set_sort_order(True)
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_order_free_sort_matches_single_instances(zp, persistent):
+    """With the order-free tile sort (zpic_set_sort_order(0)) the batch runs
+    the staged sort for every instance and still equals each instance's own
+    run with that sort."""
+    steps = 10
+    old = zp.get_sort_order()
+    zp.set_sort_order(False)
     try:
-        b = zp.Batch([a], persistent=True)
-        with pytest.raises(zp.ZpicError, match="order-free"):
-            b.advance(1)
-        b.close()
+        ref = singles(zp, 256, steps)
+        sims = [make(zp, sh) for sh in SHAPES]
+        batch = zp.Batch(sims, threads_per_tile=256, persistent=persistent)
+        batch.advance(steps)
+        batch.status()
+        for k, (s, (st, en)) in enumerate(zip(sims, ref)):
+            assert_same(state(s), st, f"instance {k}")
+        batch.close()
+        for s in sims:
+            s.close()
     finally:
-        zp.set_sort_order(False)
-    a.close()
+        zp.set_sort_order(old)

@@ -26,7 +26,7 @@ def test_cxx_dropin_weibel_matches_reference(tmp_path):
     print("[gpu-dropin] " + r.stdout.strip(), flush=True)
     gold = np.load(GOLD)
     keys = [k for k in gold.files if k.startswith("s")]
-    assert len(keys) > 60
+    assert len(keys) == 60  # 3 snapshots x (4 field arrays + 2 species x 8 arrays)
     for key in keys:
         ref = gold[key]
         got = np.fromfile(os.path.join(tmp_path, key + ".bin"), dtype=ref.dtype)

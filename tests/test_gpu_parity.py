@@ -38,6 +38,25 @@ def zp():
     return pyzpic
 
 
+@pytest.fixture(params=["reference", "free"])
+def sort_order(request, zp):
+    """Runs a fast-mode test with each tile-sort order (the reference's
+    within-tile order, the default, and the order-free placement), restoring
+    the library setting afterwards."""
+    old = zp.get_sort_order()
+    zp.set_sort_order(request.param == "reference")
+    yield request.param
+    zp.set_sort_order(old)
+
+
+@pytest.fixture
+def order_free(zp):
+    old = zp.get_sort_order()
+    zp.set_sort_order(False)
+    yield
+    zp.set_sort_order(old)
+
+
 def gpu_weibel(zp, cfg, det=False):
     return zp.weibel(ntiles=cfg["ntiles"], nx=cfg["nx"], ppc=cfg["ppc"],
                      box=tuple(np.float32(b) for b in cfg["box"]), dt=cfg["dt"], uth=cfg["uth"],
@@ -180,7 +199,7 @@ def test_fast_boundaries_close_to_reference(zp, case):
     sim.close()
 
 
-def test_fast_first_step(zp):
+def test_fast_first_step(zp, sort_order):
     cfg = cfg_of((4, 4), (16, 16), (4, 4))
     say("fast 64x64 ppc 4x4 one step vs oracle")
     sim = gpu_weibel(zp, cfg)
@@ -201,7 +220,7 @@ def test_fast_first_step(zp):
     assert sim.get_nmove() == osim.nmove(0) + osim.nmove(1)
 
 
-def test_fast_step_from_evolved_state(zp):
+def test_fast_step_from_evolved_state(zp, sort_order):
     """One step from the GPU's own state after 30 steps: the oracle gets the
     same particles and fields, so cells, positions and momenta must agree bit
     for bit and J, E, B to summation-order rounding."""
@@ -229,6 +248,7 @@ def test_fast_reference_order_sort(zp):
     (particles.cpp:379-614)."""
     cfg = cfg_of((4, 4), (16, 16), (4, 4), uth=(0.3, 0.3, 0.3))
     say("fast mode, reference sort order: one step vs oracle")
+    old = zp.get_sort_order()
     zp.set_sort_order(True)
     try:
         sim = gpu_weibel(zp, cfg)
@@ -241,10 +261,10 @@ def test_fast_reference_order_sort(zp):
                 assert same_bits(p[name], q[name]), f"species {k} {name}"
         sim.close()
     finally:
-        zp.set_sort_order(False)
+        zp.set_sort_order(old)
 
 
-def test_fast_sort_more_leavers_than_slot_rounds(zp):
+def test_fast_sort_more_leavers_than_slot_rounds(zp, order_free):
     """Order-free sort with more than 2048 leavers per tile (4x4-cell tiles,
     ppc 32x32, relativistic thermal spread): the placement runs in several
     rounds of hole slots.  Per-tile particle sets equal the oracle's after one
@@ -277,7 +297,7 @@ def test_fast_sort_more_leavers_than_slot_rounds(zp):
     sim.close()
 
 
-def test_fast_sort_is_reproducible(zp):
+def test_fast_sort_is_reproducible(zp, sort_order):
     """Two fast-mode runs of the same set-up leave identical buffers (the
     order-free placement is a fixed enumeration, not a race)."""
     cfg = cfg_of((4, 3), (16, 16), (3, 3), uth=(0.4, 0.4, 0.4), ufl=(0.2, 0.1, 0.5))
@@ -293,7 +313,7 @@ def test_fast_sort_is_reproducible(zp):
 
 
 @pytest.mark.parametrize("case", ["hot_small_tiles", "wrap_1x3_by2", "odd_tiles_bx1", "comp_cx1"])
-def test_fast_sort_edge_cases(zp, case):
+def test_fast_sort_edge_cases(zp, case, sort_order):
     cfg = driver_args_to_config(DIGESTS[case]["args"])
     say(f"fast mode edge case {case}: 3 steps, particles vs oracle")
     sim = gpu_weibel(zp, cfg)

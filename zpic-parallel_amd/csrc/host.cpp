@@ -197,7 +197,8 @@ void swap_buffers( Particles & a, Particles & b ) {
     std::swap( a.offset, b.offset );
 }
 
-ParticleSort::ParticleSort( uint2 const nt2, uint32_t const max_part ) : ntiles( nt2.x * nt2.y ), idx_size( max_part ) {
+ParticleSort::ParticleSort( uint2 const nt2, uint32_t const max_part )
+    : buf{}, ntiles( nt2.x * nt2.y ), idx_size( max_part ) {
     buf.idx = device_alloc<int>( max_part );
     buf.npt = device_alloc<int>( 9 * (size_t) ntiles );
     buf.new_np = device_alloc<int>( ntiles );
@@ -692,6 +693,7 @@ void Species::move( vec3grid<float3> * const J ) {
     const float2 qnx = make_float2( q * dx.x / dt, q * dx.y / dt );
     zpic_geom g = J->geom();
     zpic_particles p = particles->view();
+    set_staging( sort->buf );  // the following tile_sort places what this pass stages
     check( zpic_move_deposit( &p, &g, J->fbuf(), dt_dx.x, dt_dx.y, q, qnx.x, qnx.y, &sort->buf, deterministic,
                               stream() ) );
 }

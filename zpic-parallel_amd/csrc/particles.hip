@@ -954,12 +954,12 @@ constexpr int SORT_CACHE = 4;  // iterations per wave whose list entries and cel
 // SORTED: copy_sorted (particles.cpp:628-763) instead: every particle goes to
 // T at the new offsets, the n0 stayers (after the hole fill) reserved as
 // direction 4 in the same by-source, by-direction order, and T.np is set.
+// Tile s's part (one workgroup of at least 128 threads, any number of waves).
 template <bool SORTED>
-__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const SortArgs * tab, const int2 * map ) {
+__device__ inline void sort_out_tile( const SortArgs & a, const int s ) {
     __shared__ int s_start[9];
     __shared__ int s_pre[MAX_SEG + 1];
     __shared__ int s_hdr[4];
-    const int s = select_instance( a, tab, map );
     const PartG P = gview( a.P ), T = gview( a.T );
     const SortG S = gview( a.S );
     const SortGeo q = a.q;
@@ -1038,8 +1038,8 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
     const int W = blockDim.x / WAVE, w = tid / WAVE, lane = tid % WAVE;
     const int qlen = ( ( nl + W - 1 ) / W + WAVE - 1 ) & ~( WAVE - 1 );
     const int j0 = min( nl, w * qlen ), j1 = min( nl, j0 + qlen );
-    __shared__ int s_dir[SORT_BLOCK / WAVE][9];
-    __shared__ int s_stay[SORT_BLOCK / WAVE];
+    __shared__ int s_dir[FAST_BLOCK / WAVE][9];
+    __shared__ int s_stay[FAST_BLOCK / WAVE];
     // The first SORT_CACHE iterations of a wave keep their leaver entries and
     // tail cells in registers from pass 1 for passes 2 and 3 (a wave's
     // quarter is rarely longer), so that those passes start from the records'
@@ -1172,11 +1172,16 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const So
     }
 }
 
+template <bool SORTED>
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_out( SortArgs a, const SortArgs * tab, const int2 * map ) {
+    const int s = select_instance( a, tab, map );
+    sort_out_tile<SORTED>( a, s );
+}
+
 // copy_in (particles.cpp:559-614): staged particles appended after the
 // in-place stayers, or first into the gap when the range moved down.
-__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( SortArgs a, const SortArgs * tab, const int2 * map ) {
+__device__ inline void sort_in_tile( const SortArgs & a, const int t ) {
     __shared__ int s_hdr[4];
-    const int t = select_instance( a, tab, map );
     const PartG P = gview( a.P ), T = gview( a.T );
     const SortG S = gview( a.S );
     if ( threadIdx.x == 0 ) {
@@ -1200,6 +1205,11 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( SortArgs a, const Sor
         P.np[t] = n + nin;
         P.off[t] = no;
     }
+}
+
+__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( SortArgs a, const SortArgs * tab, const int2 * map ) {
+    const int t = select_instance( a, tab, map );
+    sort_in_tile( a, t );
 }
 
 // ---------------------------------------------------------------------------
@@ -1325,7 +1335,8 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t, int *
     __shared__ int s_seg[9][MAX_SEG + 1];  // segment counts of the 9 sources' lists (e = 4: the tile)
     __shared__ int s_so[9][3];             // source tile, old offset, old count
     __shared__ int s_pre[9][MAX_SEG + 1];  // prefix of the segment counts
-    __shared__ int s_epre[10];             // prefix of the incoming lists' lengths (e != 4)
+    __shared__ int s_cin[9];               // particles source e sends to this tile
+    __shared__ int s_ebase[9];             // first rank of source e's particles
     __shared__ int s_gpre[MAX_SEG + 1];    // free-slot prefix of the tile's staging region
     __shared__ int s_hdr[3];
     __shared__ int s_w[FAST_BLOCK / WAVE];
@@ -1350,6 +1361,10 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t, int *
     } else if ( tid < 9 * 16 + 3 ) {
         const int h = tid - 9 * 16;
         s_hdr[h] = h == 0 ? new_off[t] : h == 1 ? S.new_np[t] : S.tmp_np[t];
+    } else if ( tid >= 9 * 16 + 3 && tid < 9 * 16 + 12 ) {
+        const int e = tid - ( 9 * 16 + 3 );
+        const int s = source_of( t, e, q );
+        s_cin[e] = ( e != 4 && s >= 0 ) ? S.npt[9 * s + e] : 0;
     }
     __syncthreads();
     if ( tid < 9 ) {
@@ -1364,19 +1379,20 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t, int *
     if ( tid == 16 ) gap_prefix( s_seg[4], s_so[4][2], s_seg[4][MAX_SEG], s_gpre );
     __syncthreads();
     if ( tid == 0 ) {
-        int acc = 0;
+        // ranks: the tile's staged stayers first, then the sources' particles
+        // by direction, each source's in its list order
+        int acc = s_hdr[2];
         for ( int e = 0; e < 9; e++ ) {
-            s_epre[e] = acc;
-            if ( e != 4 ) acc += s_pre[e][MAX_SEG];
+            s_ebase[e] = acc;
+            acc += s_cin[e];
         }
-        s_epre[9] = acc;
     }
     __syncthreads();
 
     const int o = s_so[4][1], np = s_so[4][2], chunk = s_seg[4][MAX_SEG];
     const int no = s_hdr[0], nn = s_hdr[1], ns = s_hdr[2];
     const int nl = s_pre[4][MAX_SEG];
-    const int etot = s_epre[9];
+    const int W = blockDim.x / WAVE, w = tid / WAVE, lane = tid % WAVE;
     // slots of the new range outside the old one: [no, no + lb1) and [b2, b2 + lb2)
     const int lb1 = max( min( no + nn, o ) - no, 0 );
     const int b2 = max( no, o + np ), lb2 = max( no + nn - b2, 0 );
@@ -1420,34 +1436,36 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t, int *
                 P.u[dst] = Gu[src];
             }
         }
-        // leavers of the sources towards this tile: ranks ns, ns + 1, ...
-        int run = ns;
-        for ( int j0 = 0; j0 < etot; j0 += blockDim.x ) {
-            const int j = j0 + tid;
-            int src = -1;
-            bool match = false;
-            if ( j < etot ) {
-                int e = 0;
-#pragma unroll
-                for ( int v = 1; v < 9; v++ ) e += s_epre[v] <= j;
-                // e is the last direction whose list starts at or before j
-                // (the tile's own list, e = 4, has zero length here)
-                const int jj = j - s_epre[e];
-                const int w = segment_of( jj, s_pre[e] );
-                src = s_so[e][1] + w * s_seg[e][MAX_SEG] + ( jj - s_pre[e][w] );
-                match = ( S.idx[src] & 15 ) == e;
-            }
-            int tot;
-            const int k = run + block_rank( match, tot, s_w );
-            if ( match ) {
-                const int dst = slot( k );
-                if ( dst >= 0 ) {
-                    P.ix[dst] = Gix[src];
-                    P.x[dst] = Gx[src];
-                    P.u[dst] = Gu[src];
+        // the sources' particles towards this tile: source e's list entries of
+        // direction e, ranks from s_ebase[e] in list order.  The eight lists
+        // are independent (their rank ranges are known): wave w walks the
+        // lists w, w + W, ... with wave ballots, no workgroup barrier.
+        for ( int li = w; li < 8; li += W ) {
+            const int e = li < 4 ? li : li + 1;
+            const int n_e = s_pre[e][MAX_SEG];
+            if ( s_cin[e] == 0 ) continue;
+            const int ch = s_seg[e][MAX_SEG], oe = s_so[e][1];
+            int run = s_ebase[e];
+            for ( int j0 = 0; j0 < n_e; j0 += WAVE ) {
+                const int jj = j0 + lane;
+                int src = -1;
+                bool match = false;
+                if ( jj < n_e ) {
+                    const int sw = segment_of( jj, s_pre[e] );
+                    src = oe + sw * ch + ( jj - s_pre[e][sw] );
+                    match = ( S.idx[src] & 15 ) == e;
                 }
+                const unsigned long long m = __ballot( match );
+                if ( match ) {
+                    const int dst = slot( run + count_below( m ) );
+                    if ( dst >= 0 ) {
+                        P.ix[dst] = Gix[src];
+                        P.x[dst] = Gx[src];
+                        P.u[dst] = Gu[src];
+                    }
+                }
+                run += __popcll( m );
             }
-            run += tot;
         }
         __syncthreads();  // s_slot is rebuilt by the next round
     }
@@ -1553,9 +1571,12 @@ int check_geom( const zpic_geom * g ) {
 // on the 2048^2 Weibel step the particle pass took 7.08 ms with 4 waves per
 // tile against 7.36 ms with 8 (fewer private J planes to clear and reduce).
 int g_threads = 256;  // library-wide setting
-// Fast-mode tile sort: 0 = order-free (staged leavers), 1 = the reference's
-// within-tile order.  The deterministic mode always keeps the reference's.
-int g_reference_order = 0;
+// Fast-mode tile sort: 1 (default) = the reference's within-tile order,
+// 0 = order-free (leavers staged by the particle pass).  The deterministic
+// mode always keeps the reference's.  On the 2048^2 Weibel step the
+// reference order is the faster: the particle pass's staging writes of
+// scattered leavers cost more than the single placement pass saves.
+int g_reference_order = 1;
 
 constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup
 
@@ -1789,6 +1810,8 @@ enum StepKind : int {
     SK_COUNT,     // offsets scan, nmove and push energy of the species
     SK_SHIFT,     // order-free sort: stayers outside the new range staged (tile t0)
     SK_PLACE,     // order-free sort: placement (tile t0)
+    SK_OUT,       // reference-order sort: copy_out (tile t0)
+    SK_IN,        // reference-order sort: copy_in (tile t0)
     SK_ADD_X,     // add_from_gc x pass of J (tiles [t0, t1))
     SK_ADD_Y,     // add_from_gc y pass of J
     SK_COPY_XY,   // copy_to_gc of one grid (tiles [t0, t1))
@@ -1902,6 +1925,8 @@ __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
             break;
         case SK_SHIFT: sort_shift_tile( uniform_load( T.sort + it.arg ), it.t0 ); break;
         case SK_PLACE: sort_in_fast_tile( uniform_load( T.sort + it.arg ), it.t0, reinterpret_cast<int *>( smem ) ); break;
+        case SK_OUT: sort_out_tile<false>( uniform_load( T.sort + it.arg ), it.t0 ); break;
+        case SK_IN: sort_in_tile( uniform_load( T.sort + it.arg ), it.t0 ); break;
         case SK_ADD_X:
         case SK_ADD_Y:
         case SK_COPY_XY:
@@ -1987,6 +2012,7 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
     std::vector<int> waves( n );
     int block = 256;
     size_t lds = sizeof( int ) * SLOT_CAP;
+    bool staged = false;  // order-free sort (every item's staging buffers set) or the reference order
     for ( int i = 0; i < n; i++ ) {
         order[i] = i;
         const zpic_batch_fields_item & f = fl[i];
@@ -2005,8 +2031,9 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
         lds = std::max( { lds, particles_lds_bytes( vol, waves[i], true, true ), sizeof( float ) * 6 * (size_t) vol } );
         for ( int k = 0; k < nslots; k++ ) {
             const zpic_batch_species & s = sp[k * n + i];
-            if ( !s.s.stage_ix || !s.s.stage_x || !s.s.stage_u )
-                return set_error( ZPIC_EINVAL, "the persistent step needs the order-free sort (staging buffers)" );
+            const bool st = s.s.stage_ix && s.s.stage_x && s.s.stage_u;
+            if ( i == 0 && k == 0 ) staged = st;
+            if ( st != staged ) return set_error( ZPIC_EINVAL, "batch entries must all use the same sort (staged or not)" );
             if ( s.fld.ntiles_x != f.g.ntiles_x || s.fld.ntiles_y != f.g.ntiles_y || s.fld.nx != f.g.nx ||
                  s.fld.ny != f.g.ny || s.E != f.E || s.B != f.B || s.J != f.J )
                 return set_error( ZPIC_EINVAL, "species and fields of an instance do not match" );
@@ -2072,8 +2099,8 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
     for ( int k = 0; k < nslots; k++ ) {
         stages.push_back( { SK_NEWNP, k } );
         stages.push_back( { SK_COUNT, k } );
-        stages.push_back( { SK_SHIFT, k } );
-        stages.push_back( { SK_PLACE, k } );
+        stages.push_back( { staged ? SK_SHIFT : SK_OUT, k } );
+        stages.push_back( { staged ? SK_PLACE : SK_IN, k } );
     }
     stages.push_back( { SK_ADD_X, 0 } );
     stages.push_back( { SK_ADD_Y, 0 } );
@@ -2127,6 +2154,8 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
             case SK_COUNT:
             case SK_SHIFT:
             case SK_PLACE:
+            case SK_OUT:
+            case SK_IN:
                 dep = ( si - 1 ) * n + i;
                 if ( st.kind == SK_COUNT ) push( SK_COUNT, st.slot * n + i, 0, nt, dep );
                 else
@@ -2326,6 +2355,50 @@ __global__ void __launch_bounds__( 256 ) k_stream_probe_u( int2 * __restrict__ i
         for ( int k = 0; k < U; k++ ) {
             const size_t i = base + (size_t) k * 256;
             if ( i < n ) {
+                c[k].x ^= 1;
+                p[k].x = p[k].x + 1.0f;
+                v[k].z = v[k].z + 1.0f;
+                gix[i] = c[k];
+                gx[i] = p[k];
+                gu[i] = v[k];
+            }
+        }
+    }
+}
+
+// The particle records of `gridDim.x` tiles of tnp records each, read and
+// written back changed, one 256-lane workgroup per tile, U iterations of 64
+// records in flight per wave.  ROUNDS = false: wave w streams the contiguous
+// quarter [w c, (w + 1) c) of its tile (the particle pass's order); ROUNDS =
+// true: the four waves walk the tile together, 256 records per round.
+// Measurement only.
+template <int U, bool ROUNDS>
+__global__ void __launch_bounds__( 256 ) k_tile_probe( int2 * __restrict__ ix, float2 * __restrict__ x,
+                                                       float3 * __restrict__ u, int tnp ) {
+    const size_t base = (size_t) blockIdx.x * tnp;
+    const gptr<int2> gix = gbl( ix + base );
+    const gptr<float2> gx = gbl( x + base );
+    const gptr<float3> gu = gbl( u + base );
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int chunk = ROUNDS ? tnp : ( ( ( tnp + 3 ) / 4 + WAVE - 1 ) & ~( WAVE - 1 ) );
+    const int beg = ROUNDS ? w * WAVE : w * chunk;
+    const int end = ROUNDS ? tnp : min( tnp, beg + chunk );
+    const int step = ROUNDS ? 4 * WAVE : WAVE;
+    for ( int i0 = beg; i0 < end; i0 += step * U ) {
+        int2 c[U];
+        float2 p[U];
+        float3 v[U];
+#pragma unroll
+        for ( int k = 0; k < U; k++ ) {
+            const int i = min( i0 + k * step + lane, end - 1 );
+            c[k] = gix[i];
+            p[k] = gx[i];
+            v[k] = gu[i];
+        }
+#pragma unroll
+        for ( int k = 0; k < U; k++ ) {
+            const int i = i0 + k * step + lane;
+            if ( i < end ) {
                 c[k].x ^= 1;
                 p[k].x = p[k].x + 1.0f;
                 v[k].z = v[k].z + 1.0f;
@@ -2545,11 +2618,36 @@ int zpic_particle_stream_probe( int * ix, float * x, float * u, size_t n, void *
 
 int zpic_bandwidth_probe( int kind, void * a, void * b, void * c, size_t n, int loads_per_lane, int blocks,
                           void * stream ) {
-    if ( kind < 0 || kind > 1 ) return set_error( ZPIC_EINVAL, "probe kind must be 0 (copy) or 1 (particle records)" );
-    if ( ( !a || !b || ( kind == 1 && !c ) ) && n ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( kind < 0 || kind > 3 )
+        return set_error( ZPIC_EINVAL, "probe kind must be 0 (copy), 1 (particle records), 2 or 3 (tiles)" );
+    if ( ( !a || !b || ( kind >= 1 && !c ) ) && n ) return set_error( ZPIC_EINVAL, "null buffer" );
     if ( blocks < 1 ) return set_error( ZPIC_EINVAL, "blocks must be positive" );
     if ( n == 0 ) return ZPIC_OK;
     hipStream_t st = as_stream( stream );
+    if ( kind >= 2 ) {
+        // blocks = tiles of n / blocks records each
+        if ( n % (size_t) blocks || n / (size_t) blocks > ( 1u << 30 ) )
+            return set_error( ZPIC_EINVAL, "tile probe: n must be a multiple of blocks" );
+        const int tnp = (int) ( n / (size_t) blocks );
+#define ZPIC_TPROBE( U )                                                                                               \
+    case U:                                                                                                            \
+        if ( kind == 2 )                                                                                               \
+            hipLaunchKernelGGL( ( k_tile_probe<U, false> ), dim3( blocks ), dim3( 256 ), 0, st, (int2 *) a,            \
+                                (float2 *) b, (float3 *) c, tnp );                                                     \
+        else                                                                                                           \
+            hipLaunchKernelGGL( ( k_tile_probe<U, true> ), dim3( blocks ), dim3( 256 ), 0, st, (int2 *) a,             \
+                                (float2 *) b, (float3 *) c, tnp );                                                     \
+        break;
+        switch ( loads_per_lane ) {
+            ZPIC_TPROBE( 1 )
+            ZPIC_TPROBE( 2 )
+            ZPIC_TPROBE( 4 )
+        default: return set_error( ZPIC_EINVAL, "loads_per_lane must be 1, 2 or 4 for tile probes" );
+        }
+#undef ZPIC_TPROBE
+        ZPIC_LAUNCH_CHECK( "tile probe" );
+        return ZPIC_OK;
+    }
 #define ZPIC_PROBE( U )                                                                                                \
     case U:                                                                                                            \
         if ( kind == 0 )                                                                                               \

@@ -842,8 +842,7 @@ class Batch {
     // Capture the step in a hipGraph on the first advance and replay it.
     void set_graph( bool on );
     // One persistent launch per step (zpic_batch_step) instead of one launch
-    // per stage; needs the fast mode's order-free sort and periodic
-    // boundaries.
+    // per stage (periodic boundaries).
     void set_persistent( bool on );
     // Synchronises and throws if a persistent step reported an error.
     void check_status();

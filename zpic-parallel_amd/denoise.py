@@ -15,8 +15,9 @@ the network is worse than the filter by more than `tol`, the production
 run's filter falls back to the binomial filter for good.
 
 Judge.  The growth rate of the magnetic-field energy during the linear
-phase of the instability, fitted per run, compared with the plain high-ppc
-run's.
+phase of the instability, fitted over one band of B energy common to every
+run compared (above the noisiest run's noise floor, below the lowest
+saturation level), against the plain high-ppc run's.
 """
 import time
 
@@ -159,6 +160,17 @@ def growth_rate(eb, dt, lo_factor=30.0, hi_factor=30.0, band=None):
     t = np.arange(a, b + 1) * dt
     slope = np.polyfit(t, np.log(eb[a:b + 1]), 1)[0]
     return 0.5 * slope, a, b
+
+
+def common_band(series, lo_factor=30.0, hi_factor=30.0):
+    """One B-energy band for a set of runs: from lo_factor times the highest
+    initial level (the noisiest run's noise floor) to 1/hi_factor of the
+    lowest peak.  Every run is then fitted over the same amplitudes, above
+    every run's noise and below every run's saturation."""
+    series = [np.asarray(e, np.float64) for e in series]
+    lo = lo_factor * max(float(np.median(e[:5])) for e in series)
+    hi = min(float(e.max()) for e in series) / hi_factor
+    return lo, hi
 
 
 def field_b_energy(sim):

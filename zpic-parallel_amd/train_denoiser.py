@@ -10,8 +10,9 @@
    and reported (network vs raw subsample vs binomial filters).
 3. Judge (`--judge`): on a `--judge-grid`^2 grid, the plain 32x32-ppc run
    against 2x2-ppc runs with the network (with the shadow check), with the
-   binomial filter, and unfiltered: growth rate of the magnetic energy during
-   the linear phase, and wall time per step.
+   binomial filter, and unfiltered: growth rate of the magnetic energy over a
+   band of B energy common to all runs (denoise.common_band), per-run
+   linear-phase fits beside it, and wall time per step.
 
 Prints progress lines and ends with one JSON line.
 """
@@ -87,8 +88,8 @@ def judge(params, scale, n, steps, shadow_every, shadow_n):
     walls = {name: wall_per_step(make, steps) for name, make in makers.items()}
     walls["lo_learned_with_shadow"] = wall_per_step(learned, steps, shadow_args)
     log("ms/step " + ", ".join(f"{k} {1e3 * v:.3f}" for k, v in walls.items()))
-    top = float(series["plain_32x32"].max())
-    band = (top * 1e-4, top / 30.0)
+    # the judge's band: common to every run (denoise.common_band)
+    band = denoise.common_band(series.values())
     ref = denoise.growth_rate(series["plain_32x32"], dt)
     ref_band = denoise.growth_rate(series["plain_32x32"], dt, band=band)
     res.update({"growth_rate_plain": ref[0], "window_plain": ref[1:], "band": band,
@@ -100,6 +101,7 @@ def judge(params, scale, n, steps, shadow_every, shadow_n):
                      "growth_rate_band": gb[0], "window_band": gb[1:],
                      "rel_err_band": abs(gb[0] - ref_band[0]) / ref_band[0],
                      "ms_per_step": 1e3 * walls[name], "eb_first": float(eb[0]), "eb_max": float(eb.max())}
+    res["series"] = {k: [float(v) for v in eb] for k, eb in series.items()}
     res["lo_learned"]["ms_per_step_with_shadow"] = 1e3 * walls["lo_learned_with_shadow"]
     res["speedup_learned_vs_plain"] = walls["plain_32x32"] / walls["lo_learned"]
     res["speedup_learned_with_shadow_vs_plain"] = walls["plain_32x32"] / walls["lo_learned_with_shadow"]
