@@ -245,6 +245,17 @@ int zpic_yee(float *E, float *B, const float *J, const zpic_geom *g, float dt_dx
 int zpic_field_energy(const float *E, const float *B, const zpic_geom *g, double *tile_scratch,
                       double *out6, void *stream);
 
+/* No counterpart in the reference.  zpic_field_energy for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the per-tile sums live in
+ * a library-held device array of 6 * 65536 doubles (3 MiB per device, kept
+ * until the process ends), so no tile_scratch argument and no allocation.
+ * Grids of more than 65536 tiles give ZPIC_ERANGE.  Results equal
+ * zpic_field_energy's bit for bit.  Every call on one device shares that
+ * array: calls that overlap on one device (other streams, other host
+ * threads) race and must be serialised by the caller.  `stream` must belong
+ * to the current device. */
+int zpic_field_energy_noalloc(const float *E, const float *B, const zpic_geom *g, double *out6, void *stream);
 
 /* Sum of per-tile doubles (the OpenMP reduction of Species::push,
  * species.cpp:2311): in tile order when deterministic != 0 (the reference's

@@ -979,21 +979,87 @@ int zpic_current_bc( float * J, const zpic_geom * g, const int bc[4], void * str
     return physical_bc( J, nullptr, g, bc, false, stream );
 }
 
-int zpic_field_energy( const float * E, const float * B, const zpic_geom * gg, double * scratch, double * out6,
-                       void * stream ) {
+}  // extern "C"
+
+namespace zpic {
+namespace {
+
+int field_energy_args( const float * E, const float * B, const zpic_geom * gg, const double * out6 ) {
     if ( int rc = check( gg ) ) return rc;
-    if ( !E || !B || !scratch || !out6 ) return set_error( ZPIC_EINVAL, "null buffer" );
+    if ( !E || !B || !out6 ) return set_error( ZPIC_EINVAL, "null buffer" );
     // the reference's inner bound nx.y must stay inside the tile
     if ( gg->offset + ( gg->ny - 1 ) * gg->ext_x + gg->ny - 1 >= gg->tile_vol )
         return set_error( ZPIC_ERANGE, "field energy loop would leave the tile (ny too large for nx)" );
+    return ZPIC_OK;
+}
+
+// Per-tile sums into tile_sums[6 * ntiles] (every entry written), then their
+// sums in tile order into out6.
+int field_energy_launch( const float * E, const float * B, const zpic_geom * gg, double * tile_sums, double * out6,
+                         hipStream_t st ) {
     const G g = make( gg );
     const int nt = g.ntx * g.nty;
-    hipStream_t st = as_stream( stream );
-    hipLaunchKernelGGL( k_energy_tiles, dim3( ( 6 * nt + 255 ) / 256 ), dim3( 256 ), 0, st, E, B, g, scratch );
+    hipLaunchKernelGGL( k_energy_tiles, dim3( ( 6 * nt + 255 ) / 256 ), dim3( 256 ), 0, st, E, B, g, tile_sums );
     ZPIC_LAUNCH_CHECK( "k_energy_tiles" );
-    hipLaunchKernelGGL( k_sum_tiles, dim3( 1 ), dim3( SUM_BLOCK ), 0, st, scratch, nt, 6, out6 );
+    hipLaunchKernelGGL( k_sum_tiles, dim3( 1 ), dim3( SUM_BLOCK ), 0, st, tile_sums, nt, 6, out6 );
     ZPIC_LAUNCH_CHECK( "k_sum_tiles" );
     return ZPIC_OK;
+}
+
+// Tile sums of the allocation-free field energy (zpic_field_energy_noalloc):
+// 6 doubles for each of at most NOALLOC_MAX_TILES tiles, 3 MiB, held by every
+// device the library runs on until the process ends.  Host code never uses
+// the symbol's name as an address: scratch_address() looks up each device's
+// own copy.
+constexpr int NOALLOC_MAX_TILES = 65536;
+static __device__ __align__( 256 ) double scratch[6 * NOALLOC_MAX_TILES];
+
+// Device address of `scratch` on the current device, looked up with
+// hipGetSymbolAddress on the first call made with that device current and
+// kept per device; null (and rc set) on failure.
+double * scratch_address( int & rc ) {
+    static std::mutex m;
+    static std::map<int, double *> addr;
+    int dev = 0;
+    hipError_t e = hipGetDevice( &dev );
+    if ( e != hipSuccess ) {
+        rc = hip_error( e, "hipGetDevice" );
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk( m );
+    auto it = addr.find( dev );
+    if ( it != addr.end() ) return it->second;
+    double * p = nullptr;
+    e = hipGetSymbolAddress( (void **) &p, HIP_SYMBOL( scratch ) );
+    if ( e != hipSuccess || !p ) {
+        rc = e != hipSuccess ? hip_error( e, "hipGetSymbolAddress(scratch)" )
+                             : set_error( ZPIC_EHIP, "hipGetSymbolAddress(scratch) returned null" );
+        return nullptr;
+    }
+    addr[dev] = p;
+    return p;
+}
+
+}  // namespace
+}  // namespace zpic
+
+extern "C" {
+
+int zpic_field_energy( const float * E, const float * B, const zpic_geom * gg, double * scratch_, double * out6,
+                       void * stream ) {
+    if ( int rc = field_energy_args( E, B, gg, out6 ) ) return rc;
+    if ( !scratch_ ) return set_error( ZPIC_EINVAL, "null buffer" );
+    return field_energy_launch( E, B, gg, scratch_, out6, as_stream( stream ) );
+}
+
+int zpic_field_energy_noalloc( const float * E, const float * B, const zpic_geom * gg, double * out6, void * stream ) {
+    if ( int rc = field_energy_args( E, B, gg, out6 ) ) return rc;
+    if ( (long long) gg->ntiles_x * gg->ntiles_y > NOALLOC_MAX_TILES )
+        return set_error( ZPIC_ERANGE, "zpic_field_energy_noalloc: more than 65536 tiles" );
+    int rc = ZPIC_OK;
+    double * tile_sums = scratch_address( rc );
+    if ( !tile_sums ) return rc;
+    return field_energy_launch( E, B, gg, tile_sums, out6, as_stream( stream ) );
 }
 
 int zpic_sum_tiles( const double * v, int n, double * out, int deterministic, void * stream ) {
