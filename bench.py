@@ -172,15 +172,16 @@ def bandwidth_ceilings():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the 2048^2 Weibel configuration is quoted on 100 steps (0.8 s of GPU time)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="weibel2048", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-mode", choices=("persistent", "stage"), default="persistent",
                     help="batch64: one persistent launch per step (default) or one launch per stage")
     ap.add_argument("--untuned", action="store_true", help="batch64: 16x16 tiles instead of the autotuner's")
-    ap.add_argument("--sort-order", choices=("free", "reference"), default="free",
-                    help="fast-mode tile sort: order-free placement (default) or the reference's within-tile order")
+    ap.add_argument("--sort-order", choices=("reference", "free"), default="reference",
+                    help="fast-mode tile sort: the reference's within-tile order (default) or order-free placement")
     ap.add_argument("--threads", type=int, default=0,
                     help="threads per tile of the particle pass (multiple of 64 in [64, 512]; 0 = library default)")
     ap.add_argument("--bandwidth", action="store_true",
@@ -280,7 +281,8 @@ def main():
             "workload": args.workload, "grid": [wl["ntiles"][0] * wl["nx"][0], wl["ntiles"][1] * wl["nx"][1]],
             "tiles": list(wl["ntiles"]), "tile": list(wl["nx"]), "ppc": list(wl["ppc"]), "species": nsp,
             "particles": npart, "dt": 0.07, "dx": 0.1, "filter": "binomial x order 1" if wl["filter"] else "none",
-            "threads_per_tile": pyzpic.get_threads_per_tile(), "parallelism": "single GPU",
+            "threads_per_tile": pyzpic.get_threads_per_tile(), "sort_order": args.sort_order,
+            "parallelism": "single GPU",
         },
         "push_deposit_sort_particles_per_ns": moved / args.steps / (pds_ms * 1e6) if pds_ms > 0 else None,
         "stage_ms": {k: round(v, 4) for k, v in tm.items()},
@@ -416,14 +418,14 @@ def run_lwfa(args, pyzpic):
 def run_batch(args, pyzpic):
     """The batched parameter scan: 64 Weibel instances (pyzpic.batch64_instances,
     seed 2024), tile shape and threads per tile of each (grid, ppc) from the
-    autotuner (timed on a 128x128 sample, 2 steps), advanced by one persistent
+    autotuner (timed on a 256x256 sample, 3 steps), advanced by one persistent
     launch per hipGraph-captured step (--batch-mode stage: one launch per
     stage instead)."""
     wl = WORKLOADS["batch64"]
     inst = pyzpic.batch64_instances(wl["instances"], wl["seed"])
     t_tune = time.perf_counter()
     if not args.untuned:
-        pyzpic.tune_instances(inst)
+        pyzpic.tune_instances(inst, steps=3, max_cells=256 * 256)
     t_tune = time.perf_counter() - t_tune
     print(f"[bench] building batch64: {len(inst)} instances (autotune {t_tune:.1f} s)", file=sys.stderr, flush=True)
     t_init = time.perf_counter()

@@ -1805,20 +1805,20 @@ namespace {
 // particle pass.
 enum StepKind : int {
     SK_ZERO = 0,  // Current::zero of tiles [t0, t1)
-    SK_PART,      // particle pass of tile t0
+    SK_PART,      // particle pass of tiles [t0, t1), one after the other
     SK_NEWNP,     // new per-tile counts of tiles [t0, t1)
     SK_COUNT,     // offsets scan, nmove and push energy of the species
-    SK_SHIFT,     // order-free sort: stayers outside the new range staged (tile t0)
-    SK_PLACE,     // order-free sort: placement (tile t0)
-    SK_OUT,       // reference-order sort: copy_out (tile t0)
-    SK_IN,        // reference-order sort: copy_in (tile t0)
+    SK_SHIFT,     // order-free sort: stayers outside the new range staged (tiles [t0, t1))
+    SK_PLACE,     // order-free sort: placement
+    SK_OUT,       // reference-order sort: copy_out
+    SK_IN,        // reference-order sort: copy_in
     SK_ADD_X,     // add_from_gc x pass of J (tiles [t0, t1))
     SK_ADD_Y,     // add_from_gc y pass of J
     SK_COPY_XY,   // copy_to_gc of one grid (tiles [t0, t1))
     SK_COPY_X,    // copy_to_gc_x of J after an x filter pass
     SK_COPY_Y,    // copy_to_gc_y of J after a y filter pass
-    SK_K3,        // one filter pass (tile t0)
-    SK_YEE,       // Yee update (tile t0)
+    SK_K3,        // one filter pass (tiles [t0, t1))
+    SK_YEE,       // Yee update (tiles [t0, t1))
 };
 
 struct StepItem {
@@ -1859,10 +1859,14 @@ __device__ inline X uniform_load( const X * p ) {
     return out;
 }
 
+// Polls with relaxed loads: an acquire load invalidates the L2 of the
+// polling workgroup's XCD on every poll, which evicted every other
+// workgroup's data and made the step an order of magnitude slower; the
+// caller issues one acquire fence after the waits instead.
 __device__ inline void step_wait( const StepTables & T, int c, int need ) {
     if ( c < 0 ) return;
     int spins = 0;
-    while ( __hip_atomic_load( T.counters + c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT ) < need ) {
+    while ( __hip_atomic_load( T.counters + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) < need ) {
         if ( ++spins > STEP_SPIN_LIMIT ) {
             __hip_atomic_fetch_or( T.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
             return;
@@ -1903,16 +1907,25 @@ __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
         if ( threadIdx.x == 0 ) {
             step_wait( T, it.dep0, it.need0 );
             step_wait( T, it.dep1, it.need1 );
+            // the caches this workgroup reads through (its CU's L1, its XCD's
+            // L2) are dropped once, before any wave loads the item's inputs
+            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
         }
         __syncthreads();
-        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
         switch ( it.kind ) {
         case SK_ZERO: {
             const grid_ops::GcArgs g = uniform_load( T.gc + it.arg );
             for ( int t = it.t0; t < it.t1; t++ ) grid_ops::zero_tile( g.buf, 3 * g.g.vol, t );
             break;
         }
-        case SK_PART: particles_tile<true, true, false>( uniform_load( T.part + it.arg ), it.t0, smem ); break;
+        case SK_PART: {
+            const PartArgs a = uniform_load( T.part + it.arg );
+            for ( int t = it.t0; t < it.t1; t++ ) {
+                if ( t > it.t0 ) __syncthreads();  // LDS of the previous tile read out
+                particles_tile<true, true, false>( a, t, smem );
+            }
+            break;
+        }
         case SK_NEWNP: {
             const SortArgs a = uniform_load( T.sort + it.arg );
             for ( int t = it.t0 + (int) threadIdx.x; t < it.t1; t += BLOCK ) sort_newnp_tile( a, t );
@@ -1923,10 +1936,20 @@ __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
             __syncthreads();
             batch_energy_body( uniform_load( T.energy + it.arg ) );
             break;
-        case SK_SHIFT: sort_shift_tile( uniform_load( T.sort + it.arg ), it.t0 ); break;
-        case SK_PLACE: sort_in_fast_tile( uniform_load( T.sort + it.arg ), it.t0, reinterpret_cast<int *>( smem ) ); break;
-        case SK_OUT: sort_out_tile<false>( uniform_load( T.sort + it.arg ), it.t0 ); break;
-        case SK_IN: sort_in_tile( uniform_load( T.sort + it.arg ), it.t0 ); break;
+        case SK_SHIFT:
+        case SK_PLACE:
+        case SK_OUT:
+        case SK_IN: {
+            const SortArgs a = uniform_load( T.sort + it.arg );
+            for ( int t = it.t0; t < it.t1; t++ ) {
+                if ( t > it.t0 ) __syncthreads();
+                if ( it.kind == SK_SHIFT ) sort_shift_tile( a, t );
+                else if ( it.kind == SK_PLACE ) sort_in_fast_tile( a, t, reinterpret_cast<int *>( smem ) );
+                else if ( it.kind == SK_OUT ) sort_out_tile<false>( a, t );
+                else sort_in_tile( a, t );
+            }
+            break;
+        }
         case SK_ADD_X:
         case SK_ADD_Y:
         case SK_COPY_XY:
@@ -1942,14 +1965,34 @@ __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
             }
             break;
         }
-        case SK_K3: grid_ops::kernel3_tile( uniform_load( T.k3 + it.arg ), it.t0, smem ); break;
-        case SK_YEE: grid_ops::yee_tile( uniform_load( T.yee + it.arg ), it.t0, smem ); break;
+        case SK_K3: {
+            const grid_ops::K3Args a = uniform_load( T.k3 + it.arg );
+            for ( int t = it.t0; t < it.t1; t++ ) {
+                if ( t > it.t0 ) __syncthreads();
+                grid_ops::kernel3_tile( a, t, smem );
+            }
+            break;
+        }
+        case SK_YEE: {
+            const grid_ops::YeeArgs a = uniform_load( T.yee + it.arg );
+            for ( int t = it.t0; t < it.t1; t++ ) {
+                if ( t > it.t0 ) __syncthreads();
+                grid_ops::yee_tile( a, t, smem );
+            }
+            break;
+        }
         default: break;
         }
-        // every wave's writes reach the device scope before the item counts as finished
-        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "agent" );
+        // Every wave waits for its stores to complete at the L2, then one
+        // device-scope release writes the L2 back before the item counts as
+        // finished (one L2 write-back per item, not one per wave: each is a
+        // whole-cache operation that serialises in the L2).
+        asm volatile( "s_waitcnt vmcnt(0)" ::: "memory" );
         __syncthreads();
-        if ( threadIdx.x == 0 ) __hip_atomic_fetch_add( T.counters + it.done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT );
+        if ( threadIdx.x == 0 ) {
+            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "agent" );
+            __hip_atomic_fetch_add( T.counters + it.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+        }
     }
 }
 
@@ -2039,7 +2082,9 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
                 return set_error( ZPIC_EINVAL, "species and fields of an instance do not match" );
             if ( !s.periodic_x || !s.periodic_y ) return set_error( ZPIC_EINVAL, "the persistent step runs periodic boundaries only" );
         }
-        weight[i] = (long long) f.g.ntiles_x * f.g.ntiles_y * f.g.tile_vol;
+        // work of the instance: its particles (buffer sizes) and its cells
+        weight[i] = 4LL * f.g.ntiles_x * f.g.ntiles_y * f.g.tile_vol;
+        for ( int k = 0; k < nslots; k++ ) weight[i] += sp[k * n + i].p.max_part;
     }
     if ( lds > LDS_LIMIT ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     std::stable_sort( order.begin(), order.end(), [&]( int a, int b ) { return weight[a] > weight[b]; } );
@@ -2114,7 +2159,23 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
     const int ns = (int) stages.size();
 
     // tiles per item of the light stages
-    constexpr int ZERO_TILES = 8, NEWNP_TILES = 1024, GC_TILES = 4;
+    constexpr int ZERO_TILES = 32, NEWNP_TILES = 1024, GC_TILES = 16;
+    // tiles per item of the per-tile stages: small tiles are grouped so that
+    // an item moves at least ~16384 particles (particle pass, sort) or covers
+    // at least 1024 cells (Yee, filter pass), amortising the claim, the waits
+    // and above all the fences (an L2 write-back each) over more work
+    std::vector<int> part_group( n ), cell_group( n );
+    for ( int i = 0; i < n; i++ ) {
+        const int cells = fl[i].g.nx * fl[i].g.ny;
+        long long ppt = 0;  // particles per tile of the largest species slot
+        for ( int k = 0; k < nslots; k++ ) {
+            const zpic_batch_species & it = sp[k * n + i];
+            const long long nt = (long long) it.fld.ntiles_x * it.fld.ntiles_y;
+            ppt = std::max( ppt, (long long) it.p.max_part / std::max( nt, 1LL ) );
+        }
+        part_group[i] = (int) std::min( 64LL, std::max( 1LL, 16384 / std::max( ppt, 1LL ) ) );
+        cell_group[i] = std::min( 16, std::max( 1, 1024 / std::max( cells, 1 ) ) );
+    }
     std::vector<int> need( (size_t) ns * n, 0 );  // items per (stage, instance)
     std::vector<std::vector<StepItem>> per_stage( ns );
     std::vector<int> last( n, -1 );  // last stage with items, per instance (current chain)
@@ -2142,7 +2203,8 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
                 break;
             case SK_PART:
                 dep = last[i] * n + i;
-                for ( int t = 0; t < nt; t++ ) push( SK_PART, st.slot * n + i, t, t + 1, dep );
+                for ( int t = 0; t < nt; t += part_group[i] )
+                    push( SK_PART, st.slot * n + i, t, std::min( nt, t + part_group[i] ), dep );
                 last[i] = si;
                 part_stage[i] = si;
                 break;
@@ -2159,7 +2221,8 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
                 dep = ( si - 1 ) * n + i;
                 if ( st.kind == SK_COUNT ) push( SK_COUNT, st.slot * n + i, 0, nt, dep );
                 else
-                    for ( int t = 0; t < nt; t++ ) push( st.kind, st.slot * n + i, t, t + 1, dep );
+                    for ( int t = 0; t < nt; t += part_group[i] )
+                        push( st.kind, st.slot * n + i, t, std::min( nt, t + part_group[i] ), dep );
                 break;
             case SK_ADD_X:
             case SK_ADD_Y:
@@ -2174,7 +2237,7 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
                         push( SK_COPY_XY, 3 * i + 2, t, std::min( nt, t + GC_TILES ), dep );
                     }
                 } else if ( st.kind == SK_YEE ) {
-                    for ( int t = 0; t < nt; t++ ) push( SK_YEE, arg, t, t + 1, dep );
+                    for ( int t = 0; t < nt; t += cell_group[i] ) push( SK_YEE, arg, t, std::min( nt, t + cell_group[i] ), dep );
                 } else {
                     for ( int t = 0; t < nt; t += GC_TILES ) push( st.kind, arg, t, std::min( nt, t + GC_TILES ), dep );
                 }
@@ -2186,7 +2249,8 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
                 if ( (size_t) st.slot >= passes[i].size() ) break;  // no such pass: the chain skips it
                 dep = last[i] * n + i;
                 if ( st.kind == SK_K3 ) {
-                    for ( int t = 0; t < nt; t++ ) push( SK_K3, k3_of[i][st.slot], t, t + 1, dep );
+                    for ( int t = 0; t < nt; t += cell_group[i] )
+                        push( SK_K3, k3_of[i][st.slot], t, std::min( nt, t + cell_group[i] ), dep );
                 } else {
                     const int kind = passes[i][st.slot].dir == 0 ? SK_COPY_X : SK_COPY_Y;
                     for ( int t = 0; t < nt; t += GC_TILES ) push( kind, 3 * i, t, std::min( nt, t + GC_TILES ), dep );
