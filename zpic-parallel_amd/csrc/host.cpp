@@ -787,10 +787,12 @@ void Species::advance_timed( EMF const & emf, Current & current, hipEvent_t afte
 
 void Species::advance_mov_window( EMF const & emf, Current & current ) { step( emf, current, nullptr, true ); }
 
-// Fast mode with the order-free sort: the particle pass stages leaver
-// records in the sort buffer (`tmp`), and the sort places them from there.
-void Species::set_staging( zpic_sort & s ) const {
-    const bool staged = !deterministic && zpic_get_sort_order() == 0;
+// Fast mode with the order-free sort: the particle pass compacts stayers and
+// stages leaver records in the sort buffer (`tmp`), and the sort places them
+// from there.  Not on moving-window steps: their sort may have to take the
+// copy_sorted path, which reads the particles in place.
+void Species::set_staging( zpic_sort & s, bool window ) const {
+    const bool staged = !deterministic && !window && zpic_get_sort_order() == 0;
     s.stage_ix = staged ? reinterpret_cast<int *>( tmp->ix ) : nullptr;
     s.stage_x = staged ? reinterpret_cast<float *>( tmp->x ) : nullptr;
     s.stage_u = staged ? reinterpret_cast<float *>( tmp->u ) : nullptr;
@@ -802,7 +804,7 @@ void Species::particle_pass( EMF const & emf, Current & current, bool shift ) {
     const float2 qnx = make_float2( q * dx.x / dt, q * dx.y / dt );
     zpic_geom g = emf.E->geom();
     zpic_particles p = particles->view();
-    set_staging( sort->buf );
+    set_staging( sort->buf, shift );
     check( zpic_push_move_deposit_shift( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha, dt_dx.x,
                                          dt_dx.y, q, qnx.x, qnx.y, d_tile_energy, &sort->buf, reflect_mask(),
                                          shift ? -1 : 0, 0, deterministic, stream() ) );

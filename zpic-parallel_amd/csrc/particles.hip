@@ -387,10 +387,18 @@ __device__ inline bool reflect_walls( int refl, int tx, int ty, int ntx, int nty
 constexpr int QCAP = WAVE;  // crossing-queue entries per wave
 constexpr int QWORDS = 6;   // cell, x0.x, x0.y, d.x, d.y, qvz
 constexpr int PREFETCH = 2;  // iterations of 64 particles loaded ahead
+// Order-free sort: leavers found by a wave are buffered in LDS and written
+// to the staging arrays LBCAP at a time (whole cache lines), LBWORDS words
+// each (cell x, y, position x, y, momentum x, y, z).
+constexpr int LBCAP = 32;
+constexpr int LBWORDS = 7;
 
-__host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move ) {
+// LDS of the particle pass: E and B planes (PUSH), W private J triples, W
+// tag arrays and W crossing queues (MOVE), W leaver buffers (order-free sort).
+__host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move, bool stage = false ) {
     return sizeof( float ) * (size_t) vol * ( ( push ? 6 : 0 ) + ( move ? 3 * W : 0 ) ) +
-           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) + sizeof( float ) * (size_t) W * QCAP * QWORDS : 0 );
+           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) + sizeof( float ) * (size_t) W * QCAP * QWORDS : 0 ) +
+           ( move && stage ? sizeof( float ) * (size_t) W * LBCAP * LBWORDS : 0 );
 }
 
 // Arguments of one instance of the particle pass.
@@ -490,9 +498,10 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     float * Jy = Jx + vol;
     float * Jz = Jy + vol;
     unsigned char * tags = reinterpret_cast<unsigned char *>( Jall + 3 * vol * W ) + (size_t) vol * w;
-    float * queue = reinterpret_cast<float *>( reinterpret_cast<unsigned char *>( Jall + 3 * vol * W ) +
-                                               ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) ) +
-                    (size_t) w * QCAP * QWORDS;
+    float * const qbase = reinterpret_cast<float *>( reinterpret_cast<unsigned char *>( Jall + 3 * vol * W ) +
+                                                     ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) );
+    float * queue = qbase + (size_t) w * QCAP * QWORDS;
+    float * lbuf = qbase + (size_t) W * QCAP * QWORDS + (size_t) w * LBCAP * LBWORDS;  // order-free sort only
     __shared__ int s_cnt[9];
     __shared__ double s_en[FAST_BLOCK / WAVE];
     __shared__ int s_nl[FAST_BLOCK / WAVE];
@@ -523,9 +532,11 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     const TileArray<float2> px{ (gptr<char>) gbl( P.x + off ) };
     const TileArray<float3> pu{ (gptr<char>) gbl( P.u + off ) };
     const gptr<int> sidx = gbl( S.idx + off );
-    // order-free sort: leaver records also go to the staging arrays at their
-    // leaver-list positions (fast mode)
-    const bool stage = !DET && S.stage_ix != nullptr;
+    // Order-free sort (fast mode): each wave compacts its stayers to the front
+    // of its chunk and writes its leavers, cell still in this tile's frame, to
+    // the staging arrays at the same chunk positions, through an LDS buffer
+    // of LBCAP records (whole cache lines); no leaver list.
+    const bool stage = MOVE && !DET && S.stage_ix != nullptr;
     const TileArray<int2> six{ (gptr<char>) gbl( S.stage_ix + off ) };
     const TileArray<float2> sx{ (gptr<char>) gbl( S.stage_x + off ) };
     const TileArray<float3> su{ (gptr<char>) gbl( S.stage_u + off ) };
@@ -541,6 +552,22 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     int nl = 0;
     int qn = 0;  // queued crossing paths of this wave
     volatile lds_f32 * q = (volatile lds_f32 *) queue;
+    int nst = 0;  // order-free sort: stayers compacted so far
+    int lbn = 0;  // leavers in the LDS buffer
+    int lbw = 0;  // leavers written to the staging arrays
+    volatile lds_f32 * lb = (volatile lds_f32 *) lbuf;
+
+    // order-free sort: the first `count` buffered leavers to the staging
+    // arrays at chunk positions beg + lbw, ...
+    auto flush_leavers = [&]( int count ) {
+        if ( lane < count ) {
+            const unsigned k = (unsigned) ( beg + lbw + lane );
+            six.set( k, make_int2( __float_as_int( lb[lane] ), __float_as_int( lb[LBCAP + lane] ) ) );
+            sx.set( k, make_float2( lb[2 * LBCAP + lane], lb[3 * LBCAP + lane] ) );
+            su.set( k, make_float3( lb[4 * LBCAP + lane], lb[5 * LBCAP + lane], lb[6 * LBCAP + lane] ) );
+        }
+        lbw += count;
+    };
 
     // split + deposit of the first `count` queued paths, one per lane
     auto flush_queue = [&]( int count ) {
@@ -608,7 +635,7 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
                 float3 e, b;
                 interpolate( e0x, e0y, e0z, b0x, b0y, b0z, ys, c, x, e, b );
                 u = boris<DET>( alpha, e, b, u, ene );
-                pu.store( o12, u );
+                if ( !stage ) pu.store( o12, u );  // order-free: written where the particle goes
             }
             if constexpr ( DET ) {
                 // energy += term, in particle order
@@ -636,10 +663,13 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
             int2 c1 = make_int2( c.x + di.x + m.shx, c.y + di.y + m.shy );
             float2 x1 = make_float2( s0.b.x - di.x, s0.b.y - di.y );
             if ( act ) {
-                if ( m.refl && reflect_walls( m.refl, t % g.ntx, t / g.ntx, g.ntx, g.nty, g.nx, g.ny, c1, x1, u ) )
-                    pu.store( o12, u );
-                px.store( o8, x1 );
-                pix.store( o8, c1 );
+                const bool refl =
+                    m.refl && reflect_walls( m.refl, t % g.ntx, t / g.ntx, g.ntx, g.nty, g.nx, g.ny, c1, x1, u );
+                if ( !stage ) {
+                    if ( refl ) pu.store( o12, u );
+                    px.store( o8, x1 );
+                    pix.store( o8, c1 );
+                }
             }
             const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
             const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
@@ -672,15 +702,46 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
             }
             const bool leave = act && dir != 4;
             const unsigned long long mask = __ballot( leave );
-            if ( leave ) {
-                const int pos = nl + count_below( mask );
-                sidx[beg + pos] = ( i << 4 ) | dir;
-                atomicAdd( &s_cnt[dir], 1 );
-                if ( stage ) {
-                    const unsigned k = (unsigned) ( beg + pos );
-                    six.set( k, make_int2( c1.x - xc * g.nx, c1.y - yc * g.ny ) );
-                    sx.set( k, x1 );
-                    su.set( k, u );
+            if ( !stage ) {
+                if ( leave ) {
+                    const int pos = nl + count_below( mask );
+                    sidx[beg + pos] = ( i << 4 ) | dir;
+                    atomicAdd( &s_cnt[dir], 1 );
+                }
+            } else {
+                // stayers: compacted to the front of the chunk (positions at
+                // or below the ones already read)
+                const bool stay = act && dir == 4;
+                const unsigned long long sm = __ballot( stay );
+                if ( stay ) {
+                    const unsigned r = (unsigned) ( beg + nst + count_below( sm ) );
+                    pix.set( r, c1 );
+                    px.set( r, x1 );
+                    pu.set( r, u );
+                }
+                nst += __popcll( sm );
+                // leavers: to the LDS buffer, written out LBCAP at a time
+                if ( mask ) {
+                    if ( leave ) atomicAdd( &s_cnt[dir], 1 );
+                    const int nlm = __popcll( mask );
+                    const int pos = lbn + count_below( mask );
+                    int base = 0;
+                    for ( ;; ) {
+                        if ( leave && pos >= base && pos < base + LBCAP ) {
+                            const int k = pos - base;
+                            lb[k] = __int_as_float( c1.x );
+                            lb[LBCAP + k] = __int_as_float( c1.y );
+                            lb[2 * LBCAP + k] = x1.x;
+                            lb[3 * LBCAP + k] = x1.y;
+                            lb[4 * LBCAP + k] = u.x;
+                            lb[5 * LBCAP + k] = u.y;
+                            lb[6 * LBCAP + k] = u.z;
+                        }
+                        if ( lbn + nlm - base < LBCAP ) break;
+                        flush_leavers( LBCAP );
+                        base += LBCAP;
+                    }
+                    lbn += nlm - base;
                 }
             }
             nl += __popcll( mask );
@@ -737,6 +798,7 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     }
     if constexpr ( MOVE && !DET ) {
         if ( qn > 0 ) flush_queue( qn );
+        if ( stage && lbn > 0 ) flush_leavers( lbn );
     }
 
     if constexpr ( PUSH ) {
@@ -1229,7 +1291,6 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( SortArgs a, const Sor
 // within-tile order is not the reference's (counts, offsets and per-tile
 // multisets are).
 constexpr int SEG_OLD_OFF = 13, SEG_OLD_NP = 14;
-constexpr int SLOT_CAP = 2048;  // hole slots held in LDS per round of k_sort_in_fast
 
 // Workgroup-wide exclusive rank of `pred` among the threads; `total` gets
 // the count.  Called by every thread of the workgroup.
@@ -1300,23 +1361,24 @@ __device__ inline void sort_shift_tile( const SortArgs & a, const int t ) {
     const int a0 = o, la = max( min( o + np, no ) - o, 0 );
     const int b0 = max( o, no + nn ), lb = max( o + np - b0, 0 );
     const int nout = la + lb;
-    const int nx = a.q.nx, ny = a.q.ny;
     int ns = 0;
     for ( int j0 = 0; j0 < nout; j0 += blockDim.x ) {
         const int j = j0 + tid;
+        bool stay = false;
         int p = 0;
-        int2 c = make_int2( -1, -1 );
         if ( j < nout ) {
             p = j < la ? a0 + j : b0 + ( j - la );
-            c = P.ix[p];
+            // the particle pass compacted each chunk's stayers to its front:
+            // chunk v holds them at [v chunk, v chunk + len_v - seg[v])
+            const int r = p - o, v = r / chunk, len = min( np - v * chunk, chunk );
+            stay = r - v * chunk < len - s_seg[v];
         }
-        const bool stay = c.x >= 0 && c.x < nx && c.y >= 0 && c.y < ny;
         int tot;
         const int k = ns + block_rank( stay, tot, s_w );
         if ( stay ) {
             const int w = segment_of( k, s_gpre );
             const int dst = o + w * chunk + s_seg[w] + ( k - s_gpre[w] );
-            Gix[dst] = c;
+            Gix[dst] = P.ix[p];
             Gx[dst] = P.x[p];
             Gu[dst] = P.u[p];
         }
@@ -1330,16 +1392,22 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_shift( SortArgs a, const 
     sort_shift_tile( a, t );
 }
 
-// s_slot: LDS room for SLOT_CAP ints.
-__device__ inline void sort_in_fast_tile( const SortArgs & a, const int t, int * s_slot ) {
-    __shared__ int s_seg[9][MAX_SEG + 1];  // segment counts of the 9 sources' lists (e = 4: the tile)
+// Placement of the order-free sort for tile t.  The slots of the new range
+// [no, no + nn) that none of the tile's stayers holds are runs: the part
+// before the old range, the tail of every chunk (behind its compacted
+// stayers) and the part after the old range.  They are filled in order with
+// the tile's staged stayers, then with its eight sources' leavers towards
+// it, source by source (each in its staging order); every write is a run.
+__device__ inline void sort_in_fast_tile( const SortArgs & a, const int t ) {
+    __shared__ int s_seg[9][MAX_SEG + 1];  // leavers per chunk of the 9 sources (e = 4: the tile)
     __shared__ int s_so[9][3];             // source tile, old offset, old count
-    __shared__ int s_pre[9][MAX_SEG + 1];  // prefix of the segment counts
+    __shared__ int s_pre[9][MAX_SEG + 1];  // prefix of the leaver counts
     __shared__ int s_cin[9];               // particles source e sends to this tile
     __shared__ int s_ebase[9];             // first rank of source e's particles
     __shared__ int s_gpre[MAX_SEG + 1];    // free-slot prefix of the tile's staging region
     __shared__ int s_hdr[3];
-    __shared__ int s_w[FAST_BLOCK / WAVE];
+    __shared__ int s_run[MAX_SEG + 2][2];  // hole runs (start, length) in position order
+    __shared__ int s_rpre[MAX_SEG + 3];    // prefix of their lengths
     const PartG P = gview( a.P );
     const SortG S = gview( a.S );
     const gptr<const int2> Gix = gbl( (const int2 *) a.S.stage_ix );
@@ -1377,10 +1445,34 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t, int *
         s_pre[e][MAX_SEG] = acc;
     }
     if ( tid == 16 ) gap_prefix( s_seg[4], s_so[4][2], s_seg[4][MAX_SEG], s_gpre );
-    __syncthreads();
+    if ( tid == 32 ) {
+        // hole runs of the new range, in position order
+        const int o = s_so[4][1], np = s_so[4][2], chunk = s_seg[4][MAX_SEG];
+        const int no = s_hdr[0], nn = s_hdr[1];
+        int nr = 0;
+        auto add = [&]( int b, int e ) {  // [b, e) clipped to the new range
+            b = max( b, no );
+            e = min( e, no + nn );
+            s_run[nr][0] = b;
+            s_run[nr][1] = max( e - b, 0 );
+            nr++;
+        };
+        add( no, o );  // before the old range
+        for ( int v = 0; v < MAX_SEG; v++ ) {
+            const int len = chunk > 0 ? min( max( np - v * chunk, 0 ), chunk ) : 0;
+            add( o + v * chunk + ( len - s_seg[4][v] ), o + v * chunk + len );  // behind chunk v's stayers
+        }
+        add( o + np, no + nn );  // after the old range
+        int acc = 0;
+        for ( int r = 0; r < MAX_SEG + 2; r++ ) {
+            s_rpre[r] = acc;
+            acc += s_run[r][1];
+        }
+        s_rpre[MAX_SEG + 2] = acc;
+    }
     if ( tid == 0 ) {
         // ranks: the tile's staged stayers first, then the sources' particles
-        // by direction, each source's in its list order
+        // by direction, each source's in its staging order
         int acc = s_hdr[2];
         for ( int e = 0; e < 9; e++ ) {
             s_ebase[e] = acc;
@@ -1389,96 +1481,72 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t, int *
     }
     __syncthreads();
 
-    const int o = s_so[4][1], np = s_so[4][2], chunk = s_seg[4][MAX_SEG];
-    const int no = s_hdr[0], nn = s_hdr[1], ns = s_hdr[2];
-    const int nl = s_pre[4][MAX_SEG];
+    const int o = s_so[4][1], chunk = s_seg[4][MAX_SEG];
+    const int ns = s_hdr[2], nholes = s_rpre[MAX_SEG + 2];
     const int W = blockDim.x / WAVE, w = tid / WAVE, lane = tid % WAVE;
-    // slots of the new range outside the old one: [no, no + lb1) and [b2, b2 + lb2)
-    const int lb1 = max( min( no + nn, o ) - no, 0 );
-    const int b2 = max( no, o + np ), lb2 = max( no + nn - b2, 0 );
-
-    int nA = 0;  // the tile's holes inside its new range
-    for ( int r0 = 0; r0 == 0 || r0 < nA; r0 += SLOT_CAP ) {
-        // this round's hole slots, in list order
-        int seen = 0;
-        for ( int j0 = 0; j0 < nl; j0 += blockDim.x ) {
-            const int j = j0 + tid;
-            int p = -1;
-            if ( j < nl ) {
-                const int w = segment_of( j, s_pre[4] );
-                p = o + ( S.idx[o + w * chunk + ( j - s_pre[4][w] )] >> 4 );
-            }
-            const bool in = p >= no && p < no + nn;
-            int tot;
-            const int r = seen + block_rank( in, tot, s_w );
-            if ( in && r >= r0 && r < r0 + SLOT_CAP ) s_slot[r - r0] = p;
-            seen += tot;
+    // position of fill rank k (-1 past the holes: the counts always balance)
+    auto slot = [&]( int k ) {
+        if ( k >= nholes ) return -1;
+        int r = 0;
+#pragma unroll
+        for ( int v = 1; v < MAX_SEG + 2; v++ ) r += s_rpre[v] <= k;
+        return s_run[r][0] + ( k - s_rpre[r] );
+    };
+    // the tile's staged stayers: ranks [0, ns)
+    for ( int k = tid; k < ns; k += blockDim.x ) {
+        const int dst = slot( k );
+        if ( dst >= 0 ) {
+            const int v = segment_of( k, s_gpre );
+            const int src = o + v * chunk + s_seg[4][v] + ( k - s_gpre[v] );
+            P.ix[dst] = Gix[src];
+            P.x[dst] = Gx[src];
+            P.u[dst] = Gu[src];
         }
-        nA = seen;
-        __syncthreads();
-        // destination of record rank k, or -1 outside this round
-        auto slot = [&]( int k ) {
-            if ( k < nA ) return ( k >= r0 && k < r0 + SLOT_CAP ) ? s_slot[k - r0] : -1;
-            if ( r0 != 0 ) return -1;
-            const int k2 = k - nA;
-            // counts always balance (nA + lb1 + lb2 = staged stayers + incoming);
-            // a rank past them would not be written
-            return k2 < lb1 ? no + k2 : ( k2 - lb1 < lb2 ? b2 + ( k2 - lb1 ) : -1 );
-        };
-        // the tile's staged stayers: ranks [0, ns)
-        for ( int k = tid; k < ns; k += blockDim.x ) {
-            const int dst = slot( k );
-            if ( dst >= 0 ) {
-                const int w = segment_of( k, s_gpre );
-                const int src = o + w * chunk + s_seg[4][w] + ( k - s_gpre[w] );
-                P.ix[dst] = Gix[src];
-                P.x[dst] = Gx[src];
-                P.u[dst] = Gu[src];
-            }
-        }
-        // the sources' particles towards this tile: source e's list entries of
-        // direction e, ranks from s_ebase[e] in list order.  The eight lists
-        // are independent (their rank ranges are known): wave w walks the
-        // lists w, w + W, ... with wave ballots, no workgroup barrier.
-        for ( int li = w; li < 8; li += W ) {
-            const int e = li < 4 ? li : li + 1;
-            const int n_e = s_pre[e][MAX_SEG];
-            if ( s_cin[e] == 0 ) continue;
-            const int ch = s_seg[e][MAX_SEG], oe = s_so[e][1];
-            int run = s_ebase[e];
-            for ( int j0 = 0; j0 < n_e; j0 += WAVE ) {
-                const int jj = j0 + lane;
-                int src = -1;
-                bool match = false;
-                if ( jj < n_e ) {
-                    const int sw = segment_of( jj, s_pre[e] );
-                    src = oe + sw * ch + ( jj - s_pre[e][sw] );
-                    match = ( S.idx[src] & 15 ) == e;
-                }
-                const unsigned long long m = __ballot( match );
-                if ( match ) {
-                    const int dst = slot( run + count_below( m ) );
-                    if ( dst >= 0 ) {
-                        P.ix[dst] = Gix[src];
-                        P.x[dst] = Gx[src];
-                        P.u[dst] = Gu[src];
-                    }
-                }
-                run += __popcll( m );
-            }
-        }
-        __syncthreads();  // s_slot is rebuilt by the next round
     }
+    // the sources' leavers towards this tile: the entries of source e's
+    // staging runs whose cell left it in direction e, ranks from s_ebase[e]
+    // in staging order; the eight lists are independent, wave w walks the
+    // lists w, w + W, ... with wave ballots
+    for ( int li = w; li < 8; li += W ) {
+        const int e = li < 4 ? li : li + 1;
+        if ( s_cin[e] == 0 ) continue;
+        const int n_e = s_pre[e][MAX_SEG];
+        const int ch = s_seg[e][MAX_SEG], oe = s_so[e][1];
+        const int dx = e % 3 - 1, dy = e / 3 - 1;
+        int run = s_ebase[e];
+        for ( int j0 = 0; j0 < n_e; j0 += WAVE ) {
+            const int jj = j0 + lane;
+            int src = -1;
+            int2 c = make_int2( 0, 0 );
+            bool match = false;
+            if ( jj < n_e ) {
+                const int sv = segment_of( jj, s_pre[e] );
+                src = oe + sv * ch + ( jj - s_pre[e][sv] );
+                c = Gix[src];
+                match = ( ( c.x >= q.nx ) - ( c.x < 0 ) ) == dx && ( ( c.y >= q.ny ) - ( c.y < 0 ) ) == dy;
+            }
+            const unsigned long long m = __ballot( match );
+            if ( match ) {
+                const int dst = slot( run + count_below( m ) );
+                if ( dst >= 0 ) {
+                    P.ix[dst] = make_int2( c.x - dx * q.nx, c.y - dy * q.ny );
+                    P.x[dst] = Gx[src];
+                    P.u[dst] = Gu[src];
+                }
+            }
+            run += __popcll( m );
+        }
+    }
+    __syncthreads();
     if ( tid == 0 ) {
-        P.np[t] = nn;
-        P.off[t] = no;
+        P.np[t] = s_hdr[1];
+        P.off[t] = s_hdr[0];
     }
 }
 
 __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in_fast( SortArgs a, const SortArgs * tab, const int2 * map ) {
-    __shared__ int s_slot[SLOT_CAP];
     const int t = select_instance( a, tab, map );
-    sort_in_fast_tile( a, t, s_slot );
+    sort_in_fast_tile( a, t );
 }
 
 // ---------------------------------------------------------------------------
@@ -1582,9 +1650,9 @@ constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup
 
 // Threads per tile the particle pass runs with for tiles of `vol` elements
 // when asked for `threads` (reduced until the LDS staging fits).
-int fitted_threads( int threads, int vol ) {
-    while ( threads > WAVE && particles_lds_bytes( vol, threads / WAVE, true, true ) > LDS_LIMIT ) threads -= WAVE;
-    return particles_lds_bytes( vol, threads / WAVE, true, true ) > LDS_LIMIT ? 0 : threads;
+int fitted_threads( int threads, int vol, bool stage = false ) {
+    while ( threads > WAVE && particles_lds_bytes( vol, threads / WAVE, true, true, stage ) > LDS_LIMIT ) threads -= WAVE;
+    return particles_lds_bytes( vol, threads / WAVE, true, true, stage ) > LDS_LIMIT ? 0 : threads;
 }
 
 int check_threads( int threads ) {
@@ -1601,10 +1669,11 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
         return set_error( ZPIC_EINVAL, "null buffer" );
     const int nt = fld->ntiles_x * fld->ntiles_y;
     // Large tiles: fewer wavefronts (each holds a private J copy) until LDS fits.
+    const bool stage = !det && s && s->stage_ix && s->stage_x && s->stage_u;
     int threads = det ? WAVE : g_threads;
-    while ( threads > WAVE && particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE ) > LDS_LIMIT )
+    while ( threads > WAVE && particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE, stage ) > LDS_LIMIT )
         threads -= WAVE;
-    const size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE );
+    const size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE, stage );
     if ( lds > LDS_LIMIT ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     PartArgs a{};
     a.P = view( p ); a.g = make_geo( fld );
@@ -1682,11 +1751,12 @@ int batch_particles_create( BatchParticlePlan ** out, const zpic_batch_species *
         // each entry runs with the threads that fit its own tiles, as its own
         // single-instance pass does
         if ( it.threads && check_threads( it.threads ) ) { delete plan; return ZPIC_EINVAL; }
-        const int th = fitted_threads( it.threads ? it.threads : threads, it.fld.tile_vol );
+        const bool st = a.S.stage_ix != nullptr;
+        const int th = fitted_threads( it.threads ? it.threads : threads, it.fld.tile_vol, st );
         if ( th == 0 ) { delete plan; return set_error( ZPIC_ERANGE, "tile too large for LDS staging" ); }
         a.waves = th / WAVE;
         plan->threads = std::max( plan->threads, th );
-        plan->lds = std::max( plan->lds, particles_lds_bytes( it.fld.tile_vol, a.waves, true, true ) );
+        plan->lds = std::max( plan->lds, particles_lds_bytes( it.fld.tile_vol, a.waves, true, true, st ) );
         plan->h_part.push_back( a );
         zpic_particles p = it.p, t = it.tmp;
         sorts[e] = sort_args( &p, &t, &s, it.fld.ntiles_x, it.fld.ntiles_y, it.fld.nx, it.fld.ny, it.periodic_x,
@@ -1944,7 +2014,7 @@ __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
             for ( int t = it.t0; t < it.t1; t++ ) {
                 if ( t > it.t0 ) __syncthreads();
                 if ( it.kind == SK_SHIFT ) sort_shift_tile( a, t );
-                else if ( it.kind == SK_PLACE ) sort_in_fast_tile( a, t, reinterpret_cast<int *>( smem ) );
+                else if ( it.kind == SK_PLACE ) sort_in_fast_tile( a, t );
                 else if ( it.kind == SK_OUT ) sort_out_tile<false>( a, t );
                 else sort_in_tile( a, t );
             }
@@ -2054,8 +2124,9 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
     std::vector<long long> weight( n, 0 );
     std::vector<int> waves( n );
     int block = 256;
-    size_t lds = sizeof( int ) * SLOT_CAP;
-    bool staged = false;  // order-free sort (every item's staging buffers set) or the reference order
+    size_t lds = 0;
+    // order-free sort (every item's staging buffers set) or the reference order
+    const bool staged = sp[0].s.stage_ix && sp[0].s.stage_x && sp[0].s.stage_u;
     for ( int i = 0; i < n; i++ ) {
         order[i] = i;
         const zpic_batch_fields_item & f = fl[i];
@@ -2067,15 +2138,14 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
         const int vol = f.g.tile_vol;
         const int req = sp[i].threads ? sp[i].threads : threads;  // slot 0 of the instance
         if ( int rc = check_threads( req ) ) return rc;
-        const int th = fitted_threads( req, vol );
+        const int th = fitted_threads( req, vol, staged );
         if ( th == 0 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
         waves[i] = th / WAVE;
         block = std::max( block, th );
-        lds = std::max( { lds, particles_lds_bytes( vol, waves[i], true, true ), sizeof( float ) * 6 * (size_t) vol } );
+        lds = std::max( { lds, particles_lds_bytes( vol, waves[i], true, true, staged ), sizeof( float ) * 6 * (size_t) vol } );
         for ( int k = 0; k < nslots; k++ ) {
             const zpic_batch_species & s = sp[k * n + i];
             const bool st = s.s.stage_ix && s.s.stage_x && s.s.stage_u;
-            if ( i == 0 && k == 0 ) staged = st;
             if ( st != staged ) return set_error( ZPIC_EINVAL, "batch entries must all use the same sort (staged or not)" );
             if ( s.fld.ntiles_x != f.g.ntiles_x || s.fld.ntiles_y != f.g.ntiles_y || s.fld.nx != f.g.nx ||
                  s.fld.ny != f.g.ny || s.E != f.E || s.B != f.B || s.J != f.J )

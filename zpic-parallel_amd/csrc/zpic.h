@@ -687,7 +687,7 @@ class Species {
     // and the sort, the injection when `shift`, the iteration count and the
     // energy sum.
     void particle_pass( EMF const & emf, Current & current, bool shift );
-    void set_staging( zpic_sort & s ) const;
+    void set_staging( zpic_sort & s, bool window = false ) const;
     void finish_step( bool shift );
     // Whether this step shifts the particles (the window moves), for a step
     // of advance_mov_window (window) or advance (not window)
