@@ -82,12 +82,13 @@ typedef struct {
                                   pass moved (Species::get_nmove); may be NULL */
     /* Order-free sort of the fast mode (may be NULL): buffers of the
      * particle buffer's size (int2, float2, float3 per particle; the
-     * staging buffer `tmp` serves).  When set, a fast-mode particle pass also
-     * writes every tile leaver's record there (at its leaver-list position,
-     * cell index in the target tile's frame) and the next zpic_tile_sort /
-     * zpic_tile_sort_copy mode 0 places particles without keeping the
-     * reference's within-tile order; counts, offsets and the per-tile
-     * multisets are the reference's. */
+     * staging buffer `tmp` serves).  When set, a fast-mode particle pass
+     * compacts each wave segment's stayers to the segment's front and writes
+     * the segment's leaver records (cell in the source tile's frame) to these
+     * buffers at the same segment positions, and writes no leaver list; the
+     * next zpic_tile_sort / zpic_tile_sort_copy mode 0 places particles
+     * without keeping the reference's within-tile order; counts, offsets and
+     * the per-tile multisets are the reference's. */
     int *stage_ix;
     float *stage_x;
     float *stage_u;
@@ -233,6 +234,17 @@ int zpic_filter_x(float *buf, const zpic_geom *g, int passes, const float *coef,
  * 1 binomial, 2 compensated; filter_dir 0 x / 1 y. */
 int zpic_current_advance(float *J, const zpic_geom *g, int filter_kind, int filter_dir,
                          int filter_order, int periodic_x, int periodic_y, void *stream);
+
+/* Current::advance (current.cpp:226-246) on a grid periodic along x and y
+ * with no filter or a filter along x (filter.h:22-78): add_from_gc,
+ * copy_to_gc and `passes` kernel3_x passes (coef[3p..3p+2] = a, b, c), each
+ * followed by copy_to_gc_x, as one tile-parallel launch that reads the
+ * deposit `dep` and writes the result to `J` (a different buffer of the same
+ * geometry; `dep` is left as it is).  Every element of J, guards and tile
+ * padding included, is identical to what that sequence leaves in place.
+ * Needs nx >= gc_lower + gc_upper + passes and ny >= gc_lower + gc_upper. */
+int zpic_current_update(const float *dep, float *J, const zpic_geom *g, int passes, const float *coef,
+                        void *stream);
 
 /* EMF::advance(Current&) (emf.cpp:513-580): B half step, E step with J,
  * B half step per tile, then guard copy of E and B. */

@@ -216,3 +216,33 @@ def test_batch_order_free_sort_matches_single_instances(zp, persistent):
             s.close()
     finally:
         zp.set_sort_order(old)
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_after_single_steps_matches_single_instances(zp, persistent):
+    """Single-instance steps between batch steps on the same instances: the
+    single steps exchange the current's two buffers (the one-pass current
+    update on periodic grids), so the batch rebuilds its tables on the
+    buffers in use; the result equals the instances advanced alone."""
+    steps = (3, 4, 3, 2)  # single, batch, single, batch
+    ref = singles(zp, 256, sum(steps))
+    old = zp.get_threads_per_tile()
+    zp.set_threads_per_tile(256)
+    try:
+        sims = [make(zp, sh) for sh in SHAPES]
+        batch = zp.Batch(sims, threads_per_tile=256, graph=True, persistent=persistent)
+        for k, n in enumerate(steps):
+            if k % 2 == 0:
+                for s in sims:
+                    s.advance(n)
+            else:
+                batch.advance(n)
+        batch.status()
+        say(f"persistent={persistent}: single / batch steps {steps}")
+        for k, (s, (st, en)) in enumerate(zip(sims, ref)):
+            assert_same(state(s), st, f"instance {k}")
+        batch.close()
+        for s in sims:
+            s.close()
+    finally:
+        zp.set_threads_per_tile(old)

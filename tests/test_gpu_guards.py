@@ -61,11 +61,11 @@ def add_from_gc(A, g):
     return A
 
 
-def copy_to_gc(A, g):
+def copy_to_gc(A, g, axes=(0, 1)):
     """grid.h:253-340: x guard columns from the x neighbours' edge columns
     (all rows), then y guard rows (full width) from the y neighbours."""
     ntx, nty, nx, ny, gl, gu, px, py = g
-    for axis in (0, 1):
+    for axis in axes:
         S = A.copy()
         for t in range(ntx * nty):
             tx, ty = t % ntx, t // ntx
@@ -221,3 +221,68 @@ def test_copy_to_gc2_matches_two_copies(lib, per):
     d0.free()
     d1.free()
     say(f"copy_to_gc2 periodic {per}: identical")
+
+
+def kernel3_x(A, g, a, b, c):
+    """grid.h:495-545: every row (guard rows included), interior columns:
+    A[i-1]*a + A[i]*b + A[i+1]*c in float32, from the values before the pass."""
+    ntx, nty, nx, ny, gl, gu, px, py = g
+    S = A.copy()
+    A[:, :, gl:gl + nx] = (S[:, :, gl - 1:gl + nx - 1] * a + S[:, :, gl:gl + nx] * b) + S[:, :, gl + 1:gl + nx + 1] * c
+    return A
+
+
+def _filter_passes(kind, order):
+    """(a, b, c) of the kernel3_x passes of Filter::Binomial / Compensated
+    along x (filter.h:22-78), with the compensator's float arithmetic."""
+    if kind == "none":
+        return []
+    c = [(np.float32(0.25), np.float32(0.5), np.float32(0.25))] * order
+    if kind == "compensated":
+        b = np.float32((4.0 + 2.0 * order) / order)
+        norm = np.float32(np.float32(-2.0) + b)
+        c.append((np.float32(-1.0) / norm, b / norm, np.float32(-1.0) / norm))
+    return c
+
+
+UPDATE_CASES = [(geom, filt) for geom in GEOMS for filt in
+                [("none", 0), ("binomial", 1), ("binomial", 2), ("compensated", 1), ("binomial", 3),
+                 ("compensated", 3)]
+                if geom[2] >= geom[4] + geom[5] + len(_filter_passes(*filt)) and geom[3] >= geom[4] + geom[5]
+                and (filt[0] == "none" or min(geom[4], geom[5]) >= 1)]
+
+
+@pytest.mark.parametrize("case", UPDATE_CASES, ids=lambda c: f"{c[0]}-{c[1][0]}{c[1][1]}")
+def test_one_pass_current_update(lib, case):
+    """zpic_current_update (Current::advance in one launch on a periodic grid,
+    deposit buffer in, current out) against the reference's sequence restated
+    in numpy: add_from_gc, copy_to_gc, then every kernel3_x pass followed by
+    copy_to_gc_x; on random deposits (guards unrelated to their neighbours'
+    interiors, random tile padding), tile counts of 1 (a tile its own
+    neighbour) to 20, tiles as narrow as the guard bands plus the filter
+    passes: bit-identical over the whole buffer, guards and padding
+    included."""
+    geom, filt = case
+    L, zp, hip = lib
+    L.zpic_current_update.restype = ctypes.c_int
+    L.zpic_current_update.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_void_p, ctypes.c_void_p]
+    coef = _filter_passes(*filt)
+    say(f"one-pass current update {geom} {filt}: {len(coef)} passes")
+    gfull = (*geom, 1, 1)
+    g, flat, dep = _setup(lib, geom, 3, 7)
+    out = DeviceBuffer(hip, np.full_like(flat, np.nan))
+    want = flat.copy()
+    v = copy_to_gc(add_from_gc(_grid_view(want, g, 3), gfull), gfull)
+    for a, b, c in coef:
+        copy_to_gc(kernel3_x(v, gfull, a, b, c), gfull, axes=(0,))
+    cf = np.array(coef, np.float32).reshape(-1)
+    _run(lib, L.zpic_current_update, dep.ptr, out.ptr, ctypes.byref(g), len(coef),
+         cf.ctypes.data_as(ctypes.c_void_p) if len(coef) else None)
+    assert _ndiff(out.numpy(), want) == 0
+    # the deposit is read only
+    assert _ndiff(dep.numpy(), flat) == 0
+    # the same buffer in and out is refused
+    assert L.zpic_current_update(dep.ptr, dep.ptr, ctypes.byref(g), 0, None, None) != 0
+    dep.free()
+    out.free()

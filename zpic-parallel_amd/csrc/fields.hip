@@ -79,6 +79,11 @@ __global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( K3Args args, const K3
     kernel3_tile( args, t, A );
 }
 
+__global__ void __launch_bounds__( TILE_BLOCK ) k_current( CurArgs a ) {
+    extern __shared__ float sm[];
+    current_tile( a, blockIdx.x, sm );
+}
+
 __global__ void __launch_bounds__( TILE_BLOCK ) k_yee( YeeArgs a, const YeeArgs * tab, const int2 * map ) {
     extern __shared__ float sm[];
     const int t = select_instance( a, tab, map );
@@ -922,6 +927,36 @@ int zpic_current_advance( float * J, const zpic_geom * g, int kind, int dir, int
     for ( int p = 0; p < passes; p++ )
         if ( int rc = zpic_kernel3( J, g, dir, coef[3 * p], coef[3 * p + 1], coef[3 * p + 2], px, py, stream ) )
             return rc;
+    return ZPIC_OK;
+}
+
+int zpic_current_update( const float * dep, float * J, const zpic_geom * gg, int passes, const float * coef,
+                         void * stream ) {
+    if ( int rc = check( gg ) ) return rc;
+    if ( !dep || !J || ( passes > 0 && !coef ) ) return set_error( ZPIC_EINVAL, "null argument" );
+    if ( dep == J ) return set_error( ZPIC_EINVAL, "current update: deposit and current buffers must differ" );
+    if ( passes < 0 || passes > CUR_MAX_PASSES ) return set_error( ZPIC_EINVAL, "current update: 0 to 16 filter passes" );
+    if ( passes > 0 && ( gg->gc_lower < 1 || gg->gc_upper < 1 ) )
+        return set_error( ZPIC_EINVAL, "kernel3 requires at least 1 guard cell at both boundaries" );
+    const G g = make( gg );
+    if ( g.nx < g.gl + g.gu + passes || g.ny < g.gl + g.gu )
+        return set_error( ZPIC_EINVAL, "current update: tiles narrower than both guard bands plus the filter passes" );
+    if ( 3 * ( g.vol - g.ext_x * g.ext_y ) > TILE_BLOCK ) return set_error( ZPIC_EINVAL, "current update: tile padding" );
+    const size_t lds = current_lds_bytes( g, passes );
+    if ( lds > 65536 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
+    CurArgs a{};
+    a.dep = dep;
+    a.J = J;
+    a.g = g;
+    a.passes = passes;
+    for ( int p = 0; p < passes; p++ ) {
+        a.a[p] = coef[3 * p];
+        a.b[p] = coef[3 * p + 1];
+        a.c[p] = coef[3 * p + 2];
+    }
+    hipStream_t st = as_stream( stream );
+    hipLaunchKernelGGL( k_current, dim3( g.ntx * g.nty ), dim3( TILE_BLOCK ), lds, st, a );
+    ZPIC_LAUNCH_CHECK( "k_current" );
     return ZPIC_OK;
 }
 

@@ -297,6 +297,124 @@ __device__ inline void yee_tile( const YeeArgs & a, const int t, float * sm ) {
     }
 }
 
+// Current::advance on a grid periodic along x and y (current.cpp:226-246:
+// add_from_gc, copy_to_gc, then the filter's kernel3_x passes, each followed
+// by copy_to_gc_x, grid.h:367-444, 253-340, 495-545) for tile t in one pass,
+// from the deposit buffer `dep` into `J` (distinct buffers: neighbours read
+// `dep` while J is written).  After that sequence on a periodic grid every
+// element of a tile's region, guards included, holds the value of the global
+// cell it stands for, so tile t computes its whole region itself:
+//  * the summed current of every cell in columns [-gl - P, nx + gu + P) and
+//    rows [-gl, ny + gu) (tile-local), with the additions of the x pass
+//    (own value, + lower neighbour's upper guard, + upper neighbour's lower
+//    guard) then the y pass (+ the y neighbours' guard rows as their own x
+//    pass left them), in that order;
+//  * P filter passes in LDS, A[i-1]*a + A[i]*b + A[i+1]*c, the valid columns
+//    shrinking by one on each side per pass;
+//  * the region [-gl, nx + gu) x [-gl, ny + gu) written, and the tile's
+//    padding elements copied from `dep` (the sequence never touches them).
+// With nx >= gl + gu + P and ny >= gl + gu every value a cell of the region
+// needs lies in tile t's 3x3 neighbourhood.  LDS: current_lds_bytes.
+constexpr int CUR_MAX_PASSES = 16;
+struct CurArgs {
+    const float * dep;
+    float * J;
+    G g;
+    int passes;
+    float a[CUR_MAX_PASSES], b[CUR_MAX_PASSES], c[CUR_MAX_PASSES];
+};
+
+inline size_t current_lds_bytes( const G & g, int passes ) {
+    return sizeof( float ) * 2 * 3 * (size_t) ( g.nx + g.gl + g.gu + 2 * passes ) * g.ext_y;
+}
+
+// Loop over the n = rows x len elements of a 2-D range, row-major, thread
+// tid taking elements tid, tid + blockDim.x, ...: (row, col) carried without
+// a division per element.
+template <class F>
+__device__ inline void for_2d( int rows, int len, F && f ) {
+    const int step = blockDim.x, dcol = step % len, drow = step / len;
+    int row = threadIdx.x / len, col = threadIdx.x % len;
+    for ( ; row < rows; ) {
+        f( row, col );
+        col += dcol;
+        row += drow;
+        if ( col >= len ) {
+            col -= len;
+            row++;
+        }
+    }
+}
+
+__device__ inline void current_tile( const CurArgs & a, const int t, float * sm ) {
+    const G g = a.g;
+    const int P = a.passes;
+    const int E = g.ext_x;
+    const int XW = g.nx + g.gl + g.gu + 2 * P, plane = XW * g.ext_y;
+    float * A = sm;
+    float * Bf = sm + 3 * plane;
+    const gptr<const float> dep = gbl( a.dep );
+    // first float of each deposit region of the 3x3 neighbourhood, index
+    // (dy + 1) * 3 + dx + 1
+    __shared__ long long s_nb[9];
+    if ( threadIdx.x < 9 ) {
+        const int k = threadIdx.x;
+        s_nb[k] = (long long) nbr( t % g.ntx, t / g.ntx, k % 3 - 1, k / 3 - 1, g, 1, 1 ) * g.vol * 3;
+    }
+    __syncthreads();
+    // deposit of neighbour k at tile-local (x, y), component c
+    auto raw = [&]( int k, int x, int y, int c ) { return dep[s_nb[k] + ( ( x + g.gl ) + ( y + g.gl ) * E ) * 3 + c]; };
+    // neighbour k (at dx, dy) at interior column i, any row y, after the x
+    // pass; its x neighbours are k - 1 and k + 1 (inside the 3x3 block
+    // whenever the terms apply)
+    auto xsum = [&]( int k, int i, int y, int c ) {
+        float v = raw( k, i, y, c );
+        if ( i < g.gu ) v += raw( k - 1, g.nx + i, y, c );
+        if ( i >= g.nx - g.gl ) v += raw( k + 1, i - g.nx, y, c );
+        return v;
+    };
+    // summed current on the region, SoA planes A[c][row][col]
+    for_2d( g.ext_y, 3 * XW, [&]( int row, int q ) {
+        const int col = q / 3, c = q - 3 * col;
+        const int xx = col - g.gl - P, yy = row - g.gl;
+        const int dx = xx < 0 ? -1 : ( xx >= g.nx ? 1 : 0 );
+        const int dy = yy < 0 ? -1 : ( yy >= g.ny ? 1 : 0 );
+        const int i = xx - dx * g.nx, j = yy - dy * g.ny;
+        const int k = ( dy + 1 ) * 3 + dx + 1;
+        float s = xsum( k, i, j, c );
+        if ( j < g.gu ) s += xsum( k - 3, i, g.ny + j, c );
+        if ( j >= g.ny - g.gl ) s += xsum( k + 3, i, j - g.ny, c );
+        A[c * plane + row * XW + col] = s;
+    } );
+    for ( int p = 0; p < P; p++ ) {
+        __syncthreads();
+        const float ca = a.a[p], cb = a.b[p], cc = a.c[p];
+        float * const In = A;
+        float * const Out = Bf;
+        for_2d( 3 * g.ext_y, XW, [&]( int r, int x ) {
+            if ( x > p && x < XW - 1 - p ) {
+                const int k = r * XW + x;
+                Out[k] = In[k - 1] * ca + In[k] * cb + In[k + 1] * cc;
+            }
+        } );
+        A = Out;
+        Bf = In;
+    }
+    __syncthreads();
+    const gptr<float> Jt = gbl( a.J ) + (size_t) t * g.vol * 3;
+    float * const R = A;
+    for_2d( g.ext_y, 3 * E, [&]( int row, int q ) {
+        const int col = q / 3, c = q - 3 * col;
+        Jt[row * 3 * E + q] = R[c * plane + row * XW + col + P];
+    } );
+    // padding
+    const int npad = 3 * ( g.vol - E * g.ext_y );
+    if ( (int) threadIdx.x < npad ) {
+        const size_t k = (size_t) 3 * E * g.ext_y + threadIdx.x;
+        Jt[k] = dep[s_nb[4] + k];
+    }
+}
+
 // Current::zero (grid.h:169-172) of tile t: every element of its region,
 // guards and padding included.
 __device__ inline void zero_tile( float * buf, const int vol3, const int t ) {

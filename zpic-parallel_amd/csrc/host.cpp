@@ -398,7 +398,7 @@ void UDistribution::ThermalCorr::set( int nt, const int * np, const int * off, c
 // ------------------------------------------------------------------ Current
 Current::Current( uint2 const ntiles, uint2 const nx, float2 const box_, double const dt_ )
     : dx( make_float2( box_.x / ( nx.x * ntiles.x ), box_.y / ( nx.y * ntiles.y ) ) ), dt( dt_ ), iter( 0 ),
-      box( box_ ) {
+      Jalt( nullptr ), box( box_ ) {
     bnd<unsigned int> gc;
     gc.x = { 1, 2 };
     gc.y = { 1, 2 };
@@ -411,6 +411,7 @@ Current::Current( uint2 const ntiles, uint2 const nx, float2 const box_, double 
 
 Current::~Current() {
     delete J;
+    delete Jalt;
     delete filter;
 }
 
@@ -426,8 +427,40 @@ void Current::set_bc( current::bc_type new_bc ) {
     J->set_periodic( make_int2( bc.x.lower == current::bc::periodic, bc.y.lower == current::bc::periodic ) );
 }
 
+bool Current::one_pass_ok( std::vector<float> & coef ) const {
+    coef.clear();
+    const int2 p = J->get_periodic();
+    if ( !p.x || !p.y ) return false;
+    if ( filter->kind() == 1 || filter->kind() == 2 ) {
+        const auto * f = dynamic_cast<const Filter::Binomial *>( filter );
+        if ( !f || f->get_dir() != coord::x ) return false;
+        coef = f->passes();
+    } else if ( filter->kind() != 0 ) {
+        return false;
+    }
+    const int passes = (int) coef.size() / 3;
+    const unsigned gw = J->gc.x.lower + J->gc.x.upper;
+    const size_t lds = sizeof( float ) * 6 * ( J->ext_nx.x + 2 * passes ) * J->ext_nx.y;
+    return passes <= 16 && J->nx.x >= gw + passes && J->nx.y >= gw && lds <= 65536;
+}
+
 // Current::advance (current.cpp:227-246)
 void Current::advance() {
+    std::vector<float> coef;
+    if ( one_pass_ok( coef ) ) {
+        // guard add, guard copy and the x filter in one launch into the
+        // second buffer, which then becomes J's
+        if ( !Jalt ) {
+            Jalt = new vec3grid<float3>( J->get_ntiles(), J->nx, J->gc );
+            Jalt->name = J->name;
+        }
+        zpic_geom g = J->geom();
+        check( zpic_current_update( J->fbuf(), Jalt->fbuf(), &g, (int) coef.size() / 3, coef.data(), stream() ) );
+        std::swap( J->d_buffer, Jalt->d_buffer );
+        iter++;
+        if ( moving_window.needs_move( iter * dt ) ) moving_window.advance();
+        return;
+    }
     J->add_copy_gc();
     if ( bc.x.lower > current::bc::periodic || bc.x.upper > current::bc::periodic ||
          bc.y.lower > current::bc::periodic || bc.y.upper > current::bc::periodic ) {
@@ -1194,6 +1227,15 @@ void Batch::check_status() {
     if ( persistent_step ) check( zpic_batch_step_status( persistent_step ) );
 }
 
+std::vector<const void *> Batch::buffers() const {
+    std::vector<const void *> out;
+    for ( auto * s : sims ) {
+        out.push_back( s->current.J->fbuf() );
+        for ( auto * sp : s->species ) sp->buffers( out );
+    }
+    return out;
+}
+
 void Batch::build() {
     if ( sims.empty() ) throw std::runtime_error( "(*error*) empty batch" );
     release();
@@ -1210,6 +1252,7 @@ void Batch::build() {
         for ( auto * s : sims ) fitems.push_back( s->batch_fields() );
         check( zpic_batch_step_create( &persistent_step, items.data(), (int) nsp, fitems.data(), (int) fitems.size(),
                                        threads ) );
+        bound = buffers();
         return;
     }
     for ( size_t k = 0; k < nsp; k++ ) {
@@ -1226,6 +1269,7 @@ void Batch::build() {
     std::vector<zpic_batch_fields_item> fitems;
     for ( auto * s : sims ) fitems.push_back( s->batch_fields() );
     check( zpic_batch_fields_create( &fields, fitems.data(), (int) fitems.size() ) );
+    bound = buffers();
 }
 
 // Simulation::advance (simulation.h:90-107) for every instance: each stage is
@@ -1242,6 +1286,9 @@ void Batch::step_kernels() {
 }
 
 void Batch::advance() {
+    // rebuilt when an instance's buffers moved since (single-instance steps
+    // exchange the current's buffers and may grow or swap particle buffers)
+    if ( ( fields || persistent_step ) && buffers() != bound ) build();
     if ( !fields && !persistent_step ) build();
     if ( use_graph ) {
         if ( !exec ) {

@@ -438,6 +438,21 @@ __device__ inline SortG gview( const SortView & v ) {
                   gbl( v.nmove ) };
 }
 
+// float3 records element by element: an aggregate copy of a float3 between
+// global arrays is otherwise staged in a private temporary, which the
+// compiler places in LDS (12 bytes per thread) or scratch.
+__device__ inline float3 ld3( gptr<const float3> p ) {
+    const gptr<const float> f = (gptr<const float>) p;
+    return make_float3( f[0], f[1], f[2] );
+}
+__device__ inline float3 ld3( gptr<float3> p ) { return ld3( (gptr<const float3>) p ); }
+__device__ inline void st3( gptr<float3> p, const float3 v ) {
+    const gptr<float> f = (gptr<float>) p;
+    f[0] = v.x;
+    f[1] = v.y;
+    f[2] = v.z;
+}
+
 // Element i of one tile's particle array through the tile's (uniform) base
 // address and a 32-bit byte offset: the global load / store form with a
 // scalar base and a vector offset, no 64-bit address arithmetic per lane.
@@ -702,12 +717,9 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
             }
             const bool leave = act && dir != 4;
             const unsigned long long mask = __ballot( leave );
+            if ( leave ) atomicAdd( &s_cnt[dir], 1 );
             if ( !stage ) {
-                if ( leave ) {
-                    const int pos = nl + count_below( mask );
-                    sidx[beg + pos] = ( i << 4 ) | dir;
-                    atomicAdd( &s_cnt[dir], 1 );
-                }
+                if ( leave ) sidx[beg + nl + count_below( mask )] = ( i << 4 ) | dir;
             } else {
                 // stayers: compacted to the front of the chunk (positions at
                 // or below the ones already read)
@@ -720,9 +732,10 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
                     pu.set( r, u );
                 }
                 nst += __popcll( sm );
+            }
+            if ( stage ) {
                 // leavers: to the LDS buffer, written out LBCAP at a time
                 if ( mask ) {
-                    if ( leave ) atomicAdd( &s_cnt[dir], 1 );
                     const int nlm = __popcll( mask );
                     const int pos = lbn + count_below( mask );
                     int base = 0;
@@ -907,13 +920,15 @@ __global__ void k_sort_newnp( SortArgs a, const SortArgs * tab ) {
 }
 
 // Exclusive scan of the new counts into the new tile offsets
-// (update_tile_info, particles.cpp:314-357).  One workgroup per instance
-// walks the tiles in chunks of SCAN_BLOCK, one tile per thread (coalesced
-// loads, the next chunk's loads issued before the current chunk is
-// scanned): a wave scan by shuffles, the wave totals scanned by the first
-// wave, and a running carry between chunks.
+// (update_tile_info, particles.cpp:314-357).  One workgroup per instance;
+// each thread takes SCAN_K consecutive tiles (16-byte loads and stores when
+// the arrays allow), sums them, and one workgroup scan of the thread sums
+// (wave shuffles, then the wave totals by the first wave) gives every
+// thread its start; rounds of BLOCK * SCAN_K tiles follow with a running
+// carry.  All loads of a round are issued before its scan.
 // BLOCK: the workgroup size (a multiple of 64, at most 64 waves).
 constexpr int SCAN_BLOCK = 1024;
+constexpr int SCAN_K = 16;
 template <int BLOCK>
 __device__ inline void sort_count_body( const SortArgs & a ) {
     const SortG S = gview( a.S );
@@ -926,23 +941,46 @@ __device__ inline void sort_count_body( const SortArgs & a ) {
     __shared__ int s_wave[BLOCK / WAVE];
     __shared__ unsigned long long s_moved;
     if ( tid == 0 ) s_moved = 0;
-    auto load = [&]( int t, int & v, int & m ) {
-        v = 0;
-        m = 0;
-        if ( t < ntiles ) {
-            v = S.new_np[t] + ( a.extra ? extra[t] : 0 );
-            if ( count_moved ) m = np_old[t];
-        }
-    };
-    int v, m;
-    load( tid, v, m );
+    // 16-byte accesses need every array 16-byte aligned (tile starts are
+    // multiples of SCAN_K)
+    const bool vec = ( ( reinterpret_cast<uintptr_t>( a.S.new_np ) | reinterpret_cast<uintptr_t>( a.T.off ) |
+                         reinterpret_cast<uintptr_t>( a.extra ) | reinterpret_cast<uintptr_t>( a.P.np ) ) & 15 ) == 0;
     int carry = 0;
     unsigned long long moved = 0;
-    for ( int t0 = 0; t0 < ntiles; t0 += BLOCK ) {
-        int vn, mn;
-        load( t0 + BLOCK + tid, vn, mn );
-        // inclusive scan within the wave
-        int x = v;
+    for ( int base = 0; base < ntiles; base += BLOCK * SCAN_K ) {
+        const int t0 = base + tid * SCAN_K;
+        int v[SCAN_K];
+        if ( vec && t0 + SCAN_K <= ntiles ) {
+#pragma unroll
+            for ( int q = 0; q < SCAN_K / 4; q++ ) {
+                const int4 n4 = *(const gptr<int4>) ( S.new_np + t0 + 4 * q );
+                int4 e4 = make_int4( 0, 0, 0, 0 );
+                if ( a.extra ) e4 = *(const gptr<int4>) ( extra + t0 + 4 * q );
+                v[4 * q] = n4.x + e4.x;
+                v[4 * q + 1] = n4.y + e4.y;
+                v[4 * q + 2] = n4.z + e4.z;
+                v[4 * q + 3] = n4.w + e4.w;
+                if ( count_moved ) {
+                    const int4 m4 = *(const gptr<int4>) ( np_old + t0 + 4 * q );
+                    moved += (unsigned) m4.x + (unsigned long long) (unsigned) m4.y + (unsigned) m4.z + (unsigned) m4.w;
+                }
+            }
+        } else {
+#pragma unroll
+            for ( int k = 0; k < SCAN_K; k++ ) {
+                const int t = t0 + k;
+                v[k] = 0;
+                if ( t < ntiles ) {
+                    v[k] = S.new_np[t] + ( a.extra ? extra[t] : 0 );
+                    if ( count_moved ) moved += (unsigned) np_old[t];
+                }
+            }
+        }
+        int sum = 0;
+#pragma unroll
+        for ( int k = 0; k < SCAN_K; k++ ) sum += v[k];
+        // inclusive scan of the thread sums within the wave
+        int x = sum;
         for ( int o = 1; o < WAVE; o <<= 1 ) {
             const int y = __shfl_up( x, o );
             if ( lane >= o ) x += y;
@@ -958,13 +996,24 @@ __device__ inline void sort_count_body( const SortArgs & a ) {
             if ( lane < BLOCK / WAVE ) s_wave[lane] = z;  // inclusive wave totals
         }
         __syncthreads();
-        const int before = carry + ( w > 0 ? s_wave[w - 1] : 0 ) + x - v;
-        if ( t0 + tid < ntiles ) new_off[t0 + tid] = before;
+        int off = carry + ( w > 0 ? s_wave[w - 1] : 0 ) + x - sum;
+        int o[SCAN_K];
+#pragma unroll
+        for ( int k = 0; k < SCAN_K; k++ ) {
+            o[k] = off;
+            off += v[k];
+        }
+        if ( vec && t0 + SCAN_K <= ntiles ) {
+#pragma unroll
+            for ( int q = 0; q < SCAN_K / 4; q++ )
+                *(gptr<int4>) ( new_off + t0 + 4 * q ) = make_int4( o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3] );
+        } else {
+#pragma unroll
+            for ( int k = 0; k < SCAN_K; k++ )
+                if ( t0 + k < ntiles ) new_off[t0 + k] = o[k];
+        }
         carry += s_wave[BLOCK / WAVE - 1];
-        moved += (unsigned) m;
-        __syncthreads();  // s_wave is rewritten by the next chunk
-        v = vn;
-        m = mn;
+        __syncthreads();  // s_wave is rewritten by the next round
     }
     if ( tid == 0 ) S.total[0] = carry;
     // Species::move's d_nmove += np (species.cpp:2162): the counts the
@@ -1161,13 +1210,13 @@ __device__ inline void sort_out_tile( const SortArgs & a, const int s ) {
                 const int k = o + ( ent >> 4 );
                 const int2 c = P.ix[k];
                 const float2 x = P.x[k];
-                const float3 u = P.u[k];
+                const float3 u = ld3( P.u + k );
                 const int xc = ( c.x >= q.nx ) - ( c.x < 0 );
                 const int yc = ( c.y >= q.ny ) - ( c.y < 0 );
                 const int dst = s_start[d] + rank;
                 T.ix[dst] = make_int2( c.x - xc * q.nx, c.y - yc * q.ny );
                 T.x[dst] = x;
-                T.u[dst] = u;
+                st3( T.u + dst, u );
             }
         };
 #pragma unroll
@@ -1191,12 +1240,12 @@ __device__ inline void sort_out_tile( const SortArgs & a, const int s ) {
             const unsigned long long mk = __ballot( stay );
             if ( stay ) {
                 const float2 x = P.x[o + n0 + j];
-                const float3 u = P.u[o + n0 + j];
+                const float3 u = ld3( P.u + ( o + n0 + j ) );
                 const int r = r0 + count_below( mk );
                 const int hole = o + ( leaver_at( r, pre, chunk, S.idx, o ) >> 4 );
                 P.ix[hole] = c;
                 P.x[hole] = x;
-                P.u[hole] = u;
+                st3( P.u + hole, u );
             }
             r0 += __popcll( mk );
         };
@@ -1216,7 +1265,7 @@ __device__ inline void sort_out_tile( const SortArgs & a, const int s ) {
         for ( int i = tid; i < n0; i += blockDim.x ) {
             T.ix[dst0 + i] = P.ix[o + i];
             T.x[dst0 + i] = P.x[o + i];
-            T.u[dst0 + i] = P.u[o + i];
+            st3( T.u + ( dst0 + i ), ld3( P.u + ( o + i ) ) );
         }
         if ( tid == 0 ) T.np[s] = S.new_np[s];
     } else {
@@ -1227,7 +1276,7 @@ __device__ inline void sort_out_tile( const SortArgs & a, const int s ) {
         for ( int i = tid; i < ns; i += blockDim.x ) {
             T.ix[dst0 + i] = P.ix[o + first + i];
             T.x[dst0 + i] = P.x[o + first + i];
-            T.u[dst0 + i] = P.u[o + first + i];
+            st3( T.u + ( dst0 + i ), ld3( P.u + ( o + first + i ) ) );
         }
         __syncthreads();
         if ( tid == 0 ) P.np[s] = n0 - ns;
@@ -1260,7 +1309,7 @@ __device__ inline void sort_in_tile( const SortArgs & a, const int t ) {
         const int dst = ( i < front ) ? no + i : no + n + i;
         P.ix[dst] = T.ix[no + i];
         P.x[dst] = T.x[no + i];
-        P.u[dst] = T.u[no + i];
+        st3( P.u + dst, ld3( T.u + ( no + i ) ) );
     }
     __syncthreads();
     if ( threadIdx.x == 0 ) {
@@ -1380,7 +1429,7 @@ __device__ inline void sort_shift_tile( const SortArgs & a, const int t ) {
             const int dst = o + w * chunk + s_seg[w] + ( k - s_gpre[w] );
             Gix[dst] = P.ix[p];
             Gx[dst] = P.x[p];
-            Gu[dst] = P.u[p];
+            st3( Gu + dst, ld3( P.u + p ) );
         }
         ns += tot;
     }
@@ -1500,7 +1549,7 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t ) {
             const int src = o + v * chunk + s_seg[4][v] + ( k - s_gpre[v] );
             P.ix[dst] = Gix[src];
             P.x[dst] = Gx[src];
-            P.u[dst] = Gu[src];
+            st3( P.u + dst, ld3( Gu + src ) );
         }
     }
     // the sources' leavers towards this tile: the entries of source e's
@@ -1531,7 +1580,7 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t ) {
                 if ( dst >= 0 ) {
                     P.ix[dst] = make_int2( c.x - dx * q.nx, c.y - dy * q.ny );
                     P.x[dst] = Gx[src];
-                    P.u[dst] = Gu[src];
+                    st3( P.u + dst, ld3( Gu + src ) );
                 }
             }
             run += __popcll( m );
@@ -1625,6 +1674,7 @@ SortView view( const zpic_sort * s ) {
     }
     return v;
 }
+
 
 int check_geom( const zpic_geom * g ) {
     if ( !g || g->ntiles_x < 1 || g->ntiles_y < 1 || g->nx < 1 || g->ny < 1 )
@@ -2230,10 +2280,15 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
 
     // tiles per item of the light stages
     constexpr int ZERO_TILES = 32, NEWNP_TILES = 1024, GC_TILES = 16;
+    constexpr long long PART_GRAIN = 65536;
+    constexpr int CELL_GRAIN = 4096;
     // tiles per item of the per-tile stages: small tiles are grouped so that
-    // an item moves at least ~16384 particles (particle pass, sort) or covers
-    // at least 1024 cells (Yee, filter pass), amortising the claim, the waits
-    // and above all the fences (an L2 write-back each) over more work
+    // an item moves at least ~65536 particles (particle pass, sort) or covers
+    // at least 4096 cells (Yee, filter pass), amortising the claim, the waits
+    // and above all the fences (an L2 write-back each) over more work (on the
+    // 64-instance scan: 17.8 ms per step at 16384 / 1024, 14.1 ms at 65536 /
+    // 4096, 18.7 ms at 262144 / 16384, where too few items remain to fill the
+    // device)
     std::vector<int> part_group( n ), cell_group( n );
     for ( int i = 0; i < n; i++ ) {
         const int cells = fl[i].g.nx * fl[i].g.ny;
@@ -2243,8 +2298,8 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
             const long long nt = (long long) it.fld.ntiles_x * it.fld.ntiles_y;
             ppt = std::max( ppt, (long long) it.p.max_part / std::max( nt, 1LL ) );
         }
-        part_group[i] = (int) std::min( 64LL, std::max( 1LL, 16384 / std::max( ppt, 1LL ) ) );
-        cell_group[i] = std::min( 16, std::max( 1, 1024 / std::max( cells, 1 ) ) );
+        part_group[i] = (int) std::min( 1024LL, std::max( 1LL, PART_GRAIN / std::max( ppt, 1LL ) ) );
+        cell_group[i] = std::min( 256, std::max( 1, CELL_GRAIN / std::max( cells, 1 ) ) );
     }
     std::vector<int> need( (size_t) ns * n, 0 );  // items per (stage, instance)
     std::vector<std::vector<StepItem>> per_stage( ns );

@@ -539,6 +539,13 @@ class Current {
     double dt;
     current::bc_type bc;
     int iter;
+    // Second buffer of the one-pass update (zpic_current_update): on periodic
+    // grids with no filter or a filter along x, advance() forms the new
+    // current from J's deposit into this grid's buffer in one launch, then
+    // J and this grid exchange buffers (allocated on first use).
+    vec3grid<float3> * Jalt;
+    // whether advance() may run as one pass, and the filter's x passes
+    bool one_pass_ok( std::vector<float> & coef ) const;
 
   public:
     vec3grid<float3> * J;
@@ -736,6 +743,11 @@ class Species {
 
     // Arguments of this species for a batched advance with the given fields
     zpic_batch_species batch_item( EMF const & emf, Current & current ) const;
+    // device particle buffers (they change when the buffers are swapped)
+    void buffers( std::vector<const void *> & out ) const {
+        out.push_back( particles->ix );
+        out.push_back( tmp->ix );
+    }
     // Host-side bookkeeping of one step advanced by a batch (iteration, nmove)
     void note_step() { iter++; }
 
@@ -824,8 +836,10 @@ class Batch {
     bool persistent;
     hipGraph_t graph;
     hipGraphExec_t exec;
+    std::vector<const void *> bound;  // device buffers the tables were built on
     void release();
     void step_kernels();
+    std::vector<const void *> buffers() const;
 
   public:
     explicit Batch( int threads_per_tile = 256 );
