@@ -14,6 +14,12 @@ beside it), with the binomial x filter, and unfiltered are compared with it:
   (denoise.common_band): the learned run closer to the plain run than the
   filtered and the unfiltered 2x2 runs (what the network gains over the
   method without it); the error itself is printed;
+* k-resolved: growth rates of the plain run's 8 fastest growing |k| shells
+  of the B spectrum, each run fitted over its own linear phase: the learned
+  run within 5 % of the plain run on average over the modes, and closer
+  than the filtered and unfiltered runs; the spectrum's mean |k| where the
+  linear phase ends within 15 % of the plain run's, and closer than theirs
+  (noise-seeded runs put their energy at shorter wavelengths);
 * wall time: the learned run, shadow box included, at least 8x faster per
   step than the plain run.
 
@@ -39,8 +45,11 @@ def test_denoiser_judge_weibel512():
     print("[gpu-judge] Weibel 512x512, 500 steps: plain 32x32 ppc vs 2x2 ppc (learned, binomial, none)", flush=True)
     r = train_denoiser.judge(w["params"], float(w["scale"]), 512, 500, 25, 64)
     lo = r["lo_learned"]
-    for k in ("lo_learned", "lo_binomial_x", "lo_none"):
+    print(f"[gpu-judge] dominant modes {r['modes']}", flush=True)
+    for k in ("plain_32x32", "lo_learned", "lo_binomial_x", "lo_none"):
         print(f"[gpu-judge] {k}: own window {r[k]['rel_err']:.3f}, common band {r[k]['rel_err_band']:.3f}, "
+              f"modes {r[k]['mode_rel_err']:.3f}, spectrum peak {r[k]['spectrum_peak']} mean |k| "
+              f"{r[k]['spectrum_mean_k']:.2f} ({r[k]['spectrum_mean_k_rel_err']:.3f}), "
               f"{r[k]['ms_per_step']:.3f} ms/step", flush=True)
     print(f"[gpu-judge] plain {r['plain_32x32']['ms_per_step']:.3f} ms/step; learned with shadow "
           f"{lo['ms_per_step_with_shadow']:.3f} ms/step; speed-up {r['speedup_learned_with_shadow_vs_plain']:.1f}x",
@@ -49,4 +58,9 @@ def test_denoiser_judge_weibel512():
     assert lo["rel_err"] <= 0.05
     assert lo["rel_err_band"] < r["lo_binomial_x"]["rel_err_band"]
     assert lo["rel_err_band"] < r["lo_none"]["rel_err_band"]
+    assert lo["mode_rel_err"] <= 0.05
+    assert lo["mode_rel_err"] < min(r["lo_binomial_x"]["mode_rel_err"], r["lo_none"]["mode_rel_err"])
+    assert lo["spectrum_mean_k_rel_err"] <= 0.15
+    assert lo["spectrum_mean_k_rel_err"] < min(r["lo_binomial_x"]["spectrum_mean_k_rel_err"],
+                                               r["lo_none"]["spectrum_mean_k_rel_err"])
     assert r["speedup_learned_with_shadow_vs_plain"] >= 8.0

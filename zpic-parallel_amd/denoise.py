@@ -179,15 +179,76 @@ def field_b_energy(sim):
     return float(np.sum(e[ns + 3:ns + 6]))
 
 
-def run_series(sim, steps, shadow=None):
-    """B-field energy after every step and the wall time of the run."""
+def interior_field(sim, which):
+    """Field `which` ("E", "B", "J") of a simulation as a global (ny, nx, 3)
+    array of its cells (guard cells and tile padding dropped)."""
+    g = sim.geom
+    buf = sim.field(which).reshape(sim.ntiles, g.tile_vol, 3)[:, :g.ext_x * g.ext_y]
+    a = buf.reshape(g.ntiles_y, g.ntiles_x, g.ext_y, g.ext_x, 3)
+    a = a[:, :, g.gc_lower:g.gc_lower + g.ny, g.gc_lower:g.gc_lower + g.nx]
+    return a.transpose(0, 2, 1, 3, 4).reshape(g.ntiles_y * g.ny, g.ntiles_x * g.nx, 3)
+
+
+def b_spectrum(sim):
+    """shell_spectrum of a simulation's magnetic field."""
+    return shell_spectrum(interior_field(sim, "B"))
+
+
+def shell_spectrum(B):
+    """Energy per |k| shell of a (ny, nx, 3) field: sum over the three
+    components of |FFT|^2 over the modes with round(|k|) = m (k in units of
+    the box's fundamental mode), m = 0 .. nx/2 - 1."""
+    B = np.asarray(B, np.float64)
+    ny, nx = B.shape[:2]
+    e = np.zeros((ny, nx // 2 + 1))
+    for c in range(3):
+        e += np.abs(np.fft.rfft2(B[..., c])) ** 2
+    ky = np.fft.fftfreq(ny) * ny
+    kx = np.arange(nx // 2 + 1)
+    kr = np.rint(np.sqrt(ky[:, None] ** 2 + kx[None, :] ** 2)).astype(int)
+    w = np.where(kx[None, :] == 0, 1.0, 2.0)  # the half plane rfft2 leaves out
+    return np.bincount(kr.ravel(), (e * w).ravel(), minlength=nx // 2)[:nx // 2]
+
+
+def mode_growth(spec, every, dt, window, modes):
+    """Growth rates (half the slope of ln energy) of the |k| shells `modes`
+    over the steps `window` = (first, last) of a run's linear phase, from
+    spectra recorded after every `every`-th step (spec[i] after step
+    (i + 1) * every)."""
+    spec = np.asarray(spec, np.float64)
+    t = (np.arange(spec.shape[0]) + 1) * every
+    i0, i1 = np.searchsorted(t, window[0] + 1), np.searchsorted(t, window[1] + 1)
+    if i1 - i0 < 3:
+        return np.full(len(modes), np.nan)
+    return np.array([0.5 * np.polyfit(t[i0:i1] * dt, np.log(spec[i0:i1, m]), 1)[0] for m in modes])
+
+
+def spectrum_shape(spec, every, step, kmax):
+    """Peak shell and energy-weighted mean |k| (shells 1 .. kmax - 1) of the
+    spectrum recorded at or before `step`."""
+    spec = np.asarray(spec, np.float64)
+    i = min(max((step + 1) // every - 1, 0), spec.shape[0] - 1)
+    s = spec[i, 1:kmax]
+    s = s / s.sum()
+    return int(np.argmax(s)) + 1, float((np.arange(1, kmax) * s).sum())
+
+
+def run_series(sim, steps, shadow=None, spectrum_every=0):
+    """B-field energy after every step and the wall time of the run; with
+    spectrum_every > 0 also b_spectrum after every spectrum_every-th step."""
     sim.sync()
     out = np.zeros(steps)
+    specs = []
     t0 = time.perf_counter()
     for k in range(steps):
         sim.advance()
         if shadow is not None:
             shadow.advance()
         out[k] = field_b_energy(sim)
+        if spectrum_every and (k + 1) % spectrum_every == 0:
+            specs.append(b_spectrum(sim))
     sim.sync()
-    return out, time.perf_counter() - t0
+    wall = time.perf_counter() - t0
+    if spectrum_every:
+        return out, wall, np.array(specs)
+    return out, wall

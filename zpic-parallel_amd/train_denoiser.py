@@ -10,9 +10,11 @@
    and reported (network vs raw subsample vs binomial filters).
 3. Judge (`--judge`): on a `--judge-grid`^2 grid, the plain 32x32-ppc run
    against 2x2-ppc runs with the network (with the shadow check), with the
-   binomial filter, and unfiltered: growth rate of the magnetic energy over a
-   band of B energy common to all runs (denoise.common_band), per-run
-   linear-phase fits beside it, and wall time per step.
+   binomial filter, and unfiltered: growth rate of the magnetic energy over
+   each run's linear phase and over a band of B energy common to all runs
+   (denoise.common_band); growth rates of the plain run's dominant |k| modes
+   in each run's linear phase and the B spectrum's peak and mean |k| where
+   that phase ends; wall time per step.
 
 Prints progress lines and ends with one JSON line.
 """
@@ -29,6 +31,12 @@ sys.path.insert(0, HERE)
 
 import denoise  # noqa: E402
 import pyzpic as zp  # noqa: E402
+
+
+# k-resolved judge: B spectrum every SPECTRUM_EVERY steps, growth of the
+# plain run's DOMINANT_MODES fastest growing |k| shells
+SPECTRUM_EVERY = 5
+DOMINANT_MODES = 8
 
 
 def log(msg):
@@ -73,12 +81,13 @@ def judge(params, scale, n, steps, shadow_every, shadow_n):
         "lo_binomial_x": lambda: denoise.weibel_grid(n, denoise.LO_PPC, filter=("binomial", 0, 1)),
         "lo_learned": learned,
     }
-    series = {}
+    series, spectra = {}, {}
     for name, make in makers.items():
         sim = make()
         sh = denoise.ShadowCheck(sim, **shadow_args) if name == "lo_learned" else None
-        eb, wall = denoise.run_series(sim, steps, shadow=sh)
+        eb, wall, sp = denoise.run_series(sim, steps, shadow=sh, spectrum_every=SPECTRUM_EVERY)
         series[name] = eb
+        spectra[name] = sp
         if sh is not None:
             res["shadow"] = {"checks": [[int(h[0]), float(h[1]), float(h[2]), bool(h[3])] for h in sh.history],
                              "fallback": bool(sim.get_fallback())}
@@ -101,6 +110,23 @@ def judge(params, scale, n, steps, shadow_every, shadow_n):
                      "growth_rate_band": gb[0], "window_band": gb[1:],
                      "rel_err_band": abs(gb[0] - ref_band[0]) / ref_band[0],
                      "ms_per_step": 1e3 * walls[name], "eb_first": float(eb[0]), "eb_max": float(eb.max())}
+    # k-resolved: growth rates of the plain run's DOMINANT_MODES fastest
+    # growing |k| shells, each run fitted over its own linear phase, and the
+    # spectrum's peak and mean |k| where the linear phase ends
+    kmax = n // 4
+    plain_rates = denoise.mode_growth(spectra["plain_32x32"], SPECTRUM_EVERY, dt, ref[1:], range(kmax))
+    modes = sorted(int(m) for m in [m for m in np.argsort(-plain_rates) if m > 0][:DOMINANT_MODES])
+    res["modes"] = modes
+    g_ref = denoise.mode_growth(spectra["plain_32x32"], SPECTRUM_EVERY, dt, ref[1:], modes)
+    for name in series:
+        g = denoise.mode_growth(spectra[name], SPECTRUM_EVERY, dt, res[name]["window"], modes)
+        peak, mean_k = denoise.spectrum_shape(spectra[name], SPECTRUM_EVERY, res[name]["window"][1], kmax)
+        res[name].update({"mode_rates": [float(v) for v in g],
+                          "mode_rel_err": float(np.mean(np.abs(g - g_ref) / g_ref)),
+                          "spectrum_peak": peak, "spectrum_mean_k": mean_k})
+    for name in series:
+        res[name]["spectrum_mean_k_rel_err"] = (abs(res[name]["spectrum_mean_k"] - res["plain_32x32"]["spectrum_mean_k"])
+                                                / res["plain_32x32"]["spectrum_mean_k"])
     res["series"] = {k: [float(v) for v in eb] for k, eb in series.items()}
     res["lo_learned"]["ms_per_step_with_shadow"] = 1e3 * walls["lo_learned_with_shadow"]
     res["speedup_learned_vs_plain"] = walls["plain_32x32"] / walls["lo_learned"]
