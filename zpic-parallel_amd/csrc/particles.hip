@@ -1995,12 +1995,34 @@ __device__ inline void step_wait( const StepTables & T, int c, int need ) {
     }
 }
 
+// A workgroup runs up to STEP_CHAIN items of one (stage, instance) in a row
+// behind one acquire and publishes them with one release: such items wait for
+// the same counters (so the acquire of the first covers the others) and do
+// not read each other's output.  Finished items are published before the
+// workgroup waits for anything else, so that it never waits for itself.
+constexpr int STEP_CHAIN = 8;
+
 template <int BLOCK>
 __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
     extern __shared__ float smem[];
     __shared__ int s_item;
     const int home = blockIdx.x % STEP_QUEUES;
     int q = 0;  // queues exhausted so far, counted from the home queue
+    int pending = 0;      // finished items not yet published
+    int pend_done = -1;   // their counter
+    int pend_dep[4] = { -1, 0, -1, 0 };  // their dependencies
+    // every wave's stores complete at the L2, then one device-scope release
+    // (an L2 write-back) and the counter increment
+    auto publish = [&]() {
+        if ( pending == 0 ) return;
+        asm volatile( "s_waitcnt vmcnt(0)" ::: "memory" );
+        __syncthreads();
+        if ( threadIdx.x == 0 ) {
+            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "agent" );
+            __hip_atomic_fetch_add( T.counters + pend_done, pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+        }
+        pending = 0;
+    };
     for ( ;; ) {
         if ( threadIdx.x == 0 ) {
             int k = -1;
@@ -2024,12 +2046,23 @@ __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
         __syncthreads();
         if ( k < 0 ) break;
         const StepItem it = uniform_load( T.items + k );
-        if ( threadIdx.x == 0 ) {
-            step_wait( T, it.dep0, it.need0 );
-            step_wait( T, it.dep1, it.need1 );
-            // the caches this workgroup reads through (its CU's L1, its XCD's
-            // L2) are dropped once, before any wave loads the item's inputs
-            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
+        const bool chained = pending > 0 && pending < STEP_CHAIN && it.done == pend_done && it.dep0 == pend_dep[0] &&
+                             it.need0 == pend_dep[1] && it.dep1 == pend_dep[2] && it.need1 == pend_dep[3];
+        if ( !chained ) {
+            publish();
+            if ( threadIdx.x == 0 ) {
+                step_wait( T, it.dep0, it.need0 );
+                step_wait( T, it.dep1, it.need1 );
+                // the caches this workgroup reads through (its CU's L1, its
+                // XCD's L2) are dropped once, before any wave loads the
+                // item's inputs
+                __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
+            }
+            pend_done = it.done;
+            pend_dep[0] = it.dep0;
+            pend_dep[1] = it.need0;
+            pend_dep[2] = it.dep1;
+            pend_dep[3] = it.need1;
         }
         __syncthreads();
         switch ( it.kind ) {
@@ -2103,17 +2136,13 @@ __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
         }
         default: break;
         }
-        // Every wave waits for its stores to complete at the L2, then one
-        // device-scope release writes the L2 back before the item counts as
-        // finished (one L2 write-back per item, not one per wave: each is a
-        // whole-cache operation that serialises in the L2).
-        asm volatile( "s_waitcnt vmcnt(0)" ::: "memory" );
-        __syncthreads();
-        if ( threadIdx.x == 0 ) {
-            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "agent" );
-            __hip_atomic_fetch_add( T.counters + it.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
-        }
+        // published with the chain (one L2 write-back per chain, not per item
+        // or per wave: each is a whole-cache operation that serialises in the
+        // L2)
+        pending++;
+        __syncthreads();  // LDS of this item read out before the next one
     }
+    publish();
 }
 
 // Clears the claim and stage counters before each step (a kernel, so that the
