@@ -79,12 +79,18 @@ __global__ void __launch_bounds__( TILE_BLOCK ) k_kernel3( K3Args args, const K3
     kernel3_tile( args, t, A );
 }
 
-// 128 threads: 112 us on the 2048^2 grid against 115 us at 256 and 139 us
-// at 512 (more tiles in flight per CU for this latency-bound gather)
+// 128 threads and CUR_TILES tiles per workgroup: 104-108 us on the 2048^2
+// grid, against 113 us with one tile per workgroup and 115 / 139 us with
+// 256 / 512 threads (the fixed costs of a workgroup amortised; the same
+// for the Yee update and the guard copy gained nothing or lost)
 constexpr int CUR_BLOCK = 128;
+constexpr int CUR_TILES = 4;
 __global__ void __launch_bounds__( CUR_BLOCK ) k_current( CurArgs a ) {
     extern __shared__ float sm[];
-    current_tile( a, blockIdx.x, sm );
+    for ( int t = blockIdx.x; t < a.g.ntx * a.g.nty; t += gridDim.x ) {
+        if ( t != (int) blockIdx.x ) __syncthreads();  // LDS of the previous tile read out
+        current_tile( a, t, sm );
+    }
 }
 
 __global__ void __launch_bounds__( TILE_BLOCK ) k_yee( YeeArgs a, const YeeArgs * tab, const int2 * map ) {
@@ -958,7 +964,7 @@ int zpic_current_update( const float * dep, float * J, const zpic_geom * gg, int
         a.c[p] = coef[3 * p + 2];
     }
     hipStream_t st = as_stream( stream );
-    hipLaunchKernelGGL( k_current, dim3( g.ntx * g.nty ), dim3( CUR_BLOCK ), lds, st, a );
+    hipLaunchKernelGGL( k_current, dim3( ( g.ntx * g.nty + CUR_TILES - 1 ) / CUR_TILES ), dim3( CUR_BLOCK ), lds, st, a );
     ZPIC_LAUNCH_CHECK( "k_current" );
     return ZPIC_OK;
 }
