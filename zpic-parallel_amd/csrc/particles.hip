@@ -479,6 +479,17 @@ __device__ inline int select_instance( A & a, const A * tab, const int2 * map ) 
     return blockIdx.x;
 }
 
+// The same for kernels that take one item per wave: item g of the launch.
+template <class A>
+__device__ inline int select_item( A & a, const A * tab, const int2 * map, int g ) {
+    if ( map ) {
+        const int2 r = gbl( map )[g];
+        a = gbl( tab )[r.x];
+        return r.y;
+    }
+    return g;
+}
+
 // The particle pass of tile t (one workgroup).  `smem` is the workgroup's
 // LDS of particles_lds_bytes( vol, W ) bytes, W = the waves that take
 // particles: all of the workgroup's, or a.waves if that is smaller (the
@@ -1324,39 +1335,51 @@ __global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in( SortArgs a, const Sor
 }
 
 // ---------------------------------------------------------------------------
-// Order-free tile sort of the fast mode.  The particle pass has left every
-// particle in place (leavers with their cell outside the tile), the leaver
-// list in wave segments (idx, seg) and a copy of every leaver's record, cell
-// in the target tile's frame, in the staging arrays at its leaver-list
-// position.  After the new counts and offsets (k_sort_newnp, k_sort_count):
+// Order-free tile sort of the fast mode.  The particle pass has compacted
+// each wave chunk's stayers to the chunk's front (particle arrays) and
+// written its leavers, cell still in the source tile's frame, to the
+// staging arrays at the chunk's first positions; seg[v] counts chunk v's
+// leavers and npt[9 t + d] the tile's leavers of direction d.  After the new
+// counts and offsets (k_sort_newnp, k_sort_count):
 //   k_sort_shift: each tile stages the stayers that lie outside its new range
-//     into the free slots of its staging region (between the wave segments)
-//     and keeps its old offset and count in seg[SEG_OLD_OFF], seg[SEG_OLD_NP];
+//     into the free slots of its staging region (behind each chunk's
+//     leavers), keeps its old offset and count in seg[SEG_OLD_OFF],
+//     seg[SEG_OLD_NP], and buckets its leavers by direction: idx[o + start(d)
+//     + r] = the chunk-relative staging position of the r-th leaver of
+//     direction d in staging order, start(d) = the tile's leavers of the
+//     directions before d;
 //   k_sort_in_fast: each tile fills the slots of its new range that do not
-//     hold one of its stayers (its holes inside the new range, then the new
-//     range outside the old one) with its staged stayers and the leavers its
-//     nine sources staged towards it, in a fixed enumeration order.
+//     hold one of its stayers with its staged stayers, then with the leavers
+//     of its eight sources' buckets towards it, in a fixed enumeration order.
 // Every record is read once and written once after the particle pass; the
 // within-tile order is not the reference's (counts, offsets and per-tile
-// multisets are).
+// multisets are).  One wave sorts one tile: the work per tile is a few
+// hundred records behind a chain of dependent loads, so the kernels are
+// bound by latency and gain from as many tiles in flight as possible.
 constexpr int SEG_OLD_OFF = 13, SEG_OLD_NP = 14;
+constexpr int SORT_WAVES = 4;       // tiles per workgroup of k_sort_shift / k_sort_in_fast
+constexpr int SORT_WAVE_LDS = 384;  // ints of LDS per wave
+constexpr int SORT_BATCH = 4;       // records per lane loaded before any is stored
 
-// Workgroup-wide exclusive rank of `pred` among the threads; `total` gets
-// the count.  Called by every thread of the workgroup.
-__device__ inline int block_rank( bool pred, int & total, int * s_w ) {
-    const unsigned long long m = __ballot( pred );
-    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
-    if ( lane == 0 ) s_w[w] = __popcll( m );
-    __syncthreads();
-    int before = 0, tot = 0;
-    for ( int v = 0; v < nw; v++ ) {
-        const int c = s_w[v];
-        before += v < w ? c : 0;
-        tot += c;
+// Exclusive prefix sum of x over the wave's lanes; `total` gets the sum.
+__device__ inline int wave_excl_scan( int x, int & total ) {
+    const int lane = threadIdx.x % WAVE;
+    int s = x;
+#pragma unroll
+    for ( int o = 1; o < WAVE; o <<= 1 ) {
+        const int y = __shfl_up( s, o );
+        if ( lane >= o ) s += y;
     }
-    __syncthreads();
-    total = tot;
-    return before + count_below( m );
+    total = __shfl( s, WAVE - 1 );
+    return s - x;
+}
+
+// Orders the wave's LDS writes before its following LDS reads (the wave's
+// LDS accesses execute in order; this keeps the compiler from moving them).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
 }
 
 // Segment v of a prefix table pre[0..MAX_SEG] (non-decreasing) that holds
@@ -1368,95 +1391,156 @@ __device__ inline int segment_of( int k, const int * pre ) {
     return w;
 }
 
-// Free-slot prefix of a tile's staging region: the wave segments of the
-// particle pass are [v chunk, min(np, (v + 1) chunk)), the first seg[v]
-// slots of each hold leaver records.
-__device__ inline void gap_prefix( const int * seg, int np, int chunk, int * gpre ) {
-    int acc = 0;
-    for ( int v = 0; v < MAX_SEG; v++ ) {
-        gpre[v] = acc;
-        const int len = min( max( np - v * chunk, 0 ), chunk );
-        acc += max( len - seg[v], 0 );
-    }
-    gpre[MAX_SEG] = acc;
+// Direction of a cell relative to the tile (4: inside).
+__device__ inline int cell_dir( int2 c, const SortGeo & q ) {
+    return ( ( c.y >= q.ny ) - ( c.y < 0 ) + 1 ) * 3 + ( ( c.x >= q.nx ) - ( c.x < 0 ) + 1 );
 }
 
-__device__ inline void sort_shift_tile( const SortArgs & a, const int t ) {
-    __shared__ int s_seg[MAX_SEG + 1];
-    __shared__ int s_gpre[MAX_SEG + 1];
-    __shared__ int s_hdr[4];
-    __shared__ int s_w[FAST_BLOCK / WAVE];
+// k_sort_shift for tile t by one wave; L = the wave's SORT_WAVE_LDS ints.
+__device__ inline void sort_shift_wave( const SortArgs & a, const int t, int * L ) {
+    int * const seg = L;        // [16] leavers per chunk, [MAX_SEG] = chunk
+    int * const gpre = L + 16;  // [16] free-slot prefix of the staging region
+    int * const pre = L + 32;   // [16] prefix of the leaver counts
     const PartG P = gview( a.P );
     const SortG S = gview( a.S );
     const gptr<int2> Gix = gbl( a.S.stage_ix );
     const gptr<float2> Gx = gbl( a.S.stage_x );
     const gptr<float3> Gu = gbl( a.S.stage_u );
     const gptr<const int> new_off = gbl( (const int *) a.T.off );
-    const int tid = threadIdx.x;
-    if ( tid <= MAX_SEG ) s_seg[tid] = S.seg[16 * t + tid];  // [MAX_SEG] = chunk
-    else if ( tid >= WAVE && tid < WAVE + 4 ) {
-        const int h = tid - WAVE;
-        s_hdr[h] = h == 0 ? P.off[t] : h == 1 ? P.np[t] : h == 2 ? new_off[t] : S.new_np[t];
+    const int lane = threadIdx.x % WAVE;
+    wave_sync();  // the wave's previous tile has read L
+
+    // lanes 0-15: seg; 16-19: old offset, old count, new offset, new count;
+    // 20-28: direction counts
+    int val = 0;
+    if ( lane <= MAX_SEG ) val = S.seg[16 * t + lane];
+    else if ( lane == 16 ) val = P.off[t];
+    else if ( lane == 17 ) val = P.np[t];
+    else if ( lane == 18 ) val = new_off[t];
+    else if ( lane == 19 ) val = S.new_np[t];
+    else if ( lane < 29 ) val = S.npt[9 * t + lane - 20];
+    const int o = __builtin_amdgcn_readlane( val, 16 ), np = __builtin_amdgcn_readlane( val, 17 );
+    const int no = __builtin_amdgcn_readlane( val, 18 ), nn = __builtin_amdgcn_readlane( val, 19 );
+    const int chunk = __builtin_amdgcn_readlane( val, MAX_SEG );
+    {
+        // chunk v = [v chunk, min(np, (v + 1) chunk)), its first seg[v]
+        // staging slots hold leavers, the rest are free
+        const int v = lane;
+        const int s = v < MAX_SEG ? val : 0;
+        const int len = v < MAX_SEG ? min( max( np - v * chunk, 0 ), chunk ) : 0;
+        int nl, nfree;
+        const int p = wave_excl_scan( s, nl );
+        const int f = wave_excl_scan( v < MAX_SEG ? max( len - s, 0 ) : 0, nfree );
+        if ( v <= MAX_SEG ) {
+            seg[v] = val;
+            pre[v] = v < MAX_SEG ? p : nl;
+            gpre[v] = v < MAX_SEG ? f : nfree;
+        }
     }
-    __syncthreads();
-    const int o = s_hdr[0], np = s_hdr[1], no = s_hdr[2], nn = s_hdr[3], chunk = s_seg[MAX_SEG];
-    if ( tid == 0 ) {
-        gap_prefix( s_seg, np, chunk, s_gpre );
+    if ( lane == 0 ) {
         S.seg[16 * t + SEG_OLD_OFF] = o;
         S.seg[16 * t + SEG_OLD_NP] = np;
     }
-    __syncthreads();
-    // old-range positions below and above the new range
+    // bucket starts: the leavers of the directions before (lane 20 + d)
+    int dstart, dtot;
+    {
+        const int d = lane - 20;
+        dstart = wave_excl_scan( ( d >= 0 && d < 9 && d != 4 ) ? val : 0, dtot );
+    }
+    wave_sync();
+
+    // stayers at old-range positions below and above the new range: to the
+    // free staging slots in order
     const int a0 = o, la = max( min( o + np, no ) - o, 0 );
     const int b0 = max( o, no + nn ), lb = max( o + np - b0, 0 );
     const int nout = la + lb;
     int ns = 0;
-    for ( int j0 = 0; j0 < nout; j0 += blockDim.x ) {
-        const int j = j0 + tid;
+    for ( int j0 = 0; j0 < nout; j0 += WAVE ) {
+        const int j = j0 + lane;
         bool stay = false;
         int p = 0;
         if ( j < nout ) {
             p = j < la ? a0 + j : b0 + ( j - la );
-            // the particle pass compacted each chunk's stayers to its front:
-            // chunk v holds them at [v chunk, v chunk + len_v - seg[v])
             const int r = p - o, v = r / chunk, len = min( np - v * chunk, chunk );
-            stay = r - v * chunk < len - s_seg[v];
+            stay = r - v * chunk < len - seg[v];
         }
-        int tot;
-        const int k = ns + block_rank( stay, tot, s_w );
+        const unsigned long long m = __ballot( stay );
         if ( stay ) {
-            const int w = segment_of( k, s_gpre );
-            const int dst = o + w * chunk + s_seg[w] + ( k - s_gpre[w] );
-            Gix[dst] = P.ix[p];
-            Gx[dst] = P.x[p];
-            st3( Gu + dst, ld3( P.u + p ) );
+            const int k = ns + count_below( m );
+            const int w = segment_of( k, gpre );
+            const int dst = o + w * chunk + seg[w] + ( k - gpre[w] );
+            const int2 c = P.ix[p];
+            const float2 x = P.x[p];
+            const float3 u = ld3( P.u + p );
+            Gix[dst] = c;
+            Gx[dst] = x;
+            st3( Gu + dst, u );
         }
-        ns += tot;
+        ns += __popcll( m );
     }
-    if ( tid == 0 ) S.tmp_np[t] = ns;
+    if ( lane == 0 ) S.tmp_np[t] = ns;
+
+    // direction buckets of the leavers, staging order kept in each
+    const int nl = pre[MAX_SEG];
+    int run[9];
+#pragma unroll
+    for ( int d = 0; d < 9; d++ ) run[d] = __builtin_amdgcn_readlane( dstart, 20 + d );
+    for ( int j0 = 0; j0 < nl; j0 += SORT_BATCH * WAVE ) {
+        int pos[SORT_BATCH], dir[SORT_BATCH];
+#pragma unroll
+        for ( int i = 0; i < SORT_BATCH; i++ ) {
+            const int j = j0 + i * WAVE + lane;
+            pos[i] = 0;
+            dir[i] = 4;
+            if ( j < nl ) {
+                const int v = segment_of( j, pre );
+                pos[i] = v * chunk + ( j - pre[v] );
+                dir[i] = cell_dir( Gix[o + pos[i]], a.q );
+            }
+        }
+#pragma unroll
+        for ( int i = 0; i < SORT_BATCH; i++ ) {
+            int at = -1;
+#pragma unroll
+            for ( int d = 0; d < 9; d++ ) {
+                if ( d == 4 ) continue;
+                const unsigned long long m = __ballot( dir[i] == d );
+                if ( dir[i] == d ) at = run[d] + count_below( m );
+                run[d] += __popcll( m );
+            }
+            if ( at >= 0 ) S.idx[o + at] = pos[i];
+        }
+    }
 }
 
-__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_shift( SortArgs a, const SortArgs * tab, const int2 * map ) {
-    const int t = select_instance( a, tab, map );
-    sort_shift_tile( a, t );
+__global__ void __launch_bounds__( SORT_WAVES * WAVE ) k_sort_shift( SortArgs a, const SortArgs * tab, const int2 * map, int n ) {
+    __shared__ int s_lds[SORT_WAVES][SORT_WAVE_LDS];
+    const int w = __builtin_amdgcn_readfirstlane( (int) threadIdx.x / WAVE );
+    const int g = blockIdx.x * SORT_WAVES + w;
+    if ( g >= n ) return;
+    const int t = select_item( a, tab, map, g );
+    sort_shift_wave( a, t, s_lds[w] );
 }
 
-// Placement of the order-free sort for tile t.  The slots of the new range
-// [no, no + nn) that none of the tile's stayers holds are runs: the part
-// before the old range, the tail of every chunk (behind its compacted
-// stayers) and the part after the old range.  They are filled in order with
-// the tile's staged stayers, then with its eight sources' leavers towards
-// it, source by source (each in its staging order); every write is a run.
-__device__ inline void sort_in_fast_tile( const SortArgs & a, const int t ) {
-    __shared__ int s_seg[9][MAX_SEG + 1];  // leavers per chunk of the 9 sources (e = 4: the tile)
-    __shared__ int s_so[9][3];             // source tile, old offset, old count
-    __shared__ int s_pre[9][MAX_SEG + 1];  // prefix of the leaver counts
-    __shared__ int s_cin[9];               // particles source e sends to this tile
-    __shared__ int s_ebase[9];             // first rank of source e's particles
-    __shared__ int s_gpre[MAX_SEG + 1];    // free-slot prefix of the tile's staging region
-    __shared__ int s_hdr[3];
-    __shared__ int s_run[MAX_SEG + 2][2];  // hole runs (start, length) in position order
-    __shared__ int s_rpre[MAX_SEG + 3];    // prefix of their lengths
+// k_sort_in_fast for tile t by one wave; L = the wave's SORT_WAVE_LDS ints.
+// The slots of the new range [no, no + nn) that none of the tile's stayers
+// holds are runs: the part before the old range, the tail of every chunk
+// (behind its compacted stayers) and the part after the old range.  They
+// are filled in order with the tile's staged stayers, then with its eight
+// sources' leavers towards it, source by source (each in its staging order,
+// from the source's direction bucket).
+__device__ inline void sort_in_fast_wave( const SortArgs & a, const int t, int * L ) {
+    int * const seg = L;          // [9][16] leavers per chunk of the 9 sources (e = 4: the tile)
+    int * const so_s = L + 144;   // [9] source tile
+    int * const so_o = L + 153;   // [9] its old offset
+    int * const so_n = L + 162;   // [9] its old count
+    int * const npt = L + 171;    // [9][9] direction counts of the 9 sources
+    int * const hdr = L + 252;    // [4] new offset, new count, staged stayers
+    int * const gpre = L + 256;   // [16] free-slot prefix of the tile's staging region
+    int * const rb = L + 272;     // [17] hole run starts, in position order
+    int * const rpre = L + 289;   // [18] prefix of their lengths
+    int * const bst = L + 307;    // [9] start of source e's bucket towards the tile
+    int * const ebase = L + 316;  // [10] first rank of source e's particles; [9] = all
     const PartG P = gview( a.P );
     const SortG S = gview( a.S );
     const gptr<const int2> Gix = gbl( (const int2 *) a.S.stage_ix );
@@ -1464,138 +1548,162 @@ __device__ inline void sort_in_fast_tile( const SortArgs & a, const int t ) {
     const gptr<const float3> Gu = gbl( (const float3 *) a.S.stage_u );
     const gptr<const int> new_off = gbl( (const int *) a.T.off );
     const SortGeo q = a.q;
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x % WAVE;
+    wave_sync();  // the wave's previous tile has read L
 
-    // set-up: every load at once
-    if ( tid < 9 * 16 ) {
-        const int e = tid / 16, v = tid % 16;
-        const int s = ( e == 4 ) ? t : source_of( t, e, q );
-        const int val = s >= 0 ? S.seg[16 * s + v] : 0;
-        if ( v <= MAX_SEG ) s_seg[e][v] = ( v == SEG_OLD_OFF || v == SEG_OLD_NP ) ? 0 : val;
-        if ( v == SEG_OLD_OFF ) s_so[e][1] = val;
-        if ( v == SEG_OLD_NP ) s_so[e][2] = val;
-        if ( v == 0 ) s_so[e][0] = s;
-    } else if ( tid < 9 * 16 + 3 ) {
-        const int h = tid - 9 * 16;
-        s_hdr[h] = h == 0 ? new_off[t] : h == 1 ? S.new_np[t] : S.tmp_np[t];
-    } else if ( tid >= 9 * 16 + 3 && tid < 9 * 16 + 12 ) {
-        const int e = tid - ( 9 * 16 + 3 );
-        const int s = source_of( t, e, q );
-        s_cin[e] = ( e != 4 && s >= 0 ) ? S.npt[9 * s + e] : 0;
-    }
-    __syncthreads();
-    if ( tid < 9 ) {
-        const int e = tid;
-        int acc = 0;
-        for ( int v = 0; v < MAX_SEG; v++ ) {
-            s_pre[e][v] = acc;
-            acc += s_so[e][0] >= 0 ? s_seg[e][v] : 0;
+    // set-up: every load at once (228 values, four per lane)
+#pragma unroll
+    for ( int i = 0; i < 4; i++ ) {
+        const int k = i * WAVE + lane;
+        if ( k < 144 ) {
+            const int e = k / 16, v = k % 16;
+            const int s = ( e == 4 ) ? t : source_of( t, e, q );
+            const int x = s >= 0 ? S.seg[16 * s + v] : 0;
+            seg[k] = ( v == SEG_OLD_OFF || v == SEG_OLD_NP ) ? 0 : x;
+            if ( v == SEG_OLD_OFF ) so_o[e] = x;
+            if ( v == SEG_OLD_NP ) so_n[e] = x;
+            if ( v == 0 ) so_s[e] = s;
+        } else if ( k < 144 + 81 ) {
+            const int e = ( k - 144 ) / 9, d = ( k - 144 ) % 9;
+            const int s = source_of( t, e, q );
+            npt[k - 144] = ( e != 4 && s >= 0 ) ? S.npt[9 * s + d] : 0;
+        } else if ( k < 144 + 81 + 3 ) {
+            const int h = k - 225;
+            hdr[h] = h == 0 ? new_off[t] : h == 1 ? S.new_np[t] : S.tmp_np[t];
         }
-        s_pre[e][MAX_SEG] = acc;
     }
-    if ( tid == 16 ) gap_prefix( s_seg[4], s_so[4][2], s_seg[4][MAX_SEG], s_gpre );
-    if ( tid == 32 ) {
-        // hole runs of the new range, in position order
-        const int o = s_so[4][1], np = s_so[4][2], chunk = s_seg[4][MAX_SEG];
-        const int no = s_hdr[0], nn = s_hdr[1];
-        int nr = 0;
-        auto add = [&]( int b, int e ) {  // [b, e) clipped to the new range
-            b = max( b, no );
-            e = min( e, no + nn );
-            s_run[nr][0] = b;
-            s_run[nr][1] = max( e - b, 0 );
-            nr++;
-        };
-        add( no, o );  // before the old range
-        for ( int v = 0; v < MAX_SEG; v++ ) {
+    wave_sync();
+
+    const int o = so_o[4], np = so_n[4], chunk = seg[4 * 16 + MAX_SEG];
+    const int no = hdr[0], nn = hdr[1], ns = hdr[2];
+    {
+        // lanes 0-8: source e's bucket towards the tile and its first rank
+        const int e = lane;
+        int st = 0, cin = 0;
+        if ( e < 9 && e != 4 ) {
+            for ( int d = 0; d < e; d++ ) st += d == 4 ? 0 : npt[9 * e + d];
+            cin = npt[9 * e + e];
+        }
+        int all;
+        const int eb = ns + wave_excl_scan( e < 9 ? cin : 0, all );
+        if ( e < 9 ) {
+            bst[e] = st;
+            ebase[e] = eb;
+        }
+        if ( e == 9 ) ebase[9] = ns + all;
+    }
+    {
+        // lanes 0-14: chunk v's free staging slots (behind its leavers)
+        const int v = lane;
+        const int len = v < MAX_SEG && chunk > 0 ? min( max( np - v * chunk, 0 ), chunk ) : 0;
+        const int sv = v < MAX_SEG ? seg[4 * 16 + v] : 0;
+        int nfree;
+        const int f = wave_excl_scan( v < MAX_SEG ? max( len - sv, 0 ) : 0, nfree );
+        if ( v <= MAX_SEG ) gpre[v] = v < MAX_SEG ? f : nfree;
+    }
+    {
+        // lanes 0-16: hole runs of the new range, clipped to it: before the
+        // old range, behind chunk r - 1's stayers, after the old range
+        const int r = lane;
+        int b = 0, en = 0;
+        if ( r == 0 ) {
+            b = no;
+            en = o;
+        } else if ( r <= MAX_SEG ) {
+            const int v = r - 1;
             const int len = chunk > 0 ? min( max( np - v * chunk, 0 ), chunk ) : 0;
-            add( o + v * chunk + ( len - s_seg[4][v] ), o + v * chunk + len );  // behind chunk v's stayers
+            b = o + v * chunk + ( len - seg[4 * 16 + v] );
+            en = o + v * chunk + len;
+        } else if ( r == MAX_SEG + 1 ) {
+            b = o + np;
+            en = no + nn;
         }
-        add( o + np, no + nn );  // after the old range
-        int acc = 0;
-        for ( int r = 0; r < MAX_SEG + 2; r++ ) {
-            s_rpre[r] = acc;
-            acc += s_run[r][1];
+        b = max( b, no );
+        en = min( en, no + nn );
+        int nholes;
+        const int rp = wave_excl_scan( r <= MAX_SEG + 1 ? max( en - b, 0 ) : 0, nholes );
+        if ( r <= MAX_SEG + 1 ) {
+            rb[r] = b;
+            rpre[r] = rp;
         }
-        s_rpre[MAX_SEG + 2] = acc;
+        if ( r == MAX_SEG + 2 ) rpre[r] = nholes;
     }
-    if ( tid == 0 ) {
-        // ranks: the tile's staged stayers first, then the sources' particles
-        // by direction, each source's in its staging order
-        int acc = s_hdr[2];
-        for ( int e = 0; e < 9; e++ ) {
-            s_ebase[e] = acc;
-            acc += s_cin[e];
-        }
-    }
-    __syncthreads();
+    wave_sync();
 
-    const int o = s_so[4][1], chunk = s_seg[4][MAX_SEG];
-    const int ns = s_hdr[2], nholes = s_rpre[MAX_SEG + 2];
-    const int W = blockDim.x / WAVE, w = tid / WAVE, lane = tid % WAVE;
+    const int nholes = rpre[MAX_SEG + 2], ntot = ebase[9];
     // position of fill rank k (-1 past the holes: the counts always balance)
     auto slot = [&]( int k ) {
         if ( k >= nholes ) return -1;
         int r = 0;
 #pragma unroll
-        for ( int v = 1; v < MAX_SEG + 2; v++ ) r += s_rpre[v] <= k;
-        return s_run[r][0] + ( k - s_rpre[r] );
+        for ( int v = 1; v < MAX_SEG + 2; v++ ) r += rpre[v] <= k;
+        return rb[r] + ( k - rpre[r] );
     };
-    // the tile's staged stayers: ranks [0, ns)
-    for ( int k = tid; k < ns; k += blockDim.x ) {
-        const int dst = slot( k );
-        if ( dst >= 0 ) {
-            const int v = segment_of( k, s_gpre );
-            const int src = o + v * chunk + s_seg[4][v] + ( k - s_gpre[v] );
-            P.ix[dst] = Gix[src];
-            P.x[dst] = Gx[src];
-            st3( P.u + dst, ld3( Gu + src ) );
-        }
-    }
-    // the sources' leavers towards this tile: the entries of source e's
-    // staging runs whose cell left it in direction e, ranks from s_ebase[e]
-    // in staging order; the eight lists are independent, wave w walks the
-    // lists w, w + W, ... with wave ballots
-    for ( int li = w; li < 8; li += W ) {
-        const int e = li < 4 ? li : li + 1;
-        if ( s_cin[e] == 0 ) continue;
-        const int n_e = s_pre[e][MAX_SEG];
-        const int ch = s_seg[e][MAX_SEG], oe = s_so[e][1];
-        const int dx = e % 3 - 1, dy = e / 3 - 1;
-        int run = s_ebase[e];
-        for ( int j0 = 0; j0 < n_e; j0 += WAVE ) {
-            const int jj = j0 + lane;
-            int src = -1;
-            int2 c = make_int2( 0, 0 );
-            bool match = false;
-            if ( jj < n_e ) {
-                const int sv = segment_of( jj, s_pre[e] );
-                src = oe + sv * ch + ( jj - s_pre[e][sv] );
-                c = Gix[src];
-                match = ( ( c.x >= q.nx ) - ( c.x < 0 ) ) == dx && ( ( c.y >= q.ny ) - ( c.y < 0 ) ) == dy;
-            }
-            const unsigned long long m = __ballot( match );
-            if ( match ) {
-                const int dst = slot( run + count_below( m ) );
-                if ( dst >= 0 ) {
-                    P.ix[dst] = make_int2( c.x - dx * q.nx, c.y - dy * q.ny );
-                    P.x[dst] = Gx[src];
-                    st3( P.u + dst, ld3( Gu + src ) );
+    // ranks [0, ns): the tile's staged stayers; [ebase[e], ebase[e + 1]):
+    // source e's bucket towards the tile.  SORT_BATCH ranks per lane are
+    // resolved and loaded before any is written.
+    for ( int k0 = 0; k0 < ntot; k0 += SORT_BATCH * WAVE ) {
+        int src[SORT_BATCH], dst[SORT_BATCH], sh[SORT_BATCH];
+#pragma unroll
+        for ( int i = 0; i < SORT_BATCH; i++ ) {
+            const int k = k0 + i * WAVE + lane;
+            src[i] = -1;
+            dst[i] = -1;
+            sh[i] = 4;
+            if ( k < ntot ) {
+                dst[i] = slot( k );
+                if ( k < ns ) {
+                    const int v = segment_of( k, gpre );
+                    src[i] = o + v * chunk + seg[4 * 16 + v] + ( k - gpre[v] );
+                } else {
+                    int e = 0;
+#pragma unroll
+                    for ( int f = 1; f < 9; f++ ) e = ebase[f] <= k ? f : e;
+                    const int oe = so_o[e];
+                    const int pos = S.idx[oe + bst[e] + ( k - ebase[e] )];
+                    sh[i] = e;
+                    src[i] = ( pos >= 0 && pos < so_n[e] ) ? oe + pos : -1;
                 }
             }
-            run += __popcll( m );
+        }
+        int2 c[SORT_BATCH];
+        float2 x[SORT_BATCH];
+        float3 u[SORT_BATCH];
+#pragma unroll
+        for ( int i = 0; i < SORT_BATCH; i++ ) {
+            c[i] = make_int2( 0, 0 );
+            x[i] = make_float2( 0.f, 0.f );
+            u[i] = make_float3( 0.f, 0.f, 0.f );
+            if ( src[i] >= 0 && dst[i] >= 0 ) {
+                c[i] = Gix[src[i]];
+                x[i] = Gx[src[i]];
+                u[i] = ld3( Gu + src[i] );
+            }
+        }
+#pragma unroll
+        for ( int i = 0; i < SORT_BATCH; i++ ) {
+            if ( src[i] >= 0 && dst[i] >= 0 ) {
+                // a source's leavers move into this tile's frame (e = 4: none)
+                const int dx = sh[i] % 3 - 1, dy = sh[i] / 3 - 1;
+                P.ix[dst[i]] = make_int2( c[i].x - dx * q.nx, c[i].y - dy * q.ny );
+                P.x[dst[i]] = x[i];
+                st3( P.u + dst[i], u[i] );
+            }
         }
     }
-    __syncthreads();
-    if ( tid == 0 ) {
-        P.np[t] = s_hdr[1];
-        P.off[t] = s_hdr[0];
+    if ( lane == 0 ) {
+        P.np[t] = nn;
+        P.off[t] = no;
     }
 }
 
-__global__ void __launch_bounds__( SORT_BLOCK ) k_sort_in_fast( SortArgs a, const SortArgs * tab, const int2 * map ) {
-    const int t = select_instance( a, tab, map );
-    sort_in_fast_tile( a, t );
+__global__ void __launch_bounds__( SORT_WAVES * WAVE ) k_sort_in_fast( SortArgs a, const SortArgs * tab, const int2 * map, int n ) {
+    __shared__ int s_lds[SORT_WAVES][SORT_WAVE_LDS];
+    const int w = __builtin_amdgcn_readfirstlane( (int) threadIdx.x / WAVE );
+    const int g = blockIdx.x * SORT_WAVES + w;
+    if ( g >= n ) return;
+    const int t = select_item( a, tab, map, g );
+    sort_in_fast_wave( a, t, s_lds[w] );
 }
 
 // ---------------------------------------------------------------------------
@@ -1888,11 +1996,12 @@ int batch_particles_step( BatchParticlePlan * plan, void * stream ) {
                         (const SortArgs *) plan->d_sort );
     ZPIC_LAUNCH_CHECK( "k_sort_count (batch)" );
     if ( plan->staged ) {
-        hipLaunchKernelGGL( k_sort_shift, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
-                            (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
+        const dim3 grid( ( plan->nblocks + SORT_WAVES - 1 ) / SORT_WAVES );
+        hipLaunchKernelGGL( k_sort_shift, grid, dim3( SORT_WAVES * WAVE ), 0, st, sd, (const SortArgs *) plan->d_sort,
+                            (const int2 *) plan->d_map, plan->nblocks );
         ZPIC_LAUNCH_CHECK( "k_sort_shift (batch)" );
-        hipLaunchKernelGGL( k_sort_in_fast, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
-                            (const SortArgs *) plan->d_sort, (const int2 *) plan->d_map );
+        hipLaunchKernelGGL( k_sort_in_fast, grid, dim3( SORT_WAVES * WAVE ), 0, st, sd, (const SortArgs *) plan->d_sort,
+                            (const int2 *) plan->d_map, plan->nblocks );
         ZPIC_LAUNCH_CHECK( "k_sort_in_fast (batch)" );
     } else {
         hipLaunchKernelGGL( k_sort_out<false>, dim3( plan->nblocks ), dim3( SORT_BLOCK ), 0, st, sd,
@@ -2090,15 +2199,23 @@ __global__ void __launch_bounds__( BLOCK, 4 ) k_batch_step( StepTables T ) {
             batch_energy_body( uniform_load( T.energy + it.arg ) );
             break;
         case SK_SHIFT:
-        case SK_PLACE:
+        case SK_PLACE: {
+            // one tile per wave, in the wave's part of the LDS
+            const SortArgs a = uniform_load( T.sort + it.arg );
+            const int w = __builtin_amdgcn_readfirstlane( (int) threadIdx.x / WAVE );
+            int * L = reinterpret_cast<int *>( smem ) + w * SORT_WAVE_LDS;
+            for ( int t = it.t0 + w; t < it.t1; t += BLOCK / WAVE ) {
+                if ( it.kind == SK_SHIFT ) sort_shift_wave( a, t, L );
+                else sort_in_fast_wave( a, t, L );
+            }
+            break;
+        }
         case SK_OUT:
         case SK_IN: {
             const SortArgs a = uniform_load( T.sort + it.arg );
             for ( int t = it.t0; t < it.t1; t++ ) {
                 if ( t > it.t0 ) __syncthreads();
-                if ( it.kind == SK_SHIFT ) sort_shift_tile( a, t );
-                else if ( it.kind == SK_PLACE ) sort_in_fast_tile( a, t );
-                else if ( it.kind == SK_OUT ) sort_out_tile<false>( a, t );
+                if ( it.kind == SK_OUT ) sort_out_tile<false>( a, t );
                 else sort_in_tile( a, t );
             }
             break;
@@ -2222,6 +2339,7 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
         waves[i] = th / WAVE;
         block = std::max( block, th );
         lds = std::max( { lds, particles_lds_bytes( vol, waves[i], true, true, staged ), sizeof( float ) * 6 * (size_t) vol } );
+        if ( staged ) lds = std::max( lds, sizeof( int ) * SORT_WAVE_LDS * ( 512 / WAVE ) );  // order-free sort, a tile per wave
         for ( int k = 0; k < nslots; k++ ) {
             const zpic_batch_species & s = sp[k * n + i];
             const bool st = s.s.stage_ix && s.s.stage_x && s.s.stage_u;
@@ -2374,9 +2492,11 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
             case SK_IN:
                 dep = ( si - 1 ) * n + i;
                 if ( st.kind == SK_COUNT ) push( SK_COUNT, st.slot * n + i, 0, nt, dep );
-                else
-                    for ( int t = 0; t < nt; t += part_group[i] )
-                        push( st.kind, st.slot * n + i, t, std::min( nt, t + part_group[i] ), dep );
+                else {
+                    // the order-free sort takes a tile per wave: at least two per wave and item
+                    const int grp = ( st.kind == SK_SHIFT || st.kind == SK_PLACE ) ? std::max( part_group[i], 16 ) : part_group[i];
+                    for ( int t = 0; t < nt; t += grp ) push( st.kind, st.slot * n + i, t, std::min( nt, t + grp ), dep );
+                }
                 break;
             case SK_ADD_X:
             case SK_ADD_Y:
@@ -2705,9 +2825,10 @@ int zpic_tile_sort_copy( zpic_particles * p, zpic_particles * tmp, zpic_sort * s
     }
     if ( a.S.stage_ix ) {
         // order-free placement from the particle pass's staging
-        hipLaunchKernelGGL( k_sort_shift, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
+        const dim3 grid( ( nt + SORT_WAVES - 1 ) / SORT_WAVES );
+        hipLaunchKernelGGL( k_sort_shift, grid, dim3( SORT_WAVES * WAVE ), 0, st, a, tab, map, nt );
         ZPIC_LAUNCH_CHECK( "k_sort_shift" );
-        hipLaunchKernelGGL( k_sort_in_fast, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
+        hipLaunchKernelGGL( k_sort_in_fast, grid, dim3( SORT_WAVES * WAVE ), 0, st, a, tab, map, nt );
         ZPIC_LAUNCH_CHECK( "k_sort_in_fast" );
         return ZPIC_OK;
     }
