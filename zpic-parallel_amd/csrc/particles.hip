@@ -387,18 +387,12 @@ __device__ inline bool reflect_walls( int refl, int tx, int ty, int ntx, int nty
 constexpr int QCAP = WAVE;  // crossing-queue entries per wave
 constexpr int QWORDS = 6;   // cell, x0.x, x0.y, d.x, d.y, qvz
 constexpr int PREFETCH = 2;  // iterations of 64 particles loaded ahead
-// Order-free sort: leavers found by a wave are buffered in LDS and written
-// to the staging arrays LBCAP at a time (whole cache lines), LBWORDS words
-// each (cell x, y, position x, y, momentum x, y, z).
-constexpr int LBCAP = 32;
-constexpr int LBWORDS = 7;
 
 // LDS of the particle pass: E and B planes (PUSH), W private J triples, W
-// tag arrays and W crossing queues (MOVE), W leaver buffers (order-free sort).
-__host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move, bool stage = false ) {
+// tag arrays and W crossing queues (MOVE).
+__host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move ) {
     return sizeof( float ) * (size_t) vol * ( ( push ? 6 : 0 ) + ( move ? 3 * W : 0 ) ) +
-           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) + sizeof( float ) * (size_t) W * QCAP * QWORDS : 0 ) +
-           ( move && stage ? sizeof( float ) * (size_t) W * LBCAP * LBWORDS : 0 );
+           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) + sizeof( float ) * (size_t) W * QCAP * QWORDS : 0 );
 }
 
 // Arguments of one instance of the particle pass.
@@ -527,7 +521,6 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     float * const qbase = reinterpret_cast<float *>( reinterpret_cast<unsigned char *>( Jall + 3 * vol * W ) +
                                                      ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) );
     float * queue = qbase + (size_t) w * QCAP * QWORDS;
-    float * lbuf = qbase + (size_t) W * QCAP * QWORDS + (size_t) w * LBCAP * LBWORDS;  // order-free sort only
     __shared__ int s_cnt[9];
     __shared__ double s_en[FAST_BLOCK / WAVE];
     __shared__ int s_nl[FAST_BLOCK / WAVE];
@@ -560,8 +553,7 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     const gptr<int> sidx = gbl( S.idx + off );
     // Order-free sort (fast mode): each wave compacts its stayers to the front
     // of its chunk and writes its leavers, cell still in this tile's frame, to
-    // the staging arrays at the same chunk positions, through an LDS buffer
-    // of LBCAP records (whole cache lines); no leaver list.
+    // the staging arrays at the same chunk positions; no leaver list.
     const bool stage = MOVE && !DET && S.stage_ix != nullptr;
     const TileArray<int2> six{ (gptr<char>) gbl( S.stage_ix + off ) };
     const TileArray<float2> sx{ (gptr<char>) gbl( S.stage_x + off ) };
@@ -579,21 +571,6 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     int qn = 0;  // queued crossing paths of this wave
     volatile lds_f32 * q = (volatile lds_f32 *) queue;
     int nst = 0;  // order-free sort: stayers compacted so far
-    int lbn = 0;  // leavers in the LDS buffer
-    int lbw = 0;  // leavers written to the staging arrays
-    volatile lds_f32 * lb = (volatile lds_f32 *) lbuf;
-
-    // order-free sort: the first `count` buffered leavers to the staging
-    // arrays at chunk positions beg + lbw, ...
-    auto flush_leavers = [&]( int count ) {
-        if ( lane < count ) {
-            const unsigned k = (unsigned) ( beg + lbw + lane );
-            six.set( k, make_int2( __float_as_int( lb[lane] ), __float_as_int( lb[LBCAP + lane] ) ) );
-            sx.set( k, make_float2( lb[2 * LBCAP + lane], lb[3 * LBCAP + lane] ) );
-            su.set( k, make_float3( lb[4 * LBCAP + lane], lb[5 * LBCAP + lane], lb[6 * LBCAP + lane] ) );
-        }
-        lbw += count;
-    };
 
     // split + deposit of the first `count` queued paths, one per lane
     auto flush_queue = [&]( int count ) {
@@ -700,6 +677,35 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
             const int xc = ( c1.x >= g.nx ) - ( c1.x < 0 );
             const int yc = ( c1.y >= g.ny ) - ( c1.y < 0 );
             const int dir = act ? ( yc + 1 ) * 3 + ( xc + 1 ) : 4;
+            // the stores go out before the deposit: the iteration ends by
+            // waiting for the wave's outstanding memory operations (one
+            // counter tracks loads and stores), and the deposit hides them
+            const bool leave = act && dir != 4;
+            const unsigned long long mask = __ballot( leave );
+            if ( leave ) atomicAdd( &s_cnt[dir], 1 );
+            if ( !stage ) {
+                if ( leave ) sidx[beg + nl + count_below( mask )] = ( i << 4 ) | dir;
+            } else {
+                // leavers: to the staging arrays at chunk positions beg + nl, ...
+                if ( leave ) {
+                    const unsigned k = (unsigned) ( beg + nl + count_below( mask ) );
+                    six.set( k, c1 );
+                    sx.set( k, x1 );
+                    su.set( k, u );
+                }
+                // stayers: compacted to the front of the chunk (positions at
+                // or below the ones already read)
+                const bool stay = act && dir == 4;
+                const unsigned long long sm = __ballot( stay );
+                if ( stay ) {
+                    const unsigned r = (unsigned) ( beg + nst + count_below( sm ) );
+                    pix.set( r, c1 );
+                    px.set( r, x1 );
+                    pu.set( r, u );
+                }
+                nst += __popcll( sm );
+            }
+            nl += __popcll( mask );
             // paths inside one cell: their single segment now
             deposit_elected( act && !cross, s0, m.qnx, m.qny, j0x, j0y, j0z, tag0, ys, lane );
             // paths that cross: queued, split and deposited 64 at a time
@@ -726,49 +732,6 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
                 }
                 qn += nc - base;
             }
-            const bool leave = act && dir != 4;
-            const unsigned long long mask = __ballot( leave );
-            if ( leave ) atomicAdd( &s_cnt[dir], 1 );
-            if ( !stage ) {
-                if ( leave ) sidx[beg + nl + count_below( mask )] = ( i << 4 ) | dir;
-            } else {
-                // stayers: compacted to the front of the chunk (positions at
-                // or below the ones already read)
-                const bool stay = act && dir == 4;
-                const unsigned long long sm = __ballot( stay );
-                if ( stay ) {
-                    const unsigned r = (unsigned) ( beg + nst + count_below( sm ) );
-                    pix.set( r, c1 );
-                    px.set( r, x1 );
-                    pu.set( r, u );
-                }
-                nst += __popcll( sm );
-            }
-            if ( stage ) {
-                // leavers: to the LDS buffer, written out LBCAP at a time
-                if ( mask ) {
-                    const int nlm = __popcll( mask );
-                    const int pos = lbn + count_below( mask );
-                    int base = 0;
-                    for ( ;; ) {
-                        if ( leave && pos >= base && pos < base + LBCAP ) {
-                            const int k = pos - base;
-                            lb[k] = __int_as_float( c1.x );
-                            lb[LBCAP + k] = __int_as_float( c1.y );
-                            lb[2 * LBCAP + k] = x1.x;
-                            lb[3 * LBCAP + k] = x1.y;
-                            lb[4 * LBCAP + k] = u.x;
-                            lb[5 * LBCAP + k] = u.y;
-                            lb[6 * LBCAP + k] = u.z;
-                        }
-                        if ( lbn + nlm - base < LBCAP ) break;
-                        flush_leavers( LBCAP );
-                        base += LBCAP;
-                    }
-                    lbn += nlm - base;
-                }
-            }
-            nl += __popcll( mask );
         }
         if constexpr ( MOVE && DET ) {
             int dir = 4;
@@ -822,7 +785,6 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     }
     if constexpr ( MOVE && !DET ) {
         if ( qn > 0 ) flush_queue( qn );
-        if ( stage && lbn > 0 ) flush_leavers( lbn );
     }
 
     if constexpr ( PUSH ) {
@@ -1808,9 +1770,9 @@ constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup
 
 // Threads per tile the particle pass runs with for tiles of `vol` elements
 // when asked for `threads` (reduced until the LDS staging fits).
-int fitted_threads( int threads, int vol, bool stage = false ) {
-    while ( threads > WAVE && particles_lds_bytes( vol, threads / WAVE, true, true, stage ) > LDS_LIMIT ) threads -= WAVE;
-    return particles_lds_bytes( vol, threads / WAVE, true, true, stage ) > LDS_LIMIT ? 0 : threads;
+int fitted_threads( int threads, int vol ) {
+    while ( threads > WAVE && particles_lds_bytes( vol, threads / WAVE, true, true ) > LDS_LIMIT ) threads -= WAVE;
+    return particles_lds_bytes( vol, threads / WAVE, true, true ) > LDS_LIMIT ? 0 : threads;
 }
 
 int check_threads( int threads ) {
@@ -1827,11 +1789,10 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
         return set_error( ZPIC_EINVAL, "null buffer" );
     const int nt = fld->ntiles_x * fld->ntiles_y;
     // Large tiles: fewer wavefronts (each holds a private J copy) until LDS fits.
-    const bool stage = !det && s && s->stage_ix && s->stage_x && s->stage_u;
     int threads = det ? WAVE : g_threads;
-    while ( threads > WAVE && particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE, stage ) > LDS_LIMIT )
+    while ( threads > WAVE && particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE ) > LDS_LIMIT )
         threads -= WAVE;
-    const size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE, stage );
+    const size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE );
     if ( lds > LDS_LIMIT ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     PartArgs a{};
     a.P = view( p ); a.g = make_geo( fld );
@@ -1909,12 +1870,11 @@ int batch_particles_create( BatchParticlePlan ** out, const zpic_batch_species *
         // each entry runs with the threads that fit its own tiles, as its own
         // single-instance pass does
         if ( it.threads && check_threads( it.threads ) ) { delete plan; return ZPIC_EINVAL; }
-        const bool st = a.S.stage_ix != nullptr;
-        const int th = fitted_threads( it.threads ? it.threads : threads, it.fld.tile_vol, st );
+        const int th = fitted_threads( it.threads ? it.threads : threads, it.fld.tile_vol );
         if ( th == 0 ) { delete plan; return set_error( ZPIC_ERANGE, "tile too large for LDS staging" ); }
         a.waves = th / WAVE;
         plan->threads = std::max( plan->threads, th );
-        plan->lds = std::max( plan->lds, particles_lds_bytes( it.fld.tile_vol, a.waves, true, true, st ) );
+        plan->lds = std::max( plan->lds, particles_lds_bytes( it.fld.tile_vol, a.waves, true, true ) );
         plan->h_part.push_back( a );
         zpic_particles p = it.p, t = it.tmp;
         sorts[e] = sort_args( &p, &t, &s, it.fld.ntiles_x, it.fld.ntiles_y, it.fld.nx, it.fld.ny, it.periodic_x,
@@ -2334,11 +2294,11 @@ int batch_step_create( BatchStepPlan ** out, const zpic_batch_species * sp, int 
         const int vol = f.g.tile_vol;
         const int req = sp[i].threads ? sp[i].threads : threads;  // slot 0 of the instance
         if ( int rc = check_threads( req ) ) return rc;
-        const int th = fitted_threads( req, vol, staged );
+        const int th = fitted_threads( req, vol );
         if ( th == 0 ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
         waves[i] = th / WAVE;
         block = std::max( block, th );
-        lds = std::max( { lds, particles_lds_bytes( vol, waves[i], true, true, staged ), sizeof( float ) * 6 * (size_t) vol } );
+        lds = std::max( { lds, particles_lds_bytes( vol, waves[i], true, true ), sizeof( float ) * 6 * (size_t) vol } );
         if ( staged ) lds = std::max( lds, sizeof( int ) * SORT_WAVE_LDS * ( 512 / WAVE ) );  // order-free sort, a tile per wave
         for ( int k = 0; k < nslots; k++ ) {
             const zpic_batch_species & s = sp[k * n + i];
