@@ -180,8 +180,8 @@ def main():
     ap.add_argument("--batch-mode", choices=("persistent", "stage"), default="persistent",
                     help="batch64: one persistent launch per step (default) or one launch per stage")
     ap.add_argument("--untuned", action="store_true", help="batch64: 16x16 tiles instead of the autotuner's")
-    ap.add_argument("--sort-order", choices=("reference", "free"), default="reference",
-                    help="fast-mode tile sort: the reference's within-tile order (default) or order-free placement")
+    ap.add_argument("--sort-order", choices=("reference", "free"), default="free",
+                    help="fast-mode tile sort: order-free placement (default) or the reference's within-tile order")
     ap.add_argument("--threads", type=int, default=0,
                     help="threads per tile of the particle pass (multiple of 64 in [64, 512]; 0 = library default)")
     ap.add_argument("--bandwidth", action="store_true",

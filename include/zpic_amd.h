@@ -360,12 +360,13 @@ int zpic_div_corr_x(float *E, float *B, const zpic_geom *g, float dx, float dy, 
 int zpic_set_threads_per_tile(int threads);
 int zpic_get_threads_per_tile(void);
 
-/* Tile sort of the fast mode, library-wide: 1 (default) = the reference's
- * single-thread within-tile order (particles.cpp:379-614), as in the
- * deterministic mode, which always keeps it; 0 = order-free (the species
- * objects stage leavers in the particle pass, see zpic_sort.stage_ix, and
- * one placement pass fills each tile; counts, offsets and per-tile multisets
- * are the reference's, the order within a tile is not). */
+/* Tile sort of the fast mode, library-wide: 0 (default) = order-free (the
+ * species objects stage leavers in the particle pass, see
+ * zpic_sort.stage_ix, and one placement pass fills each tile; counts,
+ * offsets and per-tile multisets are the reference's, the order within a
+ * tile is not); 1 = the reference's single-thread within-tile order
+ * (particles.cpp:379-614), as in the deterministic mode, which always keeps
+ * it. */
 int zpic_set_sort_order(int reference_order);
 int zpic_get_sort_order(void);
 /* Threads per tile the fast particle pass actually uses for tiles of

@@ -196,13 +196,14 @@ def test_persistent_batch_graph_replays(zp):
 
 
 @pytest.mark.parametrize("persistent", [False, True])
-def test_batch_order_free_sort_matches_single_instances(zp, persistent):
-    """With the order-free tile sort (zpic_set_sort_order(0)) the batch runs
-    the staged sort for every instance and still equals each instance's own
-    run with that sort."""
+def test_batch_reference_order_sort_matches_single_instances(zp, persistent):
+    """With the reference's within-tile order (zpic_set_sort_order(1)) the
+    batch runs that sort (k_sort_out, k_sort_in) for every instance and still
+    equals each instance's own run with it; the other batch tests run the
+    default order-free sort."""
     steps = 10
     old = zp.get_sort_order()
-    zp.set_sort_order(False)
+    zp.set_sort_order(True)
     try:
         ref = singles(zp, 256, steps)
         sims = [make(zp, sh) for sh in SHAPES]

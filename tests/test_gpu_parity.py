@@ -41,7 +41,7 @@ def zp():
 @pytest.fixture(params=["reference", "free"])
 def sort_order(request, zp):
     """Runs a fast-mode test with each tile-sort order (the reference's
-    within-tile order, the default, and the order-free placement), restoring
+    within-tile order and the order-free placement, the default), restoring
     the library setting afterwards."""
     old = zp.get_sort_order()
     zp.set_sort_order(request.param == "reference")
@@ -53,6 +53,14 @@ def sort_order(request, zp):
 def order_free(zp):
     old = zp.get_sort_order()
     zp.set_sort_order(False)
+    yield
+    zp.set_sort_order(old)
+
+
+@pytest.fixture
+def reference_order(zp):
+    old = zp.get_sort_order()
+    zp.set_sort_order(True)
     yield
     zp.set_sort_order(old)
 
@@ -386,7 +394,12 @@ def test_push_stage_bit_exact(zp):
 
 
 @pytest.mark.parametrize("det", [False, True])
-def test_move_deposit_stage(zp, det):
+def test_move_deposit_stage(zp, det, reference_order):
+    """The move + deposit stage alone, particles compared in buffer order:
+    with the reference's within-tile order the fast pass moves every
+    particle in place, as the reference does (the order-free sort instead
+    compacts stayers and stages leavers in this pass; the full-step tests
+    cover it)."""
     cfg = cfg_of((2, 3), (8, 16), (2, 3), uth=(0.4, 0.4, 0.4), ufl=(0.5, -0.4, 0.3))
     say(f"move+deposit stage, deterministic={det}")
     sim = gpu_weibel(zp, cfg, det=det)

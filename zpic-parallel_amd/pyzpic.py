@@ -709,8 +709,8 @@ def get_threads_per_tile():
 
 
 def set_sort_order(reference):
-    """Fast-mode tile sort: True (default) the reference's within-tile order
-    (the deterministic mode always keeps it), False order-free."""
+    """Fast-mode tile sort: True the reference's within-tile order (the
+    deterministic mode always keeps it), False (default) order-free."""
     _chk(lib().zpic_set_sort_order(1 if reference else 0))
 
 
