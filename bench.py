@@ -253,10 +253,11 @@ def main():
     launch_ms = tm["push_move"] / nsp
     bytes_per_launch = 56.0 * (npart / nsp) + 36.0 * ncells
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    # counter-measured HBM bytes of one launch, per workload and sort order
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
-            traffic = json.load(open(TRAFFIC_FILE)).get(args.workload)
+            traffic = json.load(open(TRAFFIC_FILE)).get(args.workload, {}).get(args.sort_order)
         except Exception:
             traffic = None
     # whole-step algorithmic bytes: 56 B per particle + 56 B per tile-crossing

@@ -1411,6 +1411,27 @@ __device__ inline void sort_shift_wave( const SortArgs & a, const int t, int * L
     }
     wave_sync();
 
+    // the leavers' cells, SORT_BATCH per lane at a time; the first batch is
+    // loaded before the stayer copies below (which write other staging
+    // slots), so that the two round trips overlap
+    const int nl = pre[MAX_SEG];
+    int pos[SORT_BATCH];
+    int2 cl[SORT_BATCH];
+    auto load_batch = [&]( int j0 ) {
+#pragma unroll
+        for ( int i = 0; i < SORT_BATCH; i++ ) {
+            const int j = j0 + i * WAVE + lane;
+            pos[i] = -1;
+            cl[i] = make_int2( 0, 0 );
+            if ( j < nl ) {
+                const int v = segment_of( j, pre );
+                pos[i] = v * chunk + ( j - pre[v] );
+                cl[i] = Gix[o + pos[i]];
+            }
+        }
+    };
+    load_batch( 0 );
+
     // stayers at old-range positions below and above the new range: to the
     // free staging slots in order
     const int a0 = o, la = max( min( o + np, no ) - o, 0 );
@@ -1443,23 +1464,14 @@ __device__ inline void sort_shift_wave( const SortArgs & a, const int t, int * L
     if ( lane == 0 ) S.tmp_np[t] = ns;
 
     // direction buckets of the leavers, staging order kept in each
-    const int nl = pre[MAX_SEG];
     int run[9];
 #pragma unroll
     for ( int d = 0; d < 9; d++ ) run[d] = __builtin_amdgcn_readlane( dstart, 20 + d );
     for ( int j0 = 0; j0 < nl; j0 += SORT_BATCH * WAVE ) {
-        int pos[SORT_BATCH], dir[SORT_BATCH];
+        if ( j0 > 0 ) load_batch( j0 );
+        int dir[SORT_BATCH];
 #pragma unroll
-        for ( int i = 0; i < SORT_BATCH; i++ ) {
-            const int j = j0 + i * WAVE + lane;
-            pos[i] = 0;
-            dir[i] = 4;
-            if ( j < nl ) {
-                const int v = segment_of( j, pre );
-                pos[i] = v * chunk + ( j - pre[v] );
-                dir[i] = cell_dir( Gix[o + pos[i]], a.q );
-            }
-        }
+        for ( int i = 0; i < SORT_BATCH; i++ ) dir[i] = pos[i] >= 0 ? cell_dir( cl[i], a.q ) : 4;
 #pragma unroll
         for ( int i = 0; i < SORT_BATCH; i++ ) {
             int at = -1;
