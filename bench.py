@@ -419,16 +419,20 @@ def run_lwfa(args, pyzpic):
 def run_batch(args, pyzpic):
     """The batched parameter scan: 64 Weibel instances (pyzpic.batch64_instances,
     seed 2024), tile shape and threads per tile of each (grid, ppc) from the
-    autotuner (timed on a 256x256 sample, 3 steps), advanced by one persistent
+    autotuner (every shape timed at full occupancy, about four tiles per CU
+    within 1024x1024 cells, 3 steps), advanced by one persistent
     launch per hipGraph-captured step (--batch-mode stage: one launch per
     stage instead)."""
     wl = WORKLOADS["batch64"]
     inst = pyzpic.batch64_instances(wl["instances"], wl["seed"])
     t_tune = time.perf_counter()
     if not args.untuned:
-        pyzpic.tune_instances(inst, steps=3, max_cells=256 * 256)
+        pyzpic.tune_instances(inst, steps=3, max_cells=1024 * 1024)
     t_tune = time.perf_counter() - t_tune
     print(f"[bench] building batch64: {len(inst)} instances (autotune {t_tune:.1f} s)", file=sys.stderr, flush=True)
+    if not args.untuned:
+        picks = sorted({(c["ppc"][0], tuple(c["nx"]), c["threads"]) for c in inst})
+        print(f"[bench] autotuner picks (ppc, tile, threads): {picks}", file=sys.stderr, flush=True)
     t_init = time.perf_counter()
     sims = [pyzpic.scan_simulation(c) for c in inst]
     batch = pyzpic.Batch(threads_per_tile=pyzpic.get_threads_per_tile(), graph=True,

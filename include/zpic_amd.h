@@ -374,12 +374,13 @@ int zpic_get_sort_order(void);
  * LDS staging of a large tile would not fit); 0 if nothing fits. */
 int zpic_fitted_threads_per_tile(int threads, int tile_vol);
 
-/* Autotuner (no reference counterpart): times short Weibel-like runs of a
- * grid_x x grid_y grid at ppc for every tile shape that divides the grid
- * ({8,16,32} x {8,16,32} cells) and every threads-per-tile in {64, 128, 256,
- * 512} that fits, on a sample of at most max_cells cells (0 = 512*512), and
- * returns the fastest (particles per ns over push, deposit, sort and field
- * stages).  Results are cached in the text file cache_path (NULL = none)
+/* Autotuner (no reference counterpart): times short Weibel-like runs at ppc
+ * for every tile shape that divides the grid_x x grid_y grid ({8,16,32} x
+ * {8,16,32} cells) and every threads-per-tile in {64, 128, 256, 512} that
+ * fits, each on a sample of about four tiles per compute unit (the shapes
+ * are compared at full occupancy, whatever the grid's own size), at most
+ * max_cells cells (0 = 1024*1024), and returns the fastest (particles per ns
+ * over push, deposit, sort and field stages).  Results are cached in the text file cache_path (NULL = none)
  * keyed by device name, grid and ppc; from_cache reports a hit.  Leaves the
  * library's threads per tile unchanged. */
 typedef struct {

@@ -60,7 +60,8 @@ def same(a, b):
 def test_batch64_persistent_matches_single_instances(zp, tmp_path):
     t0 = time.perf_counter()
     inst = zp.batch64_instances(64, 2024)
-    zp.tune_instances(inst, cache=str(tmp_path / "tune.txt"))
+    # a small sample (the test is about bit-exactness, not the choice)
+    zp.tune_instances(inst, max_cells=128 * 128, cache=str(tmp_path / "tune.txt"))
     shapes = sorted({(c["grid"][0], c["ppc"][0], c["nx"], c["threads"]) for c in inst})
     say(f"autotuned {len(shapes)} (grid, ppc) pairs in {time.perf_counter() - t0:.1f} s: {shapes}")
     old = zp.get_threads_per_tile()

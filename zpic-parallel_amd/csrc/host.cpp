@@ -1585,22 +1585,32 @@ extern "C" int zpic_autotune( unsigned grid_x, unsigned grid_y, unsigned ppc_x, 
                               long long max_cells, const char * cache_path, zpic_tune_result * out ) {
     if ( !out || grid_x < 8 || grid_y < 8 || ppc_x < 1 || ppc_y < 1 || steps < 1 )
         return zpic::set_error( ZPIC_EINVAL, "invalid autotune request" );
-    if ( max_cells <= 0 ) max_cells = 512 * 512;
+    if ( max_cells <= 0 ) max_cells = 1024 * 1024;
     const int saved = zpic_get_threads_per_tile();
     try {
         const uint2 grid = make_uint2( grid_x, grid_y ), ppc = make_uint2( ppc_x, ppc_y );
         const std::string key = tune_key( grid, ppc );
         if ( cache_path && tune_lookup( cache_path, key, *out ) ) return ZPIC_OK;
+        int dev = 0, cus = 0;
+        hip_check( hipGetDevice( &dev ), "hipGetDevice" );
+        hip_check( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ), "hipDeviceGetAttribute" );
+        // A tile shape is judged at full occupancy, as it runs in a batch of
+        // many instances or in a large grid: the sample has about four tiles
+        // per compute unit whatever the instance's own grid (which only has
+        // to be divisible by the shape), within max_cells cells.  (A sample
+        // cut from a small grid leaves most CUs idle and favours small tiles,
+        // which are the slower ones when the GPU is full.)
+        const long long want = 4LL * std::max( cus, 1 );
         zpic_tune_result best{};
         const unsigned sizes[] = { 8, 16, 32 };
         for ( unsigned sy : sizes )
             for ( unsigned sx : sizes ) {
                 if ( grid_x % sx || grid_y % sy ) continue;
-                // sample: whole tiles, at most max_cells cells, same tile shape
-                uint2 nt = make_uint2( grid_x / sx, grid_y / sy );
-                while ( (long long) nt.x * sx * nt.y * sy > max_cells && ( nt.x > 2 || nt.y > 2 ) ) {
-                    if ( nt.x >= nt.y && nt.x > 2 ) nt.x = ( nt.x + 1 ) / 2;
-                    else nt.y = ( nt.y + 1 ) / 2;
+                uint2 nt = make_uint2( 2, 2 );
+                while ( (long long) nt.x * nt.y < want ) {
+                    const uint2 next = nt.x <= nt.y ? make_uint2( 2 * nt.x, nt.y ) : make_uint2( nt.x, 2 * nt.y );
+                    if ( (long long) next.x * sx * next.y * sy > max_cells ) break;
+                    nt = next;
                 }
                 const int vol = ( ( sx + 3 ) * ( sy + 3 ) + 3 ) & ~3;
                 for ( int threads : { 64, 128, 256, 512 } ) {

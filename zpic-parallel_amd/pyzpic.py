@@ -746,13 +746,16 @@ def batch64_instances(n=64, seed=2024):
     return out
 
 
-def tune_instances(instances, steps=2, max_cells=128 * 128, cache=None):
-    """Autotuned tile shape and threads per tile (zpic_autotune) of every
-    distinct (grid, ppc) of a parameter scan; adds "nx", "ntiles" and
-    "threads" to each instance (in place) and returns them."""
+def tune_instances(instances, steps=2, max_cells=1024 * 1024, cache=None):
+    """Autotuned tile shape and threads per tile (zpic_autotune) for a
+    parameter scan; adds "nx", "ntiles" and "threads" to each instance (in
+    place) and returns them.  The autotuner's samples do not depend on the
+    grid beyond which tile shapes divide it, so instances are tuned once per
+    ppc and set of dividing shapes."""
     tuned = {}
     for c in instances:
-        key = (c["grid"], c["ppc"])
+        fits = tuple(tuple(s for s in (8, 16, 32) if n % s == 0) for n in c["grid"])
+        key = (c["ppc"], fits)
         if key not in tuned:
             tuned[key] = autotune(c["grid"], c["ppc"], steps=steps, max_cells=max_cells, cache=cache)
         r = tuned[key]
