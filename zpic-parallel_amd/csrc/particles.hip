@@ -387,12 +387,20 @@ __device__ inline bool reflect_walls( int refl, int tx, int ty, int ntx, int nty
 constexpr int QCAP = WAVE;  // crossing-queue entries per wave
 constexpr int QWORDS = 6;   // cell, x0.x, x0.y, d.x, d.y, qvz
 constexpr int PREFETCH = 2;  // iterations of 64 particles loaded ahead
+// Order-free sort: where the LDS allows it without costing a workgroup per
+// CU, a wave buffers the leavers it finds and writes them to the staging
+// arrays LBCAP at a time, LBWORDS words each (cell x, y, position x, y,
+// momentum x, y, z): whole cache lines instead of a few records per
+// iteration, whose lines leave the L2 partly written.
+constexpr int LBCAP = 32;
+constexpr int LBWORDS = 7;
 
 // LDS of the particle pass: E and B planes (PUSH), W private J triples, W
-// tag arrays and W crossing queues (MOVE).
-__host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move ) {
+// tag arrays and W crossing queues (MOVE), W leaver buffers (lbuf).
+__host__ __device__ inline size_t particles_lds_bytes( int vol, int W, bool push, bool move, bool lbuf = false ) {
     return sizeof( float ) * (size_t) vol * ( ( push ? 6 : 0 ) + ( move ? 3 * W : 0 ) ) +
-           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) + sizeof( float ) * (size_t) W * QCAP * QWORDS : 0 );
+           ( move ? ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) + sizeof( float ) * (size_t) W * QCAP * QWORDS : 0 ) +
+           ( move && lbuf ? sizeof( float ) * (size_t) W * LBCAP * LBWORDS : 0 );
 }
 
 // Arguments of one instance of the particle pass.
@@ -407,6 +415,7 @@ struct PartArgs {
     double * tile_energy;
     SortView S;
     int waves;  // waves of the workgroup that take particles (0: all)
+    int lbuf;   // order-free sort: leavers buffered in LDS (LDS sized for it)
 };
 
 struct PartG {
@@ -521,6 +530,7 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     float * const qbase = reinterpret_cast<float *>( reinterpret_cast<unsigned char *>( Jall + 3 * vol * W ) +
                                                      ( ( (size_t) W * vol + 3 ) & ~(size_t) 3 ) );
     float * queue = qbase + (size_t) w * QCAP * QWORDS;
+    float * lbuf = qbase + (size_t) W * QCAP * QWORDS + (size_t) w * LBCAP * LBWORDS;  // if a.lbuf
     __shared__ int s_cnt[9];
     __shared__ double s_en[FAST_BLOCK / WAVE];
     __shared__ int s_nl[FAST_BLOCK / WAVE];
@@ -571,6 +581,22 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     int qn = 0;  // queued crossing paths of this wave
     volatile lds_f32 * q = (volatile lds_f32 *) queue;
     int nst = 0;  // order-free sort: stayers compacted so far
+    const bool lbon = stage && a.lbuf;
+    int lbn = 0;  // leavers in the LDS buffer
+    int lbw = 0;  // buffered leavers written to the staging arrays
+    volatile lds_f32 * lb = (volatile lds_f32 *) lbuf;
+
+    // the first `count` buffered leavers to the staging arrays at chunk
+    // positions beg + lbw, ...
+    auto flush_leavers = [&]( int count ) {
+        if ( lane < count ) {
+            const unsigned k = (unsigned) ( beg + lbw + lane );
+            six.set( k, make_int2( __float_as_int( lb[lane] ), __float_as_int( lb[LBCAP + lane] ) ) );
+            sx.set( k, make_float2( lb[2 * LBCAP + lane], lb[3 * LBCAP + lane] ) );
+            su.set( k, make_float3( lb[4 * LBCAP + lane], lb[5 * LBCAP + lane], lb[6 * LBCAP + lane] ) );
+        }
+        lbw += count;
+    };
 
     // split + deposit of the first `count` queued paths, one per lane
     auto flush_queue = [&]( int count ) {
@@ -687,11 +713,34 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
                 if ( leave ) sidx[beg + nl + count_below( mask )] = ( i << 4 ) | dir;
             } else {
                 // leavers: to the staging arrays at chunk positions beg + nl, ...
-                if ( leave ) {
-                    const unsigned k = (unsigned) ( beg + nl + count_below( mask ) );
-                    six.set( k, c1 );
-                    sx.set( k, x1 );
-                    su.set( k, u );
+                if ( !lbon ) {
+                    if ( leave ) {
+                        const unsigned k = (unsigned) ( beg + nl + count_below( mask ) );
+                        six.set( k, c1 );
+                        sx.set( k, x1 );
+                        su.set( k, u );
+                    }
+                } else if ( mask ) {
+                    // through the LDS buffer, flushed whenever full
+                    const int nlm = __popcll( mask );
+                    const int pos = lbn + count_below( mask );
+                    int base = 0;
+                    for ( ;; ) {
+                        if ( leave && pos >= base && pos < base + LBCAP ) {
+                            const int k = pos - base;
+                            lb[k] = __int_as_float( c1.x );
+                            lb[LBCAP + k] = __int_as_float( c1.y );
+                            lb[2 * LBCAP + k] = x1.x;
+                            lb[3 * LBCAP + k] = x1.y;
+                            lb[4 * LBCAP + k] = u.x;
+                            lb[5 * LBCAP + k] = u.y;
+                            lb[6 * LBCAP + k] = u.z;
+                        }
+                        if ( lbn + nlm - base < LBCAP ) break;
+                        flush_leavers( LBCAP );
+                        base += LBCAP;
+                    }
+                    lbn += nlm - base;
                 }
                 // stayers: compacted to the front of the chunk (positions at
                 // or below the ones already read)
@@ -785,6 +834,7 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
     }
     if constexpr ( MOVE && !DET ) {
         if ( qn > 0 ) flush_queue( qn );
+        if ( lbon && lbn > 0 ) flush_leavers( lbn );
     }
 
     if constexpr ( PUSH ) {
@@ -1805,12 +1855,24 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
     int threads = det ? WAVE : g_threads;
     while ( threads > WAVE && particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE ) > LDS_LIMIT )
         threads -= WAVE;
-    const size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE );
+    size_t lds = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE );
     if ( lds > LDS_LIMIT ) return set_error( ZPIC_ERANGE, "tile too large for LDS staging" );
     PartArgs a{};
     a.P = view( p ); a.g = make_geo( fld );
     a.E = (const float3 *) E; a.B = (const float3 *) B; a.J = (float3 *) J;
     a.alpha = alpha; a.m = m; a.tile_energy = tile_energy; a.S = view( s );
+    if ( MOVE && !det && s->stage_ix ) {
+        // the leaver buffers, where they cost no workgroup per CU
+        const size_t with = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE, true );
+        int dev = 0, cu_lds = 0;
+        ZPIC_HIP_TRY( hipGetDevice( &dev ) );
+        ZPIC_HIP_TRY( hipDeviceGetAttribute( &cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev ) );
+        const size_t stat = 256;  // the kernel's static LDS, rounded up
+        if ( with <= LDS_LIMIT && cu_lds / ( with + stat ) >= cu_lds / ( lds + stat ) ) {
+            a.lbuf = 1;
+            lds = with;
+        }
+    }
     hipStream_t st = as_stream( stream );
     if ( det )
         hipLaunchKernelGGL( ( k_particles<PUSH, MOVE, true> ), dim3( nt ), dim3( threads ), lds, st, a,
