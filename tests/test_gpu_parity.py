@@ -274,9 +274,11 @@ def test_fast_reference_order_sort(zp):
 
 def test_fast_sort_more_leavers_than_slot_rounds(zp, order_free):
     """Order-free sort with more than 2048 leavers per tile (4x4-cell tiles,
-    ppc 32x32, relativistic thermal spread): the placement runs in several
-    rounds of hole slots.  Per-tile particle sets equal the oracle's after one
-    step, and after a second step from the GPU's own state."""
+    ppc 32x32, relativistic thermal spread): the particle pass flushes its
+    leaver buffers many times, and the direction bucketing and the placement
+    each take several passes of 256 records per wave.  Per-tile particle sets
+    equal the oracle's after one step, and after a second step from the GPU's
+    own state."""
     cfg = cfg_of((3, 3), (4, 4), (32, 32), uth=(2.0, 2.0, 2.0), ufl=(0.3, -0.2, 0.0))
     say("order-free sort, > 2048 leavers per tile")
     sim = gpu_weibel(zp, cfg)
