@@ -157,7 +157,9 @@ int zpic_tile_sort(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, int nti
  *    (copy_sorted, :628-763), the path the reference takes when the total
  *    exceeds p->max_part; the caller then swaps the two buffers
  *    (swap_buffers, particles.h:195-206).  In mode 1 tmp must hold at least
- *    s->total[0] particles. */
+ *    s->total[0] particles, and s->stage_ix must be NULL (mode 1 reads the
+ *    particles in place; a particle pass that staged leavers is answered
+ *    with ZPIC_EINVAL). */
 int zpic_tile_sort_count(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, int ntiles_x, int ntiles_y,
                          int nx, int ny, int periodic_x, int periodic_y, const int *extra, void *stream);
 int zpic_tile_sort_copy(zpic_particles *p, zpic_particles *tmp, zpic_sort *s, int ntiles_x, int ntiles_y,

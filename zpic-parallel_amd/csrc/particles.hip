@@ -2854,6 +2854,9 @@ int zpic_tile_sort_copy( zpic_particles * p, zpic_particles * tmp, zpic_sort * s
     const int2 * map = nullptr;
     hipStream_t st = as_stream( stream );
     if ( mode == 1 ) {
+        // copy_sorted reads every particle in place: not after a particle
+        // pass that staged leavers
+        if ( a.S.stage_ix ) return set_error( ZPIC_EINVAL, "copy_sorted after a staging particle pass" );
         hipLaunchKernelGGL( k_sort_out<true>, dim3( nt ), dim3( SORT_BLOCK ), 0, st, a, tab, map );
         ZPIC_LAUNCH_CHECK( "k_sort_out (copy_sorted)" );
         return ZPIC_OK;
