@@ -371,13 +371,16 @@ __device__ inline bool reflect_walls( int refl, int tx, int ty, int ntx, int nty
 // serialised deposit and energy sum (reference order).
 //
 // Wave w of W handles the contiguous chunk [w*chunk, (w+1)*chunk) of the
-// tile's particles, so that the leaver indices it appends to the tile's list
-// (idx[offset + w*chunk ...]) are in increasing order; seg[16 t + w] keeps the
-// count of each wave's segment, seg[16 t + 15] the chunk.
+// tile's particles.  With the reference order the leaver indices it appends
+// to the tile's list (idx[offset + w*chunk ...]) are in increasing order;
+// with the order-free sort it compacts its stayers to the chunk's front and
+// stages its leavers at the chunk's first positions.  seg[16 t + w] keeps
+// the count of each wave's leavers, seg[16 t + 15] the chunk.
 //
 // LDS: E and B planes (6*vol floats, PUSH), then W private J plane triples
 // (3*vol floats each, MOVE), then W tag byte arrays (vol bytes each, MOVE),
-// then W crossing queues (QCAP entries of QWORDS words, MOVE).
+// then W crossing queues (QCAP entries of QWORDS words, MOVE), then W
+// leaver buffers (LBCAP records of LBWORDS words, order-free sort, lbuf).
 //
 // Fast mode: a particle that stays in its cell deposits its single segment
 // in the iteration that moves it; one that crosses a cell edge (about one in
