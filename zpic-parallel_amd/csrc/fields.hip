@@ -142,19 +142,31 @@ __global__ void __launch_bounds__( SUM_BLOCK ) k_sum_tiles( const double * __res
     }
 }
 
-// Unordered parallel sum (per-thread runs, then a tree): for the fast mode.
+// Unordered parallel sum for the fast mode: per-thread runs (four partial
+// sums, so that four loads are in flight), a butterfly within each wave,
+// then the wave sums in wave order; one workgroup barrier.
 __global__ void __launch_bounds__( SUM_BLOCK ) k_sum_tiles_fast( const double * __restrict__ v, int n,
                                                                  double * __restrict__ out ) {
-    __shared__ double buf[SUM_BLOCK];
-    double acc = 0;
-    for ( int t = threadIdx.x; t < n; t += SUM_BLOCK ) acc += v[t];
-    buf[threadIdx.x] = acc;
-    __syncthreads();
-    for ( int o = SUM_BLOCK / 2; o > 0; o >>= 1 ) {
-        if ( (int) threadIdx.x < o ) buf[threadIdx.x] += buf[threadIdx.x + o];
-        __syncthreads();
+    constexpr int W64 = 64;
+    __shared__ double wsum[SUM_BLOCK / W64];
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int t = threadIdx.x;
+    for ( ; t + 3 * SUM_BLOCK < n; t += 4 * SUM_BLOCK ) {
+        a0 += v[t];
+        a1 += v[t + SUM_BLOCK];
+        a2 += v[t + 2 * SUM_BLOCK];
+        a3 += v[t + 3 * SUM_BLOCK];
     }
-    if ( threadIdx.x == 0 ) out[0] = buf[0];
+    for ( ; t < n; t += SUM_BLOCK ) a0 += v[t];
+    double acc = ( a0 + a1 ) + ( a2 + a3 );
+    for ( int o = W64 / 2; o > 0; o >>= 1 ) acc += __shfl_xor( acc, o );
+    if ( threadIdx.x % W64 == 0 ) wsum[threadIdx.x / W64] = acc;
+    __syncthreads();
+    if ( threadIdx.x == 0 ) {
+        double s = 0;
+        for ( int w = 0; w < SUM_BLOCK / W64; w++ ) s += wsum[w];
+        out[0] = s;
+    }
 }
 
 __global__ void k_gather( const float * __restrict__ buf, G g, int ncomp, int fc, float * __restrict__ out ) {
