@@ -135,6 +135,16 @@ int zpic_push_move_deposit_shift(const zpic_particles *p, const zpic_geom *fld, 
                                  float *J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
                                  double *tile_energy, zpic_sort *s, int reflect, int shift_x, int shift_y,
                                  int deterministic, void *stream);
+/* No reference counterpart: zpic_push_move_deposit_shift in the fast mode
+ * with each tile's region of J (guard cells included) written instead of
+ * added to, for the first species of a step in place of zeroing J first
+ * (Simulation::advance: current.zero(), then the species' advance); the
+ * result is the same, and a J fill and a J read are saved.  The tile
+ * padding of J is left as it is. */
+int zpic_push_move_deposit_overwrite(const zpic_particles *p, const zpic_geom *fld, const float *E, const float *B,
+                                     float *J, float alpha, float dt_dx, float dt_dy, float q, float qnx, float qny,
+                                     double *tile_energy, zpic_sort *s, int reflect, int shift_x, int shift_y,
+                                     void *stream);
 
 /* Particles::tile_sort (particles.cpp:855-904: bnd_check, update_tile_info,
  * copy_out, copy_in) as per-tile counting + prefix scan.  Uses the

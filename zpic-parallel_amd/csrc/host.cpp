@@ -814,8 +814,8 @@ int Species::set_moving_window() {
 // move run as one pass over the particles.
 void Species::advance( EMF const & emf, Current & current ) { step( emf, current, nullptr, false ); }
 
-void Species::advance_timed( EMF const & emf, Current & current, hipEvent_t after_push, bool window ) {
-    step( emf, current, after_push, window );
+void Species::advance_timed( EMF const & emf, Current & current, hipEvent_t after_push, bool window, bool set_j ) {
+    step( emf, current, after_push, window, set_j );
 }
 
 void Species::advance_mov_window( EMF const & emf, Current & current ) { step( emf, current, nullptr, true ); }
@@ -834,16 +834,21 @@ void Species::set_staging( zpic_sort & s, bool window ) const {
     s.stage_u = staged ? reinterpret_cast<float *>( tmp->u ) : nullptr;
 }
 
-void Species::particle_pass( EMF const & emf, Current & current, bool shift ) {
+void Species::particle_pass( EMF const & emf, Current & current, bool shift, bool set_j ) {
     const float alpha = 0.5 * dt / m_q;
     const float2 dt_dx = make_float2( dt / dx.x, dt / dx.y );
     const float2 qnx = make_float2( q * dx.x / dt, q * dx.y / dt );
     zpic_geom g = emf.E->geom();
     zpic_particles p = particles->view();
     set_staging( sort->buf, shift );
-    check( zpic_push_move_deposit_shift( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha, dt_dx.x,
-                                         dt_dx.y, q, qnx.x, qnx.y, d_tile_energy, &sort->buf, reflect_mask(),
-                                         shift ? -1 : 0, 0, deterministic, stream() ) );
+    if ( set_j && !deterministic )
+        check( zpic_push_move_deposit_overwrite( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha,
+                                                 dt_dx.x, dt_dx.y, q, qnx.x, qnx.y, d_tile_energy, &sort->buf,
+                                                 reflect_mask(), shift ? -1 : 0, 0, stream() ) );
+    else
+        check( zpic_push_move_deposit_shift( &p, &g, emf.E->fbuf(), emf.B->fbuf(), current.J->fbuf(), alpha,
+                                             dt_dx.x, dt_dx.y, q, qnx.x, qnx.y, d_tile_energy, &sort->buf,
+                                             reflect_mask(), shift ? -1 : 0, 0, deterministic, stream() ) );
 }
 
 void Species::finish_step( bool shift ) {
@@ -866,9 +871,9 @@ void Species::finish_step( bool shift ) {
 // species.cpp:2088: the window time is (iter + 1) * dt in float
 bool Species::shifts( bool window ) const { return window && moving_window.needs_move( ( iter + 1 ) * dt ); }
 
-void Species::step( EMF const & emf, Current & current, hipEvent_t after_push, bool window ) {
+void Species::step( EMF const & emf, Current & current, hipEvent_t after_push, bool window, bool set_j ) {
     const bool shift = shifts( window );
-    particle_pass( emf, current, shift );
+    particle_pass( emf, current, shift, set_j );
     if ( after_push ) hip_check( hipEventRecord( after_push, stream() ), "hipEventRecord" );
     finish_step( shift );
 }
@@ -1034,16 +1039,19 @@ void Simulation::step( bool window ) {
     auto rec = [&]( size_t i, hipStream_t st ) {
         if ( timing ) hip_check( hipEventRecord( E[i], st ), "hipEventRecord" );
     };
-    current.zero();
+    // fast mode: the first species' particle pass writes J instead of a
+    // fill zeroing it first
+    const bool set_j = nsp > 0 && !species[0]->is_deterministic();
+    if ( !set_j ) current.zero();
     rec( 0, stream() );
     for ( size_t k = 0; k + ( overlap ? 1 : 0 ) < nsp; k++ ) {
-        species[k]->advance_timed( emf, current, timing ? E[1 + 2 * k] : nullptr, window );
+        species[k]->advance_timed( emf, current, timing ? E[1 + 2 * k] : nullptr, window, set_j && k == 0 );
         rec( 2 + 2 * k, stream() );
     }
     if ( overlap ) {
         Species & sp = *species[nsp - 1];
         const bool shift = sp.shifts( window );
-        sp.particle_pass( emf, current, shift );
+        sp.particle_pass( emf, current, shift, set_j && nsp == 1 );
         rec( 2 * nsp - 1, stream() );
         hipStream_t side = side_stream();
         hip_check( hipEventRecord( dep[0], stream() ), "hipEventRecord" );

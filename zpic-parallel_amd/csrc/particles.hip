@@ -419,6 +419,7 @@ struct PartArgs {
     SortView S;
     int waves;  // waves of the workgroup that take particles (0: all)
     int lbuf;   // order-free sort: leavers buffered in LDS (LDS sized for it)
+    int jset;   // fast mode: the tile's J region is written, not added to
 };
 
 struct PartG {
@@ -859,7 +860,11 @@ __device__ inline void particles_tile( const PartArgs & a, const int t, float * 
             const int el = k / 3, c = k - 3 * el;
             float s = Jall[c * vol + el];
             for ( int v = 1; v < W; v++ ) s += Jall[3 * vol * v + c * vol + el];
-            Jt[k] += s;
+            // jset (the first species of a step, fast mode): the region is
+            // written instead of zeroed and added to; the same value, since
+            // s is never -0 (every plane element starts at +0)
+            if ( !DET && a.jset ) Jt[k] = s;
+            else Jt[k] += s;
         }
         if ( tid == 0 ) {
             const gptr<int> seg = gbl( S.seg + 16 * t );
@@ -1849,7 +1854,8 @@ int check_threads( int threads ) {
 
 template <bool PUSH, bool MOVE>
 int launch_particles( const zpic_particles * p, const zpic_geom * fld, const float * E, const float * B, float * J,
-                      float alpha, MoveArgs m, double * tile_energy, zpic_sort * s, int det, void * stream ) {
+                      float alpha, MoveArgs m, double * tile_energy, zpic_sort * s, int det, void * stream,
+                      int jset = 0 ) {
     if ( int rc = check_geom( fld ) ) return rc;
     if ( !p || ( PUSH && ( !E || !B || !tile_energy ) ) || ( MOVE && ( !J || !s ) ) )
         return set_error( ZPIC_EINVAL, "null buffer" );
@@ -1864,6 +1870,7 @@ int launch_particles( const zpic_particles * p, const zpic_geom * fld, const flo
     a.P = view( p ); a.g = make_geo( fld );
     a.E = (const float3 *) E; a.B = (const float3 *) B; a.J = (float3 *) J;
     a.alpha = alpha; a.m = m; a.tile_energy = tile_energy; a.S = view( s );
+    a.jset = MOVE && !det && jset;
     if ( MOVE && !det && s->stage_ix ) {
         // the leaver buffers, where they cost no workgroup per CU
         const size_t with = particles_lds_bytes( fld->tile_vol, threads / WAVE, PUSH, MOVE, true );
@@ -2828,6 +2835,17 @@ int zpic_push_move_deposit_shift( const zpic_particles * p, const zpic_geom * fl
         return set_error( ZPIC_EINVAL, "cell shift must be -1, 0 or 1 per axis" );
     MoveArgs m{ dt_dx, dt_dy, q, qnx, qny, reflect, shift_x, shift_y };
     return launch_particles<true, true>( p, fld, E, B, J, alpha, m, tile_energy, s, deterministic, stream );
+}
+
+int zpic_push_move_deposit_overwrite( const zpic_particles * p, const zpic_geom * fld, const float * E,
+                                      const float * B, float * J, float alpha, float dt_dx, float dt_dy, float q,
+                                      float qnx, float qny, double * tile_energy, zpic_sort * s, int reflect,
+                                      int shift_x, int shift_y, void * stream ) {
+    if ( reflect < 0 || reflect > 15 ) return set_error( ZPIC_EINVAL, "reflect mask must be in [0, 15]" );
+    if ( shift_x < -1 || shift_x > 1 || shift_y < -1 || shift_y > 1 )
+        return set_error( ZPIC_EINVAL, "cell shift must be -1, 0 or 1 per axis" );
+    MoveArgs m{ dt_dx, dt_dy, q, qnx, qny, reflect, shift_x, shift_y };
+    return launch_particles<true, true>( p, fld, E, B, J, alpha, m, tile_energy, s, 0, stream, 1 );
 }
 
 int zpic_tile_sort_count( zpic_particles * p, zpic_particles * tmp, zpic_sort * s, int ntiles_x, int ntiles_y, int nx,

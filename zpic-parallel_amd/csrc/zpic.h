@@ -673,7 +673,11 @@ class Species {
     virtual void advance( EMF const & emf, Current & current );
     // advance() (window: advance_mov_window()) recording `after_push` (may be
     // null) between the particle pass and the sort
-    void advance_timed( EMF const & emf, Current & current, hipEvent_t after_push, bool window = false );
+    // set_j (fast mode): the particle pass writes J's tile regions instead
+    // of adding to them, for the first species of a step whose J was not
+    // zeroed
+    void advance_timed( EMF const & emf, Current & current, hipEvent_t after_push, bool window = false,
+                        bool set_j = false );
     // Species::advance_mov_window (species.cpp:2083-2128): on the steps where
     // the window moves, the particles shift one cell left and the density
     // profile is injected into the rightmost column of cells
@@ -693,7 +697,7 @@ class Species {
     // leaver classification; with the window's cell shift when `shift`),
     // and the sort, the injection when `shift`, the iteration count and the
     // energy sum.
-    void particle_pass( EMF const & emf, Current & current, bool shift );
+    void particle_pass( EMF const & emf, Current & current, bool shift, bool set_j = false );
     void set_staging( zpic_sort & s, bool window = false ) const;
     void finish_step( bool shift );
     // Whether this step shifts the particles (the window moves), for a step
@@ -702,6 +706,7 @@ class Species {
 
     // Deterministic (reference single-thread) deposit / energy order
     void set_deterministic( bool on ) { deterministic = on; }
+    bool is_deterministic() const { return deterministic; }
 
     void deposit_charge( grid<float> & charge ) const;
     // Diagnostics (species.cpp:2417-3100): PARTICLES/<name>/particles-<name>-<iter>.zdf,
@@ -752,7 +757,7 @@ class Species {
     void note_step() { iter++; }
 
   private:
-    void step( EMF const & emf, Current & current, hipEvent_t after_push, bool window );
+    void step( EMF const & emf, Current & current, hipEvent_t after_push, bool window, bool set_j = false );
 };
 
 // --------------------------------------------------------------- Simulation
